@@ -1,0 +1,76 @@
+"""Shared configurations and fixtures for the parity tests.
+
+The configurations follow the reference's parameter-file tags
+(Aarambam/LPT/Basis/read_param.c:27-184).  Data fixtures under tests/golden/
+are the reference's shipped tables (Aarambam/defaults/).
+"""
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+UNITS = dict(UnitLength_in_cm=3.085678e24, UnitMass_in_g=1.989e43, UnitVelocity_in_cm_per_s=1e5,
+             InputSpectrum_UnitLength_in_cm=3.085678e24)
+
+
+def base_config(**kw):
+    """Plain FP64 CPU-runnable case: Nmesh=64, Lbox=1000, EH P(k), local template."""
+    cfg = dict(Nmesh=64, Nsample=64, Nmax=64, N_modes=8, Seed=1234, SphereMode=0,
+               FixedAmplitude=0, PhaseFlip=0, WhichSpectrum=0, WhichTransfer=1, GlassTileFac=1,
+               WDM_On=0, Box=1000.0, Omega=0.3, OmegaLambda=0.7, OmegaBaryon=0.05,
+               HubbleParam=0.7, Sigma8=0.8, PrimordialIndex=0.9649, Redshift=50.0, Fnl=100.0,
+               ShapeGamma=0.0, WDM_PartMass_in_kev=10.0, **UNITS)
+    cfg.update(kw)
+    return cfg
+
+
+def load_transfer():
+    d = np.loadtxt(os.path.join(GOLDEN, "TransferFunctionConverted.dat"))
+    return d[:, 0].copy(), d[:, 1].copy()
+
+
+def alpha_rows_file(path=None):
+    path = path or os.path.join(GOLDEN, "AlphaTable.dat")
+    return np.loadtxt(path).reshape(-1, 4)
+
+
+def x0_file(Nm, path=None):
+    path = path or os.path.join(GOLDEN, "AiTable.dat")
+    d = np.loadtxt(path).reshape(-1, 3)
+    x0 = np.zeros((Nm, 2))
+    for r in d:
+        n = int(r[0])
+        if n < Nm:
+            x0[n] = r[1:]
+    return x0
+
+
+def local_rows():
+    """Local template as one separable triple (SingleField.py:19-27)."""
+    return np.array([[3, 0, 0, 1.0]])
+
+
+def equilateral_rows():
+    """Equilateral template triples (SingleField.py:45-78)."""
+    return np.array([[3, 0, 0, -3.0], [1, 1, 1, -2.0], [2, 1, 0, 6.0]])
+
+
+def x0_closed_form(Nm, ns, Box, Nmax):
+    """AiTable x0[p] = P_p(-1 - 2 kmin^s/(kmax^s - kmin^s)), decompose.py:176-216."""
+    from scipy import special
+    s = (4 - ns) / 3
+    kmin = 2 * np.pi / Box
+    kmax = kmin * Nmax
+    kbar = -1 - 2 * kmin ** s / (kmax ** s - kmin ** s)
+    x0 = np.zeros((Nm, 2))
+    for p in range(Nm):
+        x0[p, 0] = special.eval_legendre(p, kbar)
+    return x0
+
+
+def lattice_glass(n=64, box=100.0):
+    """A regular n^3 lattice in the layout of Aarambam/defaults/dummy_glass_dmonly_64.dat."""
+    g = np.arange(n, dtype=np.float64) * (box / n)
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    return np.stack([x, y, z], -1).reshape(-1, 3).astype(np.float32)
