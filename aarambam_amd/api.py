@@ -1,0 +1,293 @@
+"""ctypes binding of libaarambam_hip.so (include/aarambam.h).
+
+Fails loudly when the library has not been built; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libaarambam_hip.so")
+
+AAR_FP64, AAR_FP32 = 64, 32
+AAR_SCHED_PLAIN, AAR_SCHED_BATCHED = 0, 1
+FIELDS = dict(phig_spec=0, phi_spec=1, disp_x=2, disp_y=3, disp_z=4, disp2_x=5, disp2_y=6, disp2_z=7,
+              phig_real=8, phing_real=9)
+
+_INTS = ["Nmesh", "Nsample", "Nmax", "N_modes", "Seed", "SphereMode", "FixedAmplitude", "PhaseFlip",
+         "SavePotentialField", "WhichSpectrum", "WhichTransfer", "GlassTileFac",
+         "NumFilesWrittenInParallel", "WDM_On", "WDM_Vtherm_On"]
+_DBLS = ["Box", "Omega", "OmegaLambda", "OmegaBaryon", "OmegaDM_2ndSpecies", "HubbleParam", "Sigma8",
+         "PrimordialIndex", "Redshift", "Fnl", "ShapeGamma", "UnitLength_in_cm", "UnitMass_in_g",
+         "UnitVelocity_in_cm_per_s", "InputSpectrum_UnitLength_in_cm", "WDM_PartMass_in_kev"]
+_STRS = ["GlassFile", "FileWithInputSpectrum", "FileWithInputTransfer", "FileWithAlphaCoeff",
+         "FileWithX0Coeff", "OutputDir", "FileBase"]
+
+
+class AarParams(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in _INTS] + [(n, C.c_double) for n in _DBLS] +
+                [(n, C.c_char * 500) for n in _STRS])
+
+
+class AarTables(C.Structure):
+    _fields_ = [("alpha", C.POINTER(C.c_double)), ("x0", C.POINTER(C.c_double)),
+                ("ntransfer", C.c_int), ("transfer_k", C.POINTER(C.c_double)),
+                ("transfer_t", C.POINTER(C.c_double)), ("glass", C.POINTER(C.c_float)),
+                ("nglass", C.c_int), ("glass_box", C.c_double), ("glass_header", C.POINTER(C.c_uint8))]
+
+
+class AarCosmo(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ["UnitTime_in_s", "G", "Hubble", "InitTime", "Norm", "sigma8_in", "D0", "Dplus", "Anorm",
+                 "Beta", "hubble_a", "vel_prefac", "vel_prefac2", "fac"]]
+
+
+EXPORTS = ["aar_status_string", "aar_read_params", "aar_load_alpha", "aar_load_x0", "aar_load_transfer",
+           "aar_read_glass", "aar_cosmology", "aar_seedtable", "aar_ranlxd_nth", "aar_write_gadget",
+           "aar_write_phi", "aar_write_inputspec", "aar_ctx_create", "aar_ctx_destroy", "aar_ctx_sync", "aar_job_bytes",
+           "aar_job_prepare", "aar_job_run", "aar_job_sync", "aar_job_device_output", "aar_job_fetch",
+           "aar_job_fetch_field", "aar_job_last_seconds", "aar_job_destroy", "aar_fft3d",
+           "aar_gaussian_field", "aar_ranlxd_device", "aar_job_set_profiling", "aar_job_kernel_stats"]
+KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
+
+
+class AarError(RuntimeError):
+    def __init__(self, code, what=""):
+        msg = lib().aar_status_string(code).decode() if _lib is not None else ""
+        super().__init__(f"{what}: status {code} ({msg})")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m aarambam_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, fp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_float)
+        PP, TP, CP = C.POINTER(AarParams), C.POINTER(AarTables), C.POINTER(AarCosmo)
+        sig = {
+            "aar_status_string": (C.c_char_p, [C.c_int]),
+            "aar_read_params": (C.c_int, [C.c_char_p, PP, C.c_char_p, C.c_size_t]),
+            "aar_load_alpha": (C.c_int, [C.c_char_p, C.c_int, dp]),
+            "aar_load_x0": (C.c_int, [C.c_char_p, C.c_int, dp]),
+            "aar_load_transfer": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), dp, dp]),
+            "aar_read_glass": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), fp, dp, C.POINTER(C.c_uint8)]),
+            "aar_cosmology": (C.c_int, [PP, TP, CP]),
+            "aar_seedtable": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_uint32)]),
+            "aar_ranlxd_nth": (C.c_ulong, [C.c_ulong, C.c_int, C.c_int]),
+            "aar_write_gadget": (C.c_int, [PP, CP, fp, fp, C.c_longlong, C.POINTER(C.c_uint8)]),
+            "aar_write_phi": (C.c_int, [PP, dp, C.c_int]),
+            "aar_write_inputspec": (C.c_int, [PP, TP, CP]),
+            "aar_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+            "aar_ctx_destroy": (C.c_int, [vp]),
+            "aar_ctx_sync": (C.c_int, [vp]),
+            "aar_job_bytes": (C.c_longlong, [PP, TP, C.c_int, C.c_int]),
+            "aar_job_prepare": (C.c_int, [vp, PP, TP, C.c_int, C.c_int, C.POINTER(vp)]),
+            "aar_job_run": (C.c_int, [vp, vp]),
+            "aar_job_sync": (C.c_int, [vp]),
+            "aar_job_device_output": (C.c_int, [vp, C.POINTER(fp), C.POINTER(fp), C.POINTER(C.c_longlong)]),
+            "aar_job_fetch": (C.c_int, [vp, fp, fp]),
+            "aar_job_fetch_field": (C.c_int, [vp, C.c_int, dp]),
+            "aar_job_last_seconds": (C.c_double, [vp]),
+            "aar_job_destroy": (C.c_int, [vp]),
+            "aar_fft3d": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp]),
+            "aar_gaussian_field": (C.c_int, [vp, PP, TP, C.c_int, dp]),
+            "aar_ranlxd_device": (C.c_int, [vp, C.c_uint32, C.c_int, dp]),
+            "aar_job_set_profiling": (C.c_int, [vp, C.c_int]),
+            "aar_job_kernel_stats": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), dp, dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise AarError(rc, what)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def params_from_dict(cfg):
+    p = AarParams()
+    for n in _INTS:
+        setattr(p, n, int(cfg.get(n, 0)))
+    for n in _DBLS:
+        setattr(p, n, float(cfg.get(n, 0.0)))
+    for n in _STRS:
+        setattr(p, n, str(cfg.get(n, "")).encode())
+    return p
+
+
+def params_to_dict(p):
+    d = {n: getattr(p, n) for n in _INTS + _DBLS}
+    d.update({n: getattr(p, n).decode() for n in _STRS})
+    return d
+
+
+def read_params(path):
+    p = AarParams()
+    msg = C.create_string_buffer(4096)
+    rc = lib().aar_read_params(path.encode(), C.byref(p), msg, 4096)
+    if rc:
+        raise AarError(rc, msg.value.decode())
+    return p
+
+
+class tables:
+    """Holds the host arrays of one job and the aar_tables view of them."""
+
+    def __init__(self, alpha, x0, glass, glass_box, transfer_k=None, transfer_t=None, glass_header=None):
+        self.alpha = np.ascontiguousarray(alpha, dtype=np.float64).reshape(-1)
+        self.x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1)
+        self.glass = np.ascontiguousarray(glass, dtype=np.float32).reshape(-1)
+        self.tk = None if transfer_k is None else np.ascontiguousarray(transfer_k, dtype=np.float64)
+        self.tt = None if transfer_t is None else np.ascontiguousarray(transfer_t, dtype=np.float64)
+        self.hdr = None if glass_header is None else np.ascontiguousarray(glass_header, dtype=np.uint8)
+        t = AarTables()
+        t.alpha = _dptr(self.alpha)
+        t.x0 = _dptr(self.x0)
+        t.glass = self.glass.ctypes.data_as(C.POINTER(C.c_float))
+        t.nglass = self.glass.size // 3
+        t.glass_box = float(glass_box)
+        if self.tk is not None:
+            t.ntransfer = len(self.tk)
+            t.transfer_k = _dptr(self.tk)
+            t.transfer_t = _dptr(self.tt)
+        if self.hdr is not None:
+            t.glass_header = self.hdr.ctypes.data_as(C.POINTER(C.c_uint8))
+        self.c = t
+
+    @property
+    def nglass(self):
+        return self.glass.size // 3
+
+
+def cosmology(params, tabs):
+    c = AarCosmo()
+    _check(lib().aar_cosmology(C.byref(params), C.byref(tabs.c), C.byref(c)), "aar_cosmology")
+    return {n: getattr(c, n) for n, _ in AarCosmo._fields_}
+
+
+def seedtable(N, seed):
+    out = np.empty(N * N, np.uint32)
+    _check(lib().aar_seedtable(N, seed, out.ctypes.data_as(C.POINTER(C.c_uint32))), "aar_seedtable")
+    return out.reshape(N, N)
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().aar_ctx_create(device, C.byref(h)), "aar_ctx_create")
+        self.h = h
+
+    def sync(self):
+        _check(lib().aar_ctx_sync(self.h), "aar_ctx_sync")
+
+    def close(self):
+        if self.h:
+            lib().aar_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fft3d(self, data, dir, precision=AAR_FP64):
+        data = np.asarray(data)
+        N = data.shape[0]
+        if dir > 0:
+            inp = np.ascontiguousarray(data, dtype=np.complex128).view(np.float64).reshape(-1)
+            out = np.empty(N ** 3, np.float64)
+        else:
+            inp = np.ascontiguousarray(data, dtype=np.float64).reshape(-1)
+            out = np.empty(2 * N * N * (N // 2 + 1), np.float64)
+        _check(lib().aar_fft3d(self.h, N, precision, dir, _dptr(inp), _dptr(out)), "aar_fft3d")
+        if dir > 0:
+            return out.reshape(N, N, N)
+        return out.view(np.complex128).reshape(N, N, N // 2 + 1)
+
+    def gaussian_field(self, params, tabs, precision=AAR_FP64):
+        N = params.Nmesh
+        out = np.empty(2 * N * N * (N // 2 + 1), np.float64)
+        _check(lib().aar_gaussian_field(self.h, C.byref(params), C.byref(tabs.c), precision, _dptr(out)),
+               "aar_gaussian_field")
+        return out.view(np.complex128).reshape(N, N, N // 2 + 1)
+
+    def ranlxd(self, seed, n):
+        out = np.empty(n, np.float64)
+        _check(lib().aar_ranlxd_device(self.h, seed, n, _dptr(out)), "aar_ranlxd_device")
+        return out
+
+
+class Job:
+    def __init__(self, ctx, params, tabs, precision=AAR_FP64, schedule=AAR_SCHED_BATCHED):
+        self.ctx, self.params, self.tabs = ctx, params, tabs
+        h = C.c_void_p()
+        _check(lib().aar_job_prepare(ctx.h, C.byref(params), C.byref(tabs.c), precision, schedule,
+                                     C.byref(h)), "aar_job_prepare")
+        self.h = h
+        self.N = params.Nmesh
+        self.npart = params.GlassTileFac ** 3 * tabs.nglass
+
+    def run(self, stream=None):
+        _check(lib().aar_job_run(self.h, stream), "aar_job_run")
+        return self
+
+    def sync(self):
+        _check(lib().aar_job_sync(self.h), "aar_job_sync")
+        return lib().aar_job_last_seconds(self.h)
+
+    def fetch(self):
+        pos = np.empty(3 * self.npart, np.float32)
+        vel = np.empty(3 * self.npart, np.float32)
+        fpt = C.POINTER(C.c_float)
+        _check(lib().aar_job_fetch(self.h, pos.ctypes.data_as(fpt), vel.ctypes.data_as(fpt)), "aar_job_fetch")
+        return pos.reshape(-1, 3), vel.reshape(-1, 3)
+
+    def field(self, name):
+        N = self.N
+        which = FIELDS[name]
+        if which <= 1:
+            out = np.empty(2 * N * N * (N // 2 + 1), np.float64)
+        else:
+            out = np.empty(N ** 3, np.float64)
+        _check(lib().aar_job_fetch_field(self.h, which, _dptr(out)), "aar_job_fetch_field")
+        if which <= 1:
+            return out.view(np.complex128).reshape(N, N, N // 2 + 1)
+        return out.reshape(N, N, N)
+
+    def set_profiling(self, on=True):
+        _check(lib().aar_job_set_profiling(self.h, int(on)), "aar_job_set_profiling")
+
+    def kernel_stats(self):
+        out = {}
+        for k, name in enumerate(KINDS):
+            n, s, b = C.c_int(0), C.c_double(0), C.c_double(0)
+            _check(lib().aar_job_kernel_stats(self.h, k, C.byref(n), C.byref(s), C.byref(b)), "stats")
+            if n.value:
+                out[name] = dict(launches=n.value, seconds=s.value, bytes=b.value)
+        return out
+
+    def close(self):
+        if self.h:
+            lib().aar_job_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,75 @@
+// Shared types and helpers for the MI355X NG-2LPT library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#define AAR_PI 3.14159265358979323846  // Aarambam/LPT/Basis/allvars.h:4
+
+// Status codes.  Numbers below 100000 are the reference's FatalError codes
+// (Aarambam/LPT/Basis/main.c:1612-1618 and the call sites listed there).
+enum {
+  AAR_OK = 0,
+  AAR_ERR_ALLOC = 1,          // main.c ASSERT_ALLOC -> FatalError(1)
+  AAR_ERR_TABLE_FILE = 17,    // missing alpha / X0 / transfer file (main.c:230,270; power.c:81)
+  AAR_ERR_KLOWER = 1111,      // transfer table does not reach k < 10^-4.6 (power.c:117-121)
+  AAR_ERR_WRITE_PARALLEL = 24131,  // NumFilesWrittenInParallel > NTask (save.c:112-117)
+  AAR_ERR_BAD_ARG = 100001,   // invalid argument at the C-ABI
+  AAR_ERR_HIP = 100002,       // a HIP runtime call failed
+  AAR_ERR_HBM_BUDGET = 100003,  // job does not fit the admitted HBM budget
+  AAR_ERR_NMAX = 100004,      // Nmax < Nsample (main.c:200 assert)
+  AAR_ERR_UNSUPPORTED = 100005,  // a configuration the reference does not run either
+  AAR_ERR_NO_DEVICE = 100006,
+};
+
+#define AAR_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, \
+              __LINE__);                                                           \
+      return AAR_ERR_HIP;                                                          \
+    }                                                                              \
+  } while (0)
+
+template <typename T> struct Cx;
+template <> struct Cx<float> { using t = float2; };
+template <> struct Cx<double> { using t = double2; };
+template <typename T> using cx_t = typename Cx<T>::t;
+
+template <typename C> __host__ __device__ __forceinline__ C cmake(decltype(C::x) a, decltype(C::x) b) {
+  C r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmake<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmake<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  return cmake<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename C> __device__ __forceinline__ C cconj(C a) { return cmake<C>(a.x, -a.y); }
+template <typename C> __device__ __forceinline__ C cscale(C a, decltype(C::x) s) { return cmake<C>(a.x * s, a.y * s); }
+// multiply by DIR*i (DIR = +1 or -1)
+template <int DIR, typename C> __device__ __forceinline__ C cmul_i(C a) {
+  return DIR > 0 ? cmake<C>(-a.y, a.x) : cmake<C>(a.y, -a.x);
+}
+
+// Grid geometry of one field in HBM.  A spectrum holds the kz < N/2 bins as
+// main[x][y][kz] (N*N*H complex, H = N/2) followed by the kz = N/2 bins as
+// nyq[x][y] (N*N complex).  The same buffer holds a real field as
+// real[x][y][z] (N^3 reals) in the bytes of the main block.
+struct Geom {
+  int N, H;
+  long long nq;  // offset (complex elements) of the Nyquist plane = N*N*H
+  long long fs;  // field stride (complex elements) = N*N*(H+1)
+};
+
+static inline Geom make_geom(int N) {
+  Geom g;
+  g.N = N;
+  g.H = N / 2;
+  g.nq = (long long)N * N * (N / 2);
+  g.fs = (long long)N * N * (N / 2 + 1);
+  return g;
+}

@@ -1,0 +1,194 @@
+// Batched Stockham FFT on an LDS tile for gfx950.
+//
+// A tile holds C independent transforms ("columns") of length L, element
+// (pos, col) at tile[pos*CP + col] (CP = C + 1, padded against bank
+// conflicts).  NT threads cooperate; thread t owns column t % C and
+// E = C*L/NT elements per stage.  Each stage of radix R performs, for every
+// butterfly index i in [0, L/R) with k = i mod P (P = product of the earlier
+// radices):  v_t = x[i + t L/R] * w^(t k L/(P R)),  y = DFT_R(v),
+// x'[(i-k) R + k + u P] = y_u   (Stockham autosort, no bit reversal).
+// The first stage reads through a caller-supplied loader (global memory or
+// LDS) and the last stage writes through a caller-supplied storer, so loads
+// and stores to HBM are coalesced across the C adjacent columns and the
+// intermediate stages stay in LDS.
+#pragma once
+#include "aar_common.h"
+
+__host__ __device__ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
+
+// Balanced radix plan: ceil(log2 L / log2 Rmax) stages, bits spread evenly,
+// larger radices first.
+__host__ __device__ constexpr int plan_nstages(int L, int Rmax) {
+  return (ilog2c(L) + ilog2c(Rmax) - 1) / ilog2c(Rmax);
+}
+__host__ __device__ constexpr int plan_radix(int L, int Rmax, int s) {
+  return 1 << (ilog2c(L) / plan_nstages(L, Rmax) + (s < ilog2c(L) % plan_nstages(L, Rmax) ? 1 : 0));
+}
+__host__ __device__ constexpr int plan_prefix(int L, int Rmax, int s) {
+  return s == 0 ? 1 : plan_prefix(L, Rmax, s - 1) * plan_radix(L, Rmax, s - 1);
+}
+
+// LDS row pitch of a tile: one padding column when the tile plus its twiddle
+// table stay below 152 KiB of the 160 KiB of LDS of a gfx950 CU.
+template <typename T> __host__ __device__ constexpr int tile_cp(int L, int C) {
+  return ((long)L * (C + 1) + L) * (long)sizeof(cx_t<T>) <= 152 * 1024 ? C + 1 : C;
+}
+
+// ---------------------------------------------------------------- DFT kernels
+template <int DIR, typename C> __device__ __forceinline__ void dft2(C& a, C& b) {
+  C t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <int DIR, typename C> __device__ __forceinline__ void dft4(C& x0, C& x1, C& x2, C& x3) {
+  C a = cadd(x0, x2), b = csub(x0, x2), c = cadd(x1, x3), d = cmul_i<DIR>(csub(x1, x3));
+  x0 = cadd(a, c);
+  x2 = csub(a, c);
+  x1 = cadd(b, d);
+  x3 = csub(b, d);
+}
+
+template <int DIR, typename C> __device__ __forceinline__ void dft4v(C* v) { dft4<DIR>(v[0], v[1], v[2], v[3]); }
+
+template <int DIR, typename C> __device__ __forceinline__ void dft8v(C* v) {
+  using T = decltype(C::x);
+  const T h = (T)0.70710678118654752440;
+  C e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  C o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<DIR>(e0, e1, e2, e3);
+  dft4<DIR>(o0, o1, o2, o3);
+  // twiddles W8^u, W8 = exp(DIR * 2 pi i / 8)
+  C t1 = cmake<C>(h * (o1.x - (T)DIR * o1.y), h * (o1.y + (T)DIR * o1.x));
+  C t2 = cmul_i<DIR>(o2);
+  C t3 = cmake<C>(h * (-o3.x - (T)DIR * o3.y), h * (-o3.y + (T)DIR * o3.x));
+  v[0] = cadd(e0, o0);
+  v[4] = csub(e0, o0);
+  v[1] = cadd(e1, t1);
+  v[5] = csub(e1, t1);
+  v[2] = cadd(e2, t2);
+  v[6] = csub(e2, t2);
+  v[3] = cadd(e3, t3);
+  v[7] = csub(e3, t3);
+}
+
+template <int DIR, typename C> __device__ __forceinline__ void dft16v(C* v) {
+  using T = decltype(C::x);
+  // n = 4 n1 + n2, u = u1 + 4 u2
+  C a[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    a[n2][0] = v[n2];
+    a[n2][1] = v[4 + n2];
+    a[n2][2] = v[8 + n2];
+    a[n2][3] = v[12 + n2];
+    dft4<DIR>(a[n2][0], a[n2][1], a[n2][2], a[n2][3]);
+  }
+  // twiddle W16^(n2 u1)
+  const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173;
+  const T h = (T)0.70710678118654752440;
+#pragma unroll
+  for (int n2 = 1; n2 < 4; ++n2) {
+#pragma unroll
+    for (int u1 = 1; u1 < 4; ++u1) {
+      const int m = n2 * u1;  // 1..9, angle DIR * 2 pi m / 16
+      T cr, ci;
+      switch (m) {
+        case 1: cr = c1; ci = s1; break;
+        case 2: cr = h; ci = h; break;
+        case 3: cr = s1; ci = c1; break;
+        case 4: cr = 0; ci = 1; break;
+        case 6: cr = -h; ci = h; break;
+        case 9: cr = -c1; ci = -s1; break;
+        default: cr = 1; ci = 0; break;
+      }
+      ci *= (T)DIR;
+      C x = a[n2][u1];
+      a[n2][u1] = cmake<C>(x.x * cr - x.y * ci, x.x * ci + x.y * cr);
+    }
+  }
+#pragma unroll
+  for (int u1 = 0; u1 < 4; ++u1) {
+    C b0 = a[0][u1], b1 = a[1][u1], b2 = a[2][u1], b3 = a[3][u1];
+    dft4<DIR>(b0, b1, b2, b3);
+    v[u1] = b0;
+    v[u1 + 4] = b1;
+    v[u1 + 8] = b2;
+    v[u1 + 12] = b3;
+  }
+}
+
+template <int R, int DIR, typename C> __device__ __forceinline__ void dftR(C* v) {
+  if constexpr (R == 2) dft2<DIR>(v[0], v[1]);
+  else if constexpr (R == 4) dft4v<DIR>(v);
+  else if constexpr (R == 8) dft8v<DIR>(v);
+  else dft16v<DIR>(v);
+}
+
+// ---------------------------------------------------------------- tile FFT
+// tw: LDS table with tw[m*TWS] = exp(-2 pi i m / L) for m in [0, L).
+template <typename T, int L, int C, int NT, int DIR, int TWS, int S, bool SRC_LDS, class LoadF,
+          class StoreF>
+__device__ __forceinline__ void fft_stage(cx_t<T>* tile, const cx_t<T>* tw, LoadF& load, StoreF& store) {
+  using C2 = cx_t<T>;
+  constexpr int CP = tile_cp<T>(L, C);
+  constexpr int E = C * L / NT;
+  constexpr int RMAX = E < 16 ? E : 16;
+  constexpr int NS = plan_nstages(L, RMAX);
+  constexpr int R = plan_radix(L, RMAX, S);
+  constexpr int P = plan_prefix(L, RMAX, S);
+  constexpr int NB = E / R;
+  constexpr int STR = L / R;
+  constexpr int ROWS = NT / C;
+  static_assert(E % R == 0, "per-thread elements must be a multiple of the radix");
+  const int col = threadIdx.x % C;
+  const int i0 = threadIdx.x / C;
+  C2 v[E];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int i = i0 + q * ROWS;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      const int pos = i + t * STR;
+      if constexpr (S == 0) v[q * R + t] = load(pos, col);
+      else v[q * R + t] = tile[pos * CP + col];
+    }
+  }
+  if constexpr (S > 0 || SRC_LDS) __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int i = i0 + q * ROWS;
+    const int k = i % P;
+    if constexpr (S > 0) {
+#pragma unroll
+      for (int t = 1; t < R; ++t) {
+        C2 w = tw[(t * k * (L / (P * R))) * TWS];
+        if constexpr (DIR > 0) w.y = -w.y;
+        v[q * R + t] = cmul(v[q * R + t], w);
+      }
+    }
+    dftR<R, DIR>(&v[q * R]);
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const int pos = (i - k) * R + k + u * P;
+      if constexpr (S == NS - 1) store(pos, col, v[q * R + u]);
+      else tile[pos * CP + col] = v[q * R + u];
+    }
+  }
+  if constexpr (S < NS - 1) {
+    __syncthreads();
+    fft_stage<T, L, C, NT, DIR, TWS, S + 1, SRC_LDS, LoadF, StoreF>(tile, tw, load, store);
+  }
+}
+
+template <typename T, int L, int C, int NT, int DIR, int TWS, bool SRC_LDS, class LoadF, class StoreF>
+__device__ __forceinline__ void fft_tile(cx_t<T>* tile, const cx_t<T>* tw, LoadF load, StoreF store) {
+  __syncthreads();  // the tile may still be read by the previous transform
+  fft_stage<T, L, C, NT, DIR, TWS, 0, SRC_LDS, LoadF, StoreF>(tile, tw, load, store);
+}
+
+// Copy a global twiddle table (exp(-2 pi i m / n), m < n) into LDS.
+template <typename C2>
+__device__ __forceinline__ void load_twiddles(C2* dst, const C2* src, int n, int nt) {
+  for (int m = threadIdx.x; m < n; m += nt) dst[m] = src[m];
+}

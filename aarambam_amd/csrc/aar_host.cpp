@@ -1,0 +1,687 @@
+// Host side of the drop-in: parameter file, coefficient / transfer / glass
+// readers, cosmology scalars, seed table and the Gadget, potential and
+// input-spectrum writers.  Semantics follow Aarambam/LPT/Basis for one
+// process holding the whole grid (NTask = NTaskWithN = 1).
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/aarambam.h"
+#include "aar_common.h"
+#include "aar_ranlxd.h"
+
+static const double kGravity = 6.672e-8;    // allvars.h:5
+static const double kHubble = 3.2407789e-18;  // allvars.h:6
+static const double kPi = AAR_PI;
+
+extern "C" const char* aar_status_string(int s) {
+  switch (s) {
+    case AAR_OK: return "ok";
+    case AAR_ERR_ALLOC: return "allocation failed (FatalError 1)";
+    case 2: return "incorrect glass header size (FatalError 2)";
+    case 3: return "incorrect glass position block (FatalError 3)";
+    case 10: return "cannot write output file (FatalError 10)";
+    case AAR_ERR_TABLE_FILE: return "cannot read coefficient or transfer table (FatalError 17)";
+    case 121: return "glass file not found (FatalError 121)";
+    case 777: return "duplicate k in transfer table (FatalError 777)";
+    case 778: return "read error (FatalError 778)";
+    case AAR_ERR_KLOWER: return "transfer table klower too large (FatalError 1111)";
+    case AAR_ERR_WRITE_PARALLEL:
+      return "NumFilesWrittenInParallel exceeds the number of tasks (FatalError 24131)";
+    case AAR_ERR_BAD_ARG: return "invalid argument";
+    case AAR_ERR_HIP: return "HIP runtime error";
+    case AAR_ERR_HBM_BUDGET: return "job does not fit the HBM budget";
+    case AAR_ERR_NMAX: return "Nmax < Nsample (main.c:200 assert)";
+    case AAR_ERR_UNSUPPORTED: return "configuration not supported";
+    case AAR_ERR_NO_DEVICE: return "no HIP device";
+    case 100010: return "parameter file error";
+    default: return "unknown status";
+  }
+}
+
+// ------------------------------------------------------------------ parameter file
+extern "C" int aar_read_params(const char* path, aar_params* out, char* msg, size_t msglen) {
+  if (!path || !out) return AAR_ERR_BAD_ARG;
+  enum { F, S, I };
+  struct Tag {
+    const char* name;
+    int type;
+    void* addr;
+    bool seen;
+  };
+  aar_params p;
+  memset(&p, 0, sizeof(p));
+  std::vector<Tag> tags = {
+      {"Omega", F, &p.Omega},
+      {"OmegaLambda", F, &p.OmegaLambda},
+      {"OmegaBaryon", F, &p.OmegaBaryon},
+      {"OmegaDM_2ndSpecies", F, &p.OmegaDM_2ndSpecies},
+      {"HubbleParam", F, &p.HubbleParam},
+      {"ShapeGamma", F, &p.ShapeGamma},
+      {"Sigma8", F, &p.Sigma8},
+      {"PrimordialIndex", F, &p.PrimordialIndex},
+      {"Box", F, &p.Box},
+      {"Redshift", F, &p.Redshift},
+      {"Fnl", F, &p.Fnl},
+      {"N_modes", I, &p.N_modes},
+      {"FileWithAlphaCoeff", S, p.FileWithAlphaCoeff},
+      {"FileWithX0Coeff", S, p.FileWithX0Coeff},
+      {"SavePotentialField", I, &p.SavePotentialField},
+      {"FixedAmplitude", I, &p.FixedAmplitude},
+      {"PhaseFlip", I, &p.PhaseFlip},
+      {"Nmesh", I, &p.Nmesh},
+      {"Nsample", I, &p.Nsample},
+      {"Nmax", I, &p.Nmax},
+      {"GlassFile", S, p.GlassFile},
+      {"FileWithInputSpectrum", S, p.FileWithInputSpectrum},
+      {"FileWithInputTransfer", S, p.FileWithInputTransfer},
+      {"GlassTileFac", I, &p.GlassTileFac},
+      {"Seed", I, &p.Seed},
+      {"SphereMode", I, &p.SphereMode},
+      {"NumFilesWrittenInParallel", I, &p.NumFilesWrittenInParallel},
+      {"OutputDir", S, p.OutputDir},
+      {"FileBase", S, p.FileBase},
+      {"WhichSpectrum", I, &p.WhichSpectrum},
+      {"WhichTransfer", I, &p.WhichTransfer},
+      {"UnitVelocity_in_cm_per_s", F, &p.UnitVelocity_in_cm_per_s},
+      {"UnitLength_in_cm", F, &p.UnitLength_in_cm},
+      {"UnitMass_in_g", F, &p.UnitMass_in_g},
+      {"InputSpectrum_UnitLength_in_cm", F, &p.InputSpectrum_UnitLength_in_cm},
+      {"WDM_On", I, &p.WDM_On},
+      {"WDM_Vtherm_On", I, &p.WDM_Vtherm_On},
+      {"WDM_PartMass_in_kev", F, &p.WDM_PartMass_in_kev},
+  };
+  std::string report;
+  bool error = false;
+  FILE* fd = fopen(path, "r");
+  if (!fd) {
+    report += std::string("Parameter file ") + path + " not found.\n";
+    error = true;
+  } else {
+    // read_param.c:186-229: lines of at most 199 characters, first two
+    // whitespace-separated tokens, '%' comments; each tag exactly once.
+    char buf[200], b1[200], b2[200], b3[200];
+    while (!feof(fd)) {
+      buf[0] = 0;
+      if (!fgets(buf, 200, fd)) buf[0] = 0;
+      if (sscanf(buf, "%199s%199s%199s", b1, b2, b3) < 2) continue;
+      if (b1[0] == '%') continue;
+      Tag* hit = nullptr;
+      for (auto& t : tags)
+        if (!t.seen && strcmp(b1, t.name) == 0) {
+          hit = &t;
+          break;
+        }
+      if (!hit) {
+        report += std::string("Error in file ") + path + ":   Tag '" + b1 +
+                  "' not allowed or multiple defined.\n";
+        error = true;
+        continue;
+      }
+      hit->seen = true;
+      if (hit->type == F) *(double*)hit->addr = atof(b2);
+      else if (hit->type == I) *(int*)hit->addr = atoi(b2);
+      else {
+        strncpy((char*)hit->addr, b2, AAR_STR - 1);
+        ((char*)hit->addr)[AAR_STR - 1] = 0;
+      }
+    }
+    fclose(fd);
+  }
+  for (auto& t : tags)
+    if (!t.seen) {
+      report += std::string("Error. I miss a value for tag '") + t.name + "' in parameter file '" +
+                path + "'.\n";
+      error = true;
+    }
+  if (msg && msglen) {
+    strncpy(msg, report.c_str(), msglen - 1);
+    msg[msglen - 1] = 0;
+  }
+  *out = p;
+  return error ? 100010 : AAR_OK;
+}
+
+// ------------------------------------------------------------------ tables
+extern "C" int aar_load_alpha(const char* path, int nm, double* a) {
+  if (!path || !a || nm < 3) return AAR_ERR_BAD_ARG;
+  FILE* fd = fopen(path, "r");
+  if (!fd) return AAR_ERR_TABLE_FILE;
+  std::fill(a, a + (size_t)nm * nm * nm, 0.0);
+  int n1, n2, n3;
+  double v;
+  int rc = AAR_OK;
+  auto at = [&](int x, int y, int z) -> double& { return a[((size_t)x * nm + y) * nm + z]; };
+  while (fscanf(fd, " %d %d %d %lf ", &n1, &n2, &n3, &v) == 4) {  // main.c:231
+    if (n1 < 0 || n2 < 0 || n3 < 0 || n1 >= nm || n2 >= nm || n3 >= nm) {
+      rc = AAR_ERR_BAD_ARG;  // the reference writes outside its table here
+      break;
+    }
+    at(n3, n2, n1) = v;
+    at(n3, n1, n2) = v;
+    at(n2, n3, n1) = v;
+    at(n2, n1, n3) = v;
+    at(n1, n3, n2) = v;
+    at(n1, n2, n3) = v;
+  }
+  fclose(fd);
+  return rc;
+}
+
+extern "C" int aar_load_x0(const char* path, int nm, double* x0) {
+  if (!path || !x0 || nm < 1) return AAR_ERR_BAD_ARG;
+  FILE* fd = fopen(path, "r");
+  if (!fd) return AAR_ERR_TABLE_FILE;
+  std::fill(x0, x0 + 2 * (size_t)nm, 0.0);
+  int n;
+  double a, b;
+  int rc = AAR_OK;
+  while (fscanf(fd, " %d %lf %lf ", &n, &a, &b) == 3) {  // main.c:274-285
+    if (n < 0 || n >= nm) continue;  // rows beyond N_modes are never read by the sum
+    x0[2 * n] = a;
+    x0[2 * n + 1] = b;
+  }
+  fclose(fd);
+  return rc;
+}
+
+extern "C" int aar_load_transfer(const char* path, int* nrows, double* k, double* t) {
+  if (!path || !nrows) return AAR_ERR_BAD_ARG;
+  FILE* fd = fopen(path, "r");
+  if (!fd) return AAR_ERR_TABLE_FILE;
+  double a, b;
+  int n = 0;
+  while (fscanf(fd, " %le %le ", &a, &b) == 2) {  // power.c:85-109
+    if (k && t) {
+      if (n >= *nrows) break;
+      k[n] = a;
+      t[n] = b;
+    }
+    ++n;
+  }
+  fclose(fd);
+  *nrows = n;
+  return AAR_OK;
+}
+
+// Gadget-1 glass, possibly split over name.0, name.1, ... (read_glass.c:25-88, 226-260).
+extern "C" int aar_read_glass(const char* path, int* nglass, float* pos, double* box,
+                              uint8_t* header_out) {
+  if (!path || !nglass) return AAR_ERR_BAD_ARG;
+  std::string base(path);
+  int numfiles = 0;
+  uint8_t hdr[256];
+  {
+    std::string f0 = base + ".0";
+    FILE* fd = fopen(f0.c_str(), "rb");
+    int dummy;
+    if (fd) {
+      if (fread(&dummy, 4, 1, fd) != 1 || fread(hdr, 256, 1, fd) != 1) {
+        fclose(fd);
+        return 778;
+      }
+      fclose(fd);
+      memcpy(&numfiles, hdr + 124, 4);  // num_files
+    } else if ((fd = fopen(base.c_str(), "rb"))) {
+      fclose(fd);
+      numfiles = 1;
+    } else {
+      return 121;
+    }
+  }
+  long long total = 0, filled = 0;
+  for (int num = 0; num < numfiles; ++num) {
+    std::string fn = numfiles > 1 ? base + "." + std::to_string(num) : base;
+    FILE* fd = fopen(fn.c_str(), "rb");
+    if (!fd) return 1;
+    unsigned d1 = 0, d2 = 0;
+    uint8_t h[256];
+    if (fread(&d1, 4, 1, fd) != 1 || fread(h, 256, 1, fd) != 1 || fread(&d2, 4, 1, fd) != 1) {
+      fclose(fd);
+      return 778;
+    }
+    if (d1 != 256 || d2 != 256) {
+      fclose(fd);
+      return 2;
+    }
+    unsigned npart[6], nall[6];
+    memcpy(npart, h, 24);
+    memcpy(nall, h + 96, 24);
+    long long nlocal = 0;
+    for (int q = 0; q < 6; ++q) nlocal += npart[q];
+    if (num == 0) {
+      total = 0;
+      for (int q = 0; q < 6; ++q) total += nall[q];
+      if (box) memcpy(box, h + 128, 8);
+      if (header_out) memcpy(header_out, h, 256);
+      if (!pos) {
+        fclose(fd);
+        *nglass = (int)total;
+        return AAR_OK;
+      }
+      if (total > *nglass) {
+        fclose(fd);
+        return AAR_ERR_BAD_ARG;
+      }
+    }
+    if (fread(&d1, 4, 1, fd) != 1) {
+      fclose(fd);
+      return 778;
+    }
+    if (filled + nlocal > total ||
+        fread(pos + 3 * filled, sizeof(float), 3 * nlocal, fd) != (size_t)(3 * nlocal) ||
+        fread(&d2, 4, 1, fd) != 1) {
+      fclose(fd);
+      return 778;
+    }
+    if (d1 != 12 * nlocal || d2 != 12 * nlocal) {
+      fclose(fd);
+      return 3;
+    }
+    filled += nlocal;
+    fclose(fd);
+  }
+  *nglass = (int)total;
+  return AAR_OK;
+}
+
+// ------------------------------------------------------------------ cosmology
+namespace {
+struct Cosmo {
+  const aar_params& p;
+  const aar_tables* t;
+  double norm = 1.0, r_tophat = 0.0;
+  std::vector<double> logk, logT;
+  int status = AAR_OK;
+
+  Cosmo(const aar_params& pp, const aar_tables* tt) : p(pp), t(tt) {
+    if (p.WhichTransfer == 2) {  // read_transfer_table, power.c:111-131
+      if (!t || t->ntransfer < 2 || !t->transfer_k || !t->transfer_t) {
+        status = AAR_ERR_TABLE_FILE;
+        return;
+      }
+      std::vector<std::pair<double, double>> rows;
+      for (int i = 0; i < t->ntransfer; ++i)
+        rows.push_back({log10(t->transfer_k[i]), log10(t->transfer_t[i])});
+      std::stable_sort(rows.begin(), rows.end(),
+                       [](const std::pair<double, double>& a, const std::pair<double, double>& b) {
+                         return a.first < b.first;
+                       });
+      if (rows[0].first >= -4.6) {
+        status = AAR_ERR_KLOWER;
+        return;
+      }
+      for (size_t i = 1; i < rows.size(); ++i)
+        if (rows[i].first == rows[i - 1].first) status = 777;
+      for (auto& r : rows) {
+        logk.push_back(r.first);
+        logT.push_back(r.second - rows[0].second);
+      }
+    }
+  }
+
+  double tk_eh(double k) const {  // power.c:519-548
+    double hubble = p.HubbleParam, omegam = p.Omega;
+    double ombh2 = p.OmegaBaryon * p.HubbleParam * p.HubbleParam;
+    if (p.OmegaBaryon == 0) ombh2 = 0.04 * p.HubbleParam * p.HubbleParam;
+    k *= (3.085678e24 / p.UnitLength_in_cm);
+    double theta = 2.728 / 2.7;
+    double ommh2 = omegam * hubble * hubble;
+    double s = 44.5 * log(9.83 / ommh2) / sqrt(1. + 10. * exp(0.75 * log(ombh2))) * hubble;
+    double a = 1. - 0.328 * log(431. * ommh2) * ombh2 / ommh2 +
+               0.380 * log(22.3 * ommh2) * (ombh2 / ommh2) * (ombh2 / ommh2);
+    double gamma = a + (1. - a) / (1. + exp(4 * log(0.43 * k * s)));
+    gamma *= omegam * hubble;
+    double q = k * theta * theta / gamma;
+    double L0 = log(2. * exp(1.) + 1.8 * q);
+    double C0 = 14.2 + 731. / (1. + 62.5 * q);
+    return L0 / (L0 + C0 * q * q);
+  }
+
+  double transfer(double k) const {  // power.c:32-63
+    if (p.WhichTransfer == 1) return tk_eh(k);
+    if (p.WhichTransfer == 2) {  // power.c:181-217
+      k *= (p.InputSpectrum_UnitLength_in_cm / p.UnitLength_in_cm);
+      double lk = log10(k);
+      if (lk < logk.front() || lk > logk.back()) return 0;
+      int lo = 0, hi = (int)logk.size() - 1;
+      while (hi - lo > 1) {
+        int mid = (hi + lo) / 2;
+        if (lk < logk[mid]) hi = mid;
+        else lo = mid;
+      }
+      double u = (lk - logk[lo]) / (logk[hi] - logk[lo]);
+      return pow(10.0, (1 - u) * logT[lo] + u * logT[hi]);
+    }
+    return 1.0;  // tk_Efstathiou
+  }
+
+  double powerspec(double k) const {  // power.c:240-288
+    double power;
+    if (p.WhichSpectrum == 0) {
+      power = norm * exp(p.PrimordialIndex * log(k)) * transfer(k) * transfer(k);
+    } else {  // 1: Eisenstein & Hu spectrum
+      power = norm * k * pow(tk_eh(k), 2);
+      power *= pow(k, p.PrimordialIndex - 1.0);
+    }
+    if (p.WDM_On == 1) {
+      double alpha = 0.048 * pow((p.Omega - p.OmegaBaryon) / 0.4, 0.15) *
+                     pow(p.HubbleParam / 0.65, 1.3) * pow(1.0 / p.WDM_PartMass_in_kev, 1.15);
+      double Tf = pow(1 + pow(alpha * k * (3.085678e24 / p.UnitLength_in_cm), 2 * 1.2), -5.0 / 1.2);
+      power *= Tf * Tf;
+    }
+    return power;
+  }
+
+  double sigma2_int(double k) const {  // power.c:561-576
+    double kr = r_tophat * k, kr2 = kr * kr, kr3 = kr2 * kr;
+    if (kr < 1e-8) return 0;
+    double w = 3 * (sin(kr) / kr3 - cos(kr) / kr2);
+    return 4 * kPi * k * k * w * w * powerspec(k);
+  }
+
+  double growth_int(double a) const {  // power.c:595-598
+    return pow(a / (p.Omega + (1 - p.Omega - p.OmegaLambda) * a + p.OmegaLambda * a * a * a), 1.5);
+  }
+
+  // Romberg integration of Numerical Recipes (nrsrc/qromb.c, trapzd.c,
+  // polint.c): trapezoid refinements, K=5 point polynomial extrapolation to
+  // h=0, stop when |error estimate| <= 1e-8 |value|, at most 40 steps.
+  template <class Fn> double romberg(Fn f, double a, double b) const {
+    double s[42], h[43];
+    double st = 0;
+    h[1] = 1.0;
+    for (int j = 1; j <= 40; ++j) {
+      if (j == 1) {
+        st = 0.5 * (b - a) * (f(a) + f(b));
+      } else {
+        int it = 1;
+        for (int q = 1; q < j - 1; ++q) it <<= 1;
+        double tnm = it, del = (b - a) / tnm, x = a + 0.5 * del, sum = 0.0;
+        for (int q = 1; q <= it; ++q, x += del) sum += f(x);
+        st = 0.5 * (st + (b - a) * sum / tnm);
+      }
+      s[j] = st;
+      if (j >= 5) {
+        double y, dy;
+        neville(&h[j - 5], &s[j - 5], 5, 0.0, &y, &dy);
+        if (fabs(dy) <= 1.0e-8 * fabs(y)) return y;
+      }
+      h[j + 1] = 0.25 * h[j];
+    }
+    fprintf(stderr, "Too many steps in routine qromb\n");
+    return 0.0;
+  }
+
+  static void neville(const double xa[], const double ya[], int n, double x, double* y, double* dy) {
+    int ns = 1;
+    double c[8], d[8];
+    double dif = fabs(x - xa[1]);
+    for (int i = 1; i <= n; i++) {
+      double dift = fabs(x - xa[i]);
+      if (dift < dif) {
+        ns = i;
+        dif = dift;
+      }
+      c[i] = ya[i];
+      d[i] = ya[i];
+    }
+    *y = ya[ns--];
+    for (int m = 1; m < n; m++) {
+      for (int i = 1; i <= n - m; i++) {
+        double ho = xa[i] - x, hp = xa[i + m] - x;
+        double den = (c[i + 1] - d[i]) / (ho - hp);
+        d[i] = hp * den;
+        c[i] = ho * den;
+      }
+      *y += (*dy = (2 * ns < (n - m) ? c[ns + 1] : d[ns--]));
+    }
+  }
+
+  double growth(double a) const {  // power.c:585-592
+    double ha = sqrt(p.Omega / (a * a * a) + (1 - p.Omega - p.OmegaLambda) / (a * a) + p.OmegaLambda);
+    return ha * romberg([this](double x) { return growth_int(x); }, 0, a);
+  }
+  double growth_factor(double a0, double a1) const { return growth(a1) / growth(a0); }
+
+  double f_omega(double a) const {
+    double om = p.Omega / (p.Omega + a * (1 - p.Omega - p.OmegaLambda) + a * a * a * p.OmegaLambda);
+    return pow(om, 0.6);
+  }
+  double f2_omega(double a) const {
+    double om = p.Omega / (p.Omega + a * (1 - p.Omega - p.OmegaLambda) + a * a * a * p.OmegaLambda);
+    return 2 * pow(om, 4. / 7.);
+  }
+};
+}  // namespace
+
+extern "C" int aar_cosmology(const aar_params* p, const aar_tables* t, aar_cosmo* c) {
+  if (!p || !c) return AAR_ERR_BAD_ARG;
+  if (p->WhichSpectrum != 0 && p->WhichSpectrum != 1) return AAR_ERR_UNSUPPORTED;
+  if (p->WhichTransfer == 3) return 117;  // power.c:45-50
+  Cosmo cm(*p, t);
+  if (cm.status) return cm.status;
+  memset(c, 0, sizeof(*c));
+  c->UnitTime_in_s = p->UnitLength_in_cm / p->UnitVelocity_in_cm_per_s;
+  c->G = kGravity / pow(p->UnitLength_in_cm, 3) * p->UnitMass_in_g * pow(c->UnitTime_in_s, 2);
+  c->Hubble = kHubble * c->UnitTime_in_s;
+  c->InitTime = 1 / (1 + p->Redshift);
+  const double R8 = 8 * (3.085678e24 / p->UnitLength_in_cm);
+  cm.norm = 1.0;
+  cm.r_tophat = R8;
+  const double res = cm.romberg([&cm](double k) { return cm.sigma2_int(k); }, 0, 500.0 * 1 / R8);
+  c->sigma8_in = sqrt(res);
+  c->Norm = p->Sigma8 * p->Sigma8 / res;
+  c->D0 = cm.growth_factor(1.e-8, 1.0) * 1.e-8;
+  c->Dplus = cm.growth_factor(c->InitTime, 1.0);
+  c->Anorm = p->WhichSpectrum == 0 ? c->Norm : 0.0;
+  const double a = c->InitTime;
+  c->hubble_a = c->Hubble * sqrt(p->Omega / pow(a, 3) + (1 - p->Omega - p->OmegaLambda) / pow(a, 2) +
+                                 p->OmegaLambda);
+  c->vel_prefac = a * c->hubble_a * cm.f_omega(a) / sqrt(a);
+  c->vel_prefac2 = a * c->hubble_a * cm.f2_omega(a) / sqrt(a);
+  c->fac = pow(2 * kPi / p->Box, 1.5);
+  c->Beta = 1.5 * p->Omega /
+            (2998. * 2998. / p->UnitLength_in_cm / p->UnitLength_in_cm * 3.085678e24 * 3.085678e24) /
+            c->D0;
+  return AAR_OK;
+}
+
+// Host evaluation of T(k) for the device tables (used by aar_job.cpp).
+double aar_host_transfer(const aar_params* p, const aar_tables* t, double k) {
+  Cosmo cm(*p, t);
+  return cm.transfer(k);
+}
+
+int aar_host_transfer_tables(const aar_params* p, const aar_tables* t, std::vector<double>& logk,
+                             std::vector<double>& logT) {
+  Cosmo cm(*p, t);
+  if (cm.status) return cm.status;
+  logk = cm.logk;
+  logT = cm.logT;
+  return AAR_OK;
+}
+
+// ------------------------------------------------------------------ seed table
+extern "C" int aar_seedtable(int N, int seed, uint32_t* st) {
+  if (N < 2 || !st) return AAR_ERR_BAD_ARG;
+  RanlxdStream g;
+  g.seed((uint32_t)seed);
+  auto draw = [&]() { return (uint32_t)(0x7fffffff * g.uniform()); };
+  for (int i = 0; i < N / 2; i++) {  // main.c:321-349
+    for (int j = 0; j < i; j++) st[i * N + j] = draw();
+    for (int j = 0; j < i + 1; j++) st[j * N + i] = draw();
+    for (int j = 0; j < i; j++) st[(N - 1 - i) * N + j] = draw();
+    for (int j = 0; j < i + 1; j++) st[(N - 1 - j) * N + i] = draw();
+    for (int j = 0; j < i; j++) st[i * N + (N - 1 - j)] = draw();
+    for (int j = 0; j < i + 1; j++) st[j * N + (N - 1 - i)] = draw();
+    for (int j = 0; j < i; j++) st[(N - 1 - i) * N + (N - 1 - j)] = draw();
+    for (int j = 0; j < i + 1; j++) st[(N - 1 - j) * N + (N - 1 - i)] = draw();
+  }
+  return AAR_OK;
+}
+
+extern "C" unsigned long aar_ranlxd_nth(unsigned long seed, int luxury, int n) {
+  // GSL rng_test semantics: the n-th gsl_rng_get = (unsigned long)(u * 2^32),
+  // with the generator's register bookkeeping of rng/ranlxd.c (ir, jr, ir_old).
+  Ranlxd r;
+  ranlxd_seed(r, (uint32_t)seed);
+  int ir = 11, jr = 7, ir_old = 0;
+  unsigned long k = 0;
+  for (int i = 0; i < n; ++i) {
+    ir = (ir + 1) % 12;
+    if (ir == ir_old) {
+      for (int q = 0; q < luxury; ++q) {
+        ranlxd_step(r.s[ir], r.s[jr], r.carry);
+        ir = (ir + 1) % 12;
+        jr = (jr + 1) % 12;
+      }
+      ir_old = ir;
+    }
+    k = (unsigned long)((double)r.s[ir] * kTwoM48 * 4294967296.0);
+  }
+  return k;
+}
+
+// ------------------------------------------------------------------ writers
+static int fwrite_all(const void* p, size_t sz, size_t n, FILE* fd) {
+  return fwrite(p, sz, n, fd) == n ? AAR_OK : 777;
+}
+
+// Gadget-1 snapshot of one task (save.c:162-520, NO64BITID).
+extern "C" int aar_write_gadget(const aar_params* p, const aar_cosmo* c, const float* pos,
+                                const float* vel, long long npart, const uint8_t* glass_header) {
+  if (!p || !c || !pos || !vel) return AAR_ERR_BAD_ARG;
+  if (p->NumFilesWrittenInParallel > 1) return AAR_ERR_WRITE_PARALLEL;  // save.c:112-117, NTask = 1
+  if (npart == 0) return AAR_OK;
+  std::string fn = std::string(p->OutputDir) + "/" + p->FileBase;
+  FILE* fd = fopen(fn.c_str(), "wb");
+  if (!fd) return 10;
+  uint8_t hdr[256];
+  if (glass_header) memcpy(hdr, glass_header, 256);  // fields not set below keep the glass's bytes
+  else memset(hdr, 0, 256);
+  unsigned npart6[6] = {0, 0, 0, 0, 0, 0}, ntot6[6] = {0, 0, 0, 0, 0, 0};
+  double mass6[6] = {0, 0, 0, 0, 0, 0};
+  npart6[1] = (unsigned)npart;
+  ntot6[1] = (unsigned)npart;
+  ntot6[2] = (unsigned)(npart >> 32);
+  mass6[1] = (p->Omega) * 3 * c->Hubble * c->Hubble / (8 * kPi * c->G) * pow(p->Box, 3) / npart;
+  double time = c->InitTime, redshift = 1.0 / c->InitTime - 1;
+  int zero = 0, numfiles = 1;
+  memcpy(hdr + 0, npart6, 24);
+  memcpy(hdr + 24, mass6, 48);
+  memcpy(hdr + 72, &time, 8);
+  memcpy(hdr + 80, &redshift, 8);
+  memcpy(hdr + 88, &zero, 4);   // flag_sfr
+  memcpy(hdr + 92, &zero, 4);   // flag_feedback
+  memcpy(hdr + 96, ntot6, 24);
+  memcpy(hdr + 120, &zero, 4);  // flag_cooling
+  memcpy(hdr + 124, &numfiles, 4);
+  memcpy(hdr + 128, &p->Box, 8);
+  memcpy(hdr + 136, &p->Omega, 8);
+  memcpy(hdr + 144, &p->OmegaLambda, 8);
+  memcpy(hdr + 152, &p->HubbleParam, 8);
+  memcpy(hdr + 160, &zero, 4);  // flag_stellarage
+  memcpy(hdr + 164, &zero, 4);  // flag_metals
+  memcpy(hdr + 168, &zero, 4);  // hashtabsize
+  int rc = AAR_OK;
+  int dummy = 256;
+  rc |= fwrite_all(&dummy, 4, 1, fd);
+  rc |= fwrite_all(hdr, 256, 1, fd);
+  rc |= fwrite_all(&dummy, 4, 1, fd);
+  // record markers are 4-byte ints, truncated as in the reference for > 2^31 bytes
+  dummy = (int)(uint32_t)(sizeof(float) * 3 * (size_t)npart);
+  rc |= fwrite_all(&dummy, 4, 1, fd);
+  rc |= fwrite_all(pos, sizeof(float), 3 * (size_t)npart, fd);
+  rc |= fwrite_all(&dummy, 4, 1, fd);
+  rc |= fwrite_all(&dummy, 4, 1, fd);
+  rc |= fwrite_all(vel, sizeof(float), 3 * (size_t)npart, fd);
+  rc |= fwrite_all(&dummy, 4, 1, fd);
+  dummy = (int)(uint32_t)(sizeof(int) * (size_t)npart);
+  rc |= fwrite_all(&dummy, 4, 1, fd);
+  std::vector<int> ids(1 << 20);
+  for (long long s = 0; s < npart; s += (long long)ids.size()) {
+    const long long n = std::min<long long>((long long)ids.size(), npart - s);
+    for (long long q = 0; q < n; ++q) ids[q] = (int)(s + q + 1);  // IDs from 1 (read_glass.c:176)
+    rc |= fwrite_all(ids.data(), sizeof(int), (size_t)n, fd);
+  }
+  rc |= fwrite_all(&dummy, 4, 1, fd);
+  fclose(fd);
+  return rc ? 777 : AAR_OK;
+}
+
+// Real-space potential dump (save.c:14-105), one task.
+extern "C" int aar_write_phi(const aar_params* p, const double* real, int non_gaussian) {
+  if (!p || !real) return AAR_ERR_BAD_ARG;
+  const int N = p->Nmesh, W = 2 * (N / 2 + 1);
+  char name[1200];
+  snprintf(name, sizeof(name), "%s/%s_potential_%d", p->OutputDir, non_gaussian ? "Nongauss" : "Gauss",
+           p->Seed);
+  FILE* fd = fopen(name, "wb");
+  if (!fd) return 10;
+  int total = N * N * W + N * W;  // TotalSizePlusAdditional of one task
+  int rc = fwrite_all(&total, 4, 1, fd);
+  rc |= fwrite_all(&N, 4, 1, fd);
+  rc |= fwrite_all(&p->Box, 8, 1, fd);
+  std::vector<unsigned char> rec((size_t)N * 12);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) {
+      for (int k = 0; k < N; ++k) {
+        int coord = (i * N + j) * W + k;
+        double v = real[((size_t)i * N + j) * N + k];
+        memcpy(&rec[(size_t)k * 12], &coord, 4);
+        memcpy(&rec[(size_t)k * 12 + 4], &v, 8);
+      }
+      rc |= fwrite_all(rec.data(), 1, rec.size(), fd);
+    }
+  fclose(fd);
+  return rc ? 777 : AAR_OK;
+}
+
+// inputspec_<FileBase>.txt with the Peacock & Dodds nonlinear spectrum (main.c:1620-1694).
+extern "C" int aar_write_inputspec(const aar_params* p, const aar_tables* t, const aar_cosmo* c) {
+  if (!p || !c) return AAR_ERR_BAD_ARG;
+  Cosmo cm(*p, t);
+  if (cm.status) return cm.status;
+  cm.norm = c->Norm;
+  char name[1200];
+  snprintf(name, sizeof(name), "%s/inputspec_%s.txt", p->OutputDir, p->FileBase);
+  FILE* fd = fopen(name, "w");
+  if (!fd) return 10;
+  double gf = cm.growth_factor(0.001, 1.0) / (1.0 / 0.001);
+  double DDD = cm.growth_factor(1.0 / (p->Redshift + 1), 1.0);
+  fprintf(fd, "%12g %12g\n", p->Redshift, DDD);
+  double kstart = 2 * kPi / (1000.0 * (3.085678e24 / p->UnitLength_in_cm));
+  double kend = 2 * kPi / (0.001 * (3.085678e24 / p->UnitLength_in_cm));
+  for (double k = kstart; k < kend; k *= 1.025) {
+    double po = cm.powerspec(k);
+    double dl = 4.0 * kPi * k * k * k * po;
+    double kf = 0.5;
+    double po2 = cm.powerspec(1.001 * k * kf);
+    double po1 = cm.powerspec(k * kf);
+    double dnl = 0, knl = 0;
+    if (po != 0 && po1 != 0 && po2 != 0) {
+      double neff = (log(po2) - log(po1)) / (log(1.001 * k * kf) - log(k * kf));
+      if (1 + neff / 3 > 0) {
+        double A = 0.482 * pow(1 + neff / 3, -0.947);
+        double B = 0.226 * pow(1 + neff / 3, -1.778);
+        double alpha = 3.310 * pow(1 + neff / 3, -0.244);
+        double beta = 0.862 * pow(1 + neff / 3, -0.287);
+        double V = 11.55 * pow(1 + neff / 3, -0.423) * 1.2;
+        double x = dl;
+        dnl = x * pow((1 + B * beta * x + pow(A * x, alpha * beta)) /
+                          (1 + pow(pow(A * x, alpha) * gf * gf * gf / (V * sqrt(x)), beta)),
+                      1 / beta);
+        knl = k * pow(1 + dnl, 1.0 / 3);
+      }
+    }
+    fprintf(fd, "%12g %12g    %12g %12g\n", k, dl, knl, dnl);
+  }
+  fclose(fd);
+  return AAR_OK;
+}

@@ -1,0 +1,716 @@
+// Job orchestration and the device half of the C-ABI.
+//
+// A job owns every device buffer of one initial-condition run: a pool of
+// spectrum-sized field slots, the seed table, the k-space tables, the glass
+// and the particle arrays.  aar_job_run only enqueues kernels on a stream,
+// so several jobs can be in flight on separate streams.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <set>
+#include <vector>
+
+#include "../../include/aarambam.h"
+#include "aar_common.h"
+#include "aar_kernels.h"
+
+int aar_host_transfer_tables(const aar_params* p, const aar_tables* t, std::vector<double>& logk,
+                             std::vector<double>& logT);
+
+struct aar_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  float2* tw32 = nullptr;
+  double2* tw64 = nullptr;
+};
+
+namespace {
+
+struct PairWork {  // one (N_i, N_j) product of the pairwise schedule
+  int i, j;
+  int w_off, w_n;  // slice of the weight list
+};
+
+template <typename T> struct Exec;
+
+}  // namespace
+
+struct aar_job {
+  aar_ctx* ctx = nullptr;
+  aar_params p{};
+  aar_cosmo c{};
+  int prec = AAR_FP64, sched = AAR_SCHED_BATCHED;
+  Geom g{};
+  size_t cxsize = 16;
+  // device buffers
+  void* pool = nullptr;
+  int nslots = 0;
+  uint32_t* seeds = nullptr;
+  double *kv = nullptr, *sinc = nullptr, *x0 = nullptr, *logk = nullptr, *logT = nullptr;
+  unsigned char* outc = nullptr;
+  int *modes = nullptr, *kmodes = nullptr, *groups = nullptr;
+  NgTerm* terms = nullptr;
+  NgWeight* weights = nullptr;
+  double2* acc = nullptr;
+  float* glass = nullptr;
+  float *pos = nullptr, *vel = nullptr;
+  long long npart = 0;
+  int nglass = 0;
+  double glass_box = 0;
+  KSpace ks{};
+  TransferDev tf{};
+  // schedule
+  int M = 0, K = 0;
+  std::vector<int> modes_h, kmodes_h;
+  std::vector<PairWork> pairs;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_seconds = 0;
+  // per-kind launch timing
+  bool profiling = false;
+  struct Rec {
+    int kind;
+    double bytes;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> evpool;
+  size_t evnext = 0;
+  int k_launch[AAR_NKINDS] = {};
+  double k_sec[AAR_NKINDS] = {}, k_bytes[AAR_NKINDS] = {};
+  hipStream_t last_stream = nullptr;
+  bool ran = false;
+  // slot map
+  int s_phig = 0, s_phi = 1, s_reg = 2;
+  void* slot(int s) const { return (char*)pool + (size_t)s * g.fs * cxsize; }
+};
+
+namespace {
+
+bool pow2(int n) { return n >= 16 && n <= 1024 && (n & (n - 1)) == 0; }
+
+// Build the regrouped term list and the pairwise schedule from the alpha table
+// (main.c:564-1063): main triples i <= j <= K with alpha != 0, plus the two
+// Beta terms -alpha012 t012 g_2 [f0 f1] and +alpha012 t012 g_1 [f0 f2].
+struct Schedule {
+  std::vector<int> modes;                 // distinct field modes, ascending
+  std::vector<int> kmodes;                // distinct outer indices, ascending
+  std::vector<NgTerm> terms;              // slots: i,j -> mode slot; grouped by K slot
+  std::vector<int> groups;                // K+1 offsets
+  std::vector<PairWork> pairs;            // pairwise schedule
+  std::vector<NgWeight> weights;
+};
+
+Schedule build_schedule(const aar_params& p, const double* alpha) {
+  const int Nm = p.N_modes;
+  auto A = [&](int a, int b, int c) { return alpha[((size_t)a * Nm + b) * Nm + c]; };
+  double t012 = -A(1, 1, 1) / A(0, 1, 2);
+  if (A(0, 1, 2) == 0) t012 = 0;
+  struct Tri {
+    int i, j, K;
+    double c;
+  };
+  std::vector<Tri> tris;
+  for (int i = 0; i < Nm; ++i)
+    for (int j = i; j < Nm; ++j)
+      for (int K = j; K < Nm; ++K)
+        if (A(i, j, K) != 0) tris.push_back({i, j, K, A(i, j, K)});
+  std::vector<Tri> beta;
+  if (t012 != 0) {  // main.c:1043-1052
+    beta.push_back({0, 1, 2, -A(0, 1, 2) * t012});
+    beta.push_back({0, 2, 1, A(0, 1, 2) * t012});
+  }
+  Schedule s;
+  std::set<int> ms, ks;
+  for (auto* v : {&tris, &beta})
+    for (auto& t : *v) {
+      ms.insert(t.i);
+      ms.insert(t.j);
+      ks.insert(t.K);
+    }
+  s.modes.assign(ms.begin(), ms.end());
+  s.kmodes.assign(ks.begin(), ks.end());
+  std::map<int, int> mslot, kslot;
+  for (size_t q = 0; q < s.modes.size(); ++q) mslot[s.modes[q]] = (int)q;
+  for (size_t q = 0; q < s.kmodes.size(); ++q) kslot[s.kmodes[q]] = (int)q;
+  std::vector<std::vector<NgTerm>> byk(s.kmodes.size());
+  for (auto* v : {&tris, &beta})
+    for (auto& t : *v) byk[kslot[t.K]].push_back({mslot[t.i], mslot[t.j], t.c});
+  s.groups.push_back(0);
+  for (auto& v : byk) {
+    for (auto& t : v) s.terms.push_back(t);
+    s.groups.push_back((int)s.terms.size());
+  }
+  // pairwise schedule in the reference's loop order: main pairs, then Beta pairs
+  for (int i = 0; i < Nm; ++i)
+    for (int j = i; j < Nm; ++j) {
+      PairWork pw{i, j, (int)s.weights.size(), 0};
+      for (int K = j; K < Nm; ++K)
+        if (A(i, j, K) != 0) s.weights.push_back({K, A(i, j, K)});
+      pw.w_n = (int)s.weights.size() - pw.w_off;
+      if (pw.w_n) s.pairs.push_back(pw);
+    }
+  for (auto& t : beta) {
+    PairWork pw{t.i, t.j, (int)s.weights.size(), 1};
+    s.weights.push_back({t.K, t.c});
+    s.pairs.push_back(pw);
+  }
+  return s;
+}
+
+hipEvent_t next_event(aar_job* j) {
+  if (j->evnext == j->evpool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    j->evpool.push_back(e);
+  }
+  return j->evpool[j->evnext++];
+}
+
+// Run one launcher; with profiling on, bracket it with events on `st`.
+template <class Fn> int timed(aar_job* j, hipStream_t st, int kind, double bytes, Fn&& fn) {
+  if (!j->profiling) return fn();
+  hipEvent_t a = next_event(j), b = next_event(j);
+  if (!a || !b) return AAR_ERR_HIP;
+  hipEventRecord(a, st);
+  int rc = fn();
+  hipEventRecord(b, st);
+  j->recs.push_back({kind, bytes, a, b});
+  return rc;
+}
+
+template <typename T> struct Exec {
+  using C2 = cx_t<T>;
+  static C2* F(aar_job* j, int s) { return (C2*)j->slot(s); }
+  static const C2* tw(aar_job* j) {
+    if constexpr (sizeof(T) == 4) return j->ctx->tw32;
+    else return j->ctx->tw64;
+  }
+
+  static int run(aar_job* j, hipStream_t st) {
+    using P = Pipeline<T>;
+    const Geom& g = j->g;
+    const aar_params& p = j->p;
+    const aar_cosmo& c = j->c;
+    int rc;
+    const double S = (double)g.fs * sizeof(C2);              // one spectrum / padded field
+    const double R = (double)g.N * g.N * g.N * sizeof(T);     // one real field
+    const double phase_shift = p.PhaseFlip == 1 ? AAR_PI : 0.0;
+    auto c2r = [&](C2* f, int nf) {
+      int r;
+      if ((r = timed(j, st, AAR_K_XY, 2 * nf * S, [&] { return P::xy_pass(f, nf, g, tw(j), 0, +1, st); })))
+        return r;
+      if ((r = timed(j, st, AAR_K_XY, 2 * nf * S, [&] { return P::xy_pass(f, nf, g, tw(j), 1, +1, st); })))
+        return r;
+      return timed(j, st, AAR_K_Z, nf * (S + R), [&] { return P::z_pass(f, nf, g, tw(j), +1, st); });
+    };
+    auto r2c = [&](C2* f, int nf) {
+      int r;
+      if ((r = timed(j, st, AAR_K_Z, nf * (S + R), [&] { return P::z_pass(f, nf, g, tw(j), -1, st); })))
+        return r;
+      if ((r = timed(j, st, AAR_K_XY, 2 * nf * S, [&] { return P::xy_pass(f, nf, g, tw(j), 1, -1, st); })))
+        return r;
+      return timed(j, st, AAR_K_XY, 2 * nf * S, [&] { return P::xy_pass(f, nf, g, tw(j), 0, -1, st); });
+    };
+    if ((rc = timed(j, st, AAR_K_GAUSS, S + 4.0 * g.N * g.N, [&] {
+           return P::gauss(F(j, j->s_phig), g, j->ks, j->seeds, c.Anorm, p.PrimordialIndex, c.fac, c.Beta,
+                           phase_shift, p.FixedAmplitude, st);
+         })))
+      return rc;
+    C2* reg = F(j, j->s_reg);
+    const double n3 = (double)((unsigned)g.N * (unsigned)g.N * (unsigned)g.N);
+    if (j->sched == AAR_SCHED_BATCHED) {
+      if (j->M > 0) {
+        if ((rc = timed(j, st, AAR_K_MODES, (1 + j->M) * S,
+                        [&] { return P::modes_spec(F(j, j->s_phig), reg, j->M, j->modes, g, j->ks, st); })))
+          return rc;
+        if ((rc = c2r(reg, j->M))) return rc;
+        if ((rc = timed(j, st, AAR_K_PRODUCTS, (j->M + j->K) * R,
+                        [&] { return P::products(reg, j->M, reg, j->K, j->terms, j->groups, g, st); })))
+          return rc;
+        if ((rc = r2c(reg, j->K))) return rc;
+      }
+      if ((rc = timed(j, st, AAR_K_COMBINE, (j->K + 2) * S, [&] {
+             return P::ng_combine(F(j, j->s_phig), reg, j->K, j->kmodes, F(j, j->s_phi), p.Fnl, g, j->ks, st);
+           })))
+        return rc;
+    } else {
+      if ((rc = aar_zero_fill((double*)j->acc, 2 * g.fs, st))) return rc;
+      C2* fi = reg;
+      C2* fj = F(j, j->s_reg + 1);
+      int cur_i = -1;
+      for (const PairWork& pw : j->pairs) {
+        if (pw.i != cur_i) {  // main.c:571-652: build and C2R the field of N_i
+          if ((rc = timed(j, st, AAR_K_MODES, 2 * S,
+                          [&] { return P::modes_spec(F(j, j->s_phig), fi, 1, j->modes + pw.i, g, j->ks, st); })))
+            return rc;
+          if ((rc = c2r(fi, 1))) return rc;
+          cur_i = pw.i;
+        }
+        // main.c:666-781: field of N_j, C2R, product, R2C
+        if ((rc = timed(j, st, AAR_K_MODES, 2 * S,
+                        [&] { return P::modes_spec(F(j, j->s_phig), fj, 1, j->modes + pw.j, g, j->ks, st); })))
+          return rc;
+        if ((rc = c2r(fj, 1))) return rc;
+        if ((rc = timed(j, st, AAR_K_PRODUCTS, 3 * R,
+                        [&] { return P::products(fi, 2, fj, 1, j->terms, j->groups, g, st); })))
+          return rc;
+        if ((rc = r2c(fj, 1))) return rc;
+        // main.c:785-870: weighted accumulation
+        if ((rc = timed(j, st, AAR_K_ACCUM, S + 2.0 * g.fs * 16, [&] {
+               return P::ng_accumulate(j->acc, fj, j->weights + pw.w_off, pw.w_n, g, j->ks, st);
+             })))
+          return rc;
+      }
+      if ((rc = timed(j, st, AAR_K_COMBINE, 2 * S + g.fs * 16.0, [&] {
+             return P::ng_combine_acc(F(j, j->s_phig), j->acc, F(j, j->s_phi), p.Fnl, n3, g, st);
+           })))
+        return rc;
+    }
+    // 2LPT (main.c:1158-1381)
+    if ((rc = timed(j, st, AAR_K_LPT, 10 * S,
+                    [&] { return P::lpt_spec(F(j, j->s_phi), reg, g, j->ks, j->tf, c.Dplus, c.Beta, st); })))
+      return rc;
+    if ((rc = c2r(reg, 9))) return rc;
+    if ((rc = timed(j, st, AAR_K_PRODUCTS, 7 * R, [&] { return P::source(reg, g, st); }))) return rc;
+    if ((rc = r2c(reg + 3 * g.fs, 1))) return rc;
+    if ((rc = timed(j, st, AAR_K_LPT, 4 * S, [&] { return P::disp2_spec(reg + 3 * g.fs, reg, g, j->ks, st); })))
+      return rc;
+    if ((rc = c2r(reg, 3))) return rc;
+    // CIC readout and particle update (main.c:1440-1516)
+    return timed(j, st, AAR_K_CIC, 6 * R + 24.0 * j->npart, [&] {
+      return P::cic(reg + 6 * g.fs, reg, g, j->glass, j->nglass, j->glass_box, j->p.GlassTileFac, p.Box,
+                    c.vel_prefac, c.vel_prefac2, j->pos, j->vel, j->npart, st);
+    });
+  }
+};
+
+template <typename T> void to_internal_spec(const double* ref, T* out, int N) {
+  const int H = N / 2;
+  const long long nq = (long long)N * N * H;
+  for (long long r = 0; r < (long long)N * N; ++r) {
+    for (int k = 0; k < H; ++k) {
+      out[2 * (r * H + k)] = (T)ref[2 * (r * (H + 1) + k)];
+      out[2 * (r * H + k) + 1] = (T)ref[2 * (r * (H + 1) + k) + 1];
+    }
+    out[2 * (nq + r)] = (T)ref[2 * (r * (H + 1) + H)];
+    out[2 * (nq + r) + 1] = (T)ref[2 * (r * (H + 1) + H) + 1];
+  }
+}
+
+template <typename T> void from_internal_spec(const T* in, double* ref, int N) {
+  const int H = N / 2;
+  const long long nq = (long long)N * N * H;
+  for (long long r = 0; r < (long long)N * N; ++r) {
+    for (int k = 0; k < H; ++k) {
+      ref[2 * (r * (H + 1) + k)] = in[2 * (r * H + k)];
+      ref[2 * (r * (H + 1) + k) + 1] = in[2 * (r * H + k) + 1];
+    }
+    ref[2 * (r * (H + 1) + H)] = in[2 * (nq + r)];
+    ref[2 * (r * (H + 1) + H) + 1] = in[2 * (nq + r) + 1];
+  }
+}
+
+int device_alloc(void** p, size_t bytes) {
+  if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
+    *p = nullptr;
+    return AAR_ERR_ALLOC;
+  }
+  return AAR_OK;
+}
+
+template <typename V> int upload(V** dst, const std::vector<V>& src) {
+  int rc = device_alloc((void**)dst, sizeof(V) * src.size());
+  if (rc) return rc;
+  if (!src.empty() && hipMemcpy(*dst, src.data(), sizeof(V) * src.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return AAR_ERR_HIP;
+  return AAR_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ context
+extern "C" int aar_ctx_create(int device, aar_ctx** out) {
+  if (!out) return AAR_ERR_BAD_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return AAR_ERR_NO_DEVICE;
+  AAR_HIP(hipSetDevice(device));
+  aar_ctx* c = new aar_ctx();
+  c->device = device;
+  AAR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  std::vector<float2> t32(4096);
+  std::vector<double2> t64(4096);
+  make_twiddles<float>(t32.data());
+  make_twiddles<double>(t64.data());
+  int rc = upload(&c->tw32, t32);
+  if (!rc) rc = upload(&c->tw64, t64);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return AAR_OK;
+}
+
+extern "C" int aar_ctx_sync(aar_ctx* c) {
+  if (!c) return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(c->device));
+  AAR_HIP(hipDeviceSynchronize());
+  return AAR_OK;
+}
+
+extern "C" int aar_ctx_destroy(aar_ctx* c) {
+  if (!c) return AAR_OK;
+  hipSetDevice(c->device);
+  hipFree(c->tw32);
+  hipFree(c->tw64);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return AAR_OK;
+}
+
+// ------------------------------------------------------------------ jobs
+static int validate(const aar_params* p, const aar_tables* t) {
+  if (!p || !t) return AAR_ERR_BAD_ARG;
+  if (!pow2(p->Nmesh)) return AAR_ERR_UNSUPPORTED;
+  if (p->Nmax < p->Nsample) return AAR_ERR_NMAX;
+  if (p->N_modes < 3 || !t->alpha || !t->x0) return AAR_ERR_BAD_ARG;
+  if (!t->glass || t->nglass <= 0 || p->GlassTileFac <= 0) return AAR_ERR_BAD_ARG;
+  if (p->WDM_Vtherm_On == 1 && p->WDM_On == 1) return AAR_ERR_UNSUPPORTED;
+  return AAR_OK;
+}
+
+static int slots_needed(int M, int K, int sched) {
+  if (sched == AAR_SCHED_PLAIN) return 2 + 9;
+  return 2 + std::max(std::max(M, K), 9);
+}
+
+extern "C" long long aar_job_bytes(const aar_params* p, const aar_tables* t, int precision, int schedule) {
+  if (validate(p, t)) return -1;
+  Schedule s = build_schedule(*p, t->alpha);
+  const Geom g = make_geom(p->Nmesh);
+  const long long cx = precision == AAR_FP32 ? 8 : 16;
+  const long long npart = (long long)p->GlassTileFac * p->GlassTileFac * p->GlassTileFac * t->nglass;
+  long long b = slots_needed((int)s.modes.size(), (int)s.kmodes.size(), schedule) * g.fs * cx;
+  if (schedule == AAR_SCHED_PLAIN) b += g.fs * 16;  // double accumulator
+  b += 24 * npart + 12LL * t->nglass + 4LL * p->Nmesh * p->Nmesh;
+  return b;
+}
+
+extern "C" int aar_job_prepare(aar_ctx* ctx, const aar_params* p, const aar_tables* t, int precision,
+                               int schedule, aar_job** out) {
+  if (!ctx || !out) return AAR_ERR_BAD_ARG;
+  int rc = validate(p, t);
+  if (rc) return rc;
+  if (precision != AAR_FP32 && precision != AAR_FP64) return AAR_ERR_BAD_ARG;
+  if (schedule != AAR_SCHED_PLAIN && schedule != AAR_SCHED_BATCHED) return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(ctx->device));
+  aar_job* j = new aar_job();
+  j->ctx = ctx;
+  j->p = *p;
+  j->prec = precision;
+  j->sched = schedule;
+  j->cxsize = precision == AAR_FP32 ? 8 : 16;
+  j->g = make_geom(p->Nmesh);
+  auto fail = [&](int code) {
+    aar_job_destroy(j);
+    return code;
+  };
+  if ((rc = aar_cosmology(p, t, &j->c))) return fail(rc);
+  const int N = p->Nmesh;
+  // schedule
+  Schedule s = build_schedule(*p, t->alpha);
+  j->modes_h = s.modes;
+  j->kmodes_h = s.kmodes;
+  j->M = (int)s.modes.size();
+  j->K = (int)s.kmodes.size();
+  j->pairs = s.pairs;
+  if (j->M > 32) return fail(AAR_ERR_UNSUPPORTED);
+  std::vector<int> modes_dev = s.modes, groups = s.groups;
+  std::vector<NgTerm> terms = s.terms;
+  if (schedule == AAR_SCHED_PLAIN) {
+    // pairwise: fields are addressed by mode number, one product term f0*f1
+    modes_dev.resize(N > 0 ? p->N_modes : 0);
+    for (int q = 0; q < p->N_modes; ++q) modes_dev[q] = q;
+    terms = {NgTerm{0, 1, 1.0}};
+    groups = {0, 1};
+  }
+  // k-space tables (main.c:412-452, 1323-1343)
+  std::vector<double> kv(N), sinc(N), x0(p->N_modes);
+  std::vector<unsigned char> outc(N);
+  for (int i = 0; i < N; ++i) {
+    kv[i] = (i < N / 2) ? i * 2 * AAR_PI / p->Box : -(N - i) * 2 * AAR_PI / p->Box;
+    outc[i] = fabs(kv[i]) * p->Box / (2 * AAR_PI) > p->Nsample / 2;
+    double f = 1;
+    if (kv[i] != 0) {
+      f = (kv[i] * p->Box / 2) / N;
+      f = sin(f) / f;
+    }
+    sinc[i] = f;
+  }
+  for (int q = 0; q < p->N_modes; ++q) x0[q] = t->x0[2 * q];
+  std::vector<double> logk, logT;
+  if (p->WhichTransfer == 2 && (rc = aar_host_transfer_tables(p, t, logk, logT))) return fail(rc);
+  // device buffers
+  j->nslots = slots_needed(j->M, j->K, schedule);
+  if ((rc = device_alloc(&j->pool, (size_t)j->nslots * j->g.fs * j->cxsize))) return fail(rc);
+  if (schedule == AAR_SCHED_PLAIN && (rc = device_alloc((void**)&j->acc, (size_t)j->g.fs * 16)))
+    return fail(rc);
+  std::vector<uint32_t> seeds((size_t)N * N);
+  aar_seedtable(N, p->Seed, seeds.data());
+  if ((rc = upload(&j->seeds, seeds))) return fail(rc);
+  if ((rc = upload(&j->kv, kv)) || (rc = upload(&j->sinc, sinc)) || (rc = upload(&j->x0, x0)) ||
+      (rc = upload(&j->outc, outc)) || (rc = upload(&j->modes, modes_dev)) ||
+      (rc = upload(&j->kmodes, s.kmodes)) || (rc = upload(&j->groups, groups)) ||
+      (rc = upload(&j->terms, terms)) || (rc = upload(&j->weights, s.weights)))
+    return fail(rc);
+  if (p->WhichTransfer == 2 && ((rc = upload(&j->logk, logk)) || (rc = upload(&j->logT, logT))))
+    return fail(rc);
+  j->nglass = t->nglass;
+  j->glass_box = t->glass_box;
+  std::vector<float> gl(t->glass, t->glass + 3 * (size_t)t->nglass);
+  if ((rc = upload(&j->glass, gl))) return fail(rc);
+  j->npart = (long long)p->GlassTileFac * p->GlassTileFac * p->GlassTileFac * t->nglass;
+  if ((rc = device_alloc((void**)&j->pos, sizeof(float) * 3 * j->npart)) ||
+      (rc = device_alloc((void**)&j->vel, sizeof(float) * 3 * j->npart)))
+    return fail(rc);
+  // kernel-side views
+  KSpace& ks = j->ks;
+  ks.kv = j->kv;
+  ks.outc = j->outc;
+  ks.sinc = j->sinc;
+  ks.sphere = p->SphereMode;
+  ks.box = p->Box;
+  ks.nsh = (double)(p->Nsample / 2);
+  ks.s = (4 - p->PrimordialIndex) / 3;
+  const double kmin = 2 * AAR_PI / p->Box, kmax = 2 * AAR_PI / p->Box * p->Nmax;
+  ks.kmin_s = pow(kmin, (4 - p->PrimordialIndex) / 3);
+  ks.kmax_s = pow(kmax, (4 - p->PrimordialIndex) / 3);
+  ks.x0 = j->x0;
+  TransferDev& tf = j->tf;
+  tf.which = p->WhichTransfer;
+  if (p->WhichTransfer == 1) {  // k-independent part of tk_eh (power.c:526-540)
+    const double hubble = p->HubbleParam, omegam = p->Omega;
+    double ombh2 = p->OmegaBaryon * p->HubbleParam * p->HubbleParam;
+    if (p->OmegaBaryon == 0) ombh2 = 0.04 * p->HubbleParam * p->HubbleParam;
+    const double theta = 2.728 / 2.7, ommh2 = omegam * hubble * hubble;
+    tf.kconv = 3.085678e24 / p->UnitLength_in_cm;
+    tf.eh_s = 44.5 * log(9.83 / ommh2) / sqrt(1. + 10. * exp(0.75 * log(ombh2))) * hubble;
+    tf.eh_a = 1. - 0.328 * log(431. * ommh2) * ombh2 / ommh2 +
+              0.380 * log(22.3 * ommh2) * (ombh2 / ommh2) * (ombh2 / ommh2);
+    tf.eh_theta2 = theta * theta;
+    tf.eh_omh = omegam * hubble;
+    tf.eh_twoe = 2. * exp(1.);
+  } else if (p->WhichTransfer == 2) {
+    tf.kconv = p->InputSpectrum_UnitLength_in_cm / p->UnitLength_in_cm;
+    tf.nt = (int)logk.size();
+    tf.logk = j->logk;
+    tf.logT = j->logT;
+  }
+  AAR_HIP(hipEventCreate(&j->ev0));
+  AAR_HIP(hipEventCreate(&j->ev1));
+  *out = j;
+  return AAR_OK;
+}
+
+extern "C" int aar_job_run(aar_job* j, void* stream) {
+  if (!j) return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(j->ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : j->ctx->stream;
+  j->recs.clear();
+  j->evnext = 0;
+  AAR_HIP(hipEventRecord(j->ev0, st));
+  int rc = j->prec == AAR_FP32 ? Exec<float>::run(j, st) : Exec<double>::run(j, st);
+  if (rc) return rc;
+  AAR_HIP(hipEventRecord(j->ev1, st));
+  j->last_stream = st;
+  j->ran = true;
+  return AAR_OK;
+}
+
+extern "C" int aar_job_sync(aar_job* j) {
+  if (!j) return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(j->ctx->device));
+  AAR_HIP(hipEventSynchronize(j->ev1));
+  float ms = 0;
+  AAR_HIP(hipEventElapsedTime(&ms, j->ev0, j->ev1));
+  j->last_seconds = ms * 1e-3;
+  for (const auto& r : j->recs) {
+    float e = 0;
+    AAR_HIP(hipEventElapsedTime(&e, r.a, r.b));
+    j->k_launch[r.kind] += 1;
+    j->k_sec[r.kind] += e * 1e-3;
+    j->k_bytes[r.kind] += r.bytes;
+  }
+  j->recs.clear();
+  return AAR_OK;
+}
+
+extern "C" int aar_job_set_profiling(aar_job* j, int on) {
+  if (!j) return AAR_ERR_BAD_ARG;
+  j->profiling = on != 0;
+  for (int k = 0; k < AAR_NKINDS; ++k) {
+    j->k_launch[k] = 0;
+    j->k_sec[k] = 0;
+    j->k_bytes[k] = 0;
+  }
+  return AAR_OK;
+}
+
+extern "C" int aar_job_kernel_stats(aar_job* j, int kind, int* launches, double* seconds, double* bytes) {
+  if (!j || kind < 0 || kind >= AAR_NKINDS) return AAR_ERR_BAD_ARG;
+  if (launches) *launches = j->k_launch[kind];
+  if (seconds) *seconds = j->k_sec[kind];
+  if (bytes) *bytes = j->k_bytes[kind];
+  return AAR_OK;
+}
+
+extern "C" double aar_job_last_seconds(aar_job* j) { return j ? j->last_seconds : -1; }
+
+extern "C" int aar_job_device_output(aar_job* j, float** pos, float** vel, long long* npart) {
+  if (!j) return AAR_ERR_BAD_ARG;
+  if (pos) *pos = j->pos;
+  if (vel) *vel = j->vel;
+  if (npart) *npart = j->npart;
+  return AAR_OK;
+}
+
+extern "C" int aar_job_fetch(aar_job* j, float* pos, float* vel) {
+  if (!j || !j->ran) return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(j->ctx->device));
+  AAR_HIP(hipEventSynchronize(j->ev1));
+  if (pos) AAR_HIP(hipMemcpy(pos, j->pos, sizeof(float) * 3 * j->npart, hipMemcpyDeviceToHost));
+  if (vel) AAR_HIP(hipMemcpy(vel, j->vel, sizeof(float) * 3 * j->npart, hipMemcpyDeviceToHost));
+  return AAR_OK;
+}
+
+template <typename T> static int fetch_field_t(aar_job* j, int which, double* out) {
+  using C2 = cx_t<T>;
+  const Geom& g = j->g;
+  const int N = g.N;
+  std::vector<T> host((size_t)2 * g.fs);
+  const C2* src = nullptr;
+  C2* tmp = nullptr;
+  if (which == AAR_FIELD_PHIG_SPEC || which == AAR_FIELD_PHI_SPEC) {
+    src = (const C2*)j->slot(which == AAR_FIELD_PHIG_SPEC ? j->s_phig : j->s_phi);
+  } else if (which >= AAR_FIELD_DISP_X && which <= AAR_FIELD_DISP2_Z) {
+    const int a = which - AAR_FIELD_DISP_X;  // 0..5
+    src = (const C2*)j->slot(j->s_reg + (a < 3 ? 6 + a : a - 3));
+  } else if (which == AAR_FIELD_PHIG_REAL || which == AAR_FIELD_PHING_REAL) {
+    if (device_alloc((void**)&tmp, (size_t)g.fs * sizeof(C2))) return AAR_ERR_ALLOC;
+    const void* s = j->slot(which == AAR_FIELD_PHIG_REAL ? j->s_phig : j->s_phi);
+    // stream-ordered: the context stream does not synchronise with the null stream
+    AAR_HIP(hipMemcpyAsync(tmp, s, (size_t)g.fs * sizeof(C2), hipMemcpyDeviceToDevice, j->ctx->stream));
+    const C2* tw = sizeof(T) == 4 ? (const C2*)j->ctx->tw32 : (const C2*)j->ctx->tw64;
+    int rc = Pipeline<T>::c2r(tmp, 1, g, tw, j->ctx->stream);
+    if (rc) return rc;
+    AAR_HIP(hipStreamSynchronize(j->ctx->stream));
+    src = tmp;
+  } else {
+    return AAR_ERR_BAD_ARG;
+  }
+  AAR_HIP(hipMemcpy(host.data(), src, (size_t)g.fs * sizeof(C2), hipMemcpyDeviceToHost));
+  if (tmp) hipFree(tmp);
+  if (which <= AAR_FIELD_PHI_SPEC) {
+    from_internal_spec<T>(host.data(), out, N);
+  } else {
+    const long long n3 = (long long)N * N * N;
+    for (long long q = 0; q < n3; ++q) out[q] = host[q];
+  }
+  return AAR_OK;
+}
+
+extern "C" int aar_job_fetch_field(aar_job* j, int which, double* out) {
+  if (!j || !out || !j->ran) return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(j->ctx->device));
+  AAR_HIP(hipEventSynchronize(j->ev1));
+  return j->prec == AAR_FP32 ? fetch_field_t<float>(j, which, out) : fetch_field_t<double>(j, which, out);
+}
+
+extern "C" int aar_job_destroy(aar_job* j) {
+  if (!j) return AAR_OK;
+  hipSetDevice(j->ctx->device);
+  if (j->ran) hipEventSynchronize(j->ev1);
+  void* bufs[] = {j->pool, j->seeds, j->kv, j->sinc, j->x0, j->logk, j->logT, j->outc, j->modes,
+                  j->kmodes, j->groups, j->terms, j->weights, j->acc, j->glass, j->pos, j->vel};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  for (hipEvent_t e : j->evpool) hipEventDestroy(e);
+  if (j->ev0) hipEventDestroy(j->ev0);
+  if (j->ev1) hipEventDestroy(j->ev1);
+  delete j;
+  return AAR_OK;
+}
+
+// ------------------------------------------------------------------ building blocks
+template <typename T> static int fft3d_t(aar_ctx* ctx, int N, int dir, const double* in, double* out) {
+  using C2 = cx_t<T>;
+  const Geom g = make_geom(N);
+  std::vector<T> host((size_t)2 * g.fs, (T)0);
+  const long long n3 = (long long)N * N * N;
+  if (dir > 0) to_internal_spec<T>(in, host.data(), N);
+  else
+    for (long long q = 0; q < n3; ++q) host[q] = (T)in[q];
+  C2* d = nullptr;
+  if (device_alloc((void**)&d, (size_t)g.fs * sizeof(C2))) return AAR_ERR_ALLOC;
+  AAR_HIP(hipMemcpyAsync(d, host.data(), (size_t)g.fs * sizeof(C2), hipMemcpyHostToDevice, ctx->stream));
+  const C2* tw = sizeof(T) == 4 ? (const C2*)ctx->tw32 : (const C2*)ctx->tw64;
+  int rc = dir > 0 ? Pipeline<T>::c2r(d, 1, g, tw, ctx->stream) : Pipeline<T>::r2c(d, 1, g, tw, ctx->stream);
+  if (rc) {
+    hipFree(d);
+    return rc;
+  }
+  AAR_HIP(hipStreamSynchronize(ctx->stream));
+  AAR_HIP(hipMemcpy(host.data(), d, (size_t)g.fs * sizeof(C2), hipMemcpyDeviceToHost));
+  hipFree(d);
+  if (dir > 0)
+    for (long long q = 0; q < n3; ++q) out[q] = host[q];
+  else
+    from_internal_spec<T>(host.data(), out, N);
+  return AAR_OK;
+}
+
+extern "C" int aar_fft3d(aar_ctx* ctx, int n, int precision, int dir, const double* in, double* out) {
+  if (!ctx || !in || !out || !pow2(n) || (dir != 1 && dir != -1)) return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(ctx->device));
+  return precision == AAR_FP32 ? fft3d_t<float>(ctx, n, dir, in, out) : fft3d_t<double>(ctx, n, dir, in, out);
+}
+
+extern "C" int aar_gaussian_field(aar_ctx* ctx, const aar_params* p, const aar_tables* t, int precision,
+                                  double* out) {
+  aar_job* j = nullptr;
+  int rc = aar_job_prepare(ctx, p, t, precision, AAR_SCHED_BATCHED, &j);
+  if (rc) return rc;
+  const double phase_shift = p->PhaseFlip == 1 ? AAR_PI : 0.0;
+  if (precision == AAR_FP32)
+    rc = Pipeline<float>::gauss((float2*)j->slot(0), j->g, j->ks, j->seeds, j->c.Anorm, p->PrimordialIndex,
+                                j->c.fac, j->c.Beta, phase_shift, p->FixedAmplitude, ctx->stream);
+  else
+    rc = Pipeline<double>::gauss((double2*)j->slot(0), j->g, j->ks, j->seeds, j->c.Anorm,
+                                 p->PrimordialIndex, j->c.fac, j->c.Beta, phase_shift, p->FixedAmplitude,
+                                 ctx->stream);
+  if (!rc && hipEventRecord(j->ev1, ctx->stream) != hipSuccess) rc = AAR_ERR_HIP;
+  if (!rc) {
+    j->ran = true;
+    rc = aar_job_fetch_field(j, AAR_FIELD_PHIG_SPEC, out);
+  }
+  aar_job_destroy(j);
+  return rc;
+}
+
+extern "C" int aar_ranlxd_device(aar_ctx* ctx, uint32_t seed, int n, double* out) {
+  if (!ctx || !out || n <= 0) return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(ctx->device));
+  double* d = nullptr;
+  if (device_alloc((void**)&d, sizeof(double) * n)) return AAR_ERR_ALLOC;
+  int rc = aar_ranlxd_draws_dev(seed, n, d, ctx->stream);
+  if (!rc) {
+    AAR_HIP(hipStreamSynchronize(ctx->stream));
+    AAR_HIP(hipMemcpy(out, d, sizeof(double) * n, hipMemcpyDeviceToHost));
+  }
+  hipFree(d);
+  return rc;
+}

@@ -1,0 +1,84 @@
+// Kernel launchers of the NG-2LPT pipeline (implemented in aar_kernels.hip).
+#pragma once
+#include "aar_common.h"
+
+// Per-job k-space tables, device pointers (built on the host with the
+// reference's exact expressions, Aarambam/LPT/Basis/main.c:412-452).
+struct KSpace {
+  const double* kv;           // [N] kvec component of grid index i
+  const unsigned char* outc;  // [N] 1 when |kv|*Box/(2 pi) > Nsample/2 (cube cut)
+  const double* sinc;         // [N] CIC window sin(kv L/2N)/(kv L/2N), 1 at kv = 0
+  int sphere;                 // SphereMode
+  double box;                 // Box
+  double nsh;                 // (double)(Nsample/2)
+  double s;                   // (4 - ns)/3
+  double kmin_s, kmax_s;      // kmin^s, kmax^s (Legendre argument map)
+  const double* x0;           // [N_modes] X0Table[n][0]
+};
+
+// Transfer function parameters (power.c:32-63, 181-217, 519-548).
+struct TransferDev {
+  int which;           // 1 EH, 2 table, other: 1.0
+  double kconv;        // unit conversion of k before evaluation
+  double eh_s, eh_a, eh_theta2, eh_omh, eh_twoe;  // k-independent EH terms
+  int nt;
+  const double* logk;  // sorted log10 k
+  const double* logT;  // log10 T - log10 T[0]
+};
+
+// One product term of the regrouped NG sum: h[K] += c * f[i] * f[j].
+struct NgTerm {
+  int i, j;  // field slots
+  double c;
+};
+
+// Weight list entry: W(k) += c * g_n(k).
+struct NgWeight {
+  int n;     // mode index of the weight function
+  double c;  // coefficient
+};
+
+template <typename T> struct Pipeline {
+  // FFT passes on nf consecutive fields starting at f (stride g.fs)
+  static int c2r(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st);
+  static int r2c(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st);
+  static int xy_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int axis, int dir,
+                     hipStream_t st);
+  static int z_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int dir, hipStream_t st);
+
+  static int gauss(cx_t<T>* phi, const Geom& g, const KSpace& ks, const uint32_t* seeds, double anorm,
+                   double ns, double fac, double beta, double phase_shift, int fixed_amp, hipStream_t st);
+  // out[m] = q_{modes[m]}(k) * phi(k) for m < M (cut and origin -> 0)
+  static int modes_spec(const cx_t<T>* phi, cx_t<T>* out, int M, const int* modes, const Geom& g,
+                        const KSpace& ks, hipStream_t st);
+  // real points: out[K] = sum_{terms of group K} c f[i] f[j]
+  static int products(const cx_t<T>* fin, int M, cx_t<T>* fout, int K, const NgTerm* terms,
+                      const int* group_start, const Geom& g, hipStream_t st);
+  // acc(k) += W(k) * c(k), W from the weight list (pairwise path)
+  static int ng_accumulate(double2* acc, const cx_t<T>* c, const NgWeight* w, int nw, const Geom& g,
+                           const KSpace& ks, hipStream_t st);
+  // Phi = phiG + fnl/N^3 * sum_K g_{Kmode[K]} hK   (batched path)
+  static int ng_combine(const cx_t<T>* phig, const cx_t<T>* h, int K, const int* kmodes,
+                        cx_t<T>* phi, double fnl, const Geom& g, const KSpace& ks, hipStream_t st);
+  // Phi = phiG + fnl/N^3 * acc   (pairwise path)
+  static int ng_combine_acc(const cx_t<T>* phig, const double2* acc, cx_t<T>* phi, double fnl,
+                            double n3, const Geom& g, hipStream_t st);
+  // 6 displacement-gradient spectra and 3 CIC-deconvolved ZA spectra
+  static int lpt_spec(const cx_t<T>* phi, cx_t<T>* out9, const Geom& g, const KSpace& ks,
+                      const TransferDev& tf, double dplus, double beta, hipStream_t st);
+  static int source(cx_t<T>* dg6, const Geom& g, hipStream_t st);  // writes into dg6[3]
+  static int disp2_spec(const cx_t<T>* src, cx_t<T>* out3, const Geom& g, const KSpace& ks,
+                        hipStream_t st);
+  // disp3 / disp2_3: three consecutive real fields each (ZA, 2LPT)
+  static int cic(const cx_t<T>* disp3, const cx_t<T>* disp2_3, const Geom& g, const float* glass,
+                 int nglass, double gbox, int tile, double box, double vp1, double vp2, float* pos,
+                 float* vel, long long npart, hipStream_t st);
+};
+
+// zero n doubles (graph-capturable; no memset nodes)
+int aar_zero_fill(double* p, long long n, hipStream_t st);
+// n uniforms of one ranlxd1 stream on the device (test hook)
+int aar_ranlxd_draws_dev(uint32_t seed, int n, double* out, hipStream_t st);
+
+// Twiddle tables: tw[L + m] = exp(-2 pi i m / L) for every power of two L <= 2048.
+template <typename T> void make_twiddles(cx_t<T>* host_out /* 4096 entries */);

@@ -1,0 +1,833 @@
+// HIP kernels of the NG-2LPT initial-condition pipeline for gfx950.
+//
+// All k-space kernels evaluate the reference's expressions in the same
+// operation order (compiled with -ffp-contract=off), reading the per-axis
+// k components and cut flags from host-built tables, so that cut decisions
+// and radial arguments agree bit for bit with Aarambam/LPT/Basis/main.c.
+#include <math.h>
+
+#include "aar_fft.h"
+#include "aar_kernels.h"
+#include "aar_ranlxd.h"
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ void decode_k(const Geom& g, long long e, int& x, int& y, int& kz) {
+  if (e < g.nq) {
+    kz = (int)(e % g.H);
+    long long r = e / g.H;
+    y = (int)(r % g.N);
+    x = (int)(r / g.N);
+  } else {
+    long long r = e - g.nq;
+    kz = g.H;
+    y = (int)(r % g.N);
+    x = (int)(r / g.N);
+  }
+}
+
+// main.c:452-462: SphereMode 1 keeps |k| L/2pi <= Nsample/2, else the cube.
+__device__ __forceinline__ bool outside_cut(const KSpace& ks, int x, int y, int z, double kmag1) {
+  if (ks.sphere == 1) return kmag1 * ks.box / (2 * AAR_PI) > ks.nsh;
+  return (ks.outc[x] | ks.outc[y] | ks.outc[z]) != 0;
+}
+
+__device__ __forceinline__ double kmag2_of(const KSpace& ks, int x, int y, int z) {
+  const double a = ks.kv[x], b = ks.kv[y], c = ks.kv[z];
+  return a * a + b * b + c * c;
+}
+
+// Upward Legendre recurrence, GSL gsl_sf_legendre_Pl (legendre_poly.c).
+struct LegRec {
+  double x, pm2, pm1;
+  int l;
+  __device__ void init(double xx) {
+    x = xx;
+    pm2 = 1.0;
+    pm1 = xx;
+    l = 1;
+  }
+  __device__ double at(int L) {
+    if (L == 0) return 1.0;
+    while (l < L) {
+      ++l;
+      double p = (x * (2 * l - 1) * pm1 - (l - 1) * pm2) / l;
+      pm2 = pm1;
+      pm1 = p;
+    }
+    return pm1;
+  }
+};
+
+// Radial weight g_n(k) of the modal basis (main.c:835-862):
+// n <= 3: k^{s(n-3)};  n >= 4: (P_{n-1}(xbar) - X0[n-1]) / k^2.
+__device__ __forceinline__ double weight_g(int n, double ks, double kmag2, LegRec& leg,
+                                           const double* x0) {
+  if (n <= 3) {
+    double r = 1.0;
+    for (int q = n; q < 3; ++q) r *= ks;
+    return 1.0 / r;
+  }
+  return (leg.at(n - 1) - x0[n - 1]) / kmag2;
+}
+
+// ------------------------------------------------------------------ twiddles
+template <typename T> void make_twiddles(cx_t<T>* tw) {
+  const long double tp = 6.283185307179586476925286766559005768L;
+  tw[0] = cmake<cx_t<T>>(0, 0);
+  tw[1] = cmake<cx_t<T>>(0, 0);
+  for (int L = 2; L <= 2048; L *= 2)
+    for (int m = 0; m < L; ++m) {
+      long double a = tp * (long double)m / (long double)L;
+      tw[L + m] = cmake<cx_t<T>>((T)cosl(a), (T)(-sinl(a)));
+    }
+}
+template void make_twiddles<float>(float2*);
+template void make_twiddles<double>(double2*);
+
+// ------------------------------------------------------------------ launch shapes
+template <typename T> struct Shape {
+  static constexpr int CMAX = sizeof(T) == 4 ? 16 : 8;  // 128-byte row segments
+};
+__host__ __device__ constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// Threads per tile: 16 elements per thread where possible, at least 2.
+__host__ __device__ constexpr int tile_threads(int CL) {
+  return clampi(CL / 16, CL / 2 < 64 ? CL / 2 : 64, 1024);
+}
+template <typename T, int L> struct XYCfg {
+  static constexpr int C = Shape<T>::CMAX < L / 2 ? Shape<T>::CMAX : L / 2;
+  static constexpr int NT = tile_threads(C * L);
+};
+template <typename T, int H> struct ZCfg {
+  static constexpr int C = Shape<T>::CMAX;
+  static constexpr int NT = tile_threads(C * H);
+};
+
+// ------------------------------------------------------------------ x / y passes
+// One tile = C adjacent columns of one field.  Tiles [0, nmain) cover the
+// kz < N/2 block, the rest the Nyquist plane.
+template <typename T, int L, int AXIS, int DIR>
+__global__ void __launch_bounds__((XYCfg<T, L>::NT))
+    k_xy_pass(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg, int nmain) {
+  using C2 = cx_t<T>;
+  constexpr int C = XYCfg<T, L>::C, NT = XYCfg<T, L>::NT;
+  constexpr int CP = tile_cp<T>(L, C);
+  __shared__ C2 tile[L * CP];
+  __shared__ C2 tw[L];
+  load_twiddles(tw, twg + L, L, NT);
+  C2* f = base + (long long)blockIdx.y * g.fs;
+  const int t = blockIdx.x;
+  long long b0, es, cs;
+  const int N = g.N, H = g.H;
+  if (t < nmain) {
+    const int per = H / C;
+    const int a = t / per, kzt = t % per;
+    if (AXIS == 0) {  // fixed y = a, along x
+      b0 = (long long)a * H + kzt * C;
+      es = (long long)N * H;
+    } else {  // fixed x = a, along y
+      b0 = (long long)a * N * H + kzt * C;
+      es = H;
+    }
+    cs = 1;
+  } else {
+    const int u = t - nmain;  // Nyquist plane nyq[x][y]
+    if (AXIS == 0) {          // columns y, along x
+      b0 = g.nq + (long long)u * C;
+      es = N;
+      cs = 1;
+    } else {  // columns x, along y (contiguous)
+      b0 = g.nq + (long long)u * C * N;
+      es = 1;
+      cs = N;
+    }
+  }
+  C2* src = f + b0;
+  auto load = [&](int pos, int col) { return src[pos * es + col * cs]; };
+  auto store = [&](int pos, int col, C2 v) { src[pos * es + col * cs] = v; };
+  fft_tile<T, L, C, NT, DIR, 1, false>(tile, tw, load, store);
+}
+
+template <typename T, int L, int AXIS, int DIR>
+static int launch_xy(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st) {
+  constexpr int C = XYCfg<T, L>::C, NT = XYCfg<T, L>::NT;
+  const int nmain = g.N * (g.H / C);
+  const int nnyq = g.N / C;
+  dim3 grid(nmain + nnyq, nf);
+  hipLaunchKernelGGL((k_xy_pass<T, L, AXIS, DIR>), grid, dim3(NT), 0, st, f, g, tw, nmain);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+// ------------------------------------------------------------------ z passes
+// Real transforms along the contiguous axis, C rows per block, through a
+// half-length complex FFT.  c2r: Y[k] = (X_k + X*_{H-k}) + i (X_k - X*_{H-k}) e^{+2 pi i k/N},
+// z = IFFT_H(Y), real row = (Re z_m, Im z_m) interleaved; the imaginary
+// parts of X_0 and X_H are ignored (FFTW's half-complex convention).
+// r2c: Z = FFT_H(x_{2m} + i x_{2m+1}), X_k = E_k + e^{-2 pi i k/N} O_k.
+template <typename T, int H>
+__global__ void __launch_bounds__((ZCfg<T, H>::NT))
+    k_z_c2r(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg) {
+  using C2 = cx_t<T>;
+  constexpr int C = ZCfg<T, H>::C, NT = ZCfg<T, H>::NT;
+  constexpr int N = 2 * H;
+  constexpr int CP = tile_cp<T>(H, C);
+  __shared__ C2 tile[H * CP];
+  __shared__ C2 tw[N];
+  __shared__ C2 xh[C];
+  load_twiddles(tw, twg + N, N, NT);
+  C2* f = base + (long long)blockIdx.y * g.fs;
+  const long long r0 = (long long)blockIdx.x * C;
+  C2* rows = f + r0 * H;
+  for (int e = threadIdx.x; e < C * H; e += NT) {
+    const int c = e / H, m = e % H;
+    tile[m * CP + c] = rows[(long long)c * H + m];
+  }
+  if (threadIdx.x < C) xh[threadIdx.x] = f[g.nq + r0 + threadIdx.x];
+  __syncthreads();
+  // pre-processing, pairs (k, H-k)
+  for (int e = threadIdx.x; e < C * (H / 2 + 1); e += NT) {
+    const int c = e % C, k = e / C;
+    if (k == 0) {
+      const T a = tile[c].x, b = xh[c].x;
+      tile[c] = cmake<C2>(a + b, a - b);
+    } else {
+      const int k2 = H - k;
+      const C2 A = tile[k * CP + c], B = tile[k2 * CP + c];
+      // Y[k]
+      C2 w = tw[k];
+      w.y = -w.y;  // e^{+2 pi i k/N}
+      C2 s = cadd(A, cconj(B)), d = csub(A, cconj(B));
+      C2 dw = cmul(d, w);
+      C2 Yk = cmake<C2>(s.x - dw.y, s.y + dw.x);
+      if (k2 != k) {
+        C2 w2 = tw[k2];
+        w2.y = -w2.y;
+        C2 s2 = cadd(B, cconj(A)), d2 = csub(B, cconj(A));
+        C2 dw2 = cmul(d2, w2);
+        tile[k2 * CP + c] = cmake<C2>(s2.x - dw2.y, s2.y + dw2.x);
+      }
+      tile[k * CP + c] = Yk;
+    }
+  }
+  auto load = [&](int pos, int col) { return tile[pos * CP + col]; };
+  auto store = [&](int pos, int col, C2 v) { rows[(long long)col * H + pos] = v; };
+  fft_tile<T, H, C, NT, +1, 2, true>(tile, tw, load, store);
+}
+
+template <typename T, int H>
+__global__ void __launch_bounds__((ZCfg<T, H>::NT))
+    k_z_r2c(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg) {
+  using C2 = cx_t<T>;
+  constexpr int C = ZCfg<T, H>::C, NT = ZCfg<T, H>::NT;
+  constexpr int N = 2 * H;
+  constexpr int CP = tile_cp<T>(H, C);
+  __shared__ C2 tile[H * CP];
+  __shared__ C2 tw[N];
+  load_twiddles(tw, twg + N, N, NT);
+  C2* f = base + (long long)blockIdx.y * g.fs;
+  const long long r0 = (long long)blockIdx.x * C;
+  C2* rows = f + r0 * H;
+  auto load = [&](int pos, int col) { return rows[(long long)col * H + pos]; };
+  auto store = [&](int pos, int col, C2 v) { tile[pos * CP + col] = v; };
+  __syncthreads();
+  fft_tile<T, H, C, NT, -1, 2, false>(tile, tw, load, store);
+  __syncthreads();
+  for (int e = threadIdx.x; e < C * (H / 2 + 1); e += NT) {
+    const int c = e % C, k = e / C;
+    C2* out = rows + (long long)c * H;
+    if (k == 0) {
+      const C2 z = tile[c];
+      out[0] = cmake<C2>(z.x + z.y, (T)0);
+      f[g.nq + r0 + c] = cmake<C2>(z.x - z.y, (T)0);
+    } else {
+      const int k2 = H - k;
+      const C2 A = tile[k * CP + c], B = tile[k2 * CP + c];
+      // E = (A + B*)/2, O = (A - B*)/(2i), X_k = E + W^k O, W = e^{-2 pi i/N}
+      C2 E = cscale(cadd(A, cconj(B)), (T)0.5);
+      C2 dd = csub(A, cconj(B));
+      C2 O = cmake<C2>(dd.y * (T)0.5, -dd.x * (T)0.5);
+      out[k] = cadd(E, cmul(O, tw[k]));
+      if (k2 != k) {
+        C2 E2 = cscale(cadd(B, cconj(A)), (T)0.5);
+        C2 dd2 = csub(B, cconj(A));
+        C2 O2 = cmake<C2>(dd2.y * (T)0.5, -dd2.x * (T)0.5);
+        out[k2] = cadd(E2, cmul(O2, tw[k2]));
+      }
+    }
+  }
+}
+
+template <typename T, int H>
+static int launch_z(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int dir, hipStream_t st) {
+  constexpr int C = ZCfg<T, H>::C, NT = ZCfg<T, H>::NT;
+  dim3 grid((unsigned)((long long)g.N * g.N / C), nf);
+  if (dir > 0)
+    hipLaunchKernelGGL((k_z_c2r<T, H>), grid, dim3(NT), 0, st, f, g, tw);
+  else
+    hipLaunchKernelGGL((k_z_r2c<T, H>), grid, dim3(NT), 0, st, f, g, tw);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+template <typename T>
+int Pipeline<T>::xy_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int axis, int dir,
+                         hipStream_t st) {
+#define XY_CASE(LL)                                                                     \
+  case LL:                                                                              \
+    if (axis == 0)                                                                      \
+      return dir > 0 ? launch_xy<T, LL, 0, 1>(f, nf, g, tw, st)                         \
+                     : launch_xy<T, LL, 0, -1>(f, nf, g, tw, st);                       \
+    return dir > 0 ? launch_xy<T, LL, 1, 1>(f, nf, g, tw, st) : launch_xy<T, LL, 1, -1>(f, nf, g, tw, st);
+  switch (g.N) {
+    XY_CASE(16)
+    XY_CASE(32)
+    XY_CASE(64)
+    XY_CASE(128)
+    XY_CASE(256)
+    XY_CASE(512)
+    XY_CASE(1024)
+    default:
+      return AAR_ERR_UNSUPPORTED;
+  }
+#undef XY_CASE
+}
+
+template <typename T>
+int Pipeline<T>::z_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int dir, hipStream_t st) {
+  switch (g.H) {
+    case 8: return launch_z<T, 8>(f, nf, g, tw, dir, st);
+    case 16: return launch_z<T, 16>(f, nf, g, tw, dir, st);
+    case 32: return launch_z<T, 32>(f, nf, g, tw, dir, st);
+    case 64: return launch_z<T, 64>(f, nf, g, tw, dir, st);
+    case 128: return launch_z<T, 128>(f, nf, g, tw, dir, st);
+    case 256: return launch_z<T, 256>(f, nf, g, tw, dir, st);
+    case 512: return launch_z<T, 512>(f, nf, g, tw, dir, st);
+    default: return AAR_ERR_UNSUPPORTED;
+  }
+}
+
+// C2R with FFTW's order: complex inverse over x and y, then half-complex over z.
+template <typename T>
+int Pipeline<T>::c2r(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st) {
+  int rc;
+  if ((rc = xy_pass(f, nf, g, tw, 0, +1, st))) return rc;
+  if ((rc = xy_pass(f, nf, g, tw, 1, +1, st))) return rc;
+  return z_pass(f, nf, g, tw, +1, st);
+}
+
+template <typename T>
+int Pipeline<T>::r2c(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st) {
+  int rc;
+  if ((rc = z_pass(f, nf, g, tw, -1, st))) return rc;
+  if ((rc = xy_pass(f, nf, g, tw, 1, -1, st))) return rc;
+  return xy_pass(f, nf, g, tw, 0, -1, st);
+}
+
+// ------------------------------------------------------------------ Gaussian potential
+// One thread per (i, j) pencil, main.c:382-524.  The stream of pencil (i, j)
+// is reseeded with seedtable[i N + j]; two uniforms (phase, amplitude) are
+// drawn for every k < N/2, also where the mode is cut or Nyquist.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_gauss(cx_t<T>* __restrict__ phi, Geom g, KSpace ks, const uint32_t* __restrict__ seeds,
+            double anorm, double ns, double fac, double beta, double phase_shift, int fixed_amp) {
+  using C2 = cx_t<T>;
+  const int N = g.N, H = g.H;
+  const long long pid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid >= (long long)N * N) return;
+  const int i = (int)(pid / N), j = (int)(pid % N);
+  C2* row = phi + ((long long)i * N + j) * H;
+  const bool nyq_pencil = (i == H) || (j == H);
+  const int ii = (N - i) % N, jj = (N - j) % N;
+  // Which k = 0 entries this pencil owns (main.c:476-520).
+  const bool gen0 = !nyq_pencil && ((i == 0 && j > 0 && j < H) || (i > 0 && i < H));
+  const bool zero0 = nyq_pencil || (i == 0 && j == 0);
+  phi[g.nq + pid] = cmake<C2>(0, 0);
+
+  auto emit = [&](int k, double u1, double u2) {
+    double phase = u1 * 2 * AAR_PI;
+    phase += phase_shift;
+    C2 val = cmake<C2>(0, 0);
+    bool live = !nyq_pencil && !(i == 0 && j == 0 && k == 0);
+    if (live) {
+      const double kmag2 = kmag2_of(ks, i, j, k);
+      const double kmag = sqrt(kmag2);
+      if (outside_cut(ks, i, j, k, kmag)) {
+        live = false;
+      } else {
+        double phig = anorm * exp(ns * log(kmag));
+        if (!fixed_amp) phig *= -log(u2);
+        phig = sqrt(phig) * fac * beta / kmag2;
+        val = cmake<C2>((T)(phig * cos(phase)), (T)(phig * sin(phase)));
+      }
+    }
+    if (k > 0) {
+      row[k] = val;
+    } else if (gen0) {
+      row[0] = val;
+      phi[((long long)ii * N + jj) * H] = cmake<C2>(val.x, -val.y);
+    } else if (zero0) {
+      row[0] = cmake<C2>(0, 0);
+    }
+  };
+
+  Ranlxd r;
+  ranlxd_seed(r, seeds[pid]);
+  bool redo = false;
+  for (int kb = 0; kb < H; kb += 6) {
+    ranlxd_update(r);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int k = kb + q;
+      if (k < H) {
+        const double u1 = (double)r.s[2 * q] * kTwoM48;
+        const double u2 = (double)r.s[2 * q + 1] * kTwoM48;
+        if (u2 == 0) redo = true;
+        if (!redo) emit(k, u1, u2);
+      }
+    }
+    if (redo) break;
+  }
+  if (redo) {  // an amplitude draw was exactly 0: replay with GSL's redraw loop
+    Ranlxd rr;
+    ranlxd_seed(rr, seeds[pid]);
+    int pos = 12;
+    auto next = [&]() {
+      if (pos == 12) {
+        ranlxd_update(rr);
+        pos = 0;
+      }
+      uint64_t v = rr.s[0];
+#pragma unroll
+      for (int q = 1; q < 12; ++q) v = (pos == q) ? rr.s[q] : v;
+      ++pos;
+      return (double)v * kTwoM48;
+    };
+    for (int k = 0; k < H; ++k) {
+      const double u1 = next();
+      double u2;
+      do u2 = next();
+      while (u2 == 0);
+      emit(k, u1, u2);
+    }
+  }
+}
+
+template <typename T>
+int Pipeline<T>::gauss(cx_t<T>* phi, const Geom& g, const KSpace& ks, const uint32_t* seeds, double anorm,
+                       double ns, double fac, double beta, double phase_shift, int fixed_amp,
+                       hipStream_t st) {
+  const long long np = (long long)g.N * g.N;
+  hipLaunchKernelGGL((k_gauss<T>), dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, phi, g, ks,
+                     seeds, anorm, ns, fac, beta, phase_shift, fixed_amp);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+// ------------------------------------------------------------------ mode fields
+// out[m](k) = q_n(k) phi(k), n = modes[m] (ascending), q_n = g_n(k) k^{4-ns}:
+// n <= 3: k^{s n};  n >= 4: (P_{n-1}(xbar) - X0[n-1]) k^{2-ns}   (main.c:571-639).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_modes_spec(const cx_t<T>* __restrict__ phi, cx_t<T>* __restrict__ out, int M,
+                 const int* __restrict__ modes, Geom g, KSpace ks) {
+  using C2 = cx_t<T>;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.fs) return;
+  int x, y, z;
+  decode_k(g, e, x, y, z);
+  const double kmag2 = kmag2_of(ks, x, y, z);
+  const double kmag1 = sqrt(kmag2);
+  const bool dead = (x + y + z == 0) || outside_cut(ks, x, y, z, kmag1);
+  if (dead) {
+    for (int m = 0; m < M; ++m) out[(long long)m * g.fs + e] = cmake<C2>(0, 0);
+    return;
+  }
+  const C2 p = phi[e];
+  const double kss = pow(kmag1, ks.s);
+  const double xb = -1 + 2 * (kss - ks.kmin_s) / (ks.kmax_s - ks.kmin_s);
+  const double k3s = kss * kss * kss;  // k^{4-ns}
+  LegRec leg;
+  leg.init(xb);
+  for (int m = 0; m < M; ++m) {
+    const int n = modes[m];
+    double q;
+    if (n <= 3) {
+      q = 1.0;
+      for (int t = 0; t < n; ++t) q *= kss;
+    } else {
+      q = (leg.at(n - 1) - ks.x0[n - 1]) * (k3s / kmag2);
+    }
+    out[(long long)m * g.fs + e] = cmake<C2>((T)(q * (double)p.x), (T)(q * (double)p.y));
+  }
+}
+
+template <typename T>
+int Pipeline<T>::modes_spec(const cx_t<T>* phi, cx_t<T>* out, int M, const int* modes, const Geom& g,
+                            const KSpace& ks, hipStream_t st) {
+  hipLaunchKernelGGL((k_modes_spec<T>), dim3((unsigned)((g.fs + 255) / 256)), dim3(256), 0, st, phi,
+                     out, M, modes, g, ks);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+// ------------------------------------------------------------------ real-space products
+template <typename T, int MMAX>
+__global__ void __launch_bounds__(256)
+    k_products(const cx_t<T>* __restrict__ fin, int M, cx_t<T>* __restrict__ fout, int K,
+               const NgTerm* __restrict__ terms, const int* __restrict__ gs, Geom g) {
+  const long long n3 = (long long)g.N * g.N * g.N;
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n3) return;
+  T f[MMAX];
+#pragma unroll
+  for (int m = 0; m < MMAX; ++m)
+    if (m < M) f[m] = reinterpret_cast<const T*>(fin + (long long)m * g.fs)[p];
+  for (int kk = 0; kk < K; ++kk) {
+    T h = 0;
+    for (int t = gs[kk]; t < gs[kk + 1]; ++t) {
+      const NgTerm tm = terms[t];
+      h += (T)tm.c * (f[tm.i] * f[tm.j]);
+    }
+    reinterpret_cast<T*>(fout + (long long)kk * g.fs)[p] = h;
+  }
+}
+
+template <typename T>
+int Pipeline<T>::products(const cx_t<T>* fin, int M, cx_t<T>* fout, int K, const NgTerm* terms,
+                          const int* gs, const Geom& g, hipStream_t st) {
+  const long long n3 = (long long)g.N * g.N * g.N;
+  dim3 grid((unsigned)((n3 + 255) / 256));
+  if (M <= 8)
+    hipLaunchKernelGGL((k_products<T, 8>), grid, dim3(256), 0, st, fin, M, fout, K, terms, gs, g);
+  else if (M <= 16)
+    hipLaunchKernelGGL((k_products<T, 16>), grid, dim3(256), 0, st, fin, M, fout, K, terms, gs, g);
+  else if (M <= 32)
+    hipLaunchKernelGGL((k_products<T, 32>), grid, dim3(256), 0, st, fin, M, fout, K, terms, gs, g);
+  else
+    return AAR_ERR_UNSUPPORTED;
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+// ------------------------------------------------------------------ NG accumulation
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_ng_accumulate(double2* __restrict__ acc, const cx_t<T>* __restrict__ c,
+                    const NgWeight* __restrict__ w, int nw, Geom g, KSpace ks) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.fs) return;
+  int x, y, z;
+  decode_k(g, e, x, y, z);
+  if (x + y + z == 0) return;
+  const double kmag2 = kmag2_of(ks, x, y, z);
+  const double kmag1 = sqrt(kmag2);
+  if (outside_cut(ks, x, y, z, kmag1)) return;
+  const double kss = pow(kmag1, ks.s);
+  LegRec leg;
+  leg.init(-1 + 2 * (kss - ks.kmin_s) / (ks.kmax_s - ks.kmin_s));
+  double W = 0;
+  for (int l = 0; l < nw; ++l) W += w[l].c * weight_g(w[l].n, kss, kmag2, leg, ks.x0);
+  const cx_t<T> v = c[e];
+  double2 a = acc[e];
+  a.x += W * (double)v.x;
+  a.y += W * (double)v.y;
+  acc[e] = a;
+}
+
+template <typename T>
+int Pipeline<T>::ng_accumulate(double2* acc, const cx_t<T>* c, const NgWeight* w, int nw, const Geom& g,
+                               const KSpace& ks, hipStream_t st) {
+  hipLaunchKernelGGL((k_ng_accumulate<T>), dim3((unsigned)((g.fs + 255) / 256)), dim3(256), 0, st, acc,
+                     c, w, nw, g, ks);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+// Phi = phiG + Fnl NG / N^3 with NG = sum_K g_K(k) h_K(k) (main.c:1104-1126).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_ng_combine(const cx_t<T>* __restrict__ phig, const cx_t<T>* __restrict__ h, int K,
+                 const int* __restrict__ kmodes, cx_t<T>* __restrict__ phi, double fnl, double n3,
+                 Geom g, KSpace ks) {
+  using C2 = cx_t<T>;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.fs) return;
+  int x, y, z;
+  decode_k(g, e, x, y, z);
+  const C2 pg = phig[e];
+  double re = pg.x, im = pg.y;
+  if (x + y + z != 0) {
+    const double kmag2 = kmag2_of(ks, x, y, z);
+    const double kmag1 = sqrt(kmag2);
+    if (!outside_cut(ks, x, y, z, kmag1)) {
+      const double kss = pow(kmag1, ks.s);
+      LegRec leg;
+      leg.init(-1 + 2 * (kss - ks.kmin_s) / (ks.kmax_s - ks.kmin_s));
+      double nr = 0, ni = 0;
+      for (int kk = 0; kk < K; ++kk) {
+        const double w = weight_g(kmodes[kk], kss, kmag2, leg, ks.x0);
+        const C2 v = h[(long long)kk * g.fs + e];
+        nr += w * (double)v.x;
+        ni += w * (double)v.y;
+      }
+      re += fnl * nr / n3;
+      im += fnl * ni / n3;
+    }
+  } else {
+    re = 0;
+    im = 0;
+  }
+  phi[e] = cmake<C2>((T)re, (T)im);
+}
+
+template <typename T>
+int Pipeline<T>::ng_combine(const cx_t<T>* phig, const cx_t<T>* h, int K, const int* kmodes,
+                            cx_t<T>* phi, double fnl, const Geom& g, const KSpace& ks,
+                            hipStream_t st) {
+  const double n3 = (double)((unsigned)g.N * (unsigned)g.N * (unsigned)g.N);
+  hipLaunchKernelGGL((k_ng_combine<T>), dim3((unsigned)((g.fs + 255) / 256)), dim3(256), 0, st, phig, h,
+                     K, kmodes, phi, fnl, n3, g, ks);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_ng_combine_acc(const cx_t<T>* __restrict__ phig, const double2* __restrict__ acc,
+                     cx_t<T>* __restrict__ phi, double fnl, double n3, Geom g) {
+  using C2 = cx_t<T>;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.fs) return;
+  const C2 pg = phig[e];
+  const double2 a = acc[e];
+  double re = (double)pg.x + fnl * a.x / n3;
+  double im = (double)pg.y + fnl * a.y / n3;
+  if (e == 0) re = im = 0;
+  phi[e] = cmake<C2>((T)re, (T)im);
+}
+
+template <typename T>
+int Pipeline<T>::ng_combine_acc(const cx_t<T>* phig, const double2* acc, cx_t<T>* phi, double fnl,
+                                double n3, const Geom& g, hipStream_t st) {
+  hipLaunchKernelGGL((k_ng_combine_acc<T>), dim3((unsigned)((g.fs + 255) / 256)), dim3(256), 0, st, phig,
+                     acc, phi, fnl, n3, g);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+// ------------------------------------------------------------------ 2LPT
+__device__ double transfer_dev(const TransferDev& tf, double k) {
+  if (tf.which == 1) {  // Eisenstein & Hu no-wiggle, power.c:519-548
+    k *= tf.kconv;
+    double gamma = tf.eh_a + (1. - tf.eh_a) / (1. + exp(4 * log(0.43 * k * tf.eh_s)));
+    gamma *= tf.eh_omh;
+    double q = k * tf.eh_theta2 / gamma;
+    double L0 = log(tf.eh_twoe + 1.8 * q);
+    double C0 = 14.2 + 731. / (1. + 62.5 * q);
+    return L0 / (L0 + C0 * q * q);
+  }
+  if (tf.which == 2) {  // log-log table, power.c:181-217
+    k *= tf.kconv;
+    double logk = log10(k);
+    if (logk < tf.logk[0] || logk > tf.logk[tf.nt - 1]) return 0;
+    int lo = 0, hi = tf.nt - 1;
+    while (hi - lo > 1) {
+      int mid = (hi + lo) / 2;
+      if (logk < tf.logk[mid]) hi = mid;
+      else lo = mid;
+    }
+    double u = (logk - tf.logk[lo]) / (tf.logk[hi] - tf.logk[lo]);
+    double logT = (1 - u) * tf.logT[lo] + u * tf.logT[hi];
+    return pow(10.0, logT);
+  }
+  return 1.0;
+}
+
+// out9[0..5] = i k_b psi_a for ab = 00,01,02,11,12,22 (main.c:1238-1277);
+// out9[6..8] = psi_a * CIC deconvolution (main.c:1311-1369).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_lpt_spec(const cx_t<T>* __restrict__ phi, cx_t<T>* __restrict__ out, Geom g, KSpace ks,
+               TransferDev tf, double dplus, double beta) {
+  using C2 = cx_t<T>;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.fs) return;
+  int x, y, z;
+  decode_k(g, e, x, y, z);
+  const double kv[3] = {ks.kv[x], ks.kv[y], ks.kv[z]};
+  const double kmag2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+  const double kmag = sqrt(kmag2);
+  const double twb = transfer_dev(tf, kmag) / dplus / beta;
+  const C2 p = phi[e];
+  double cr[3], ci[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    ci[a] = kv[a] * twb * (double)p.x;
+    cr[a] = -kv[a] * twb * (double)p.y;
+  }
+  const int ga[6] = {0, 0, 0, 1, 1, 2}, gb[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+    out[(long long)q * g.fs + e] = cmake<C2>((T)(-ci[ga[q]] * kv[gb[q]]), (T)(cr[ga[q]] * kv[gb[q]]));
+  const double ff = 1 / (ks.sinc[x] * ks.sinc[y] * ks.sinc[z]);
+  const double smth = ff * ff;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    out[(long long)(6 + a) * g.fs + e] = cmake<C2>((T)(cr[a] * smth), (T)(ci[a] * smth));
+}
+
+template <typename T>
+int Pipeline<T>::lpt_spec(const cx_t<T>* phi, cx_t<T>* out9, const Geom& g, const KSpace& ks,
+                          const TransferDev& tf, double dplus, double beta, hipStream_t st) {
+  hipLaunchKernelGGL((k_lpt_spec<T>), dim3((unsigned)((g.fs + 255) / 256)), dim3(256), 0, st, phi, out9,
+                     g, ks, tf, dplus, beta);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+// second-order source into field 3 (main.c:1284-1294)
+template <typename T>
+__global__ void __launch_bounds__(256) k_source(cx_t<T>* __restrict__ dg, Geom g) {
+  const long long n3 = (long long)g.N * g.N * g.N;
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n3) return;
+  T d[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) d[q] = reinterpret_cast<const T*>(dg + (long long)q * g.fs)[p];
+  reinterpret_cast<T*>(dg + 3 * g.fs)[p] =
+      d[0] * (d[3] + d[5]) + d[3] * d[5] - d[1] * d[1] - d[2] * d[2] - d[4] * d[4];
+}
+
+template <typename T> int Pipeline<T>::source(cx_t<T>* dg6, const Geom& g, hipStream_t st) {
+  const long long n3 = (long long)g.N * g.N * g.N;
+  hipLaunchKernelGGL((k_source<T>), dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st, dg6, g);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+// psi2_a = src k_a / (i k^2), CIC-deconvolved (main.c:1311-1369)
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_disp2_spec(const cx_t<T>* __restrict__ src, cx_t<T>* __restrict__ out, Geom g, KSpace ks) {
+  using C2 = cx_t<T>;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.fs) return;
+  int x, y, z;
+  decode_k(g, e, x, y, z);
+  const double kv[3] = {ks.kv[x], ks.kv[y], ks.kv[z]};
+  const double kmag2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+  const double ff = 1 / (ks.sinc[x] * ks.sinc[y] * ks.sinc[z]);
+  const double smth = ff * ff;
+  const C2 s = src[e];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double re = 0, im = 0;
+    if (kmag2 > 0.0) {
+      re = (double)s.y * kv[a] / kmag2;
+      im = -(double)s.x * kv[a] / kmag2;
+    }
+    out[(long long)a * g.fs + e] = cmake<C2>((T)(re * smth), (T)(im * smth));
+  }
+}
+
+template <typename T>
+int Pipeline<T>::disp2_spec(const cx_t<T>* src, cx_t<T>* out3, const Geom& g, const KSpace& ks,
+                            hipStream_t st) {
+  hipLaunchKernelGGL((k_disp2_spec<T>), dim3((unsigned)((g.fs + 255) / 256)), dim3(256), 0, st, src,
+                     out3, g, ks);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+// ------------------------------------------------------------------ particles
+// Glass tiling (read_glass.c:176-214) and CIC readout with the ZA + 2LPT
+// update (main.c:1440-1516, periodic_wrap main.c:1534-1544).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_cic(const cx_t<T>* __restrict__ disp, const cx_t<T>* __restrict__ disp2, Geom g,
+          const float* __restrict__ glass, int nglass,
+          double gbox, int tile, double box, double vp1, double vp2, float* __restrict__ pos,
+          float* __restrict__ vel, long long npart) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npart) return;
+  const int N = g.N;
+  const long long t = p / nglass;
+  const int n = (int)(p % nglass);
+  const int ti = (int)(t / ((long long)tile * tile)), tj = (int)((t / tile) % tile), tk = (int)(t % tile);
+  const double bt = box / tile;
+  float P[3];
+  P[0] = (float)(glass[3 * n] / gbox * bt + ti * bt);
+  P[1] = (float)(glass[3 * n + 1] / gbox * bt + tj * bt);
+  P[2] = (float)(glass[3 * n + 2] / gbox * bt + tk * bt);
+  double u = P[0] / box * N, v = P[1] / box * N, w = P[2] / box * N;
+  int i = (int)u, j = (int)v, k = (int)w;
+  if (i == N) i = N - 1;
+  if (i < 0) i = 0;
+  if (j == N) j = N - 1;
+  if (k == N) k = N - 1;
+  u -= i;
+  v -= j;
+  w -= k;
+  int ii = i + 1, jj = j + 1, kk = k + 1;
+  if (ii >= N) ii -= N;
+  if (jj >= N) jj -= N;
+  if (kk >= N) kk -= N;
+  const double f1 = (1 - u) * (1 - v) * (1 - w), f2 = (1 - u) * (1 - v) * (w);
+  const double f3 = (1 - u) * (v) * (1 - w), f4 = (1 - u) * (v) * (w);
+  const double f5 = (u) * (1 - v) * (1 - w), f6 = (u) * (1 - v) * (w);
+  const double f7 = (u) * (v) * (1 - w), f8 = (u) * (v) * (w);
+  const long long NN = (long long)N * N;
+  const long long c1 = i * NN + (long long)j * N, c2 = i * NN + (long long)jj * N;
+  const long long c3 = ii * NN + (long long)j * N, c4 = ii * NN + (long long)jj * N;
+  const float n3f = (float)(N * N * N);
+  float V[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const T* D = reinterpret_cast<const T*>(disp + (long long)a * g.fs);
+    const T* D2 = reinterpret_cast<const T*>(disp2 + (long long)a * g.fs);
+    double dis = (double)D[c1 + k] * f1 + (double)D[c1 + kk] * f2 + (double)D[c2 + k] * f3 +
+                 (double)D[c2 + kk] * f4 + (double)D[c3 + k] * f5 + (double)D[c3 + kk] * f6 +
+                 (double)D[c4 + k] * f7 + (double)D[c4 + kk] * f8;
+    double dis2 = (double)D2[c1 + k] * f1 + (double)D2[c1 + kk] * f2 + (double)D2[c2 + k] * f3 +
+                  (double)D2[c2 + kk] * f4 + (double)D2[c3 + k] * f5 + (double)D2[c3 + kk] * f6 +
+                  (double)D2[c4 + k] * f7 + (double)D2[c4 + kk] * f8;
+    dis2 /= n3f;
+    P[a] = (float)((double)P[a] + (dis - 3. / 7. * dis2));
+    V[a] = (float)(dis * vp1 - 3. / 7. * dis2 * vp2);
+    P[a] = (float)fmod(fmod((double)P[a], box) + box, box);
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    pos[3 * p + a] = P[a];
+    vel[3 * p + a] = V[a];
+  }
+}
+
+template <typename T>
+int Pipeline<T>::cic(const cx_t<T>* disp3, const cx_t<T>* disp2_3, const Geom& g, const float* glass,
+                     int nglass, double gbox, int tile, double box, double vp1, double vp2, float* pos,
+                     float* vel, long long npart, hipStream_t st) {
+  hipLaunchKernelGGL((k_cic<T>), dim3((unsigned)((npart + 255) / 256)), dim3(256), 0, st, disp3, disp2_3,
+                     g, glass,
+                     nglass, gbox, tile, box, vp1, vp2, pos, vel, npart);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+__global__ void k_zero(double* p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+int aar_zero_fill(double* p, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+__global__ void k_ranlxd_draws(uint32_t seed, int n, double* out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  Ranlxd r;
+  ranlxd_seed(r, seed);
+  for (int b = 0; b < n; b += 12) {
+    ranlxd_update(r);
+#pragma unroll
+    for (int q = 0; q < 12; ++q)
+      if (b + q < n) out[b + q] = (double)r.s[q] * kTwoM48;
+  }
+}
+
+int aar_ranlxd_draws_dev(uint32_t seed, int n, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_ranlxd_draws, dim3(1), dim3(64), 0, st, seed, n, out);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+template struct Pipeline<float>;
+template struct Pipeline<double>;

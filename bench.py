@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Throughput of the NG-2LPT initial-condition stage on one MI355X.
+
+One step = one whole IC job on the device (Gaussian potential from the seed
+table, modal non-Gaussian sum, 2LPT, particle positions and velocities), with
+all inputs resident in HBM.  Workload: Nmesh = Npart = 512^3, Lbox = 2000
+Mpc/h, the reference's shipped ScalarI (mass 1) modal table
+(Aarambam/defaults/AlphaTable.dat, 15 modes) with X0 from its closed form at
+Nmax = 512, Eisenstein-Hu transfer, fNL = 100.  Headline: FP32 batched
+schedule; the FP64 pairwise schedule and the CPU oracle are reported beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(N=512, box=2000.0):
+    from tests import cases
+    cfg = cases.base_config(Nmesh=N, Nsample=N, Nmax=N, N_modes=15, Box=box, GlassTileFac=N // 64,
+                            Fnl=100.0, Seed=1234)
+    x0 = cases.x0_closed_form(15, cfg["PrimordialIndex"], box, N)
+    alpha = load_alpha(os.path.join(cases.GOLDEN, "AlphaTable.dat"), 15)
+    glass = cases.lattice_glass(64, 100.0)
+    return cfg, alpha, x0, glass, 100.0
+
+
+def load_alpha(path, nm):
+    import ctypes as C
+
+    import aarambam_amd as aar
+    a = np.empty(nm ** 3)
+    rc = aar.lib().aar_load_alpha(path.encode(), nm, a.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc:
+        raise RuntimeError(f"aar_load_alpha failed: {rc}")
+    return a.reshape(nm, nm, nm)
+
+
+def modal_sizes(alpha):
+    """(M, K): distinct mode fields and distinct outer indices of the regrouped sum."""
+    nm = alpha.shape[0]
+    modes, outer = set(), set()
+    for i in range(nm):
+        for j in range(i, nm):
+            for k in range(j, nm):
+                if alpha[i, j, k] != 0:
+                    modes |= {i, j}
+                    outer.add(k)
+    if alpha[0, 1, 2] != 0 and alpha[1, 1, 1] != 0:  # the two divergence-cancelling terms
+        modes |= {0, 1, 2}
+        outer |= {1, 2}
+    return len(modes), len(outer)
+
+
+def survey_bytes(N, M, K, b):
+    """Algorithmic bytes of one job: S_b (94 + 7M + 8K) + 24 N^3 (one spectrum S_b = N^2 (N/2+1) 2 b)."""
+    S = N * N * (N // 2 + 1) * 2 * b
+    return S * (94 + 7 * M + 8 * K) + 24 * N ** 3
+
+
+def time_job(aar, ctx, cfg, alpha, x0, glass, gbox, prec, sched, steps, warmup, profile):
+    t = aar.tables(alpha, x0, glass, gbox)
+    job = aar.Job(ctx, aar.params_from_dict(cfg), t, prec, sched)
+    for _ in range(warmup):
+        job.run()
+    job.sync()
+    if profile:
+        job.set_profiling(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    dev = 0.0
+    for _ in range(steps):
+        job.run()
+        dev += job.sync()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    stats = job.kernel_stats() if profile else {}
+    info = dict(wall=wall, device=dev, stats=stats, M=None)
+    job.close()
+    return info
+
+
+def cpu_baseline(N=128):
+    from oracle import pyoracle as orc
+    cfg, alpha, x0, glass, gbox = workload(N, 2000.0 * N / 512)
+    cfg["GlassTileFac"] = N // 64
+    t0 = time.perf_counter()
+    orc.run(cfg, alpha, x0, glass, gbox, want_fields=False)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=N ** 3 / dt / 1e6, unit="Mcells/s", cores=cores, kind="port",
+                sample=f"CPU oracle (C/OpenMP FP64, reference's pairwise loops), one {N}^3 job of the "
+                       f"same table, {dt:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nmesh", type=int, default=512)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=128)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs one IC job per GPU on a single GPU; multi-GPU runs are not "
+              "part of it (jobs shard across GPUs with no exchange).", file=sys.stderr)
+        sys.exit(2)
+    import aarambam_amd as aar
+    N = args.nmesh
+    cfg, alpha, x0, glass, gbox = workload(N)
+    ctx = aar.Context(0)
+    r32 = time_job(aar, ctx, cfg, alpha, x0, glass, gbox, aar.AAR_FP32, aar.AAR_SCHED_BATCHED, args.steps,
+                   args.warmup, True)
+    ms = r32["wall"] / args.steps * 1e3
+    value = N ** 3 * args.steps / r32["wall"] / 1e6
+    stats = r32["stats"]
+    dom = max(stats, key=lambda k: stats[k]["seconds"])
+    d = stats[dom]
+    achieved = d["bytes"] / d["seconds"] / 1e9
+    M, K = modal_sizes(alpha)
+    out = {
+        "metric": "Mcells/s for NG-2LPT IC at Nmesh=512 (FP32 batched; FP64 pairwise beside it)",
+        "value": round(value, 3), "unit": "Mcells/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Gaussian field, lattice glass)",
+        "config": {"workload": f"NG-2LPT IC, Nmesh=Npart={N}^3, Lbox=2000 Mpc/h, ScalarI mass=1 15-mode "
+                               f"table (M={M}, K={K}), EH P(k), fNL=100, seed 1234",
+                   "precision": "fp32", "schedule": "batched"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "kernel_time_share": {k: round(v["seconds"] / r32["device"] * args.steps / args.steps, 4)
+                              for k, v in stats.items()},
+        "job_roofline": {"bytes_per_job": survey_bytes(N, M, K, 4),
+                         "frac": round(survey_bytes(N, M, K, 4) / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)},
+    }
+    if not args.no_fp64:
+        r64 = time_job(aar, ctx, cfg, alpha, x0, glass, gbox, aar.AAR_FP64, aar.AAR_SCHED_PLAIN,
+                       args.fp64_steps, 1, False)
+        v64 = N ** 3 * args.fp64_steps / r64["wall"] / 1e6
+        out["fp64_plain"] = {"value": round(v64, 3), "unit": "Mcells/s",
+                             "ms_per_step": round(r64["wall"] / args.fp64_steps * 1e3, 3),
+                             "fp32_over_fp64": round(value / v64, 3)}
+    ctx.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,162 @@
+/*
+ * aarambam.h -- C-ABI of the MI355X NG-2LPT initial-condition library
+ * (libaarambam_hip.so).
+ *
+ * The library is a drop-in for the hot path of the reference binary
+ * Aarambam/LPT/Basis (`2LPTBasis <ParameterFile>`): it keeps the parameter
+ * file, the alpha / X0 coefficient tables, the transfer table, the glass
+ * file, the seed-table semantics and the Gadget / potential-dump outputs,
+ * and runs displacement_fields() on one GPU.  Plain C types only.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * reference repository root):
+ *   aar_read_params      Aarambam/LPT/Basis/read_param.c:8-262 (read_parameterfile)
+ *   aar_load_alpha       Aarambam/LPT/Basis/main.c:215-247 (AlphaTable load + symmetrisation)
+ *   aar_load_x0          Aarambam/LPT/Basis/main.c:250-289 (X0Table load)
+ *   aar_load_transfer    Aarambam/LPT/Basis/power.c:66-153 (read_transfer_table)
+ *   aar_read_glass       Aarambam/LPT/Basis/read_glass.c:10-88, 226-260 (read_glass, find_files)
+ *   aar_cosmology        Aarambam/LPT/Basis/main.c:1547-1552 (set_units),
+ *                        power.c:391-448 (initialize_powerspectrum), main.c:305-313, 372
+ *   aar_seedtable        Aarambam/LPT/Basis/main.c:315-349
+ *   aar_job_prepare/run  Aarambam/LPT/Basis/main.c:149-1532 (displacement_fields)
+ *   aar_write_gadget     Aarambam/LPT/Basis/save.c:108-520 (write_particle_data, save_local_data)
+ *   aar_write_phi        Aarambam/LPT/Basis/save.c:14-105 (write_phi)
+ *   aar_write_inputspec  Aarambam/LPT/Basis/main.c:1631-1694 (print_spec)
+ *
+ * Status codes: 0 on success; values below 100000 are the reference's
+ * FatalError numbers where one applies; see aar_status_string().
+ */
+#ifndef AARAMBAM_H
+#define AARAMBAM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AAR_STR 500
+
+/* The 38 tags of the reference parameter file (read_param.c:27-184). */
+typedef struct aar_params {
+  int Nmesh, Nsample, Nmax, N_modes, Seed;
+  int SphereMode, FixedAmplitude, PhaseFlip, SavePotentialField;
+  int WhichSpectrum, WhichTransfer, GlassTileFac, NumFilesWrittenInParallel;
+  int WDM_On, WDM_Vtherm_On;
+  double Box, Omega, OmegaLambda, OmegaBaryon, OmegaDM_2ndSpecies, HubbleParam, Sigma8;
+  double PrimordialIndex, Redshift, Fnl, ShapeGamma;
+  double UnitLength_in_cm, UnitMass_in_g, UnitVelocity_in_cm_per_s;
+  double InputSpectrum_UnitLength_in_cm, WDM_PartMass_in_kev;
+  char GlassFile[AAR_STR], FileWithInputSpectrum[AAR_STR], FileWithInputTransfer[AAR_STR];
+  char FileWithAlphaCoeff[AAR_STR], FileWithX0Coeff[AAR_STR], OutputDir[AAR_STR], FileBase[AAR_STR];
+} aar_params;
+
+/* Host-side inputs of one job.  Arrays are owned by the caller. */
+typedef struct aar_tables {
+  const double *alpha;      /* N_modes^3, symmetrised, alpha[(i*Nm + j)*Nm + k] */
+  const double *x0;         /* N_modes x 2 (X0Table rows) */
+  int ntransfer;            /* WhichTransfer 2: rows of the transfer table */
+  const double *transfer_k; /* raw k column */
+  const double *transfer_t; /* raw T column */
+  const float *glass;       /* nglass x 3 glass positions, file order */
+  int nglass;
+  double glass_box;         /* BoxSize from the glass header */
+  const uint8_t *glass_header; /* optional 256-byte glass header (fill bytes are kept) */
+} aar_tables;
+
+/* Derived scalars (set_units, initialize_powerspectrum, displacement_fields). */
+typedef struct aar_cosmo {
+  double UnitTime_in_s, G, Hubble;
+  double InitTime, Norm, sigma8_in, D0, Dplus, Anorm;
+  double Beta, hubble_a, vel_prefac, vel_prefac2, fac;
+} aar_cosmo;
+
+enum { AAR_FP64 = 64, AAR_FP32 = 32 };
+/* PLAIN: pairwise (N_i, N_j) schedule of the reference, one product and one
+ * R2C per pair.  BATCHED: the sum regrouped by outer index K, all mode
+ * fields and all K products in batched FFT launches. */
+enum { AAR_SCHED_PLAIN = 0, AAR_SCHED_BATCHED = 1 };
+
+/* Fields that aar_job_fetch_field can copy back, in the reference's
+ * layouts: spectra as N x N x (N/2+1) complex (2 doubles each), real
+ * fields as N x N x N doubles (padding dropped). */
+enum {
+  AAR_FIELD_PHIG_SPEC = 0,   /* phi_G(k) after main.c:524 */
+  AAR_FIELD_PHI_SPEC = 1,    /* Phi(k) after main.c:1150 */
+  AAR_FIELD_DISP_X = 2,      /* ZA displacement fields, real (main.c:1380) */
+  AAR_FIELD_DISP_Y = 3,
+  AAR_FIELD_DISP_Z = 4,
+  AAR_FIELD_DISP2_X = 5,     /* 2LPT displacement fields, real, not divided by N^3 (main.c:1381) */
+  AAR_FIELD_DISP2_Y = 6,
+  AAR_FIELD_DISP2_Z = 7,
+  AAR_FIELD_PHIG_REAL = 8,   /* write_phi(pot, 0) field (main.c:1096) */
+  AAR_FIELD_PHING_REAL = 9,  /* write_phi(pot, 1) field (main.c:1135) */
+};
+
+typedef struct aar_ctx aar_ctx;
+typedef struct aar_job aar_job;
+
+const char *aar_status_string(int status);
+
+/* ---- host side (no GPU needed) ---- */
+int aar_read_params(const char *path, aar_params *out, char *msg, size_t msglen);
+int aar_load_alpha(const char *path, int n_modes, double *alpha_out /* Nm^3 */);
+int aar_load_x0(const char *path, int n_modes, double *x0_out /* Nm*2 */);
+/* count first (k, t NULL), then fill */
+int aar_load_transfer(const char *path, int *nrows, double *k, double *t);
+/* count first (pos NULL), then fill; header_out (256 bytes) optional */
+int aar_read_glass(const char *path, int *nglass, float *pos, double *box, uint8_t *header_out);
+int aar_cosmology(const aar_params *p, const aar_tables *t, aar_cosmo *out);
+int aar_seedtable(int nmesh, int seed, uint32_t *out /* nmesh^2 */);
+/* n-th output of gsl_rng_get for ranlxd (luxury 202 or 397), GSL rng_test semantics */
+unsigned long aar_ranlxd_nth(unsigned long seed, int luxury, int n);
+int aar_write_gadget(const aar_params *p, const aar_cosmo *c, const float *pos, const float *vel,
+                     long long npart, const uint8_t *glass_header);
+int aar_write_phi(const aar_params *p, const double *real_nnn, int non_gaussian);
+int aar_write_inputspec(const aar_params *p, const aar_tables *t, const aar_cosmo *c);
+
+/* ---- device side ---- */
+int aar_ctx_create(int device, aar_ctx **out);
+int aar_ctx_destroy(aar_ctx *ctx);
+/* wait for all work on the context's device (hipDeviceSynchronize) */
+int aar_ctx_sync(aar_ctx *ctx);
+/* HBM bytes a job of this shape allocates (admission control). */
+long long aar_job_bytes(const aar_params *p, const aar_tables *t, int precision, int schedule);
+int aar_job_prepare(aar_ctx *ctx, const aar_params *p, const aar_tables *t, int precision,
+                    int schedule, aar_job **out);
+/* enqueue the whole job on `stream` (a hipStream_t, NULL = the context stream) */
+int aar_job_run(aar_job *job, void *stream);
+int aar_job_sync(aar_job *job);
+/* device pointers of the result arrays (npart*3 floats each), valid until destroy */
+int aar_job_device_output(aar_job *job, float **pos, float **vel, long long *npart);
+int aar_job_fetch(aar_job *job, float *pos, float *vel);
+int aar_job_fetch_field(aar_job *job, int which, double *out);
+/* seconds spent in the last run (HIP events around the enqueued work) */
+double aar_job_last_seconds(aar_job *job);
+/* Per-kernel-kind timing: when on, every launch of the next runs is bracketed
+ * by HIP events on the job's stream; stats give launches, summed seconds and
+ * summed algorithmic bytes (reads + writes of the data each launch must move). */
+enum {
+  AAR_K_GAUSS = 0, AAR_K_XY = 1, AAR_K_Z = 2, AAR_K_MODES = 3, AAR_K_PRODUCTS = 4,
+  AAR_K_COMBINE = 5, AAR_K_LPT = 6, AAR_K_CIC = 7, AAR_K_ACCUM = 8, AAR_NKINDS = 9
+};
+int aar_job_set_profiling(aar_job *job, int on);
+int aar_job_kernel_stats(aar_job *job, int kind, int *launches, double *seconds, double *bytes);
+int aar_job_destroy(aar_job *job);
+
+/* ---- building blocks (tests, tools) ---- */
+/* 3-D transforms with FFTW 2 rfftwnd conventions on a host array in the
+ * reference layout: dir=+1 C2R (in: N*N*(N/2+1) complex, out: N^3 real),
+ * dir=-1 R2C (in: N^3 real, out: complex).  precision AAR_FP64/AAR_FP32. */
+int aar_fft3d(aar_ctx *ctx, int n, int precision, int dir, const double *in, double *out);
+/* Gaussian potential phi_G(k) only (reference complex layout, doubles). */
+int aar_gaussian_field(aar_ctx *ctx, const aar_params *p, const aar_tables *t, int precision,
+                       double *out);
+/* Device ranlxd draws for tests: `n` uniforms of stream `seed`. */
+int aar_ranlxd_device(aar_ctx *ctx, uint32_t seed, int n, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

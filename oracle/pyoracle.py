@@ -206,3 +206,30 @@ def run(cfg, alpha, x0, glass, glass_box, want_fields=True):
         res["phig_real"] = outs["phig_real"].reshape(N, N, 2 * H)[:, :, :N]
         res["phing_real"] = outs["phing_real"].reshape(N, N, 2 * H)[:, :, :N]
     return res
+
+
+def lpt_fields(cfg, phi_spec):
+    """ZA and 2LPT real displacement fields (main.c:1158-1381) from Phi(k).
+
+    Returns six (N,N,N) arrays: disp x,y,z and disp2 x,y,z (disp2 not divided by N^3)."""
+    N = cfg["Nmesh"]
+    p = make_params(cfg)
+    c = OrcCosmo()
+    rc = lib().orc_cosmology(C.byref(p), C.byref(c))
+    if rc:
+        raise RuntimeError(rc)
+    nd = 2 * N * N * (N // 2 + 1)
+    buf = np.ascontiguousarray(phi_spec, dtype=np.complex128).view(np.float64).reshape(-1)
+    outs = [np.empty(nd, np.float64) for _ in range(6)]
+    lib().orc_lpt_fields(C.byref(p), C.byref(c), buf, *outs)
+    return [o.reshape(N, N, 2 * (N // 2 + 1))[:, :, :N].copy() for o in outs]
+
+
+def glass_tile(cfg, glass, glass_box):
+    p = make_params(cfg)
+    glass = np.ascontiguousarray(glass, dtype=np.float32).reshape(-1)
+    ng = glass.size // 3
+    npart = cfg["GlassTileFac"] ** 3 * ng
+    pos = np.empty(3 * npart, np.float32)
+    lib().orc_glass_tile(C.byref(p), glass, ng, float(glass_box), pos, None)
+    return pos.reshape(-1, 3)

@@ -74,3 +74,108 @@ def lattice_glass(n=64, box=100.0):
     g = np.arange(n, dtype=np.float64) * (box / n)
     x, y, z = np.meshgrid(g, g, g, indexing="ij")
     return np.stack([x, y, z], -1).reshape(-1, 3).astype(np.float32)
+
+
+def job_inputs(cfg, rows=None, x0=None, glass_n=None):
+    """(alpha table, x0 table, glass positions, glass box) for a config."""
+    from oracle import pyoracle as orc
+    Nm = cfg["N_modes"]
+    rows = local_rows() if rows is None else rows
+    alpha = orc.alpha_table(Nm, rows)
+    if x0 is None:
+        x0 = x0_closed_form(Nm, cfg["PrimordialIndex"], cfg["Box"], cfg["Nmax"])
+    n = glass_n or cfg["Nmesh"] // cfg["GlassTileFac"]
+    return alpha, x0, lattice_glass(n, 100.0), 100.0
+
+
+def shipped_table_config(N=32, **kw):
+    """The reference's shipped 15-mode ScalarI tables (ns=0.9660499, Nmax=256)."""
+    cfg = base_config(Nmesh=N, Nsample=N, Nmax=256, N_modes=15, PrimordialIndex=0.9660499, **kw)
+    return cfg, alpha_rows_file(), x0_file(15)
+
+
+PARAM_TEMPLATE = """
+Nmesh                               %(Nmesh)d
+Nsample                             %(Nsample)d
+Nmax                                %(Nmax)d
+Box                                 %(Box)f
+FileBase                            %(FileBase)s
+OutputDir                           %(OutputDir)s
+GlassFile                           %(GlassFile)s
+GlassTileFac                        %(GlassTileFac)d
+
+Seed                                %(Seed)d
+
+%% a comment line
+Omega                               %(Omega)f
+OmegaLambda                         %(OmegaLambda)f
+OmegaBaryon                         %(OmegaBaryon)f #trailing comment
+OmegaDM_2ndSpecies                  0.00
+HubbleParam                         %(HubbleParam)f
+Sigma8                              %(Sigma8)f
+PrimordialIndex                     %(PrimordialIndex)f
+Redshift                            %(Redshift)f
+Fnl                                 %(Fnl)f
+N_modes                             %(N_modes)d
+
+FixedAmplitude                      %(FixedAmplitude)d
+PhaseFlip                           %(PhaseFlip)d
+SphereMode                          %(SphereMode)d
+SavePotentialField                  %(SavePotentialField)d
+
+WhichSpectrum                       %(WhichSpectrum)d
+WhichTransfer                       %(WhichTransfer)d
+
+FileWithInputSpectrum               ThisIsAFakeStringThatIsNeverUsed
+FileWithInputTransfer               %(TransferPath)s
+FileWithAlphaCoeff                  %(AlphaPath)s
+FileWithX0Coeff                     %(AiPath)s
+
+NumFilesWrittenInParallel           %(NumFilesWrittenInParallel)d
+
+InputSpectrum_UnitLength_in_cm      3.085678e24
+UnitLength_in_cm                    3.085678e24
+UnitMass_in_g                       1.989e43
+UnitVelocity_in_cm_per_s            1e5
+
+ShapeGamma                          0.0
+WDM_On                              0
+WDM_Vtherm_On                       0
+WDM_PartMass_in_kev                 10.0
+"""
+
+
+def write_param_file(path, cfg, **files):
+    d = dict(cfg)
+    d.setdefault("FileBase", "ics")
+    d.setdefault("OutputDir", os.path.dirname(path))
+    d.setdefault("SavePotentialField", 0)
+    d.setdefault("NumFilesWrittenInParallel", 1)
+    d.update(files)
+    with open(path, "w") as f:
+        f.write(PARAM_TEMPLATE % d)
+
+
+def write_lattice_glass(path, n=64, box=100.0):
+    """Gadget-1 glass file in the layout of Aarambam/defaults/dummy_glass_dmonly_64.dat:
+    256-byte header (type-2 particles, BoxSize, num_files=1), then the position block."""
+    import struct
+    pos = lattice_glass(n, box)
+    hdr = bytearray(256)
+    npart = [0, 0, n ** 3, 0, 0, 0]
+    hdr[0:24] = struct.pack("6I", *npart)
+    hdr[96:120] = struct.pack("6I", *npart)
+    hdr[124:128] = struct.pack("i", 1)
+    hdr[128:136] = struct.pack("d", box)
+    with open(path, "wb") as f:
+        f.write(struct.pack("i", 256))
+        f.write(bytes(hdr))
+        f.write(struct.pack("i", 256))
+        f.write(struct.pack("i", 12 * n ** 3))
+        f.write(pos.tobytes())
+        f.write(struct.pack("i", 12 * n ** 3))
+    return pos
+
+
+# sha256 of Aarambam/defaults/dummy_glass_dmonly_64.dat
+REFERENCE_GLASS_SHA256 = "684dc7e1ce49983c3948cca40abc320726737090469df96c55574bdc7c6f4579"

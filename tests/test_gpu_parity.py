@@ -1,0 +1,144 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle.
+
+Tolerances: FP64 path 1e-10 relative (L2) on spectra and displacement fields;
+particle positions and velocities are float32 and agree to 1 ulp (positions
+that wrap at the box edge excepted); FP32 path <= 1e-5 relative L2.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as orc
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+aar = pytest.importorskip("aarambam_amd")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = aar.Context(0)
+    yield c
+    c.close()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def tables_for(cfg, alpha, x0, glass, gbox):
+    tk = tt = None
+    if cfg.get("WhichTransfer") == 2:
+        tk, tt = cfg["transfer_k"], cfg["transfer_t"]
+    return aar.tables(alpha, x0, glass, gbox, tk, tt)
+
+
+def test_device_ranlxd_matches_oracle_bitwise(ctx):
+    for seed in [1, 1234, 2 ** 31 - 1]:
+        dev = ctx.ranlxd(seed, 1000)
+        ref = orc.ranlxd_draws(seed, 1000)
+        np.testing.assert_array_equal(dev, ref)
+
+
+@pytest.mark.parametrize("prec,tol", [(aar.AAR_FP64, 1e-12), (aar.AAR_FP32, 2e-6)])
+@pytest.mark.parametrize("N", [16, 32, 64, 128])
+def test_fft_matches_numpy(ctx, N, prec, tol):
+    rng = np.random.default_rng(N)
+    H = N // 2 + 1
+    spec = rng.standard_normal((N, N, H)) + 1j * rng.standard_normal((N, N, H))
+    ref = np.fft.irfftn(spec, s=(N, N, N), axes=(0, 1, 2)) * N ** 3
+    out = ctx.fft3d(spec, +1, prec)
+    assert rel_l2(out, ref) < tol
+    real = rng.standard_normal((N, N, N))
+    ref2 = np.fft.rfftn(real)
+    out2 = ctx.fft3d(real, -1, prec)
+    assert rel_l2(out2, ref2) < tol
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(SphereMode=1, Nsample=24), dict(PhaseFlip=1),
+                                dict(FixedAmplitude=1, Nsample=20)])
+def test_gaussian_field_vs_oracle(ctx, kw):
+    cfg = cases.base_config(**{**dict(Nmesh=32, Nmax=32, Nsample=32), **kw})
+    alpha, x0, glass, gbox = cases.job_inputs(cfg)
+    t = tables_for(cfg, alpha, x0, glass, gbox)
+    dev = ctx.gaussian_field(aar.params_from_dict(cfg), t, aar.AAR_FP64)
+    ref = orc.gaussian(cfg)
+    np.testing.assert_array_equal(dev == 0, ref == 0)
+    assert np.max(np.abs(dev - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def _compare_job(ctx, cfg, rows, x0, prec, sched, tol_fields):
+    alpha, x0v, glass, gbox = cases.job_inputs(cfg, rows, x0)
+    ref = orc.run(cfg, alpha, x0v, glass, gbox)
+    t = tables_for(cfg, alpha, x0v, glass, gbox)
+    job = aar.Job(ctx, aar.params_from_dict(cfg), t, prec, sched)
+    job.run()
+    job.sync()
+    pos, vel = job.fetch()
+    res = dict(phig=rel_l2(job.field("phig_spec"), ref["phig_spec"]),
+               phi=rel_l2(job.field("phi_spec"), ref["phi_spec"]),
+               phing_real=rel_l2(job.field("phing_real"), ref["phing_real"]))
+    N = cfg["Nmesh"]
+    # displacement fields: reconstruct the oracle's from its particle-independent pipeline
+    for name in ["phig", "phi", "phing_real"]:
+        assert res[name] < tol_fields, (name, res)
+    box = cfg["Box"]
+    names = ["disp_x", "disp_y", "disp_z", "disp2_x", "disp2_y", "disp2_z"]
+    ref_disp = orc.lpt_fields(cfg, ref["phi_spec"])
+    for name, rd in zip(names, ref_disp):
+        res[name] = rel_l2(job.field(name), rd)
+    res["vel"] = rel_l2(vel, ref["vel"])
+    q = orc.glass_tile(cfg, glass, gbox).astype(np.float64)
+    d_dev = (pos - q + box / 2) % box - box / 2
+    d_ref = (ref["pos"] - q + box / 2) % box - box / 2
+    res["disp_particles"] = rel_l2(d_dev, d_ref)
+    for name in names:
+        assert res[name] < tol_fields, (name, res)
+    if prec == aar.AAR_FP64:
+        dpos = np.abs(pos.astype(np.float64) - ref["pos"])
+        dpos = np.minimum(dpos, box - dpos)
+        ulp = np.spacing(np.abs(ref["pos"]).astype(np.float32)).astype(np.float64)
+        assert np.all(dpos <= ulp), float((dpos / ulp).max())
+        vulp = np.spacing(np.abs(ref["vel"]).astype(np.float32)).astype(np.float64)
+        assert np.all(np.abs(vel - ref["vel"]) <= vulp), float((np.abs(vel - ref["vel"]) / vulp).max())
+    else:
+        assert res["vel"] < tol_fields, res
+    return res
+
+
+@pytest.mark.parametrize("sched", [aar.AAR_SCHED_PLAIN, aar.AAR_SCHED_BATCHED])
+@pytest.mark.parametrize("table", ["local", "equilateral"])
+def test_job_fp64_vs_oracle(ctx, sched, table):
+    cfg = cases.base_config(Nmesh=32, Nsample=32, Nmax=32, N_modes=4)
+    rows = cases.local_rows() if table == "local" else cases.equilateral_rows()
+    _compare_job(ctx, cfg, rows, None, aar.AAR_FP64, sched, 1e-10)
+
+
+@pytest.mark.parametrize("sched", [aar.AAR_SCHED_PLAIN, aar.AAR_SCHED_BATCHED])
+def test_job_fp64_shipped_table_tabulated_transfer(ctx, sched):
+    tk, tt = cases.load_transfer()
+    cfg, rows, x0 = cases.shipped_table_config(N=32, WhichTransfer=2, transfer_k=tk, transfer_t=tt,
+                                               Box=512.0)
+    _compare_job(ctx, cfg, rows, x0, aar.AAR_FP64, sched, 1e-10)
+
+
+def test_job_fp64_sphere_mode_cut(ctx):
+    cfg = cases.base_config(Nmesh=32, Nsample=20, Nmax=32, N_modes=4, SphereMode=1, Seed=77)
+    _compare_job(ctx, cfg, cases.equilateral_rows(), None, aar.AAR_FP64, aar.AAR_SCHED_BATCHED, 1e-10)
+
+
+def test_job_c1_local_64(ctx):
+    """Plain FP64 CPU-runnable case: 64^3, L=1000, EH, local fNL=100, 8 modes, seed 1234."""
+    cfg = cases.base_config()
+    _compare_job(ctx, cfg, cases.local_rows(), None, aar.AAR_FP64, aar.AAR_SCHED_PLAIN, 1e-10)
+
+
+@pytest.mark.parametrize("table", ["equilateral", "shipped"])
+def test_job_fp32_error_vs_oracle(ctx, table):
+    if table == "shipped":
+        cfg, rows, x0 = cases.shipped_table_config(N=64)
+    else:
+        cfg = cases.base_config(N_modes=4)
+        rows, x0 = cases.equilateral_rows(), None
+    res = _compare_job(ctx, cfg, rows, x0, aar.AAR_FP32, aar.AAR_SCHED_BATCHED, 1e-5)
+    print("fp32 rel L2", table, res)
