@@ -48,7 +48,7 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_load_alpha", "aar_load_x
            "aar_write_phi", "aar_write_inputspec", "aar_ctx_create", "aar_ctx_destroy", "aar_ctx_sync", "aar_job_bytes",
            "aar_job_prepare", "aar_job_run", "aar_job_sync", "aar_job_device_output", "aar_job_fetch",
            "aar_job_fetch_field", "aar_job_last_seconds", "aar_job_destroy", "aar_fft3d",
-           "aar_gaussian_field", "aar_ranlxd_device", "aar_job_set_profiling", "aar_job_kernel_stats"]
+           "aar_gaussian_field", "aar_ranlxd_device", "aar_bench_pass", "aar_job_set_profiling", "aar_job_kernel_stats"]
 KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
 
 
@@ -98,6 +98,7 @@ def lib():
             "aar_fft3d": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp]),
             "aar_gaussian_field": (C.c_int, [vp, PP, TP, C.c_int, dp]),
             "aar_ranlxd_device": (C.c_int, [vp, C.c_uint32, C.c_int, dp]),
+            "aar_bench_pass": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp]),
             "aar_job_set_profiling": (C.c_int, [vp, C.c_int]),
             "aar_job_kernel_stats": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), dp, dp]),
         }
@@ -225,6 +226,11 @@ class Context:
         _check(lib().aar_gaussian_field(self.h, C.byref(params), C.byref(tabs.c), precision, _dptr(out)),
                "aar_gaussian_field")
         return out.view(np.complex128).reshape(N, N, N // 2 + 1)
+
+    def bench_pass(self, n, kind, nf=4, iters=5, precision=AAR_FP32):
+        s = C.c_double(0)
+        _check(lib().aar_bench_pass(self.h, n, precision, kind, nf, iters, C.byref(s)), "aar_bench_pass")
+        return s.value
 
     def ranlxd(self, seed, n):
         out = np.empty(n, np.float64)

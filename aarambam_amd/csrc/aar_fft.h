@@ -126,12 +126,29 @@ template <int R, int DIR, typename C> __device__ __forceinline__ void dftR(C* v)
 }
 
 // ---------------------------------------------------------------- tile FFT
+// Two tile layouts:
+//  column-major (RM = false): element (pos, col) at pos*CP + col, thread t
+//    serves column t % C, so a wavefront touches 16 adjacent columns at a few
+//    positions -- coalesced when columns are adjacent in memory (x, y passes);
+//  row-major (RM = true): element at col*LP + pos + pos/16, thread t serves
+//    butterfly t % (L/R) of column t / (L/R), so a wavefront runs along
+//    positions -- coalesced when each column is a contiguous row (z passes).
+template <typename T, int L, int C, bool RM> struct TileIdx {
+  static constexpr int CP = tile_cp<T>(L, C);
+  static constexpr int LP = L + L / 16;
+  static constexpr int SIZE = RM ? C * LP : L * CP;
+  __device__ static __forceinline__ int at(int pos, int col) {
+    if constexpr (RM) return col * LP + pos + (pos >> 4);
+    else return pos * CP + col;
+  }
+};
+
 // tw: LDS table with tw[m*TWS] = exp(-2 pi i m / L) for m in [0, L).
-template <typename T, int L, int C, int NT, int DIR, int TWS, int S, bool SRC_LDS, class LoadF,
+template <typename T, int L, int C, int NT, int DIR, int TWS, int S, bool SRC_LDS, bool RM, class LoadF,
           class StoreF>
 __device__ __forceinline__ void fft_stage(cx_t<T>* tile, const cx_t<T>* tw, LoadF& load, StoreF& store) {
   using C2 = cx_t<T>;
-  constexpr int CP = tile_cp<T>(L, C);
+  using TI = TileIdx<T, L, C, RM>;
   constexpr int E = C * L / NT;
   constexpr int RMAX = E < 16 ? E : 16;
   constexpr int NS = plan_nstages(L, RMAX);
@@ -140,24 +157,35 @@ __device__ __forceinline__ void fft_stage(cx_t<T>* tile, const cx_t<T>* tw, Load
   constexpr int NB = E / R;
   constexpr int STR = L / R;
   constexpr int ROWS = NT / C;
+  static_assert(RM || NT % C == 0, "column-major tiles need NT to be a multiple of C");
   static_assert(E % R == 0, "per-thread elements must be a multiple of the radix");
-  const int col = threadIdx.x % C;
-  const int i0 = threadIdx.x / C;
+  auto bfly = [](int q, int& i, int& col) {
+    if constexpr (RM) {
+      const int b = threadIdx.x + q * NT;
+      i = b % STR;
+      col = b / STR;
+    } else {
+      i = threadIdx.x / C + q * ROWS;
+      col = threadIdx.x % C;
+    }
+  };
   C2 v[E];
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
-    const int i = i0 + q * ROWS;
+    int i, col;
+    bfly(q, i, col);
 #pragma unroll
     for (int t = 0; t < R; ++t) {
       const int pos = i + t * STR;
       if constexpr (S == 0) v[q * R + t] = load(pos, col);
-      else v[q * R + t] = tile[pos * CP + col];
+      else v[q * R + t] = tile[TI::at(pos, col)];
     }
   }
   if constexpr (S > 0 || SRC_LDS) __syncthreads();
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
-    const int i = i0 + q * ROWS;
+    int i, col;
+    bfly(q, i, col);
     const int k = i % P;
     if constexpr (S > 0) {
 #pragma unroll
@@ -172,19 +200,20 @@ __device__ __forceinline__ void fft_stage(cx_t<T>* tile, const cx_t<T>* tw, Load
     for (int u = 0; u < R; ++u) {
       const int pos = (i - k) * R + k + u * P;
       if constexpr (S == NS - 1) store(pos, col, v[q * R + u]);
-      else tile[pos * CP + col] = v[q * R + u];
+      else tile[TI::at(pos, col)] = v[q * R + u];
     }
   }
   if constexpr (S < NS - 1) {
     __syncthreads();
-    fft_stage<T, L, C, NT, DIR, TWS, S + 1, SRC_LDS, LoadF, StoreF>(tile, tw, load, store);
+    fft_stage<T, L, C, NT, DIR, TWS, S + 1, SRC_LDS, RM, LoadF, StoreF>(tile, tw, load, store);
   }
 }
 
-template <typename T, int L, int C, int NT, int DIR, int TWS, bool SRC_LDS, class LoadF, class StoreF>
+template <typename T, int L, int C, int NT, int DIR, int TWS, bool SRC_LDS, bool RM = false, class LoadF,
+          class StoreF>
 __device__ __forceinline__ void fft_tile(cx_t<T>* tile, const cx_t<T>* tw, LoadF load, StoreF store) {
   __syncthreads();  // the tile may still be read by the previous transform
-  fft_stage<T, L, C, NT, DIR, TWS, 0, SRC_LDS, LoadF, StoreF>(tile, tw, load, store);
+  fft_stage<T, L, C, NT, DIR, TWS, 0, SRC_LDS, RM, LoadF, StoreF>(tile, tw, load, store);
 }
 
 // Copy a global twiddle table (exp(-2 pi i m / n), m < n) into LDS.

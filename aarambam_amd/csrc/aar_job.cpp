@@ -701,6 +701,58 @@ extern "C" int aar_gaussian_field(aar_ctx* ctx, const aar_params* p, const aar_t
   return rc;
 }
 
+// Microbenchmark of one building block on nf fields of an N^3 grid:
+// kind 0 x-pass, 1 y-pass, 2 z c2r, 3 z r2c, 4 device copy of nf spectra.
+// Returns the mean seconds per launch over `iters` launches.
+template <typename T> static int bench_pass_t(aar_ctx* ctx, int N, int kind, int nf, int iters, double* sec) {
+  using C2 = cx_t<T>;
+  const Geom g = make_geom(N);
+  C2 *d = nullptr, *d2 = nullptr;
+  if (device_alloc((void**)&d, (size_t)nf * g.fs * sizeof(C2))) return AAR_ERR_ALLOC;
+  if (kind == 4 && device_alloc((void**)&d2, (size_t)nf * g.fs * sizeof(C2))) {
+    hipFree(d);
+    return AAR_ERR_ALLOC;
+  }
+  AAR_HIP(hipMemsetAsync(d, 0, (size_t)nf * g.fs * sizeof(C2), ctx->stream));
+  const C2* tw = sizeof(T) == 4 ? (const C2*)ctx->tw32 : (const C2*)ctx->tw64;
+  auto launch = [&]() -> int {
+    switch (kind) {
+      case 0: return Pipeline<T>::xy_pass(d, nf, g, tw, 0, +1, ctx->stream);
+      case 1: return Pipeline<T>::xy_pass(d, nf, g, tw, 1, +1, ctx->stream);
+      case 2: return Pipeline<T>::z_pass(d, nf, g, tw, +1, ctx->stream);
+      case 3: return Pipeline<T>::z_pass(d, nf, g, tw, -1, ctx->stream);
+      default:
+        return hipMemcpyAsync(d2, d, (size_t)nf * g.fs * sizeof(C2), hipMemcpyDeviceToDevice, ctx->stream) ==
+                       hipSuccess
+                   ? AAR_OK
+                   : AAR_ERR_HIP;
+    }
+  };
+  int rc = launch();
+  hipEvent_t a, b;
+  hipEventCreate(&a);
+  hipEventCreate(&b);
+  hipEventRecord(a, ctx->stream);
+  for (int q = 0; q < iters && !rc; ++q) rc = launch();
+  hipEventRecord(b, ctx->stream);
+  hipEventSynchronize(b);
+  float ms = 0;
+  hipEventElapsedTime(&ms, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  *sec = ms * 1e-3 / iters;
+  hipFree(d);
+  if (d2) hipFree(d2);
+  return rc;
+}
+
+extern "C" int aar_bench_pass(aar_ctx* ctx, int n, int precision, int kind, int nf, int iters, double* sec) {
+  if (!ctx || !sec || !pow2(n) || nf < 1 || iters < 1 || kind < 0 || kind > 4) return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(ctx->device));
+  return precision == AAR_FP32 ? bench_pass_t<float>(ctx, n, kind, nf, iters, sec)
+                               : bench_pass_t<double>(ctx, n, kind, nf, iters, sec);
+}
+
 extern "C" int aar_ranlxd_device(aar_ctx* ctx, uint32_t seed, int n, double* out) {
   if (!ctx || !out || n <= 0) return AAR_ERR_BAD_ARG;
   AAR_HIP(hipSetDevice(ctx->device));

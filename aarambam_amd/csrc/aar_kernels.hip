@@ -110,8 +110,7 @@ __global__ void __launch_bounds__((XYCfg<T, L>::NT))
     k_xy_pass(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg, int nmain) {
   using C2 = cx_t<T>;
   constexpr int C = XYCfg<T, L>::C, NT = XYCfg<T, L>::NT;
-  constexpr int CP = tile_cp<T>(L, C);
-  __shared__ C2 tile[L * CP];
+  __shared__ C2 tile[TileIdx<T, L, C, false>::SIZE];
   __shared__ C2 tw[L];
   load_twiddles(tw, twg + L, L, NT);
   C2* f = base + (long long)blockIdx.y * g.fs;
@@ -169,48 +168,34 @@ __global__ void __launch_bounds__((ZCfg<T, H>::NT))
   using C2 = cx_t<T>;
   constexpr int C = ZCfg<T, H>::C, NT = ZCfg<T, H>::NT;
   constexpr int N = 2 * H;
-  constexpr int CP = tile_cp<T>(H, C);
-  __shared__ C2 tile[H * CP];
+  using TI = TileIdx<T, H, C, true>;
+  __shared__ C2 tile[TI::SIZE];
   __shared__ C2 tw[N];
   __shared__ C2 xh[C];
   load_twiddles(tw, twg + N, N, NT);
   C2* f = base + (long long)blockIdx.y * g.fs;
   const long long r0 = (long long)blockIdx.x * C;
   C2* rows = f + r0 * H;
-  for (int e = threadIdx.x; e < C * H; e += NT) {
-    const int c = e / H, m = e % H;
-    tile[m * CP + c] = rows[(long long)c * H + m];
-  }
   if (threadIdx.x < C) xh[threadIdx.x] = f[g.nq + r0 + threadIdx.x];
   __syncthreads();
-  // pre-processing, pairs (k, H-k)
-  for (int e = threadIdx.x; e < C * (H / 2 + 1); e += NT) {
-    const int c = e % C, k = e / C;
+  // The first FFT stage forms Y[k] from X[k] and X[H-k], read straight from
+  // HBM along each row (the mirrored read hits the same cache lines).
+  auto load = [&](int k, int c) {
+    const C2* row = rows + (long long)c * H;
     if (k == 0) {
-      const T a = tile[c].x, b = xh[c].x;
-      tile[c] = cmake<C2>(a + b, a - b);
-    } else {
-      const int k2 = H - k;
-      const C2 A = tile[k * CP + c], B = tile[k2 * CP + c];
-      // Y[k]
-      C2 w = tw[k];
-      w.y = -w.y;  // e^{+2 pi i k/N}
-      C2 s = cadd(A, cconj(B)), d = csub(A, cconj(B));
-      C2 dw = cmul(d, w);
-      C2 Yk = cmake<C2>(s.x - dw.y, s.y + dw.x);
-      if (k2 != k) {
-        C2 w2 = tw[k2];
-        w2.y = -w2.y;
-        C2 s2 = cadd(B, cconj(A)), d2 = csub(B, cconj(A));
-        C2 dw2 = cmul(d2, w2);
-        tile[k2 * CP + c] = cmake<C2>(s2.x - dw2.y, s2.y + dw2.x);
-      }
-      tile[k * CP + c] = Yk;
+      const T a = row[0].x, b = xh[c].x;
+      return cmake<C2>(a + b, a - b);
     }
-  }
-  auto load = [&](int pos, int col) { return tile[pos * CP + col]; };
-  auto store = [&](int pos, int col, C2 v) { rows[(long long)col * H + pos] = v; };
-  fft_tile<T, H, C, NT, +1, 2, true>(tile, tw, load, store);
+    const C2 A = row[k], B = row[H - k];
+    C2 w = tw[k];
+    w.y = -w.y;  // e^{+2 pi i k/N}
+    const C2 s = cadd(A, cconj(B)), d = csub(A, cconj(B));
+    const C2 dw = cmul(d, w);
+    return cmake<C2>(s.x - dw.y, s.y + dw.x);
+  };
+  auto store = [&](int pos, int c, C2 v) { rows[(long long)c * H + pos] = v; };
+  // true: barrier after the first gather, since the transform is in place in HBM
+  fft_tile<T, H, C, NT, +1, 2, true, true>(tile, tw, load, store);
 }
 
 template <typename T, int H>
@@ -219,37 +204,37 @@ __global__ void __launch_bounds__((ZCfg<T, H>::NT))
   using C2 = cx_t<T>;
   constexpr int C = ZCfg<T, H>::C, NT = ZCfg<T, H>::NT;
   constexpr int N = 2 * H;
-  constexpr int CP = tile_cp<T>(H, C);
-  __shared__ C2 tile[H * CP];
+  using TI = TileIdx<T, H, C, true>;
+  __shared__ C2 tile[TI::SIZE];
   __shared__ C2 tw[N];
   load_twiddles(tw, twg + N, N, NT);
   C2* f = base + (long long)blockIdx.y * g.fs;
   const long long r0 = (long long)blockIdx.x * C;
   C2* rows = f + r0 * H;
-  auto load = [&](int pos, int col) { return rows[(long long)col * H + pos]; };
-  auto store = [&](int pos, int col, C2 v) { tile[pos * CP + col] = v; };
+  auto load = [&](int pos, int c) { return rows[(long long)c * H + pos]; };
+  auto store = [&](int pos, int c, C2 v) { tile[TI::at(pos, c)] = v; };
+  fft_tile<T, H, C, NT, -1, 2, false, true>(tile, tw, load, store);
   __syncthreads();
-  fft_tile<T, H, C, NT, -1, 2, false>(tile, tw, load, store);
-  __syncthreads();
-  for (int e = threadIdx.x; e < C * (H / 2 + 1); e += NT) {
-    const int c = e % C, k = e / C;
+  constexpr int HP = H / 2 + 1;
+  for (int e = threadIdx.x; e < C * HP; e += NT) {  // threads along k: coalesced stores
+    const int c = e / HP, k = e % HP;
     C2* out = rows + (long long)c * H;
     if (k == 0) {
-      const C2 z = tile[c];
+      const C2 z = tile[TI::at(0, c)];
       out[0] = cmake<C2>(z.x + z.y, (T)0);
       f[g.nq + r0 + c] = cmake<C2>(z.x - z.y, (T)0);
     } else {
       const int k2 = H - k;
-      const C2 A = tile[k * CP + c], B = tile[k2 * CP + c];
+      const C2 A = tile[TI::at(k, c)], B = tile[TI::at(k2, c)];
       // E = (A + B*)/2, O = (A - B*)/(2i), X_k = E + W^k O, W = e^{-2 pi i/N}
-      C2 E = cscale(cadd(A, cconj(B)), (T)0.5);
-      C2 dd = csub(A, cconj(B));
-      C2 O = cmake<C2>(dd.y * (T)0.5, -dd.x * (T)0.5);
+      const C2 E = cscale(cadd(A, cconj(B)), (T)0.5);
+      const C2 dd = csub(A, cconj(B));
+      const C2 O = cmake<C2>(dd.y * (T)0.5, -dd.x * (T)0.5);
       out[k] = cadd(E, cmul(O, tw[k]));
       if (k2 != k) {
-        C2 E2 = cscale(cadd(B, cconj(A)), (T)0.5);
-        C2 dd2 = csub(B, cconj(A));
-        C2 O2 = cmake<C2>(dd2.y * (T)0.5, -dd2.x * (T)0.5);
+        const C2 E2 = cscale(cadd(B, cconj(A)), (T)0.5);
+        const C2 dd2 = csub(B, cconj(A));
+        const C2 O2 = cmake<C2>(dd2.y * (T)0.5, -dd2.x * (T)0.5);
         out[k2] = cadd(E2, cmul(O2, tw[k2]));
       }
     }

@@ -153,6 +153,9 @@ int aar_fft3d(aar_ctx *ctx, int n, int precision, int dir, const double *in, dou
 /* Gaussian potential phi_G(k) only (reference complex layout, doubles). */
 int aar_gaussian_field(aar_ctx *ctx, const aar_params *p, const aar_tables *t, int precision,
                        double *out);
+/* Mean seconds per launch of one FFT pass (kind 0 x, 1 y, 2 z-C2R, 3 z-R2C)
+ * or of a device-to-device copy (kind 4) over nf fields of an N^3 grid. */
+int aar_bench_pass(aar_ctx *ctx, int n, int precision, int kind, int nf, int iters, double *sec);
 /* Device ranlxd draws for tests: `n` uniforms of stream `seed`. */
 int aar_ranlxd_device(aar_ctx *ctx, uint32_t seed, int n, double *out);
 
