@@ -48,7 +48,7 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_load_alpha", "aar_load_x
            "aar_write_phi", "aar_write_inputspec", "aar_ctx_create", "aar_ctx_destroy", "aar_ctx_sync", "aar_job_bytes",
            "aar_job_prepare", "aar_job_run", "aar_job_sync", "aar_job_device_output", "aar_job_fetch",
            "aar_job_fetch_field", "aar_job_last_seconds", "aar_job_destroy", "aar_fft3d",
-           "aar_gaussian_field", "aar_ranlxd_device", "aar_bench_pass", "aar_job_set_profiling", "aar_job_kernel_stats"]
+           "aar_gaussian_field", "aar_ranlxd_device", "aar_bench_pass", "aar_job_set_profiling", "aar_job_kernel_stats", "aar_job_set_graph"]
 KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
 
 
@@ -100,6 +100,7 @@ def lib():
             "aar_ranlxd_device": (C.c_int, [vp, C.c_uint32, C.c_int, dp]),
             "aar_bench_pass": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp]),
             "aar_job_set_profiling": (C.c_int, [vp, C.c_int]),
+            "aar_job_set_graph": (C.c_int, [vp, C.c_int]),
             "aar_job_kernel_stats": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), dp, dp]),
         }
         for name, (res, args) in sig.items():
@@ -274,6 +275,9 @@ class Job:
         if which <= 1:
             return out.view(np.complex128).reshape(N, N, N // 2 + 1)
         return out.reshape(N, N, N)
+
+    def set_graph(self, on=True):
+        _check(lib().aar_job_set_graph(self.h, int(on)), "aar_job_set_graph")
 
     def set_profiling(self, on=True):
         _check(lib().aar_job_set_profiling(self.h, int(on)), "aar_job_set_profiling")

@@ -55,15 +55,27 @@ template <int DIR, typename C> __device__ __forceinline__ C cmul_i(C a) {
   return DIR > 0 ? cmake<C>(-a.y, a.x) : cmake<C>(a.y, -a.x);
 }
 
-// Grid geometry of one field in HBM.  A spectrum holds the kz < N/2 bins as
-// main[x][y][kz] (N*N*H complex, H = N/2) followed by the kz = N/2 bins as
-// nyq[x][y] (N*N complex).  The same buffer holds a real field as
-// real[x][y][z] (N^3 reals) in the bytes of the main block.
+// Grid geometry of one field in HBM.  Grid rows (x, y) are stored x-blocked:
+// row r = ((x / 8) * N + y) * 8 + x % 8, so eight consecutive x share each
+// stretch of memory and a column along x crosses N/8 memory regions instead
+// of N (fewer pages and DRAM rows per x-pass tile).  A spectrum holds the
+// kz < N/2 bins as main[r][kz] (N*N*H complex, H = N/2) followed by the
+// kz = N/2 bins as nyq[r] (N*N complex).  The same buffer holds a real field
+// as real[r][z] (N^3 reals) in the bytes of the main block.
 struct Geom {
   int N, H;
   long long nq;  // offset (complex elements) of the Nyquist plane = N*N*H
   long long fs;  // field stride (complex elements) = N*N*(H+1)
 };
+
+__host__ __device__ __forceinline__ long long row_of(int N, int x, int y) {
+  return ((long long)(x >> 3) * N + y) * 8 + (x & 7);
+}
+__host__ __device__ __forceinline__ void row_xy(int N, long long r, int& x, int& y) {
+  const long long t = r >> 3;
+  y = (int)(t % N);
+  x = (int)(t / N) * 8 + (int)(r & 7);
+}
 
 static inline Geom make_geom(int N) {
   Geom g;

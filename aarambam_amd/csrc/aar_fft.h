@@ -216,6 +216,99 @@ __device__ __forceinline__ void fft_tile(cx_t<T>* tile, const cx_t<T>* tw, LoadF
   fft_stage<T, L, C, NT, DIR, TWS, 0, SRC_LDS, RM, LoadF, StoreF>(tile, tw, load, store);
 }
 
+// ---------------------------------------------------------------- split-exchange variant
+// Same transform with the LDS exchange between stages done one component
+// at a time (real parts, then imaginary parts) through a real-valued tile:
+// half the LDS of the complex tile, so twice as many columns fit per CU.
+// Column-major layout only; the first stage loads and the last stage stores
+// through the caller's functors.
+template <typename T, int L, int C, int NT, int DIR, int S, class StoreF>
+__device__ __forceinline__ void fft_stage_split(T* tile, const cx_t<T>* tw, cx_t<T>* v, StoreF& store) {
+  using C2 = cx_t<T>;
+  constexpr int CP = C + 1;
+  constexpr int E = C * L / NT;
+  constexpr int RMAX = E < 16 ? E : 16;
+  constexpr int NS = plan_nstages(L, RMAX);
+  constexpr int R = plan_radix(L, RMAX, S);
+  constexpr int P = plan_prefix(L, RMAX, S);
+  constexpr int NB = E / R;
+  constexpr int ROWS = NT / C;
+  const int col = threadIdx.x % C;
+  const int i0 = threadIdx.x / C;
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int i = i0 + q * ROWS;
+    const int k = i % P;
+    if constexpr (S > 0) {
+#pragma unroll
+      for (int t = 1; t < R; ++t) {
+        C2 w = tw[t * k * (L / (P * R))];
+        if constexpr (DIR > 0) w.y = -w.y;
+        v[q * R + t] = cmul(v[q * R + t], w);
+      }
+    }
+    dftR<R, DIR>(&v[q * R]);
+  }
+  if constexpr (S == NS - 1) {
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int i = i0 + q * ROWS;
+      const int k = i % P;
+#pragma unroll
+      for (int u = 0; u < R; ++u) store((i - k) * R + k + u * P, col, v[q * R + u]);
+    }
+  } else {
+    constexpr int R2 = plan_radix(L, RMAX, S + 1);
+    constexpr int NB2 = E / R2;
+    constexpr int STR2 = L / R2;
+    C2 w[E];
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+#pragma unroll
+      for (int q = 0; q < NB; ++q) {
+        const int i = i0 + q * ROWS;
+        const int k = i % P;
+#pragma unroll
+        for (int u = 0; u < R; ++u)
+          tile[((i - k) * R + k + u * P) * CP + col] = part ? v[q * R + u].y : v[q * R + u].x;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < NB2; ++q) {
+        const int i = i0 + q * ROWS;
+#pragma unroll
+        for (int t = 0; t < R2; ++t) {
+          const T x = tile[(i + t * STR2) * CP + col];
+          if (part) w[q * R2 + t].y = x;
+          else w[q * R2 + t].x = x;
+        }
+      }
+      __syncthreads();
+    }
+    fft_stage_split<T, L, C, NT, DIR, S + 1, StoreF>(tile, tw, w, store);
+  }
+}
+
+template <typename T, int L, int C, int NT, int DIR, class LoadF, class StoreF>
+__device__ __forceinline__ void fft_tile_split(T* tile, const cx_t<T>* tw, LoadF load, StoreF store) {
+  using C2 = cx_t<T>;
+  constexpr int E = C * L / NT;
+  constexpr int RMAX = E < 16 ? E : 16;
+  constexpr int R = plan_radix(L, RMAX, 0);
+  constexpr int NB = E / R;
+  constexpr int STR = L / R;
+  constexpr int ROWS = NT / C;
+  const int col = threadIdx.x % C;
+  const int i0 = threadIdx.x / C;
+  C2 v[E];
+#pragma unroll
+  for (int q = 0; q < NB; ++q)
+#pragma unroll
+    for (int t = 0; t < R; ++t) v[q * R + t] = load(i0 + q * ROWS + t * STR, col);
+  __syncthreads();  // the tile may still be read by the previous transform
+  fft_stage_split<T, L, C, NT, DIR, 0, StoreF>(tile, tw, v, store);
+}
+
 // Copy a global twiddle table (exp(-2 pi i m / n), m < n) into LDS.
 template <typename C2>
 __device__ __forceinline__ void load_twiddles(C2* dst, const C2* src, int n, int nt) {

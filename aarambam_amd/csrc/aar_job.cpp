@@ -83,6 +83,10 @@ struct aar_job {
   double k_sec[AAR_NKINDS] = {}, k_bytes[AAR_NKINDS] = {};
   hipStream_t last_stream = nullptr;
   bool ran = false;
+  // hipGraph of the whole job (captured on the first run with graph mode on)
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
   // slot map
   int s_phig = 0, s_phi = 1, s_reg = 2;
   void* slot(int s) const { return (char*)pool + (size_t)s * g.fs * cxsize; }
@@ -288,30 +292,52 @@ template <typename T> struct Exec {
   }
 };
 
+// Reference layouts (row x*N + y, kz contiguous, N/2+1 bins) <-> the x-blocked
+// device layout (aar_common.h).
 template <typename T> void to_internal_spec(const double* ref, T* out, int N) {
   const int H = N / 2;
   const long long nq = (long long)N * N * H;
-  for (long long r = 0; r < (long long)N * N; ++r) {
-    for (int k = 0; k < H; ++k) {
-      out[2 * (r * H + k)] = (T)ref[2 * (r * (H + 1) + k)];
-      out[2 * (r * H + k) + 1] = (T)ref[2 * (r * (H + 1) + k) + 1];
+  for (int x = 0; x < N; ++x)
+    for (int y = 0; y < N; ++y) {
+      const long long q = (long long)x * N + y, r = row_of(N, x, y);
+      for (int k = 0; k < H; ++k) {
+        out[2 * (r * H + k)] = (T)ref[2 * (q * (H + 1) + k)];
+        out[2 * (r * H + k) + 1] = (T)ref[2 * (q * (H + 1) + k) + 1];
+      }
+      out[2 * (nq + r)] = (T)ref[2 * (q * (H + 1) + H)];
+      out[2 * (nq + r) + 1] = (T)ref[2 * (q * (H + 1) + H) + 1];
     }
-    out[2 * (nq + r)] = (T)ref[2 * (r * (H + 1) + H)];
-    out[2 * (nq + r) + 1] = (T)ref[2 * (r * (H + 1) + H) + 1];
-  }
 }
 
 template <typename T> void from_internal_spec(const T* in, double* ref, int N) {
   const int H = N / 2;
   const long long nq = (long long)N * N * H;
-  for (long long r = 0; r < (long long)N * N; ++r) {
-    for (int k = 0; k < H; ++k) {
-      ref[2 * (r * (H + 1) + k)] = in[2 * (r * H + k)];
-      ref[2 * (r * (H + 1) + k) + 1] = in[2 * (r * H + k) + 1];
+  for (int x = 0; x < N; ++x)
+    for (int y = 0; y < N; ++y) {
+      const long long q = (long long)x * N + y, r = row_of(N, x, y);
+      for (int k = 0; k < H; ++k) {
+        ref[2 * (q * (H + 1) + k)] = in[2 * (r * H + k)];
+        ref[2 * (q * (H + 1) + k) + 1] = in[2 * (r * H + k) + 1];
+      }
+      ref[2 * (q * (H + 1) + H)] = in[2 * (nq + r)];
+      ref[2 * (q * (H + 1) + H) + 1] = in[2 * (nq + r) + 1];
     }
-    ref[2 * (r * (H + 1) + H)] = in[2 * (nq + r)];
-    ref[2 * (r * (H + 1) + H) + 1] = in[2 * (nq + r) + 1];
-  }
+}
+
+template <typename T> void to_internal_real(const double* ref, T* out, int N) {
+  for (int x = 0; x < N; ++x)
+    for (int y = 0; y < N; ++y) {
+      const long long q = ((long long)x * N + y) * N, r = row_of(N, x, y) * N;
+      for (int z = 0; z < N; ++z) out[r + z] = (T)ref[q + z];
+    }
+}
+
+template <typename T> void from_internal_real(const T* in, double* ref, int N) {
+  for (int x = 0; x < N; ++x)
+    for (int y = 0; y < N; ++y) {
+      const long long q = ((long long)x * N + y) * N, r = row_of(N, x, y) * N;
+      for (int z = 0; z < N; ++z) ref[q + z] = in[r + z];
+    }
 }
 
 int device_alloc(void** p, size_t bytes) {
@@ -521,10 +547,32 @@ extern "C" int aar_job_run(aar_job* j, void* stream) {
   AAR_HIP(hipSetDevice(j->ctx->device));
   hipStream_t st = stream ? (hipStream_t)stream : j->ctx->stream;
   j->recs.clear();
-  j->evnext = 0;
   AAR_HIP(hipEventRecord(j->ev0, st));
-  int rc = j->prec == AAR_FP32 ? Exec<float>::run(j, st) : Exec<double>::run(j, st);
-  if (rc) return rc;
+  if (j->use_graph) {
+    if (!j->graph_exec) {  // capture every launch of the job once, then replay
+      // Per-kind event timing stays outside graphs: HIP cannot time events
+      // recorded by captured event nodes.
+      const bool prof = j->profiling;
+      j->profiling = false;
+      AAR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = j->prec == AAR_FP32 ? Exec<float>::run(j, st) : Exec<double>::run(j, st);
+      hipGraph_t gr = nullptr;
+      hipError_t e = hipStreamEndCapture(st, &gr);
+      j->profiling = prof;
+      if (rc) {
+        if (gr) hipGraphDestroy(gr);
+        return rc;
+      }
+      if (e != hipSuccess) return AAR_ERR_HIP;
+      j->graph = gr;
+      AAR_HIP(hipGraphInstantiate(&j->graph_exec, gr, nullptr, nullptr, 0));
+    }
+    AAR_HIP(hipGraphLaunch(j->graph_exec, st));
+  } else {
+    j->evnext = 0;
+    int rc = j->prec == AAR_FP32 ? Exec<float>::run(j, st) : Exec<double>::run(j, st);
+    if (rc) return rc;
+  }
   AAR_HIP(hipEventRecord(j->ev1, st));
   j->last_stream = st;
   j->ran = true;
@@ -546,6 +594,21 @@ extern "C" int aar_job_sync(aar_job* j) {
     j->k_bytes[r.kind] += r.bytes;
   }
   j->recs.clear();
+  return AAR_OK;
+}
+
+static void drop_graph(aar_job* j) {
+  if (j->graph_exec) hipGraphExecDestroy(j->graph_exec);
+  if (j->graph) hipGraphDestroy(j->graph);
+  j->graph_exec = nullptr;
+  j->graph = nullptr;
+}
+
+extern "C" int aar_job_set_graph(aar_job* j, int on) {
+  if (!j) return AAR_ERR_BAD_ARG;
+  if (j->ran) hipEventSynchronize(j->ev1);
+  drop_graph(j);  // re-captured on the next run
+  j->use_graph = on != 0;
   return AAR_OK;
 }
 
@@ -614,12 +677,8 @@ template <typename T> static int fetch_field_t(aar_job* j, int which, double* ou
   }
   AAR_HIP(hipMemcpy(host.data(), src, (size_t)g.fs * sizeof(C2), hipMemcpyDeviceToHost));
   if (tmp) hipFree(tmp);
-  if (which <= AAR_FIELD_PHI_SPEC) {
-    from_internal_spec<T>(host.data(), out, N);
-  } else {
-    const long long n3 = (long long)N * N * N;
-    for (long long q = 0; q < n3; ++q) out[q] = host[q];
-  }
+  if (which <= AAR_FIELD_PHI_SPEC) from_internal_spec<T>(host.data(), out, N);
+  else from_internal_real<T>(host.data(), out, N);
   return AAR_OK;
 }
 
@@ -638,6 +697,8 @@ extern "C" int aar_job_destroy(aar_job* j) {
                   j->kmodes, j->groups, j->terms, j->weights, j->acc, j->glass, j->pos, j->vel};
   for (void* b : bufs)
     if (b) hipFree(b);
+  if (j->graph_exec) hipGraphExecDestroy(j->graph_exec);
+  if (j->graph) hipGraphDestroy(j->graph);
   for (hipEvent_t e : j->evpool) hipEventDestroy(e);
   if (j->ev0) hipEventDestroy(j->ev0);
   if (j->ev1) hipEventDestroy(j->ev1);
@@ -652,8 +713,7 @@ template <typename T> static int fft3d_t(aar_ctx* ctx, int N, int dir, const dou
   std::vector<T> host((size_t)2 * g.fs, (T)0);
   const long long n3 = (long long)N * N * N;
   if (dir > 0) to_internal_spec<T>(in, host.data(), N);
-  else
-    for (long long q = 0; q < n3; ++q) host[q] = (T)in[q];
+  else to_internal_real<T>(in, host.data(), N);
   C2* d = nullptr;
   if (device_alloc((void**)&d, (size_t)g.fs * sizeof(C2))) return AAR_ERR_ALLOC;
   AAR_HIP(hipMemcpyAsync(d, host.data(), (size_t)g.fs * sizeof(C2), hipMemcpyHostToDevice, ctx->stream));
@@ -666,10 +726,8 @@ template <typename T> static int fft3d_t(aar_ctx* ctx, int N, int dir, const dou
   AAR_HIP(hipStreamSynchronize(ctx->stream));
   AAR_HIP(hipMemcpy(host.data(), d, (size_t)g.fs * sizeof(C2), hipMemcpyDeviceToHost));
   hipFree(d);
-  if (dir > 0)
-    for (long long q = 0; q < n3; ++q) out[q] = host[q];
-  else
-    from_internal_spec<T>(host.data(), out, N);
+  if (dir > 0) from_internal_real<T>(host.data(), out, N);
+  else from_internal_spec<T>(host.data(), out, N);
   return AAR_OK;
 }
 

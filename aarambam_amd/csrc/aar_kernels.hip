@@ -12,17 +12,15 @@
 
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ void decode_k(const Geom& g, long long e, int& x, int& y, int& kz) {
+  long long r;
   if (e < g.nq) {
     kz = (int)(e % g.H);
-    long long r = e / g.H;
-    y = (int)(r % g.N);
-    x = (int)(r / g.N);
+    r = e / g.H;
   } else {
-    long long r = e - g.nq;
     kz = g.H;
-    y = (int)(r % g.N);
-    x = (int)(r / g.N);
+    r = e - g.nq;
   }
+  row_xy(g.N, r, x, y);
 }
 
 // main.c:452-462: SphereMode 1 keeps |k| L/2pi <= Nsample/2, else the cube.
@@ -93,10 +91,21 @@ __host__ __device__ constexpr int clampi(int v, int lo, int hi) { return v < lo 
 __host__ __device__ constexpr int tile_threads(int CL) {
   return clampi(CL / 16, CL / 2 < 64 ? CL / 2 : 64, 1024);
 }
+// Tile width: 128-byte column segments, halved at L = 1024 so that two
+// tiles fit one CU's LDS (the XCD-aware block order below keeps the two
+// halves of each 128-byte line on one XCD's L2).
 template <typename T, int L> struct XYCfg {
   static constexpr int C = Shape<T>::CMAX < L / 2 ? Shape<T>::CMAX : L / 2;
   static constexpr int NT = tile_threads(C * L);
+  static constexpr bool SPLIT = true;  // real-valued LDS exchange tile
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; give
+// each XCD a contiguous range of tiles so neighbouring tiles share its L2.
+__device__ __forceinline__ int xcd_tile(int b, int nblk) {
+  if (nblk % 8) return b;
+  return (b % 8) * (nblk / 8) + b / 8;
+}
 template <typename T, int H> struct ZCfg {
   static constexpr int C = Shape<T>::CMAX;
   static constexpr int NT = tile_threads(C * H);
@@ -110,40 +119,37 @@ __global__ void __launch_bounds__((XYCfg<T, L>::NT))
     k_xy_pass(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg, int nmain) {
   using C2 = cx_t<T>;
   constexpr int C = XYCfg<T, L>::C, NT = XYCfg<T, L>::NT;
-  __shared__ C2 tile[TileIdx<T, L, C, false>::SIZE];
+  constexpr bool SPLIT = XYCfg<T, L>::SPLIT;
+  __shared__ T tile_r[SPLIT ? L * (C + 1) : 1];
+  __shared__ C2 tile_c[SPLIT ? 1 : TileIdx<T, L, C, false>::SIZE];
   __shared__ C2 tw[L];
   load_twiddles(tw, twg + L, L, NT);
   C2* f = base + (long long)blockIdx.y * g.fs;
-  const int t = blockIdx.x;
-  long long b0, es, cs;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int N = g.N, H = g.H;
-  if (t < nmain) {
+  // element (pos, col) of this tile -> complex offset in the field
+  int a, c0;
+  bool main_tile = t < nmain;
+  if (main_tile) {
     const int per = H / C;
-    const int a = t / per, kzt = t % per;
-    if (AXIS == 0) {  // fixed y = a, along x
-      b0 = (long long)a * H + kzt * C;
-      es = (long long)N * H;
-    } else {  // fixed x = a, along y
-      b0 = (long long)a * N * H + kzt * C;
-      es = H;
-    }
-    cs = 1;
+    a = t / per;         // fixed y (x pass) or fixed x (y pass)
+    c0 = (t % per) * C;  // first kz of the tile
   } else {
-    const int u = t - nmain;  // Nyquist plane nyq[x][y]
-    if (AXIS == 0) {          // columns y, along x
-      b0 = g.nq + (long long)u * C;
-      es = N;
-      cs = 1;
-    } else {  // columns x, along y (contiguous)
-      b0 = g.nq + (long long)u * C * N;
-      es = 1;
-      cs = N;
-    }
+    a = 0;
+    c0 = (t - nmain) * C;  // first y (x pass) or first x (y pass) of a Nyquist tile
   }
-  C2* src = f + b0;
-  auto load = [&](int pos, int col) { return src[pos * es + col * cs]; };
-  auto store = [&](int pos, int col, C2 v) { src[pos * es + col * cs] = v; };
-  fft_tile<T, L, C, NT, DIR, 1, false>(tile, tw, load, store);
+  auto addr = [&](int pos, int col) -> long long {
+    if (main_tile) return (AXIS == 0 ? row_of(N, pos, a) : row_of(N, a, pos)) * H + c0 + col;
+    return g.nq + (AXIS == 0 ? row_of(N, pos, c0 + col) : row_of(N, c0 + col, pos));
+  };
+  auto load = [&](int pos, int col) { return f[addr(pos, col)]; };
+  auto store = [&](int pos, int col, C2 v) { f[addr(pos, col)] = v; };
+  if constexpr (SPLIT) {
+    __syncthreads();  // twiddles
+    fft_tile_split<T, L, C, NT, DIR>(tile_r, tw, load, store);
+  } else {
+    fft_tile<T, L, C, NT, DIR, 1, false>(tile_c, tw, load, store);
+  }
 }
 
 template <typename T, int L, int AXIS, int DIR>
@@ -316,16 +322,18 @@ __global__ void __launch_bounds__(256)
             double anorm, double ns, double fac, double beta, double phase_shift, int fixed_amp) {
   using C2 = cx_t<T>;
   const int N = g.N, H = g.H;
-  const long long pid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pid >= (long long)N * N) return;
-  const int i = (int)(pid / N), j = (int)(pid % N);
-  C2* row = phi + ((long long)i * N + j) * H;
+  const long long mrow = (long long)blockIdx.x * blockDim.x + threadIdx.x;  // memory row
+  if (mrow >= (long long)N * N) return;
+  int i, j;
+  row_xy(N, mrow, i, j);
+  const long long pid = (long long)i * N + j;  // seed table index
+  C2* row = phi + mrow * H;
   const bool nyq_pencil = (i == H) || (j == H);
   const int ii = (N - i) % N, jj = (N - j) % N;
   // Which k = 0 entries this pencil owns (main.c:476-520).
   const bool gen0 = !nyq_pencil && ((i == 0 && j > 0 && j < H) || (i > 0 && i < H));
   const bool zero0 = nyq_pencil || (i == 0 && j == 0);
-  phi[g.nq + pid] = cmake<C2>(0, 0);
+  phi[g.nq + mrow] = cmake<C2>(0, 0);
 
   auto emit = [&](int k, double u1, double u2) {
     double phase = u1 * 2 * AAR_PI;
@@ -348,7 +356,7 @@ __global__ void __launch_bounds__(256)
       row[k] = val;
     } else if (gen0) {
       row[0] = val;
-      phi[((long long)ii * N + jj) * H] = cmake<C2>(val.x, -val.y);
+      phi[row_of(N, ii, jj) * H] = cmake<C2>(val.x, -val.y);
     } else if (zero0) {
       row[0] = cmake<C2>(0, 0);
     }
@@ -750,9 +758,8 @@ __global__ void __launch_bounds__(256)
   const double f3 = (1 - u) * (v) * (1 - w), f4 = (1 - u) * (v) * (w);
   const double f5 = (u) * (1 - v) * (1 - w), f6 = (u) * (1 - v) * (w);
   const double f7 = (u) * (v) * (1 - w), f8 = (u) * (v) * (w);
-  const long long NN = (long long)N * N;
-  const long long c1 = i * NN + (long long)j * N, c2 = i * NN + (long long)jj * N;
-  const long long c3 = ii * NN + (long long)j * N, c4 = ii * NN + (long long)jj * N;
+  const long long c1 = row_of(N, i, j) * N, c2 = row_of(N, i, jj) * N;
+  const long long c3 = row_of(N, ii, j) * N, c4 = row_of(N, ii, jj) * N;
   const float n3f = (float)(N * N * N);
   float V[3];
 #pragma unroll

@@ -23,13 +23,34 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def workload(N=512, box=2000.0):
+TEMPLATES = {
+    "scalari": "ScalarI mass=1 15-mode table (Aarambam/defaults/AlphaTable.dat)",
+    "le": "local+equilateral closed-form table, 12 modes",
+    "local": "local closed-form table, 8 modes",
+}
+
+
+def workload(N=512, box=2000.0, template="scalari"):
     from tests import cases
-    cfg = cases.base_config(Nmesh=N, Nsample=N, Nmax=N, N_modes=15, Box=box, GlassTileFac=N // 64,
+    nm = {"scalari": 15, "le": 12, "local": 8}[template]
+    cfg = cases.base_config(Nmesh=N, Nsample=N, Nmax=N, N_modes=nm, Box=box, GlassTileFac=max(1, N // 64),
                             Fnl=100.0, Seed=1234)
-    x0 = cases.x0_closed_form(15, cfg["PrimordialIndex"], box, N)
-    alpha = load_alpha(os.path.join(cases.GOLDEN, "AlphaTable.dat"), 15)
-    glass = cases.lattice_glass(64, 100.0)
+    x0 = cases.x0_closed_form(nm, cfg["PrimordialIndex"], box, N)
+    if template == "scalari":
+        alpha = load_alpha(os.path.join(cases.GOLDEN, "AlphaTable.dat"), 15)
+    else:
+        rows = list(cases.local_rows())
+        if template == "le":
+            rows += list(cases.equilateral_rows())
+        merged = {}
+        for n1, n2, n3, a in rows:  # a sum of templates adds coefficients of the same triple
+            key = tuple(sorted((int(n1), int(n2), int(n3)), reverse=True))
+            merged[key] = merged.get(key, 0.0) + a
+        alpha = np.zeros((nm, nm, nm))
+        for (n1, n2, n3), a in merged.items():  # main.c:231-241 fills all six permutations
+            for p in [(n1, n2, n3), (n1, n3, n2), (n2, n1, n3), (n2, n3, n1), (n3, n1, n2), (n3, n2, n1)]:
+                alpha[p] = a
+    glass = cases.lattice_glass(64 if N >= 64 else N, 100.0)
     return cfg, alpha, x0, glass, 100.0
 
 
@@ -66,14 +87,15 @@ def survey_bytes(N, M, K, b):
     return S * (94 + 7 * M + 8 * K) + 24 * N ** 3
 
 
-def time_job(aar, ctx, cfg, alpha, x0, glass, gbox, prec, sched, steps, warmup, profile):
+def time_job(aar, ctx, cfg, alpha, x0, glass, gbox, prec, sched, steps, warmup, profile, graph=False):
     t = aar.tables(alpha, x0, glass, gbox)
     job = aar.Job(ctx, aar.params_from_dict(cfg), t, prec, sched)
+    job.set_graph(graph)
     for _ in range(warmup):
         job.run()
     job.sync()
     if profile:
-        job.set_profiling(True)
+        job.set_profiling(True)  # resets the counters, keeps the captured graph
     ctx.sync()
     t0 = time.perf_counter()
     dev = 0.0
@@ -91,7 +113,6 @@ def time_job(aar, ctx, cfg, alpha, x0, glass, gbox, prec, sched, steps, warmup, 
 def cpu_baseline(N=128):
     from oracle import pyoracle as orc
     cfg, alpha, x0, glass, gbox = workload(N, 2000.0 * N / 512)
-    cfg["GlassTileFac"] = N // 64
     t0 = time.perf_counter()
     orc.run(cfg, alpha, x0, glass, gbox, want_fields=False)
     dt = time.perf_counter() - t0
@@ -111,6 +132,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--template", default="scalari", choices=sorted(TEMPLATES))
+    ap.add_argument("--box", type=float, default=2000.0)
+    ap.add_argument("--no-graph", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs one IC job per GPU on a single GPU; multi-GPU runs are not "
@@ -118,10 +142,15 @@ def main():
         sys.exit(2)
     import aarambam_amd as aar
     N = args.nmesh
-    cfg, alpha, x0, glass, gbox = workload(N)
+    cfg, alpha, x0, glass, gbox = workload(N, args.box, args.template)
     ctx = aar.Context(0)
+    # timed region: eager launches, each bracketed by HIP events (per-kernel roofline)
     r32 = time_job(aar, ctx, cfg, alpha, x0, glass, gbox, aar.AAR_FP32, aar.AAR_SCHED_BATCHED, args.steps,
                    args.warmup, True)
+    rg = None
+    if not args.no_graph:  # the same job replayed as one captured hipGraph
+        rg = time_job(aar, ctx, cfg, alpha, x0, glass, gbox, aar.AAR_FP32, aar.AAR_SCHED_BATCHED, args.steps,
+                      args.warmup, False, graph=True)
     ms = r32["wall"] / args.steps * 1e3
     value = N ** 3 * args.steps / r32["wall"] / 1e6
     stats = r32["stats"]
@@ -134,8 +163,8 @@ def main():
         "value": round(value, 3), "unit": "Mcells/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Gaussian field, lattice glass)",
-        "config": {"workload": f"NG-2LPT IC, Nmesh=Npart={N}^3, Lbox=2000 Mpc/h, ScalarI mass=1 15-mode "
-                               f"table (M={M}, K={K}), EH P(k), fNL=100, seed 1234",
+        "config": {"workload": f"NG-2LPT IC, Nmesh=Npart={N}^3, Lbox={args.box:g} Mpc/h, "
+                               f"{TEMPLATES[args.template]} (M={M}, K={K}), EH P(k), fNL=100, seed 1234",
                    "precision": "fp32", "schedule": "batched"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
@@ -144,6 +173,9 @@ def main():
         "job_roofline": {"bytes_per_job": survey_bytes(N, M, K, 4),
                          "frac": round(survey_bytes(N, M, K, 4) / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)},
     }
+    if rg is not None:
+        out["hipgraph"] = {"ms_per_step": round(rg["wall"] / args.steps * 1e3, 3),
+                           "value": round(N ** 3 * args.steps / rg["wall"] / 1e6, 3)}
     if not args.no_fp64:
         r64 = time_job(aar, ctx, cfg, alpha, x0, glass, gbox, aar.AAR_FP64, aar.AAR_SCHED_PLAIN,
                        args.fp64_steps, 1, False)

@@ -142,6 +142,9 @@ enum {
   AAR_K_COMBINE = 5, AAR_K_LPT = 6, AAR_K_CIC = 7, AAR_K_ACCUM = 8, AAR_NKINDS = 9
 };
 int aar_job_set_profiling(aar_job *job, int on);
+/* Graph mode: the first run captures every launch of the job into one
+ * hipGraph, later runs replay it.  Per-kind profiling applies to eager runs only. */
+int aar_job_set_graph(aar_job *job, int on);
 int aar_job_kernel_stats(aar_job *job, int kind, int *launches, double *seconds, double *bytes);
 int aar_job_destroy(aar_job *job);
 

@@ -142,3 +142,26 @@ def test_job_fp32_error_vs_oracle(ctx, table):
         rows, x0 = cases.equilateral_rows(), None
     res = _compare_job(ctx, cfg, rows, x0, aar.AAR_FP32, aar.AAR_SCHED_BATCHED, 1e-5)
     print("fp32 rel L2", table, res)
+
+
+def test_graph_replays_match_eager(ctx):
+    """The captured hipGraph gives the eager result on every replay, with eager work between."""
+    cfg = cases.base_config(Nmesh=32, Nsample=32, Nmax=32, N_modes=4)
+    alpha, x0, glass, gbox = cases.job_inputs(cfg, cases.equilateral_rows())
+    t = tables_for(cfg, alpha, x0, glass, gbox)
+    eager = aar.Job(ctx, aar.params_from_dict(cfg), t, aar.AAR_FP32, aar.AAR_SCHED_BATCHED)
+    eager.run()
+    eager.sync()
+    pos0, vel0 = eager.fetch()
+    phi0 = eager.field("phi_spec")
+    g = aar.Job(ctx, aar.params_from_dict(cfg), t, aar.AAR_FP32, aar.AAR_SCHED_BATCHED)
+    g.set_graph(True)
+    for rep in range(3):
+        g.run()
+        g.sync()
+        pos, vel = g.fetch()
+        np.testing.assert_array_equal(pos, pos0)
+        np.testing.assert_array_equal(vel, vel0)
+        np.testing.assert_array_equal(g.field("phi_spec"), phi0)
+        eager.run()  # unrelated eager work between replays
+        eager.sync()
