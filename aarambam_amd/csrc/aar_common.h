@@ -64,6 +64,7 @@ template <int DIR, typename C> __device__ __forceinline__ C cmul_i(C a) {
 // as real[r][z] (N^3 reals) in the bytes of the main block.
 struct Geom {
   int N, H;
+  int nlog, hlog;  // log2 N, log2 H (N is a power of two)
   long long nq;  // offset (complex elements) of the Nyquist plane = N*N*H
   long long fs;  // field stride (complex elements) = N*N*(H+1)
 };
@@ -76,11 +77,20 @@ __host__ __device__ __forceinline__ void row_xy(int N, long long r, int& x, int&
   y = (int)(t % N);
   x = (int)(t / N) * 8 + (int)(r & 7);
 }
+// same, for power-of-two N given as log2 (no 64-bit division)
+__host__ __device__ __forceinline__ void row_xy_log(int nlog, unsigned r, int& x, int& y) {
+  const unsigned t = r >> 3;
+  y = (int)(t & ((1u << nlog) - 1));
+  x = (int)((t >> nlog) << 3) + (int)(r & 7);
+}
 
 static inline Geom make_geom(int N) {
   Geom g;
   g.N = N;
   g.H = N / 2;
+  g.nlog = 0;
+  while ((1 << g.nlog) < N) ++g.nlog;
+  g.hlog = g.nlog - 1;
   g.nq = (long long)N * N * (N / 2);
   g.fs = (long long)N * N * (N / 2 + 1);
   return g;

@@ -12,7 +12,20 @@
 // and stores to HBM are coalesced across the C adjacent columns and the
 // intermediate stages stay in LDS.
 #pragma once
+#include <type_traits>
+
 #include "aar_common.h"
+
+// Loaders may take (pos, col) or (pos, col, e), e = the thread's element slot.
+template <class F> __device__ __forceinline__ auto call_load(F& f, int pos, int col, int e) {
+  if constexpr (std::is_invocable_v<F&, int, int, int>) return f(pos, col, e);
+  else return f(pos, col);
+}
+// Storers may take (pos, col, v) or (pos, col, v, e), e = the thread's output slot.
+template <class F, class V> __device__ __forceinline__ void call_store(F& f, int pos, int col, V v, int e) {
+  if constexpr (std::is_invocable_v<F&, int, int, V, int>) f(pos, col, v, e);
+  else f(pos, col, v);
+}
 
 __host__ __device__ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
 
@@ -177,7 +190,7 @@ __device__ __forceinline__ void fft_stage(cx_t<T>* tile, const cx_t<T>* tw, Load
 #pragma unroll
     for (int t = 0; t < R; ++t) {
       const int pos = i + t * STR;
-      if constexpr (S == 0) v[q * R + t] = load(pos, col);
+      if constexpr (S == 0) v[q * R + t] = call_load(load, pos, col, q * R + t);
       else v[q * R + t] = tile[TI::at(pos, col)];
     }
   }
@@ -199,7 +212,7 @@ __device__ __forceinline__ void fft_stage(cx_t<T>* tile, const cx_t<T>* tw, Load
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       const int pos = (i - k) * R + k + u * P;
-      if constexpr (S == NS - 1) store(pos, col, v[q * R + u]);
+      if constexpr (S == NS - 1) call_store(store, pos, col, v[q * R + u], q * R + u);
       else tile[TI::at(pos, col)] = v[q * R + u];
     }
   }
@@ -304,9 +317,69 @@ __device__ __forceinline__ void fft_tile_split(T* tile, const cx_t<T>* tw, LoadF
 #pragma unroll
   for (int q = 0; q < NB; ++q)
 #pragma unroll
-    for (int t = 0; t < R; ++t) v[q * R + t] = load(i0 + q * ROWS + t * STR, col);
+    for (int t = 0; t < R; ++t) v[q * R + t] = call_load(load, i0 + q * ROWS + t * STR, col, q * R + t);
   __syncthreads();  // the tile may still be read by the previous transform
   fft_stage_split<T, L, C, NT, DIR, 0, StoreF>(tile, tw, v, store);
+}
+
+// Element slots of the first stage of fft_tile_split: the (pos, col) that
+// slot e of this thread is loaded from.
+template <typename T, int L, int C, int NT>
+__device__ __forceinline__ void split_slot(int e, int& pos, int& col) {
+  constexpr int E = C * L / NT;
+  constexpr int RMAX = E < 16 ? E : 16;
+  constexpr int R = plan_radix(L, RMAX, 0);
+  constexpr int STR = L / R;
+  constexpr int ROWS = NT / C;
+  col = threadIdx.x % C;
+  pos = threadIdx.x / C + (e / R) * ROWS + (e % R) * STR;
+}
+
+// Input slots of the first stage of fft_tile: the (pos, col) that slot e of
+// this thread is loaded from.
+template <typename T, int L, int C, int NT, bool RM>
+__device__ __forceinline__ void first_slot(int e, int& pos, int& col) {
+  constexpr int E = C * L / NT;
+  constexpr int RMAX = E < 16 ? E : 16;
+  constexpr int R = plan_radix(L, RMAX, 0);
+  constexpr int STR = L / R;
+  constexpr int ROWS = NT / C;
+  const int q = e / R, t = e % R;
+  int i;
+  if constexpr (RM) {
+    const int b = threadIdx.x + q * NT;
+    i = b % STR;
+    col = b / STR;
+  } else {
+    i = threadIdx.x / C + q * ROWS;
+    col = threadIdx.x % C;
+  }
+  pos = i + t * STR;
+}
+
+// Output slots of the last stage of fft_tile: the (pos, col) that slot e of
+// this thread is stored to.
+template <typename T, int L, int C, int NT, bool RM>
+__device__ __forceinline__ void last_slot(int e, int& pos, int& col) {
+  constexpr int E = C * L / NT;
+  constexpr int RMAX = E < 16 ? E : 16;
+  constexpr int NS = plan_nstages(L, RMAX);
+  constexpr int R = plan_radix(L, RMAX, NS - 1);
+  constexpr int P = plan_prefix(L, RMAX, NS - 1);
+  constexpr int STR = L / R;
+  constexpr int ROWS = NT / C;
+  const int q = e / R, u = e % R;
+  int i;
+  if constexpr (RM) {
+    const int b = threadIdx.x + q * NT;
+    i = b % STR;
+    col = b / STR;
+  } else {
+    i = threadIdx.x / C + q * ROWS;
+    col = threadIdx.x % C;
+  }
+  const int k = i % P;
+  pos = (i - k) * R + k + u * P;
 }
 
 // Copy a global twiddle table (exp(-2 pi i m / n), m < n) into LDS.

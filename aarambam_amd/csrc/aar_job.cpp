@@ -231,16 +231,42 @@ template <typename T> struct Exec {
         if ((rc = timed(j, st, AAR_K_MODES, (1 + j->M) * S,
                         [&] { return P::modes_spec(F(j, j->s_phig), reg, j->M, j->modes, g, j->ks, st); })))
           return rc;
-        if ((rc = c2r(reg, j->M))) return rc;
-        if ((rc = timed(j, st, AAR_K_PRODUCTS, (j->M + j->K) * R,
-                        [&] { return P::products(reg, j->M, reg, j->K, j->terms, j->groups, g, st); })))
-          return rc;
-        if ((rc = r2c(reg, j->K))) return rc;
+        if (j->M <= 16 && g.N >= 64) {
+          // x, y inverse passes of the M mode fields, then the fused z pass
+          // (C2R, products, R2C), then y, x forward passes of the K products
+          if ((rc = timed(j, st, AAR_K_XY, 2 * j->M * S, [&] { return P::xy_pass(reg, j->M, g, tw(j), 0, +1, st); })) ||
+              (rc = timed(j, st, AAR_K_XY, 2 * j->M * S, [&] { return P::xy_pass(reg, j->M, g, tw(j), 1, +1, st); })))
+            return rc;
+          if ((rc = timed(j, st, AAR_K_PRODUCTS, (j->M + j->K) * S,
+                          [&] { return P::z_ng(reg, j->M, j->K, j->terms, j->groups, g, tw(j), st); })))
+            return rc;
+          if ((rc = timed(j, st, AAR_K_XY, 2 * j->K * S, [&] { return P::xy_pass(reg, j->K, g, tw(j), 1, -1, st); })) ||
+              (rc = timed(j, st, AAR_K_XY, 2 * j->K * S, [&] { return P::xy_pass(reg, j->K, g, tw(j), 0, -1, st); })))
+            return rc;
+        } else {
+          if ((rc = c2r(reg, j->M))) return rc;
+          if ((rc = timed(j, st, AAR_K_PRODUCTS, (j->M + j->K) * R,
+                          [&] { return P::products(reg, j->M, reg, j->K, j->terms, j->groups, g, st); })))
+            return rc;
+          if ((rc = r2c(reg, j->K))) return rc;
+        }
       }
-      if ((rc = timed(j, st, AAR_K_COMBINE, (j->K + 2) * S, [&] {
-             return P::ng_combine(F(j, j->s_phig), reg, j->K, j->kmodes, F(j, j->s_phi), p.Fnl, g, j->ks, st);
+      // combine + the nine 2LPT spectra in one elementwise pass
+      if ((rc = timed(j, st, AAR_K_COMBINE, (j->K + 11) * S, [&] {
+             return P::ng_lpt(F(j, j->s_phig), reg, j->K, j->kmodes, F(j, j->s_phi), reg, p.Fnl, g, j->ks, j->tf,
+                              c.Dplus, c.Beta, st);
            })))
         return rc;
+      if ((rc = c2r(reg, 9))) return rc;
+      if ((rc = timed(j, st, AAR_K_PRODUCTS, 7 * R, [&] { return P::source(reg, g, st); }))) return rc;
+      if ((rc = r2c(reg + 3 * g.fs, 1))) return rc;
+      if ((rc = timed(j, st, AAR_K_LPT, 4 * S, [&] { return P::disp2_spec(reg + 3 * g.fs, reg, g, j->ks, st); })))
+        return rc;
+      if ((rc = c2r(reg, 3))) return rc;
+      return timed(j, st, AAR_K_CIC, 6 * R + 24.0 * j->npart, [&] {
+        return P::cic(reg + 6 * g.fs, reg, g, j->glass, j->nglass, j->glass_box, j->p.GlassTileFac, p.Box,
+                      c.vel_prefac, c.vel_prefac2, j->pos, j->vel, j->npart, st);
+      });
     } else {
       if ((rc = aar_zero_fill((double*)j->acc, 2 * g.fs, st))) return rc;
       C2* fi = reg;

@@ -46,6 +46,10 @@ template <typename T> struct Pipeline {
                      hipStream_t st);
   static int z_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int dir, hipStream_t st);
 
+  // fused z pass of the NG sum: M mode fields (x, y already inverse transformed)
+  // -> z C2R -> products -> z R2C -> K fields over the first K slots
+  static int z_ng(cx_t<T>* f, int M, int K, const NgTerm* terms, const int* group_start, const Geom& g,
+                  const cx_t<T>* tw, hipStream_t st);
   static int gauss(cx_t<T>* phi, const Geom& g, const KSpace& ks, const uint32_t* seeds, double anorm,
                    double ns, double fac, double beta, double phase_shift, int fixed_amp, hipStream_t st);
   // out[m] = q_{modes[m]}(k) * phi(k) for m < M (cut and origin -> 0)
@@ -67,6 +71,10 @@ template <typename T> struct Pipeline {
   static int lpt_spec(const cx_t<T>* phi, cx_t<T>* out9, const Geom& g, const KSpace& ks,
                       const TransferDev& tf, double dplus, double beta, hipStream_t st);
   static int source(cx_t<T>* dg6, const Geom& g, hipStream_t st);  // writes into dg6[3]
+  // batched path: combine and the nine 2LPT spectra in one pass (out9 may alias h)
+  static int ng_lpt(const cx_t<T>* phig, const cx_t<T>* h, int K, const int* kmodes, cx_t<T>* phi,
+                    cx_t<T>* out9, double fnl, const Geom& g, const KSpace& ks, const TransferDev& tf,
+                    double dplus, double beta, hipStream_t st);
   static int disp2_spec(const cx_t<T>* src, cx_t<T>* out3, const Geom& g, const KSpace& ks,
                         hipStream_t st);
   // disp3 / disp2_3: three consecutive real fields each (ZA, 2LPT)

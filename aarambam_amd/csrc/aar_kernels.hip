@@ -12,15 +12,16 @@
 
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ void decode_k(const Geom& g, long long e, int& x, int& y, int& kz) {
-  long long r;
+  // N*N rows fit 32 bits for N <= 1024 (2^20 rows)
+  unsigned r;
   if (e < g.nq) {
-    kz = (int)(e % g.H);
-    r = e / g.H;
+    kz = (int)(e & (g.H - 1));
+    r = (unsigned)(e >> g.hlog);
   } else {
     kz = g.H;
-    r = e - g.nq;
+    r = (unsigned)(e - g.nq);
   }
-  row_xy(g.N, r, x, y);
+  row_xy_log(g.nlog, r, x, y);
 }
 
 // main.c:452-462: SphereMode 1 keeps |k| L/2pi <= Nsample/2, else the cube.
@@ -35,37 +36,40 @@ __device__ __forceinline__ double kmag2_of(const KSpace& ks, int x, int y, int z
 }
 
 // Upward Legendre recurrence, GSL gsl_sf_legendre_Pl (legendre_poly.c).
-struct LegRec {
-  double x, pm2, pm1;
+// RT is the precision of the radial functions: FP64 on the FP64 path, FP32 on
+// the FP32 path (|x| <= 1 and orders <= 31 keep the recurrence stable).
+template <typename RT> struct LegRecT {
+  RT x, pm2, pm1;
   int l;
-  __device__ void init(double xx) {
+  __device__ void init(RT xx) {
     x = xx;
-    pm2 = 1.0;
+    pm2 = 1;
     pm1 = xx;
     l = 1;
   }
-  __device__ double at(int L) {
-    if (L == 0) return 1.0;
+  __device__ RT at(int L) {
+    if (L == 0) return 1;
     while (l < L) {
       ++l;
-      double p = (x * (2 * l - 1) * pm1 - (l - 1) * pm2) / l;
+      RT p = (x * (RT)(2 * l - 1) * pm1 - (RT)(l - 1) * pm2) / (RT)l;
       pm2 = pm1;
       pm1 = p;
     }
     return pm1;
   }
 };
+using LegRec = LegRecT<double>;
 
 // Radial weight g_n(k) of the modal basis (main.c:835-862):
 // n <= 3: k^{s(n-3)};  n >= 4: (P_{n-1}(xbar) - X0[n-1]) / k^2.
-__device__ __forceinline__ double weight_g(int n, double ks, double kmag2, LegRec& leg,
-                                           const double* x0) {
+template <typename RT>
+__device__ __forceinline__ RT weight_g(int n, RT ks, RT kmag2, LegRecT<RT>& leg, const double* x0) {
   if (n <= 3) {
-    double r = 1.0;
+    RT r = 1;
     for (int q = n; q < 3; ++q) r *= ks;
-    return 1.0 / r;
+    return (RT)1 / r;
   }
-  return (leg.at(n - 1) - x0[n - 1]) / kmag2;
+  return (leg.at(n - 1) - (RT)x0[n - 1]) / kmag2;
 }
 
 // ------------------------------------------------------------------ twiddles
@@ -258,6 +262,164 @@ static int launch_z(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int di
   return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
 }
 
+// ------------------------------------------------------------------ fused z pass of the NG sum
+// For C grid rows at a time: half-complex -> real along z for each of the M
+// mode fields (kept in registers), the products h_K = sum c f_i f_j of the
+// regrouped NG sum, and real -> half-complex along z of each h_K, written
+// over the first K field slots.  Replaces z-C2R of M fields + the product
+// kernel + z-R2C of K fields ((M + K) spectra of HBM traffic instead of
+// 3(M + K)).
+template <typename T, int H> struct ZNGCfg {
+  static constexpr int E = sizeof(T) == 4 ? 4 : 2;  // complex elements per thread
+  static constexpr int C = sizeof(T) == 4 && H < 512 ? 4 : 2;  // rows per block
+  static constexpr int NT = C * H / E;
+};
+
+template <typename T, int H, int MMAX>
+__global__ void __launch_bounds__((ZNGCfg<T, H>::NT))
+    k_z_ng(cx_t<T>* base, int M, int K, const NgTerm* __restrict__ terms, const int* __restrict__ gs, Geom g,
+           const cx_t<T>* __restrict__ twg) {
+  using C2 = cx_t<T>;
+  constexpr int C = ZNGCfg<T, H>::C, NT = ZNGCfg<T, H>::NT, E = ZNGCfg<T, H>::E;
+  constexpr int N = 2 * H;
+  using TI = TileIdx<T, H, C, true>;
+  __shared__ C2 tile[TI::SIZE];
+  __shared__ C2 tw[N];
+  load_twiddles(tw, twg + N, N, NT);
+  const long long r0 = (long long)blockIdx.x * C;
+  // Real-space values of the mode fields: (x[2 pos], x[2 pos + 1]) of slot s
+  // in fx<s>[m], fy<s>[m].  One small array per slot and component, each
+  // indexed by the uniform mode number, stays in VGPRs (indirect addressing).
+  static_assert(E == 2 || E == 4, "two or four slots per thread");
+  T fx0[MMAX], fx1[MMAX], fx2[MMAX], fx3[MMAX], fy0[MMAX], fy1[MMAX], fy2[MMAX], fy3[MMAX];
+  // Raw inputs of the first FFT stage (X[k], X[H-k] per slot, X[H] for k = 0)
+  // are prefetched one field ahead, so HBM loads overlap the previous field's
+  // transform.
+  C2 ra[E], rb[E], na[E], nb[E];
+  auto fetch = [&](int m, C2* a, C2* b) {
+    const C2* fm = base + (long long)m * g.fs;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      int pos, col;
+      first_slot<T, H, C, NT, true>(e, pos, col);
+      const C2* row = fm + (r0 + col) * H;
+      a[e] = row[pos];
+      b[e] = pos == 0 ? fm[g.nq + r0 + col] : row[H - pos];
+    }
+  };
+  fetch(0, ra, rb);
+  for (int m = 0; m < M; ++m) {
+    if (m + 1 < M) fetch(m + 1, na, nb);
+    auto load = [&](int k, int, int e) {
+      const C2 A = ra[e], B = rb[e];
+      if (k == 0) return cmake<C2>(A.x + B.x, A.x - B.x);  // B = X[H] here
+      C2 w = tw[k];
+      w.y = -w.y;
+      const C2 s = cadd(A, cconj(B)), d = csub(A, cconj(B));
+      const C2 dw = cmul(d, w);
+      return cmake<C2>(s.x - dw.y, s.y + dw.x);
+    };
+    auto store = [&](int, int, C2 v, int e) {
+      if (e == 0) { fx0[m] = v.x; fy0[m] = v.y; }
+      if (e == 1) { fx1[m] = v.x; fy1[m] = v.y; }
+      if constexpr (E == 4) {
+        if (e == 2) { fx2[m] = v.x; fy2[m] = v.y; }
+        if (e == 3) { fx3[m] = v.x; fy3[m] = v.y; }
+      }
+    };
+    fft_tile<T, H, C, NT, +1, 2, false, true>(tile, tw, load, store);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      ra[e] = na[e];
+      rb[e] = nb[e];
+    }
+  }
+  for (int kk = 0; kk < K; ++kk) {
+    C2 hv[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) hv[e] = cmake<C2>(0, 0);
+    for (int t = gs[kk]; t < gs[kk + 1]; ++t) {
+      const NgTerm tm = terms[t];
+      const T cf = (T)tm.c;
+      const int ti = tm.i, tj = tm.j;
+      hv[0].x += cf * (fx0[ti] * fx0[tj]);
+      hv[0].y += cf * (fy0[ti] * fy0[tj]);
+      hv[1].x += cf * (fx1[ti] * fx1[tj]);
+      hv[1].y += cf * (fy1[ti] * fy1[tj]);
+      if constexpr (E == 4) {
+        hv[2].x += cf * (fx2[ti] * fx2[tj]);
+        hv[2].y += cf * (fy2[ti] * fy2[tj]);
+        hv[3].x += cf * (fx3[ti] * fx3[tj]);
+        hv[3].y += cf * (fy3[ti] * fy3[tj]);
+      }
+    }
+    __syncthreads();  // the tile is free
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      int pos, col;
+      last_slot<T, H, C, NT, true>(e, pos, col);
+      tile[TI::at(pos, col)] = hv[e];
+    }
+    auto load = [&](int pos, int c) { return tile[TI::at(pos, c)]; };
+    auto store = [&](int pos, int c, C2 v) { tile[TI::at(pos, c)] = v; };
+    fft_tile<T, H, C, NT, -1, 2, true, true>(tile, tw, load, store);
+    __syncthreads();
+    C2* fo = base + (long long)kk * g.fs;
+    C2* rows = fo + r0 * H;
+    constexpr int HP = H / 2 + 1;
+    for (int e = threadIdx.x; e < C * HP; e += NT) {
+      const int c = e / HP, k = e % HP;
+      C2* out = rows + (long long)c * H;
+      if (k == 0) {
+        const C2 z = tile[TI::at(0, c)];
+        out[0] = cmake<C2>(z.x + z.y, (T)0);
+        fo[g.nq + r0 + c] = cmake<C2>(z.x - z.y, (T)0);
+      } else {
+        const int k2 = H - k;
+        const C2 A = tile[TI::at(k, c)], B = tile[TI::at(k2, c)];
+        const C2 Ev = cscale(cadd(A, cconj(B)), (T)0.5);
+        const C2 dd = csub(A, cconj(B));
+        const C2 O = cmake<C2>(dd.y * (T)0.5, -dd.x * (T)0.5);
+        out[k] = cadd(Ev, cmul(O, tw[k]));
+        if (k2 != k) {
+          const C2 E2 = cscale(cadd(B, cconj(A)), (T)0.5);
+          const C2 dd2 = csub(B, cconj(A));
+          const C2 O2 = cmake<C2>(dd2.y * (T)0.5, -dd2.x * (T)0.5);
+          out[k2] = cadd(E2, cmul(O2, tw[k2]));
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int H>
+static int launch_z_ng(cx_t<T>* base, int M, int K, const NgTerm* terms, const int* gs, const Geom& g,
+                       const cx_t<T>* tw, hipStream_t st) {
+  constexpr int C = ZNGCfg<T, H>::C, NT = ZNGCfg<T, H>::NT;
+  dim3 grid((unsigned)((long long)g.N * g.N / C));
+  if (M <= 8)
+    hipLaunchKernelGGL((k_z_ng<T, H, 8>), grid, dim3(NT), 0, st, base, M, K, terms, gs, g, tw);
+  else if (M <= 16)
+    hipLaunchKernelGGL((k_z_ng<T, H, 16>), grid, dim3(NT), 0, st, base, M, K, terms, gs, g, tw);
+  else
+    return AAR_ERR_UNSUPPORTED;
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+template <typename T>
+int Pipeline<T>::z_ng(cx_t<T>* f, int M, int K, const NgTerm* terms, const int* gs, const Geom& g,
+                      const cx_t<T>* tw, hipStream_t st) {
+  if (M > 16) return AAR_ERR_UNSUPPORTED;
+  switch (g.H) {
+    case 32: return launch_z_ng<T, 32>(f, M, K, terms, gs, g, tw, st);
+    case 64: return launch_z_ng<T, 64>(f, M, K, terms, gs, g, tw, st);
+    case 128: return launch_z_ng<T, 128>(f, M, K, terms, gs, g, tw, st);
+    case 256: return launch_z_ng<T, 256>(f, M, K, terms, gs, g, tw, st);
+    case 512: return launch_z_ng<T, 512>(f, M, K, terms, gs, g, tw, st);
+    default: return AAR_ERR_UNSUPPORTED;
+  }
+}
+
 template <typename T>
 int Pipeline<T>::xy_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int axis, int dir,
                          hipStream_t st) {
@@ -433,21 +595,23 @@ __global__ void __launch_bounds__(256)
     return;
   }
   const C2 p = phi[e];
-  const double kss = pow(kmag1, ks.s);
-  const double xb = -1 + 2 * (kss - ks.kmin_s) / (ks.kmax_s - ks.kmin_s);
-  const double k3s = kss * kss * kss;  // k^{4-ns}
-  LegRec leg;
+  using RT = T;  // radial functions in the path's precision
+  const RT k1 = (RT)kmag1, k2 = (RT)kmag2;
+  const RT kss = pow(k1, (RT)ks.s);
+  const RT xb = -1 + 2 * (kss - (RT)ks.kmin_s) / (RT)(ks.kmax_s - ks.kmin_s);
+  const RT k3s = kss * kss * kss;  // k^{4-ns}
+  LegRecT<RT> leg;
   leg.init(xb);
   for (int m = 0; m < M; ++m) {
     const int n = modes[m];
-    double q;
+    RT q;
     if (n <= 3) {
-      q = 1.0;
+      q = 1;
       for (int t = 0; t < n; ++t) q *= kss;
     } else {
-      q = (leg.at(n - 1) - ks.x0[n - 1]) * (k3s / kmag2);
+      q = (leg.at(n - 1) - (RT)ks.x0[n - 1]) * (k3s / k2);
     }
-    out[(long long)m * g.fs + e] = cmake<C2>((T)(q * (double)p.x), (T)(q * (double)p.y));
+    out[(long long)m * g.fs + e] = cmake<C2>(q * p.x, q * p.y);
   }
 }
 
@@ -547,12 +711,13 @@ __global__ void __launch_bounds__(256)
     const double kmag2 = kmag2_of(ks, x, y, z);
     const double kmag1 = sqrt(kmag2);
     if (!outside_cut(ks, x, y, z, kmag1)) {
-      const double kss = pow(kmag1, ks.s);
-      LegRec leg;
-      leg.init(-1 + 2 * (kss - ks.kmin_s) / (ks.kmax_s - ks.kmin_s));
+      using RT = T;
+      const RT kss = pow((RT)kmag1, (RT)ks.s);
+      LegRecT<RT> leg;
+      leg.init(-1 + 2 * (kss - (RT)ks.kmin_s) / (RT)(ks.kmax_s - ks.kmin_s));
       double nr = 0, ni = 0;
       for (int kk = 0; kk < K; ++kk) {
-        const double w = weight_g(kmodes[kk], kss, kmag2, leg, ks.x0);
+        const double w = (double)weight_g<RT>(kmodes[kk], kss, (RT)kmag2, leg, ks.x0);
         const C2 v = h[(long long)kk * g.fs + e];
         nr += w * (double)v.x;
         ni += w * (double)v.y;
@@ -601,31 +766,32 @@ int Pipeline<T>::ng_combine_acc(const cx_t<T>* phig, const double2* acc, cx_t<T>
 }
 
 // ------------------------------------------------------------------ 2LPT
-__device__ double transfer_dev(const TransferDev& tf, double k) {
+template <typename RT> __device__ RT transfer_dev(const TransferDev& tf, RT k) {
   if (tf.which == 1) {  // Eisenstein & Hu no-wiggle, power.c:519-548
-    k *= tf.kconv;
-    double gamma = tf.eh_a + (1. - tf.eh_a) / (1. + exp(4 * log(0.43 * k * tf.eh_s)));
-    gamma *= tf.eh_omh;
-    double q = k * tf.eh_theta2 / gamma;
-    double L0 = log(tf.eh_twoe + 1.8 * q);
-    double C0 = 14.2 + 731. / (1. + 62.5 * q);
+    k *= (RT)tf.kconv;
+    const RT a = (RT)tf.eh_a;
+    RT gamma = a + ((RT)1 - a) / ((RT)1 + exp((RT)4 * log((RT)0.43 * k * (RT)tf.eh_s)));
+    gamma *= (RT)tf.eh_omh;
+    RT q = k * (RT)tf.eh_theta2 / gamma;
+    RT L0 = log((RT)tf.eh_twoe + (RT)1.8 * q);
+    RT C0 = (RT)14.2 + (RT)731. / ((RT)1 + (RT)62.5 * q);
     return L0 / (L0 + C0 * q * q);
   }
   if (tf.which == 2) {  // log-log table, power.c:181-217
-    k *= tf.kconv;
-    double logk = log10(k);
-    if (logk < tf.logk[0] || logk > tf.logk[tf.nt - 1]) return 0;
+    k *= (RT)tf.kconv;
+    RT logk = log10(k);
+    if (logk < (RT)tf.logk[0] || logk > (RT)tf.logk[tf.nt - 1]) return 0;
     int lo = 0, hi = tf.nt - 1;
     while (hi - lo > 1) {
       int mid = (hi + lo) / 2;
-      if (logk < tf.logk[mid]) hi = mid;
+      if (logk < (RT)tf.logk[mid]) hi = mid;
       else lo = mid;
     }
-    double u = (logk - tf.logk[lo]) / (tf.logk[hi] - tf.logk[lo]);
-    double logT = (1 - u) * tf.logT[lo] + u * tf.logT[hi];
-    return pow(10.0, logT);
+    RT u = (logk - (RT)tf.logk[lo]) / (RT)(tf.logk[hi] - tf.logk[lo]);
+    RT logT = (1 - u) * (RT)tf.logT[lo] + u * (RT)tf.logT[hi];
+    return pow((RT)10, logT);
   }
-  return 1.0;
+  return 1;
 }
 
 // out9[0..5] = i k_b psi_a for ab = 00,01,02,11,12,22 (main.c:1238-1277);
@@ -642,7 +808,7 @@ __global__ void __launch_bounds__(256)
   const double kv[3] = {ks.kv[x], ks.kv[y], ks.kv[z]};
   const double kmag2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
   const double kmag = sqrt(kmag2);
-  const double twb = transfer_dev(tf, kmag) / dplus / beta;
+  const double twb = (double)transfer_dev<T>(tf, (T)kmag) / dplus / beta;
   const C2 p = phi[e];
   double cr[3], ci[3];
 #pragma unroll
@@ -666,6 +832,75 @@ int Pipeline<T>::lpt_spec(const cx_t<T>* phi, cx_t<T>* out9, const Geom& g, cons
                           const TransferDev& tf, double dplus, double beta, hipStream_t st) {
   hipLaunchKernelGGL((k_lpt_spec<T>), dim3((unsigned)((g.fs + 255) / 256)), dim3(256), 0, st, phi, out9,
                      g, ks, tf, dplus, beta);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+// Batched path: Phi = phiG + Fnl/N^3 sum_K g_K h_K (written to phi) and the
+// nine 2LPT spectra of k_lpt_spec in one elementwise pass (out9 may alias h).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_ng_lpt(const cx_t<T>* __restrict__ phig, const cx_t<T>* h, int K, const int* __restrict__ kmodes,
+             cx_t<T>* __restrict__ phi, cx_t<T>* out, double fnl, double n3, Geom g, KSpace ks,
+             TransferDev tf, double dplus, double beta) {
+  using C2 = cx_t<T>;
+  using RT = T;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.fs) return;
+  int x, y, z;
+  decode_k(g, e, x, y, z);
+  const double kv[3] = {ks.kv[x], ks.kv[y], ks.kv[z]};
+  const double kmag2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+  const double kmag1 = sqrt(kmag2);
+  const C2 pg = phig[e];
+  double re = pg.x, im = pg.y;
+  if (x + y + z == 0) {
+    re = im = 0;
+  } else if (!outside_cut(ks, x, y, z, kmag1)) {
+    const RT kss = pow((RT)kmag1, (RT)ks.s);
+    LegRecT<RT> leg;
+    leg.init(-1 + 2 * (kss - (RT)ks.kmin_s) / (RT)(ks.kmax_s - ks.kmin_s));
+    double nr = 0, ni = 0;
+    for (int kk = 0; kk < K; ++kk) {
+      const double w = (double)weight_g<RT>(kmodes[kk], kss, (RT)kmag2, leg, ks.x0);
+      const C2 v = h[(long long)kk * g.fs + e];
+      nr += w * (double)v.x;
+      ni += w * (double)v.y;
+    }
+    re += fnl * nr / n3;
+    im += fnl * ni / n3;
+  }
+  const C2 p = cmake<C2>((T)re, (T)im);
+  phi[e] = p;
+  const double twb = (double)transfer_dev<T>(tf, (T)kmag1) / dplus / beta;
+  double cr[3], ci[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    ci[a] = kv[a] * twb * (double)p.x;
+    cr[a] = -kv[a] * twb * (double)p.y;
+  }
+  const double ff = 1 / (ks.sinc[x] * ks.sinc[y] * ks.sinc[z]);
+  const double smth = ff * ff;
+  // all reads of h at e are done; the outputs may overwrite them
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int b = a; b < 3; ++b) {
+      const int q = a == 0 ? b : (a == 1 ? 2 + b : 5);
+      out[(long long)q * g.fs + e] = cmake<C2>((T)(-ci[a] * kv[b]), (T)(cr[a] * kv[b]));
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    out[(long long)(6 + a) * g.fs + e] = cmake<C2>((T)(cr[a] * smth), (T)(ci[a] * smth));
+}
+
+template <typename T>
+int Pipeline<T>::ng_lpt(const cx_t<T>* phig, const cx_t<T>* h, int K, const int* kmodes, cx_t<T>* phi,
+                        cx_t<T>* out9, double fnl, const Geom& g, const KSpace& ks, const TransferDev& tf,
+                        double dplus, double beta, hipStream_t st) {
+  const double n3 = (double)((unsigned)g.N * (unsigned)g.N * (unsigned)g.N);
+  hipLaunchKernelGGL((k_ng_lpt<T>), dim3((unsigned)((g.fs + 255) / 256)), dim3(256), 0, st, phig, h, K, kmodes,
+                     phi, out9, fnl, n3, g, ks, tf, dplus, beta);
   return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
 }
 
@@ -724,18 +959,19 @@ int Pipeline<T>::disp2_spec(const cx_t<T>* src, cx_t<T>* out3, const Geom& g, co
 // ------------------------------------------------------------------ particles
 // Glass tiling (read_glass.c:176-214) and CIC readout with the ZA + 2LPT
 // update (main.c:1440-1516, periodic_wrap main.c:1534-1544).
+// Interpolation runs in the field precision (FP32 path: FP32 weights and
+// sums); the position update and periodic wrap are always FP64.
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_cic(const cx_t<T>* __restrict__ disp, const cx_t<T>* __restrict__ disp2, Geom g,
-          const float* __restrict__ glass, int nglass,
-          double gbox, int tile, double box, double vp1, double vp2, float* __restrict__ pos,
-          float* __restrict__ vel, long long npart) {
-  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npart) return;
+          const float* __restrict__ glass, int nglass, double gbox, int tile, double box, double vp1,
+          double vp2, float* __restrict__ pos, float* __restrict__ vel, long long npart) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;  // glass particle
+  if (n >= nglass) return;
+  const int t = blockIdx.y;                              // tile (tile-major order of read_glass.c)
+  const long long p = (long long)t * nglass + n;
   const int N = g.N;
-  const long long t = p / nglass;
-  const int n = (int)(p % nglass);
-  const int ti = (int)(t / ((long long)tile * tile)), tj = (int)((t / tile) % tile), tk = (int)(t % tile);
+  const int ti = t / (tile * tile), tj = (t / tile) % tile, tk = t % tile;
   const double bt = box / tile;
   float P[3];
   P[0] = (float)(glass[3 * n] / gbox * bt + ti * bt);
@@ -754,10 +990,12 @@ __global__ void __launch_bounds__(256)
   if (ii >= N) ii -= N;
   if (jj >= N) jj -= N;
   if (kk >= N) kk -= N;
-  const double f1 = (1 - u) * (1 - v) * (1 - w), f2 = (1 - u) * (1 - v) * (w);
-  const double f3 = (1 - u) * (v) * (1 - w), f4 = (1 - u) * (v) * (w);
-  const double f5 = (u) * (1 - v) * (1 - w), f6 = (u) * (1 - v) * (w);
-  const double f7 = (u) * (v) * (1 - w), f8 = (u) * (v) * (w);
+  using W = T;  // interpolation precision
+  const W uu = (W)u, vv = (W)v, ww = (W)w;
+  const W f1 = (1 - uu) * (1 - vv) * (1 - ww), f2 = (1 - uu) * (1 - vv) * (ww);
+  const W f3 = (1 - uu) * (vv) * (1 - ww), f4 = (1 - uu) * (vv) * (ww);
+  const W f5 = (uu) * (1 - vv) * (1 - ww), f6 = (uu) * (1 - vv) * (ww);
+  const W f7 = (uu) * (vv) * (1 - ww), f8 = (uu) * (vv) * (ww);
   const long long c1 = row_of(N, i, j) * N, c2 = row_of(N, i, jj) * N;
   const long long c3 = row_of(N, ii, j) * N, c4 = row_of(N, ii, jj) * N;
   const float n3f = (float)(N * N * N);
@@ -766,12 +1004,10 @@ __global__ void __launch_bounds__(256)
   for (int a = 0; a < 3; ++a) {
     const T* D = reinterpret_cast<const T*>(disp + (long long)a * g.fs);
     const T* D2 = reinterpret_cast<const T*>(disp2 + (long long)a * g.fs);
-    double dis = (double)D[c1 + k] * f1 + (double)D[c1 + kk] * f2 + (double)D[c2 + k] * f3 +
-                 (double)D[c2 + kk] * f4 + (double)D[c3 + k] * f5 + (double)D[c3 + kk] * f6 +
-                 (double)D[c4 + k] * f7 + (double)D[c4 + kk] * f8;
-    double dis2 = (double)D2[c1 + k] * f1 + (double)D2[c1 + kk] * f2 + (double)D2[c2 + k] * f3 +
-                  (double)D2[c2 + kk] * f4 + (double)D2[c3 + k] * f5 + (double)D2[c3 + kk] * f6 +
-                  (double)D2[c4 + k] * f7 + (double)D2[c4 + kk] * f8;
+    double dis = (double)(D[c1 + k] * f1 + D[c1 + kk] * f2 + D[c2 + k] * f3 + D[c2 + kk] * f4 +
+                          D[c3 + k] * f5 + D[c3 + kk] * f6 + D[c4 + k] * f7 + D[c4 + kk] * f8);
+    double dis2 = (double)(D2[c1 + k] * f1 + D2[c1 + kk] * f2 + D2[c2 + k] * f3 + D2[c2 + kk] * f4 +
+                           D2[c3 + k] * f5 + D2[c3 + kk] * f6 + D2[c4 + k] * f7 + D2[c4 + kk] * f8);
     dis2 /= n3f;
     P[a] = (float)((double)P[a] + (dis - 3. / 7. * dis2));
     V[a] = (float)(dis * vp1 - 3. / 7. * dis2 * vp2);
@@ -788,9 +1024,10 @@ template <typename T>
 int Pipeline<T>::cic(const cx_t<T>* disp3, const cx_t<T>* disp2_3, const Geom& g, const float* glass,
                      int nglass, double gbox, int tile, double box, double vp1, double vp2, float* pos,
                      float* vel, long long npart, hipStream_t st) {
-  hipLaunchKernelGGL((k_cic<T>), dim3((unsigned)((npart + 255) / 256)), dim3(256), 0, st, disp3, disp2_3,
-                     g, glass,
-                     nglass, gbox, tile, box, vp1, vp2, pos, vel, npart);
+  const long long ntile = (long long)tile * tile * tile;
+  if (ntile * nglass != npart || ntile > 65535) return AAR_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL((k_cic<T>), dim3((unsigned)((nglass + 255) / 256), (unsigned)ntile), dim3(256), 0, st,
+                     disp3, disp2_3, g, glass, nglass, gbox, tile, box, vp1, vp2, pos, vel, npart);
   return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
 }
 
