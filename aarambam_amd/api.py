@@ -48,7 +48,9 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_load_alpha", "aar_load_x
            "aar_write_phi", "aar_write_inputspec", "aar_ctx_create", "aar_ctx_destroy", "aar_ctx_sync", "aar_job_bytes",
            "aar_job_prepare", "aar_job_run", "aar_job_sync", "aar_job_device_output", "aar_job_fetch",
            "aar_job_fetch_field", "aar_job_last_seconds", "aar_job_destroy", "aar_fft3d",
-           "aar_gaussian_field", "aar_ranlxd_device", "aar_bench_pass", "aar_job_set_profiling", "aar_job_kernel_stats", "aar_job_set_graph"]
+           "aar_gaussian_field", "aar_ranlxd_device", "aar_bench_pass", "aar_job_set_profiling", "aar_job_kernel_stats", "aar_job_set_graph",
+           "aar_server_create", "aar_server_submit", "aar_server_wait", "aar_server_fetch",
+           "aar_server_device_output", "aar_server_release", "aar_server_stats", "aar_server_destroy"]
 KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
 
 
@@ -102,6 +104,16 @@ def lib():
             "aar_job_set_profiling": (C.c_int, [vp, C.c_int]),
             "aar_job_set_graph": (C.c_int, [vp, C.c_int]),
             "aar_job_kernel_stats": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), dp, dp]),
+            "aar_server_create": (C.c_int, [vp, C.c_longlong, C.c_int, C.POINTER(vp)]),
+            "aar_server_submit": (C.c_int, [vp, PP, TP, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_longlong)]),
+            "aar_server_wait": (C.c_int, [vp, C.c_longlong, dp, dp]),
+            "aar_server_fetch": (C.c_int, [vp, C.c_longlong, fp, fp]),
+            "aar_server_device_output": (C.c_int, [vp, C.c_longlong, C.POINTER(fp), C.POINTER(fp),
+                                                   C.POINTER(C.c_longlong)]),
+            "aar_server_release": (C.c_int, [vp, C.c_longlong]),
+            "aar_server_stats": (C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                           C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
+            "aar_server_destroy": (C.c_int, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -294,6 +306,62 @@ class Job:
     def close(self):
         if self.h:
             lib().aar_job_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop on one device (aar_server_*).
+
+    submit() returns a ticket at once; wait() blocks until that job completed
+    and returns (latency_s, queue_s).  Jobs submitted with keep=True hold
+    their results until release()."""
+
+    def __init__(self, ctx, hbm_budget, max_active=4):
+        h = C.c_void_p()
+        _check(lib().aar_server_create(ctx.h, int(hbm_budget), int(max_active), C.byref(h)), "aar_server_create")
+        self.h, self.ctx = h, ctx
+        self.npart = {}
+
+    def submit(self, params, tabs, precision=AAR_FP32, schedule=AAR_SCHED_BATCHED, keep=False):
+        t = C.c_longlong(0)
+        _check(lib().aar_server_submit(self.h, C.byref(params), C.byref(tabs.c), precision, schedule, int(keep),
+                                       C.byref(t)), "aar_server_submit")
+        self.npart[t.value] = params.GlassTileFac ** 3 * tabs.nglass
+        return t.value
+
+    def wait(self, ticket):
+        lat, q = C.c_double(0), C.c_double(0)
+        _check(lib().aar_server_wait(self.h, ticket, C.byref(lat), C.byref(q)), "aar_server_wait")
+        return lat.value, q.value
+
+    def fetch(self, ticket):
+        n = self.npart[ticket]
+        pos = np.empty(3 * n, np.float32)
+        vel = np.empty(3 * n, np.float32)
+        fpt = C.POINTER(C.c_float)
+        _check(lib().aar_server_fetch(self.h, ticket, pos.ctypes.data_as(fpt), vel.ctypes.data_as(fpt)),
+               "aar_server_fetch")
+        return pos.reshape(-1, 3), vel.reshape(-1, 3)
+
+    def release(self, ticket):
+        _check(lib().aar_server_release(self.h, ticket), "aar_server_release")
+        self.npart.pop(ticket, None)
+
+    def stats(self):
+        s, c, pb = C.c_longlong(0), C.c_longlong(0), C.c_longlong(0)
+        pa = C.c_int(0)
+        _check(lib().aar_server_stats(self.h, C.byref(s), C.byref(c), C.byref(pa), C.byref(pb)), "aar_server_stats")
+        return dict(submitted=s.value, completed=c.value, peak_active=pa.value, peak_bytes=pb.value)
+
+    def close(self):
+        if self.h:
+            lib().aar_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -14,11 +14,12 @@ LIB = os.path.join(HERE, "libaarambam_hip.so")
 EXE = os.path.join(HERE, "bin", "2LPTBasis")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["aar_kernels.hip", "aar_job.cpp", "aar_host.cpp"]
+SOURCES = ["aar_kernels.hip", "aar_job.cpp", "aar_host.cpp", "aar_serve.cpp"]
 # -ffp-contract=off keeps the reference's operation order (no fused multiply-add
 # contraction); packed FP32 instructions are disabled by target feature.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
-         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument",
+         "-pthread"]
 
 
 def _run(cmd):
@@ -44,7 +45,7 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     main_src = os.path.join(CSRC, "aar_main.cpp")
     if os.path.exists(main_src):

@@ -11,11 +11,14 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <set>
+#include <string>
 #include <vector>
 
 #include "../../include/aarambam.h"
 #include "aar_common.h"
+#include "aar_internal.h"
 #include "aar_kernels.h"
 
 int aar_host_transfer_tables(const aar_params* p, const aar_tables* t, std::vector<double>& logk,
@@ -26,6 +29,11 @@ struct aar_ctx {
   hipStream_t stream = nullptr;
   float2* tw32 = nullptr;
   double2* tw64 = nullptr;
+  // cosmology scalars by the parameters they depend on: the Romberg sigma8
+  // normalisation and growth-factor integrals are the costliest host step of
+  // a job and repeat across jobs that differ only in mesh, table or seed
+  std::mutex mu;
+  std::map<std::string, aar_cosmo> cosmo_cache;
 };
 
 namespace {
@@ -87,6 +95,13 @@ struct aar_job {
   bool use_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  // device memory carved from a caller's workspace (serving arena) instead of
+  // hipMalloc; host copies of uploaded tables live until destroy when the
+  // uploads are asynchronous on `up`
+  char* ws = nullptr;
+  size_t ws_size = 0, ws_used = 0;
+  hipStream_t up = nullptr;
+  std::vector<std::vector<char>> staged;
   // slot map
   int s_phig = 0, s_phi = 1, s_reg = 2;
   void* slot(int s) const { return (char*)pool + (size_t)s * g.fs * cxsize; }
@@ -382,6 +397,31 @@ template <typename V> int upload(V** dst, const std::vector<V>& src) {
   return AAR_OK;
 }
 
+// Device memory of a job: its own hipMalloc'd buffer, or the next 256-byte
+// aligned piece of the workspace it was prepared in.
+int job_alloc(aar_job* j, void** p, size_t bytes) {
+  if (!j->ws) return device_alloc(p, bytes);
+  const size_t off = (j->ws_used + 255) & ~(size_t)255, n = bytes ? bytes : 1;
+  if (off + n > j->ws_size) {
+    *p = nullptr;
+    return AAR_ERR_HBM_BUDGET;
+  }
+  *p = j->ws + off;
+  j->ws_used = off + n;
+  return AAR_OK;
+}
+
+template <typename V> int job_upload(aar_job* j, V** dst, const std::vector<V>& src) {
+  const size_t n = sizeof(V) * src.size();
+  int rc = job_alloc(j, (void**)dst, n);
+  if (rc || !n) return rc;
+  if (!j->up) return hipMemcpy(*dst, src.data(), n, hipMemcpyHostToDevice) == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+  j->staged.emplace_back((const char*)src.data(), (const char*)src.data() + n);
+  return hipMemcpyAsync(*dst, j->staged.back().data(), n, hipMemcpyHostToDevice, j->up) == hipSuccess
+             ? AAR_OK
+             : AAR_ERR_HIP;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ context
@@ -407,6 +447,8 @@ extern "C" int aar_ctx_create(int device, aar_ctx** out) {
   return AAR_OK;
 }
 
+int aar_ctx_device(const aar_ctx* c) { return c->device; }
+
 extern "C" int aar_ctx_sync(aar_ctx* c) {
   if (!c) return AAR_ERR_BAD_ARG;
   AAR_HIP(hipSetDevice(c->device));
@@ -425,6 +467,38 @@ extern "C" int aar_ctx_destroy(aar_ctx* c) {
 }
 
 // ------------------------------------------------------------------ jobs
+// aar_cosmology through the context's cache.  The key holds every parameter
+// the cosmology code reads (and a few more) plus the transfer table rows.
+static int ctx_cosmology(aar_ctx* ctx, const aar_params* p, const aar_tables* t, aar_cosmo* c) {
+  std::string key;
+  auto put = [&](const void* v, size_t n) { key.append((const char*)v, n); };
+  const double d[] = {p->Box, p->Omega, p->OmegaLambda, p->OmegaBaryon, p->OmegaDM_2ndSpecies, p->HubbleParam,
+                      p->Sigma8, p->PrimordialIndex, p->Redshift, p->ShapeGamma, p->UnitLength_in_cm,
+                      p->UnitMass_in_g, p->UnitVelocity_in_cm_per_s, p->InputSpectrum_UnitLength_in_cm,
+                      p->WDM_PartMass_in_kev};
+  const int i[] = {p->WhichSpectrum, p->WhichTransfer, p->WDM_On, p->WDM_Vtherm_On};
+  put(d, sizeof d);
+  put(i, sizeof i);
+  if (p->WhichTransfer == 2 && t->ntransfer > 0 && t->transfer_k && t->transfer_t) {
+    put(t->transfer_k, sizeof(double) * t->ntransfer);
+    put(t->transfer_t, sizeof(double) * t->ntransfer);
+  }
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto it = ctx->cosmo_cache.find(key);
+    if (it != ctx->cosmo_cache.end()) {
+      *c = it->second;
+      return AAR_OK;
+    }
+  }
+  const int rc = aar_cosmology(p, t, c);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (ctx->cosmo_cache.size() > 256) ctx->cosmo_cache.clear();
+  ctx->cosmo_cache[key] = *c;
+  return AAR_OK;
+}
+
 static int validate(const aar_params* p, const aar_tables* t) {
   if (!p || !t) return AAR_ERR_BAD_ARG;
   if (!pow2(p->Nmesh)) return AAR_ERR_UNSUPPORTED;
@@ -440,20 +514,50 @@ static int slots_needed(int M, int K, int sched) {
   return 2 + std::max(std::max(M, K), 9);
 }
 
+// Bytes of every device buffer aar_job_prepare allocates, in its order, each
+// rounded up to 256 bytes: the exact footprint of a job carved from a workspace.
+static long long job_footprint(const aar_params* p, const aar_tables* t, int precision, int schedule,
+                               const Schedule& s) {
+  const Geom g = make_geom(p->Nmesh);
+  const long long N = p->Nmesh, Nm = p->N_modes, cx = precision == AAR_FP32 ? 8 : 16;
+  const long long npart = (long long)p->GlassTileFac * p->GlassTileFac * p->GlassTileFac * t->nglass;
+  const bool plain = schedule == AAR_SCHED_PLAIN;
+  long long b = 0;
+  auto add = [&](long long n) { b = ((b + 255) & ~255LL) + (n ? n : 1); };
+  add(slots_needed((int)s.modes.size(), (int)s.kmodes.size(), schedule) * g.fs * cx);  // field pool
+  if (plain) add(g.fs * 16);                                                           // double accumulator
+  add(4 * N * N);                                                                      // seed table
+  add(8 * N), add(8 * N), add(8 * Nm), add(N);                                         // kv, sinc, x0, cut flags
+  add(4 * (plain ? Nm : (long long)s.modes.size()));
+  add(4 * (long long)s.kmodes.size());
+  add(4 * (plain ? 2 : (long long)s.groups.size()));
+  add((long long)sizeof(NgTerm) * (plain ? 1 : (long long)s.terms.size()));
+  add((long long)sizeof(NgWeight) * (long long)s.weights.size());
+  if (p->WhichTransfer == 2) {
+    std::vector<double> logk, logT;
+    if (aar_host_transfer_tables(p, t, logk, logT)) return -1;
+    add(8 * (long long)logk.size()), add(8 * (long long)logT.size());
+  }
+  add(12LL * t->nglass);
+  add(12 * npart), add(12 * npart);  // pos, vel
+  return b;
+}
+
 extern "C" long long aar_job_bytes(const aar_params* p, const aar_tables* t, int precision, int schedule) {
   if (validate(p, t)) return -1;
-  Schedule s = build_schedule(*p, t->alpha);
-  const Geom g = make_geom(p->Nmesh);
-  const long long cx = precision == AAR_FP32 ? 8 : 16;
-  const long long npart = (long long)p->GlassTileFac * p->GlassTileFac * p->GlassTileFac * t->nglass;
-  long long b = slots_needed((int)s.modes.size(), (int)s.kmodes.size(), schedule) * g.fs * cx;
-  if (schedule == AAR_SCHED_PLAIN) b += g.fs * 16;  // double accumulator
-  b += 24 * npart + 12LL * t->nglass + 4LL * p->Nmesh * p->Nmesh;
-  return b;
+  return job_footprint(p, t, precision, schedule, build_schedule(*p, t->alpha));
 }
 
 extern "C" int aar_job_prepare(aar_ctx* ctx, const aar_params* p, const aar_tables* t, int precision,
                                int schedule, aar_job** out) {
+  return aar_job_prepare_ws(ctx, p, t, precision, schedule, nullptr, 0, nullptr, out);
+}
+
+// Prepare a job whose device buffers are carved from `ws` (ws_size bytes,
+// at least aar_job_bytes) and whose table uploads are enqueued on `up`;
+// ws = nullptr gives the job its own hipMalloc'd buffers.
+int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, int precision, int schedule,
+                       void* ws, size_t ws_size, hipStream_t up, aar_job** out) {
   if (!ctx || !out) return AAR_ERR_BAD_ARG;
   int rc = validate(p, t);
   if (rc) return rc;
@@ -467,11 +571,14 @@ extern "C" int aar_job_prepare(aar_ctx* ctx, const aar_params* p, const aar_tabl
   j->sched = schedule;
   j->cxsize = precision == AAR_FP32 ? 8 : 16;
   j->g = make_geom(p->Nmesh);
+  j->ws = (char*)ws;
+  j->ws_size = ws_size;
+  j->up = up;
   auto fail = [&](int code) {
     aar_job_destroy(j);
     return code;
   };
-  if ((rc = aar_cosmology(p, t, &j->c))) return fail(rc);
+  if ((rc = ctx_cosmology(ctx, p, t, &j->c))) return fail(rc);
   const int N = p->Nmesh;
   // schedule
   Schedule s = build_schedule(*p, t->alpha);
@@ -508,26 +615,26 @@ extern "C" int aar_job_prepare(aar_ctx* ctx, const aar_params* p, const aar_tabl
   if (p->WhichTransfer == 2 && (rc = aar_host_transfer_tables(p, t, logk, logT))) return fail(rc);
   // device buffers
   j->nslots = slots_needed(j->M, j->K, schedule);
-  if ((rc = device_alloc(&j->pool, (size_t)j->nslots * j->g.fs * j->cxsize))) return fail(rc);
-  if (schedule == AAR_SCHED_PLAIN && (rc = device_alloc((void**)&j->acc, (size_t)j->g.fs * 16)))
+  if ((rc = job_alloc(j, &j->pool, (size_t)j->nslots * j->g.fs * j->cxsize))) return fail(rc);
+  if (schedule == AAR_SCHED_PLAIN && (rc = job_alloc(j, (void**)&j->acc, (size_t)j->g.fs * 16)))
     return fail(rc);
   std::vector<uint32_t> seeds((size_t)N * N);
   aar_seedtable(N, p->Seed, seeds.data());
-  if ((rc = upload(&j->seeds, seeds))) return fail(rc);
-  if ((rc = upload(&j->kv, kv)) || (rc = upload(&j->sinc, sinc)) || (rc = upload(&j->x0, x0)) ||
-      (rc = upload(&j->outc, outc)) || (rc = upload(&j->modes, modes_dev)) ||
-      (rc = upload(&j->kmodes, s.kmodes)) || (rc = upload(&j->groups, groups)) ||
-      (rc = upload(&j->terms, terms)) || (rc = upload(&j->weights, s.weights)))
+  if ((rc = job_upload(j, &j->seeds, seeds))) return fail(rc);
+  if ((rc = job_upload(j, &j->kv, kv)) || (rc = job_upload(j, &j->sinc, sinc)) || (rc = job_upload(j, &j->x0, x0)) ||
+      (rc = job_upload(j, &j->outc, outc)) || (rc = job_upload(j, &j->modes, modes_dev)) ||
+      (rc = job_upload(j, &j->kmodes, s.kmodes)) || (rc = job_upload(j, &j->groups, groups)) ||
+      (rc = job_upload(j, &j->terms, terms)) || (rc = job_upload(j, &j->weights, s.weights)))
     return fail(rc);
-  if (p->WhichTransfer == 2 && ((rc = upload(&j->logk, logk)) || (rc = upload(&j->logT, logT))))
+  if (p->WhichTransfer == 2 && ((rc = job_upload(j, &j->logk, logk)) || (rc = job_upload(j, &j->logT, logT))))
     return fail(rc);
   j->nglass = t->nglass;
   j->glass_box = t->glass_box;
   std::vector<float> gl(t->glass, t->glass + 3 * (size_t)t->nglass);
-  if ((rc = upload(&j->glass, gl))) return fail(rc);
+  if ((rc = job_upload(j, &j->glass, gl))) return fail(rc);
   j->npart = (long long)p->GlassTileFac * p->GlassTileFac * p->GlassTileFac * t->nglass;
-  if ((rc = device_alloc((void**)&j->pos, sizeof(float) * 3 * j->npart)) ||
-      (rc = device_alloc((void**)&j->vel, sizeof(float) * 3 * j->npart)))
+  if ((rc = job_alloc(j, (void**)&j->pos, sizeof(float) * 3 * j->npart)) ||
+      (rc = job_alloc(j, (void**)&j->vel, sizeof(float) * 3 * j->npart)))
     return fail(rc);
   // kernel-side views
   KSpace& ks = j->ks;
@@ -607,6 +714,7 @@ extern "C" int aar_job_run(aar_job* j, void* stream) {
 
 extern "C" int aar_job_sync(aar_job* j) {
   if (!j) return AAR_ERR_BAD_ARG;
+  if (!j->ran) return AAR_OK;  // nothing enqueued yet
   AAR_HIP(hipSetDevice(j->ctx->device));
   AAR_HIP(hipEventSynchronize(j->ev1));
   float ms = 0;
@@ -719,10 +827,14 @@ extern "C" int aar_job_destroy(aar_job* j) {
   if (!j) return AAR_OK;
   hipSetDevice(j->ctx->device);
   if (j->ran) hipEventSynchronize(j->ev1);
-  void* bufs[] = {j->pool, j->seeds, j->kv, j->sinc, j->x0, j->logk, j->logT, j->outc, j->modes,
-                  j->kmodes, j->groups, j->terms, j->weights, j->acc, j->glass, j->pos, j->vel};
-  for (void* b : bufs)
-    if (b) hipFree(b);
+  if (!j->ws) {
+    void* bufs[] = {j->pool, j->seeds, j->kv, j->sinc, j->x0, j->logk, j->logT, j->outc, j->modes,
+                    j->kmodes, j->groups, j->terms, j->weights, j->acc, j->glass, j->pos, j->vel};
+    for (void* b : bufs)
+      if (b) hipFree(b);
+  } else if (j->up && !j->ran) {
+    hipStreamSynchronize(j->up);  // uploads from the staged host copies may still be in flight
+  }
   if (j->graph_exec) hipGraphExecDestroy(j->graph_exec);
   if (j->graph) hipGraphDestroy(j->graph);
   for (hipEvent_t e : j->evpool) hipEventDestroy(e);
@@ -786,7 +898,7 @@ extern "C" int aar_gaussian_field(aar_ctx* ctx, const aar_params* p, const aar_t
 }
 
 // Microbenchmark of one building block on nf fields of an N^3 grid:
-// kind 0 x-pass, 1 y-pass, 2 z c2r, 3 z r2c, 4 device copy of nf spectra.
+// kind 0 x-pass, 1 y-pass, 2 z c2r, 3 z r2c, 4 plain copy kernel over nf spectra.
 // Returns the mean seconds per launch over `iters` launches.
 template <typename T> static int bench_pass_t(aar_ctx* ctx, int N, int kind, int nf, int iters, double* sec) {
   using C2 = cx_t<T>;
@@ -806,10 +918,7 @@ template <typename T> static int bench_pass_t(aar_ctx* ctx, int N, int kind, int
       case 2: return Pipeline<T>::z_pass(d, nf, g, tw, +1, ctx->stream);
       case 3: return Pipeline<T>::z_pass(d, nf, g, tw, -1, ctx->stream);
       default:
-        return hipMemcpyAsync(d2, d, (size_t)nf * g.fs * sizeof(C2), hipMemcpyDeviceToDevice, ctx->stream) ==
-                       hipSuccess
-                   ? AAR_OK
-                   : AAR_ERR_HIP;
+        return aar_copy16(d, d2, (long long)nf * g.fs * (long long)sizeof(C2), ctx->stream);
     }
   };
   int rc = launch();

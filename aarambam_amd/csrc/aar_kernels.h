@@ -85,6 +85,7 @@ template <typename T> struct Pipeline {
 
 // zero n doubles (graph-capturable; no memset nodes)
 int aar_zero_fill(double* p, long long n, hipStream_t st);
+int aar_copy16(const void* src, void* dst, long long bytes, hipStream_t st);
 // n uniforms of one ranlxd1 stream on the device (test hook)
 int aar_ranlxd_draws_dev(uint32_t seed, int n, double* out, hipStream_t st);
 

@@ -27,20 +27,23 @@ TEMPLATES = {
     "scalari": "ScalarI mass=1 15-mode table (Aarambam/defaults/AlphaTable.dat)",
     "le": "local+equilateral closed-form table, 12 modes",
     "local": "local closed-form table, 8 modes",
+    "equil": "equilateral closed-form table, 12 modes",
 }
 
 
-def workload(N=512, box=2000.0, template="scalari"):
+def workload(N=512, box=2000.0, template="scalari", seed=1234, fnl=100.0):
     from tests import cases
-    nm = {"scalari": 15, "le": 12, "local": 8}[template]
+    nm = {"scalari": 15, "le": 12, "local": 8, "equil": 12}[template]
     cfg = cases.base_config(Nmesh=N, Nsample=N, Nmax=N, N_modes=nm, Box=box, GlassTileFac=max(1, N // 64),
-                            Fnl=100.0, Seed=1234)
+                            Fnl=fnl, Seed=seed)
     x0 = cases.x0_closed_form(nm, cfg["PrimordialIndex"], box, N)
     if template == "scalari":
         alpha = load_alpha(os.path.join(cases.GOLDEN, "AlphaTable.dat"), 15)
     else:
-        rows = list(cases.local_rows())
-        if template == "le":
+        rows = []
+        if template in ("local", "le"):
+            rows += list(cases.local_rows())
+        if template in ("equil", "le"):
             rows += list(cases.equilateral_rows())
         merged = {}
         for n1, n2, n3, a in rows:  # a sum of templates adds coefficients of the same triple
@@ -110,6 +113,91 @@ def time_job(aar, ctx, cfg, alpha, x0, glass, gbox, prec, sched, steps, warmup, 
     return info
 
 
+def serving(aar, ctx, n_jobs=64, load=0.75, budget_gib=96, max_active=4, seed=2024):
+    """Poisson stream of IC jobs through the continuous-batching server.
+
+    Nmesh uniform over {128, 256, 512}, template uniform over {local,
+    equilateral, ScalarI mass=1}, seeds and fNL ~ U[-200, 200] from a fixed-seed
+    PCG64.  The arrival rate is `load` times the capacity measured first: the
+    inverse of the mean time of one job of each (Nmesh, template) run alone.
+    Latency = completion - submission of each job."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes, tpls = (128, 256, 512), ("local", "equil", "scalari")
+    cache = {}
+
+    def inputs(N, tpl, sd, fnl):
+        cfg, alpha, x0, glass, gbox = workload(N, 2000.0 * N / 512, tpl, sd, fnl)
+        key = (N, tpl)
+        if key not in cache:
+            cache[key] = aar.tables(alpha, x0, glass, gbox)
+        return aar.params_from_dict(cfg), cache[key]
+
+    srv = aar.Server(ctx, int(budget_gib) << 30, max_active)
+    alone = {}
+    for N in sizes:
+        for tpl in tpls:
+            p, t = inputs(N, tpl, 1, 100.0)
+            for rep in range(2):  # the first run of a shape warms its code paths
+                tk = srv.submit(p, t, aar.AAR_FP32, aar.AAR_SCHED_BATCHED)
+                lat, _ = srv.wait(tk)
+                srv.release(tk)
+            alone[f"{N}/{tpl}"] = lat
+    mean_service = float(np.mean(list(alone.values())))
+    rate = load / mean_service
+    specs = [(int(rng.choice(sizes)), str(rng.choice(tpls)), int(rng.integers(1, 2 ** 31 - 1)),
+              float(rng.uniform(-200, 200))) for _ in range(n_jobs)]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    jobs = [inputs(*sp) for sp in specs]
+    tickets = []
+    t0 = time.perf_counter()
+    for (p, t), ta in zip(jobs, arrivals):
+        dt = t0 + ta - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        tickets.append(srv.submit(p, t, aar.AAR_FP32, aar.AAR_SCHED_BATCHED))
+    lats = []
+    for tk in tickets:
+        lats.append(srv.wait(tk)[0])
+        srv.release(tk)
+    span = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lats = np.array(lats) * 1e3
+    cells = sum(sp[0] ** 3 for sp in specs)
+    return {"jobs": n_jobs, "p50_ms": round(float(np.percentile(lats, 50)), 3),
+            "p99_ms": round(float(np.percentile(lats, 99)), 3), "mean_ms": round(float(lats.mean()), 3),
+            "alone_ms": {k: round(v * 1e3, 3) for k, v in alone.items()},
+            "arrival_rate_per_s": round(rate, 2), "load": load,
+            "throughput_mcells_s": round(cells / span / 1e6, 1), "span_s": round(span, 3),
+            "max_active": max_active, "peak_active": st["peak_active"], "hbm_budget_gib": budget_gib,
+            "peak_arena_gib": round(st["peak_bytes"] / 2 ** 30, 2),
+            "mix": "Nmesh U{128,256,512}, template U{local, equilateral, ScalarI}, seed and fNL U[-200,200] "
+                   "from PCG64(2024), FP32 batched, Lbox = 2000 Mpc/h * Nmesh/512"}
+
+
+def copy_peak(ctx, N=512, nf=8, iters=10):
+    """HBM bandwidth of a plain streaming copy kernel (reads + writes nf spectra)."""
+    sec = ctx.bench_pass(N, 4, nf, iters)
+    S = N * N * (N // 2 + 1) * 8
+    return 2 * nf * S / sec / 1e9
+
+
+def profiled_traffic(kernel, N, template):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (counters scaled by the same counters on a copy kernel of known size); None when not profiled."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(f) >= 4 and f[0] == kernel and f[1] == str(N) and f[2] == template:
+            try:
+                return float(f[3])
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(N=128):
     from oracle import pyoracle as orc
     cfg, alpha, x0, glass, gbox = workload(N, 2000.0 * N / 512)
@@ -135,6 +223,8 @@ def main():
     ap.add_argument("--template", default="scalari", choices=sorted(TEMPLATES))
     ap.add_argument("--box", type=float, default=2000.0)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs one IC job per GPU on a single GPU; multi-GPU runs are not "
@@ -159,7 +249,7 @@ def main():
     achieved = d["bytes"] / d["seconds"] / 1e9
     M, K = modal_sizes(alpha)
     out = {
-        "metric": "Mcells/s for NG-2LPT IC at Nmesh=512 (FP32 batched; FP64 pairwise beside it)",
+        "metric": "Mcells/s for NG-2LPT IC at Nmesh=512, FP32 vs FP64; p50/p99 job latency",
         "value": round(value, 3), "unit": "Mcells/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Gaussian field, lattice glass)",
@@ -167,7 +257,10 @@ def main():
                                f"{TEMPLATES[args.template]} (M={M}, K={K}), EH P(k), fNL=100, seed 1234",
                    "precision": "fp32", "schedule": "batched"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(dom, N, args.template),
+                     "algorithmic_bytes_per_launch": round(d["bytes"] / d["launches"]),
+                     "avg_launch_us": round(d["seconds"] / d["launches"] * 1e6, 2)},
         "kernel_time_share": {k: round(v["seconds"] / r32["device"] * args.steps / args.steps, 4)
                               for k, v in stats.items()},
         "job_roofline": {"bytes_per_job": survey_bytes(N, M, K, 4),
@@ -183,6 +276,9 @@ def main():
         out["fp64_plain"] = {"value": round(v64, 3), "unit": "Mcells/s",
                              "ms_per_step": round(r64["wall"] / args.fp64_steps * 1e3, 3),
                              "fp32_over_fp64": round(value / v64, 3)}
+    out["copy_peak_gbs"] = round(copy_peak(ctx, min(N, 512)), 1)
+    if not args.no_serve:
+        out["serving"] = serving(aar, ctx, args.serve_jobs)
     ctx.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)

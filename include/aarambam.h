@@ -127,6 +127,7 @@ int aar_job_prepare(aar_ctx *ctx, const aar_params *p, const aar_tables *t, int 
                     int schedule, aar_job **out);
 /* enqueue the whole job on `stream` (a hipStream_t, NULL = the context stream) */
 int aar_job_run(aar_job *job, void *stream);
+/* wait for the job's last run (returns at once if it never ran) */
 int aar_job_sync(aar_job *job);
 /* device pointers of the result arrays (npart*3 floats each), valid until destroy */
 int aar_job_device_output(aar_job *job, float **pos, float **vel, long long *npart);
@@ -148,6 +149,34 @@ int aar_job_set_graph(aar_job *job, int on);
 int aar_job_kernel_stats(aar_job *job, int kind, int *launches, double *seconds, double *bytes);
 int aar_job_destroy(aar_job *job);
 
+/* ---- serving: continuous batching of IC jobs on one device ----
+ * No counterpart in the reference, which runs one IC per process
+ * (main.c:53-102); every job keeps the semantics of aar_job_prepare/run.
+ * A server allocates one HBM arena of hbm_budget bytes once and runs up to
+ * max_active (1..16) jobs at a time, each on its own stream, carved from the
+ * arena by footprint (aar_job_bytes).  Jobs are admitted in submission order
+ * as slots and arena space free up, so jobs submitted while others run
+ * overlap with them on the device.  Results are bit-identical to the same
+ * job run alone. */
+typedef struct aar_server aar_server;
+int aar_server_create(aar_ctx *ctx, long long hbm_budget, int max_active, aar_server **out);
+/* Queue a job (tables are copied).  keep = 0: its arena piece is returned as
+ * soon as it completes and results are dropped; keep = 1: results stay on the
+ * device until aar_server_release.  AAR_ERR_HBM_BUDGET if the job can never fit. */
+int aar_server_submit(aar_server *s, const aar_params *p, const aar_tables *t, int precision, int schedule,
+                      int keep, long long *ticket);
+/* Block until the job has completed; latency_s = completion - submission,
+ * queue_s = admission - submission.  Returns the job's status. */
+int aar_server_wait(aar_server *s, long long ticket, double *latency_s, double *queue_s);
+int aar_server_fetch(aar_server *s, long long ticket, float *pos, float *vel);
+int aar_server_device_output(aar_server *s, long long ticket, float **pos, float **vel, long long *npart);
+/* Forget a completed job and return a kept job's arena piece. */
+int aar_server_release(aar_server *s, long long ticket);
+int aar_server_stats(aar_server *s, long long *submitted, long long *completed, int *peak_active,
+                     long long *peak_bytes);
+/* Cancels jobs not yet admitted, waits for running ones, frees the arena. */
+int aar_server_destroy(aar_server *s);
+
 /* ---- building blocks (tests, tools) ---- */
 /* 3-D transforms with FFTW 2 rfftwnd conventions on a host array in the
  * reference layout: dir=+1 C2R (in: N*N*(N/2+1) complex, out: N^3 real),
@@ -157,7 +186,7 @@ int aar_fft3d(aar_ctx *ctx, int n, int precision, int dir, const double *in, dou
 int aar_gaussian_field(aar_ctx *ctx, const aar_params *p, const aar_tables *t, int precision,
                        double *out);
 /* Mean seconds per launch of one FFT pass (kind 0 x, 1 y, 2 z-C2R, 3 z-R2C)
- * or of a device-to-device copy (kind 4) over nf fields of an N^3 grid. */
+ * or of a plain streaming copy kernel (kind 4, reads and writes nf spectra) over nf fields of an N^3 grid. */
 int aar_bench_pass(aar_ctx *ctx, int n, int precision, int kind, int nf, int iters, double *sec);
 /* Device ranlxd draws for tests: `n` uniforms of stream `seed`. */
 int aar_ranlxd_device(aar_ctx *ctx, uint32_t seed, int n, double *out);
