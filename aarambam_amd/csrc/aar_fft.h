@@ -156,8 +156,30 @@ template <typename T, int L, int C, bool RM> struct TileIdx {
   }
 };
 
+// Row-major tiles in which every stage has radix E (one butterfly per thread,
+// thread t on column t / (L/E)) and L/E divides 64: each column belongs to one
+// wavefront at every stage, so the exchanges between stages stay inside a
+// wavefront, whose LDS operations complete in issue order, and a
+// wavefront-scope fence replaces the workgroup barrier.
+__host__ __device__ constexpr bool wave_local(int L, int E) {
+  if (E > 16) return false;
+  for (int s = 0; s < plan_nstages(L, E); ++s)
+    if (plan_radix(L, E, s) != E) return false;
+  return L / E <= 64 && 64 % (L / E) == 0;
+}
+
+template <bool WL> __device__ __forceinline__ void tile_sync() {
+  if constexpr (WL) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // tw: LDS table with tw[m*TWS] = exp(-2 pi i m / L) for m in [0, L).
-template <typename T, int L, int C, int NT, int DIR, int TWS, int S, bool SRC_LDS, bool RM, class LoadF,
+template <typename T, int L, int C, int NT, int DIR, int TWS, int S, bool SRC_LDS, bool RM, bool WL, class LoadF,
           class StoreF>
 __device__ __forceinline__ void fft_stage(cx_t<T>* tile, const cx_t<T>* tw, LoadF& load, StoreF& store) {
   using C2 = cx_t<T>;
@@ -171,6 +193,7 @@ __device__ __forceinline__ void fft_stage(cx_t<T>* tile, const cx_t<T>* tw, Load
   constexpr int STR = L / R;
   constexpr int ROWS = NT / C;
   static_assert(RM || NT % C == 0, "column-major tiles need NT to be a multiple of C");
+  static_assert(!WL || (RM && wave_local(L, E)), "wavefront-local exchange needs whole columns per wavefront");
   static_assert(E % R == 0, "per-thread elements must be a multiple of the radix");
   auto bfly = [](int q, int& i, int& col) {
     if constexpr (RM) {
@@ -194,7 +217,7 @@ __device__ __forceinline__ void fft_stage(cx_t<T>* tile, const cx_t<T>* tw, Load
       else v[q * R + t] = tile[TI::at(pos, col)];
     }
   }
-  if constexpr (S > 0 || SRC_LDS) __syncthreads();
+  if constexpr (S > 0 || SRC_LDS) tile_sync<WL>();
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
     int i, col;
@@ -217,16 +240,16 @@ __device__ __forceinline__ void fft_stage(cx_t<T>* tile, const cx_t<T>* tw, Load
     }
   }
   if constexpr (S < NS - 1) {
-    __syncthreads();
-    fft_stage<T, L, C, NT, DIR, TWS, S + 1, SRC_LDS, RM, LoadF, StoreF>(tile, tw, load, store);
+    tile_sync<WL>();
+    fft_stage<T, L, C, NT, DIR, TWS, S + 1, SRC_LDS, RM, WL, LoadF, StoreF>(tile, tw, load, store);
   }
 }
 
-template <typename T, int L, int C, int NT, int DIR, int TWS, bool SRC_LDS, bool RM = false, class LoadF,
-          class StoreF>
+template <typename T, int L, int C, int NT, int DIR, int TWS, bool SRC_LDS, bool RM = false, bool WL = false,
+          class LoadF, class StoreF>
 __device__ __forceinline__ void fft_tile(cx_t<T>* tile, const cx_t<T>* tw, LoadF load, StoreF store) {
-  __syncthreads();  // the tile may still be read by the previous transform
-  fft_stage<T, L, C, NT, DIR, TWS, 0, SRC_LDS, RM, LoadF, StoreF>(tile, tw, load, store);
+  tile_sync<WL>();  // the tile may still be read by the previous transform
+  fft_stage<T, L, C, NT, DIR, TWS, 0, SRC_LDS, RM, WL, LoadF, StoreF>(tile, tw, load, store);
 }
 
 // ---------------------------------------------------------------- split-exchange variant

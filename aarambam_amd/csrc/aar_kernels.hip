@@ -269,29 +269,37 @@ static int launch_z(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int di
 // over the first K field slots.  Replaces z-C2R of M fields + the product
 // kernel + z-R2C of K fields ((M + K) spectra of HBM traffic instead of
 // 3(M + K)).
-template <typename T, int H> struct ZNGCfg {
-  static constexpr int E = sizeof(T) == 4 ? 4 : 2;  // complex elements per thread
-  static constexpr int C = sizeof(T) == 4 && H < 512 ? 4 : 2;  // rows per block
+template <typename T, int H, int MMAX> struct ZNGCfg {
+  // complex elements per thread: 8 keeps the 1024-point rows in one
+  // wavefront each when at most 4 mode fields sit in registers
+  static constexpr int E = sizeof(T) == 4 ? (H == 512 && MMAX <= 4 ? 8 : 4) : 2;
+  static constexpr int C = sizeof(T) == 4 && (H < 512 || E == 8) ? 4 : 2;  // rows per block
   static constexpr int NT = C * H / E;
+  // every row's exchanges inside one wavefront: no workgroup barriers
+  static constexpr bool WL = wave_local(H, E) && NT % 64 == 0;
 };
 
 template <typename T, int H, int MMAX>
-__global__ void __launch_bounds__((ZNGCfg<T, H>::NT))
+__global__ void __launch_bounds__((ZNGCfg<T, H, MMAX>::NT))
     k_z_ng(cx_t<T>* base, int M, int K, const NgTerm* __restrict__ terms, const int* __restrict__ gs, Geom g,
            const cx_t<T>* __restrict__ twg) {
   using C2 = cx_t<T>;
-  constexpr int C = ZNGCfg<T, H>::C, NT = ZNGCfg<T, H>::NT, E = ZNGCfg<T, H>::E;
+  using Cfg = ZNGCfg<T, H, MMAX>;
+  constexpr int C = Cfg::C, NT = Cfg::NT, E = Cfg::E;
+  constexpr bool WL = Cfg::WL;
   constexpr int N = 2 * H;
   using TI = TileIdx<T, H, C, true>;
   __shared__ C2 tile[TI::SIZE];
   __shared__ C2 tw[N];
   load_twiddles(tw, twg + N, N, NT);
+  __syncthreads();
   const long long r0 = (long long)blockIdx.x * C;
   // Real-space values of the mode fields: (x[2 pos], x[2 pos + 1]) of slot s
   // in fx<s>[m], fy<s>[m].  One small array per slot and component, each
   // indexed by the uniform mode number, stays in VGPRs (indirect addressing).
-  static_assert(E == 2 || E == 4, "two or four slots per thread");
+  static_assert(E == 2 || E == 4 || E == 8, "two, four or eight slots per thread");
   T fx0[MMAX], fx1[MMAX], fx2[MMAX], fx3[MMAX], fy0[MMAX], fy1[MMAX], fy2[MMAX], fy3[MMAX];
+  T fx4[MMAX], fx5[MMAX], fx6[MMAX], fx7[MMAX], fy4[MMAX], fy5[MMAX], fy6[MMAX], fy7[MMAX];
   // Raw inputs of the first FFT stage (X[k], X[H-k] per slot, X[H] for k = 0)
   // are prefetched one field ahead, so HBM loads overlap the previous field's
   // transform.
@@ -322,12 +330,18 @@ __global__ void __launch_bounds__((ZNGCfg<T, H>::NT))
     auto store = [&](int, int, C2 v, int e) {
       if (e == 0) { fx0[m] = v.x; fy0[m] = v.y; }
       if (e == 1) { fx1[m] = v.x; fy1[m] = v.y; }
-      if constexpr (E == 4) {
+      if constexpr (E >= 4) {
         if (e == 2) { fx2[m] = v.x; fy2[m] = v.y; }
         if (e == 3) { fx3[m] = v.x; fy3[m] = v.y; }
       }
+      if constexpr (E == 8) {
+        if (e == 4) { fx4[m] = v.x; fy4[m] = v.y; }
+        if (e == 5) { fx5[m] = v.x; fy5[m] = v.y; }
+        if (e == 6) { fx6[m] = v.x; fy6[m] = v.y; }
+        if (e == 7) { fx7[m] = v.x; fy7[m] = v.y; }
+      }
     };
-    fft_tile<T, H, C, NT, +1, 2, false, true>(tile, tw, load, store);
+    fft_tile<T, H, C, NT, +1, 2, false, true, WL>(tile, tw, load, store);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       ra[e] = na[e];
@@ -342,18 +356,32 @@ __global__ void __launch_bounds__((ZNGCfg<T, H>::NT))
       const NgTerm tm = terms[t];
       const T cf = (T)tm.c;
       const int ti = tm.i, tj = tm.j;
-      hv[0].x += cf * (fx0[ti] * fx0[tj]);
-      hv[0].y += cf * (fy0[ti] * fy0[tj]);
-      hv[1].x += cf * (fx1[ti] * fx1[tj]);
-      hv[1].y += cf * (fy1[ti] * fy1[tj]);
-      if constexpr (E == 4) {
-        hv[2].x += cf * (fx2[ti] * fx2[tj]);
-        hv[2].y += cf * (fy2[ti] * fy2[tj]);
-        hv[3].x += cf * (fx3[ti] * fx3[tj]);
-        hv[3].y += cf * (fy3[ti] * fy3[tj]);
+      // all reads with index ti, then all with tj: runs of one register index
+      C2 a[E], b[E];
+      auto get = [&](int m, C2* v) {
+        v[0] = cmake<C2>(fx0[m], fy0[m]);
+        v[1] = cmake<C2>(fx1[m], fy1[m]);
+        if constexpr (E >= 4) {
+          v[2] = cmake<C2>(fx2[m], fy2[m]);
+          v[3] = cmake<C2>(fx3[m], fy3[m]);
+        }
+        if constexpr (E == 8) {
+          v[4] = cmake<C2>(fx4[m], fy4[m]);
+          v[5] = cmake<C2>(fx5[m], fy5[m]);
+          v[6] = cmake<C2>(fx6[m], fy6[m]);
+          v[7] = cmake<C2>(fx7[m], fy7[m]);
+        }
+      };
+      get(ti, a);
+      get(tj, b);
+      __builtin_amdgcn_sched_barrier(0);  // keep each run of indexed reads together
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        hv[e].x = fma(cf, a[e].x * b[e].x, hv[e].x);
+        hv[e].y = fma(cf, a[e].y * b[e].y, hv[e].y);
       }
     }
-    __syncthreads();  // the tile is free
+    tile_sync<WL>();  // the tile is free
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       int pos, col;
@@ -362,48 +390,59 @@ __global__ void __launch_bounds__((ZNGCfg<T, H>::NT))
     }
     auto load = [&](int pos, int c) { return tile[TI::at(pos, c)]; };
     auto store = [&](int pos, int c, C2 v) { tile[TI::at(pos, c)] = v; };
-    fft_tile<T, H, C, NT, -1, 2, true, true>(tile, tw, load, store);
-    __syncthreads();
+    fft_tile<T, H, C, NT, -1, 2, true, true, WL>(tile, tw, load, store);
+    tile_sync<WL>();
     C2* fo = base + (long long)kk * g.fs;
     C2* rows = fo + r0 * H;
-    constexpr int HP = H / 2 + 1;
-    for (int e = threadIdx.x; e < C * HP; e += NT) {
-      const int c = e / HP, k = e % HP;
+    // Post-processing units per row: k in [1, H/2) writes bins k and H - k;
+    // unit 0 writes bins 0, H/2 and the Nyquist plane.
+    constexpr int U = H / 2;
+    // rows handled by this thread's group: the whole block, or with
+    // wavefront-local exchanges only the rows its own wavefront owns
+    constexpr int GT = WL ? 64 : NT, GC = WL ? C * 64 / NT : C;
+    const int gt = WL ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+    const int gc0 = WL ? (int)(threadIdx.x >> 6) * GC : 0;
+    auto bin = [&](int k, const C2 A, const C2 B) {  // X_k from Z_k = A and Z_{H-k} = B
+      const C2 Ev = cscale(cadd(A, cconj(B)), (T)0.5);
+      const C2 dd = csub(A, cconj(B));
+      const C2 O = cmake<C2>(dd.y * (T)0.5, -dd.x * (T)0.5);
+      return cadd(Ev, cmul(O, tw[k]));
+    };
+    for (int e = gt; e < GC * U; e += GT) {
+      const int c = gc0 + e / U, k = e % U;
       C2* out = rows + (long long)c * H;
       if (k == 0) {
         const C2 z = tile[TI::at(0, c)];
         out[0] = cmake<C2>(z.x + z.y, (T)0);
         fo[g.nq + r0 + c] = cmake<C2>(z.x - z.y, (T)0);
+        const C2 m = tile[TI::at(U, c)];
+        out[U] = bin(U, m, m);
       } else {
         const int k2 = H - k;
         const C2 A = tile[TI::at(k, c)], B = tile[TI::at(k2, c)];
-        const C2 Ev = cscale(cadd(A, cconj(B)), (T)0.5);
-        const C2 dd = csub(A, cconj(B));
-        const C2 O = cmake<C2>(dd.y * (T)0.5, -dd.x * (T)0.5);
-        out[k] = cadd(Ev, cmul(O, tw[k]));
-        if (k2 != k) {
-          const C2 E2 = cscale(cadd(B, cconj(A)), (T)0.5);
-          const C2 dd2 = csub(B, cconj(A));
-          const C2 O2 = cmake<C2>(dd2.y * (T)0.5, -dd2.x * (T)0.5);
-          out[k2] = cadd(E2, cmul(O2, tw[k2]));
-        }
+        out[k] = bin(k, A, B);
+        out[k2] = bin(k2, B, A);
       }
     }
   }
 }
 
+template <typename T, int H, int MMAX>
+static int launch_z_ng_m(cx_t<T>* base, int M, int K, const NgTerm* terms, const int* gs, const Geom& g,
+                         const cx_t<T>* tw, hipStream_t st) {
+  constexpr int C = ZNGCfg<T, H, MMAX>::C, NT = ZNGCfg<T, H, MMAX>::NT;
+  dim3 grid((unsigned)((long long)g.N * g.N / C));
+  hipLaunchKernelGGL((k_z_ng<T, H, MMAX>), grid, dim3(NT), 0, st, base, M, K, terms, gs, g, tw);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
 template <typename T, int H>
 static int launch_z_ng(cx_t<T>* base, int M, int K, const NgTerm* terms, const int* gs, const Geom& g,
                        const cx_t<T>* tw, hipStream_t st) {
-  constexpr int C = ZNGCfg<T, H>::C, NT = ZNGCfg<T, H>::NT;
-  dim3 grid((unsigned)((long long)g.N * g.N / C));
-  if (M <= 8)
-    hipLaunchKernelGGL((k_z_ng<T, H, 8>), grid, dim3(NT), 0, st, base, M, K, terms, gs, g, tw);
-  else if (M <= 16)
-    hipLaunchKernelGGL((k_z_ng<T, H, 16>), grid, dim3(NT), 0, st, base, M, K, terms, gs, g, tw);
-  else
-    return AAR_ERR_UNSUPPORTED;
-  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+  if (M <= 4) return launch_z_ng_m<T, H, 4>(base, M, K, terms, gs, g, tw, st);
+  if (M <= 8) return launch_z_ng_m<T, H, 8>(base, M, K, terms, gs, g, tw, st);
+  if (M <= 16) return launch_z_ng_m<T, H, 16>(base, M, K, terms, gs, g, tw, st);
+  return AAR_ERR_UNSUPPORTED;
 }
 
 template <typename T>
