@@ -50,7 +50,8 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_load_alpha", "aar_load_x
            "aar_job_fetch_field", "aar_job_last_seconds", "aar_job_destroy", "aar_fft3d",
            "aar_gaussian_field", "aar_ranlxd_device", "aar_bench_pass", "aar_job_set_profiling", "aar_job_kernel_stats", "aar_job_set_graph",
            "aar_server_create", "aar_server_submit", "aar_server_wait", "aar_server_fetch",
-           "aar_server_device_output", "aar_server_release", "aar_server_stats", "aar_server_destroy"]
+           "aar_server_device_output", "aar_server_release", "aar_server_stats", "aar_server_destroy",
+           "aar_autotune", "aar_xy_tile_variant", "aar_set_xy_tile_variant"]
 KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
 
 
@@ -114,6 +115,9 @@ def lib():
             "aar_server_stats": (C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                            C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
             "aar_server_destroy": (C.c_int, [vp]),
+            "aar_autotune": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, C.POINTER(C.c_int)]),
+            "aar_xy_tile_variant": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+            "aar_set_xy_tile_variant": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -244,6 +248,13 @@ class Context:
         s = C.c_double(0)
         _check(lib().aar_bench_pass(self.h, n, precision, kind, nf, iters, C.byref(s)), "aar_bench_pass")
         return s.value
+
+    def autotune(self, n, precision=AAR_FP32, nf=8):
+        """Time the x / y tile shapes at n^3 and keep the fastest; returns (seconds[2][3], chosen[2])."""
+        sec = np.zeros(6, np.float64)
+        ch = (C.c_int * 2)()
+        _check(lib().aar_autotune(self.h, n, precision, nf, _dptr(sec), ch), "aar_autotune")
+        return sec.reshape(2, 3), [ch[0], ch[1]]
 
     def ranlxd(self, seed, n):
         out = np.empty(n, np.float64)

@@ -110,6 +110,11 @@ struct aar_job {
 namespace {
 
 bool pow2(int n) { return n >= 16 && n <= 1024 && (n & (n - 1)) == 0; }
+int ilog2i(int n) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
 
 // Build the regrouped term list and the pairwise schedule from the alpha table
 // (main.c:564-1063): main triples i <= j <= K with alpha != 0, plus the two
@@ -466,6 +471,81 @@ extern "C" int aar_ctx_destroy(aar_ctx* c) {
   return AAR_OK;
 }
 
+// ------------------------------------------------------------------ autotuner
+// Times every tile variant of the x and y passes on nf fields (alternating
+// inverse and forward launches) and keeps the fastest per axis in the
+// variant table.  The variants share the radix plan, so the choice changes
+// speed only, never results.
+static std::mutex g_tune_mu;
+static bool g_tuned[2][11] = {};
+
+template <typename T>
+static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, double* sec, int* chosen) {
+  using C2 = cx_t<T>;
+  const Geom g = make_geom(N);
+  C2* d = (C2*)scratch;
+  if (!d && device_alloc((void**)&d, (size_t)nf * g.fs * sizeof(C2))) return AAR_ERR_ALLOC;
+  int rc = aar_zero_fill((double*)d, (long long)nf * g.fs * (long long)(sizeof(C2) / 8), ctx->stream);
+  const C2* tw = sizeof(T) == 4 ? (const C2*)ctx->tw32 : (const C2*)ctx->tw64;
+  hipEvent_t a = nullptr, b = nullptr;
+  hipEventCreate(&a);
+  hipEventCreate(&b);
+  const int iters = 4;
+  for (int axis = 0; axis < 2 && !rc; ++axis) {
+    double best = 1e30;
+    int bv = 0;
+    for (int v = 0; v < kXYVariantCount && !rc; ++v) {
+      double t = 1e30;
+      int r = Pipeline<T>::xy_pass(d, nf, g, tw, axis, +1, ctx->stream, v);
+      if (r == AAR_OK) {
+        hipEventRecord(a, ctx->stream);
+        for (int q = 0; q < iters && !r; ++q) r = Pipeline<T>::xy_pass(d, nf, g, tw, axis, q % 2 ? -1 : +1, ctx->stream, v);
+        hipEventRecord(b, ctx->stream);
+        if (hipEventSynchronize(b) != hipSuccess) r = AAR_ERR_HIP;
+        float ms = 0;
+        if (!r && hipEventElapsedTime(&ms, a, b) == hipSuccess) t = ms * 1e-3 / iters;
+      }
+      if (r && r != AAR_ERR_UNSUPPORTED) rc = r;
+      if (sec) sec[axis * kXYVariantCount + v] = t < 1e30 ? t : -1.0;
+      if (t < best) {
+        best = t;
+        bv = v;
+      }
+    }
+    if (!rc) aar_set_xy_variant(sizeof(T) == 8, N, axis, bv);
+    if (chosen) chosen[axis] = bv;
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (!scratch) hipFree(d);
+  if (!rc) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[sizeof(T) == 8][ilog2i(N)] = true;
+  }
+  return rc;
+}
+
+extern "C" int aar_autotune(aar_ctx* ctx, int n, int precision, int nf, double* seconds, int* variant) {
+  if (!ctx || !pow2(n) || nf < 1 || (precision != AAR_FP32 && precision != AAR_FP64)) return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(ctx->device));
+  return precision == AAR_FP32 ? autotune_t<float>(ctx, n, nf, nullptr, seconds, variant)
+                               : autotune_t<double>(ctx, n, nf, nullptr, seconds, variant);
+}
+
+extern "C" int aar_set_xy_tile_variant(int n, int precision, int axis, int variant) {
+  if (!pow2(n) || axis < 0 || axis > 1 || variant < 0 || variant >= kXYVariantCount) return AAR_ERR_BAD_ARG;
+  if (precision != AAR_FP32 && precision != AAR_FP64) return AAR_ERR_BAD_ARG;
+  aar_set_xy_variant(precision == AAR_FP64, n, axis, variant);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned[precision == AAR_FP64][ilog2i(n)] = true;  // a pinned choice is not re-tuned
+  return AAR_OK;
+}
+
+extern "C" int aar_xy_tile_variant(int n, int precision, int axis) {
+  if (!pow2(n) || axis < 0 || axis > 1) return -1;
+  return aar_xy_variant(precision == AAR_FP64, n, axis);
+}
+
 // ------------------------------------------------------------------ jobs
 // aar_cosmology through the context's cache.  The key holds every parameter
 // the cosmology code reads (and a few more) plus the transfer table rows.
@@ -618,6 +698,17 @@ int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, i
   if ((rc = job_alloc(j, &j->pool, (size_t)j->nslots * j->g.fs * j->cxsize))) return fail(rc);
   if (schedule == AAR_SCHED_PLAIN && (rc = job_alloc(j, (void**)&j->acc, (size_t)j->g.fs * 16)))
     return fail(rc);
+  if (!ws && N >= 256) {  // first job of this size and precision: tune the x / y tiles on its pool
+    bool tuned;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      tuned = g_tuned[precision == AAR_FP64][ilog2i(N)];
+    }
+    const int nf = std::min(j->nslots, 8);
+    if (!tuned && (rc = precision == AAR_FP32 ? autotune_t<float>(ctx, N, nf, j->pool, nullptr, nullptr)
+                                              : autotune_t<double>(ctx, N, nf, j->pool, nullptr, nullptr)))
+      return fail(rc);
+  }
   std::vector<uint32_t> seeds((size_t)N * N);
   aar_seedtable(N, p->Seed, seeds.data());
   if ((rc = job_upload(j, &j->seeds, seeds))) return fail(rc);

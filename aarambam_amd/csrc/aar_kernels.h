@@ -42,8 +42,9 @@ template <typename T> struct Pipeline {
   // FFT passes on nf consecutive fields starting at f (stride g.fs)
   static int c2r(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st);
   static int r2c(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st);
+  // variant < 0: the tile shape from the variant table (aar_set_xy_variant)
   static int xy_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int axis, int dir,
-                     hipStream_t st);
+                     hipStream_t st, int variant = -1);
   static int z_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int dir, hipStream_t st);
 
   // fused z pass of the NG sum: M mode fields (x, y already inverse transformed)
@@ -86,6 +87,10 @@ template <typename T> struct Pipeline {
 // zero n doubles (graph-capturable; no memset nodes)
 int aar_zero_fill(double* p, long long n, hipStream_t st);
 int aar_copy16(const void* src, void* dst, long long bytes, hipStream_t st);
+// x / y pass tile-shape variants (0 .. kXYVariantCount-1) per precision, mesh size and axis
+constexpr int kXYVariantCount = 3;
+int aar_xy_variant(int fp64, int n, int axis);
+void aar_set_xy_variant(int fp64, int n, int axis, int v);
 // n uniforms of one ranlxd1 stream on the device (test hook)
 int aar_ranlxd_draws_dev(uint32_t seed, int n, double* out, hipStream_t st);
 

@@ -95,14 +95,38 @@ __host__ __device__ constexpr int clampi(int v, int lo, int hi) { return v < lo 
 __host__ __device__ constexpr int tile_threads(int CL) {
   return clampi(CL / 16, CL / 2 < 64 ? CL / 2 : 64, 1024);
 }
-// Tile width: 128-byte column segments, halved at L = 1024 so that two
-// tiles fit one CU's LDS (the XCD-aware block order below keeps the two
-// halves of each 128-byte line on one XCD's L2).
-template <typename T, int L> struct XYCfg {
-  static constexpr int C = Shape<T>::CMAX < L / 2 ? Shape<T>::CMAX : L / 2;
+// Tile shapes of the x / y passes, picked per precision, mesh size and axis
+// by the autotuner.  All keep 16 elements per thread, hence the same radix
+// plan and bit-identical results; they differ in columns per tile:
+//   V = 0: 128-byte column segments;
+//   V = 1: 64-byte segments (half the LDS and threads per tile, so more
+//          tiles per CU at L = 1024; the XCD-aware block order keeps both
+//          halves of each 128-byte line on one XCD's L2);
+//   V = 2: 256-byte segments (fewer, larger tiles; L <= 512).
+template <typename T, int L, int V = 0> struct XYCfg {
+  static constexpr int CW = V == 1 ? Shape<T>::CMAX / 2 : (V == 2 ? Shape<T>::CMAX * 2 : Shape<T>::CMAX);
+  static constexpr int C = CW < L / 2 ? CW : L / 2;
+  static constexpr bool VALID =
+      C * L / 16 <= 1024 && (long)L * (C + 1) * sizeof(T) + (long)L * sizeof(cx_t<T>) <= 152 * 1024;
   static constexpr int NT = tile_threads(C * L);
   static constexpr bool SPLIT = true;  // real-valued LDS exchange tile
+  // minimum wavefronts per SIMD the register allocation must allow: 64-byte
+  // tiles at L = 1024 fit three per CU by LDS when they stay within 80 VGPRs
+  static constexpr int WPE = (sizeof(T) == 4 && L == 1024 && V == 1) ? 6 : 1;
 };
+
+// Variant table: [precision 0 FP32 / 1 FP64][log2 N][axis]
+static int g_xy_variant[2][11][2] = {};
+int aar_xy_variant(int fp64, int n, int axis) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return (l < 11 && axis >= 0 && axis < 2) ? g_xy_variant[fp64 ? 1 : 0][l][axis] : 0;
+}
+void aar_set_xy_variant(int fp64, int n, int axis, int v) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  if (l < 11 && axis >= 0 && axis < 2 && v >= 0 && v < kXYVariantCount) g_xy_variant[fp64 ? 1 : 0][l][axis] = v;
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; give
 // each XCD a contiguous range of tiles so neighbouring tiles share its L2.
@@ -111,19 +135,63 @@ __device__ __forceinline__ int xcd_tile(int b, int nblk) {
   return (b % 8) * (nblk / 8) + b / 8;
 }
 template <typename T, int H> struct ZCfg {
-  static constexpr int C = Shape<T>::CMAX;
-  static constexpr int NT = tile_threads(C * H);
+  // FP32 H = 512: 8 elements per thread (the radix-8 plan of 16 elements per
+  // thread, with one row per wavefront); otherwise 16 per thread
+  static constexpr bool E8 = sizeof(T) == 4 && H == 512;
+  static constexpr int C = E8 ? 8 : Shape<T>::CMAX;
+  static constexpr int NT = E8 ? C * H / 8 : tile_threads(C * H);
+  // every row's exchanges inside one wavefront: no workgroup barriers
+  static constexpr bool WL = wave_local(H, C * H / NT) && NT % 64 == 0;
 };
+
+// r2c post-processing of C transformed rows held in a row-major tile:
+// X_k = E_k + e^{-2 pi i k/N} O_k from Z_k and Z_{H-k} (E = (Z_k + Z*_{H-k})/2,
+// O = (Z_k - Z*_{H-k})/(2i)), stored along k (coalesced).  Unit k in [1, H/2)
+// writes bins k and H - k; unit 0 writes bins 0 and H/2 and the Nyquist plane
+// entry.  With wavefront-local tiles each wavefront handles its own rows.
+template <typename T, int H, int C, int NT, bool WL>
+__device__ __forceinline__ void r2c_post(const cx_t<T>* tile, const cx_t<T>* tw, cx_t<T>* f, long long r0,
+                                         const Geom& g) {
+  using C2 = cx_t<T>;
+  using TI = TileIdx<T, H, C, true>;
+  C2* rows = f + r0 * H;
+  constexpr int U = H / 2;
+  constexpr int GT = WL ? 64 : NT, GC = WL ? C * 64 / NT : C;
+  const int gt = WL ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+  const int gc0 = WL ? (int)(threadIdx.x >> 6) * GC : 0;
+  auto bin = [&](int k, const C2 A, const C2 B) {  // X_k from Z_k = A and Z_{H-k} = B
+    const C2 Ev = cscale(cadd(A, cconj(B)), (T)0.5);
+    const C2 dd = csub(A, cconj(B));
+    const C2 O = cmake<C2>(dd.y * (T)0.5, -dd.x * (T)0.5);
+    return cadd(Ev, cmul(O, tw[k]));
+  };
+  for (int e = gt; e < GC * U; e += GT) {
+    const int c = gc0 + e / U, k = e % U;
+    C2* out = rows + (long long)c * H;
+    if (k == 0) {
+      const C2 z = tile[TI::at(0, c)];
+      out[0] = cmake<C2>(z.x + z.y, (T)0);
+      f[g.nq + r0 + c] = cmake<C2>(z.x - z.y, (T)0);
+      const C2 m = tile[TI::at(U, c)];
+      out[U] = bin(U, m, m);
+    } else {
+      const int k2 = H - k;
+      const C2 A = tile[TI::at(k, c)], B = tile[TI::at(k2, c)];
+      out[k] = bin(k, A, B);
+      out[k2] = bin(k2, B, A);
+    }
+  }
+}
 
 // ------------------------------------------------------------------ x / y passes
 // One tile = C adjacent columns of one field.  Tiles [0, nmain) cover the
 // kz < N/2 block, the rest the Nyquist plane.
-template <typename T, int L, int AXIS, int DIR>
-__global__ void __launch_bounds__((XYCfg<T, L>::NT))
+template <typename T, int L, int AXIS, int DIR, int V>
+__global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_waves_per_eu(XYCfg<T, L, V>::WPE)))
     k_xy_pass(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg, int nmain) {
   using C2 = cx_t<T>;
-  constexpr int C = XYCfg<T, L>::C, NT = XYCfg<T, L>::NT;
-  constexpr bool SPLIT = XYCfg<T, L>::SPLIT;
+  constexpr int C = XYCfg<T, L, V>::C, NT = XYCfg<T, L, V>::NT;
+  constexpr bool SPLIT = XYCfg<T, L, V>::SPLIT;
   __shared__ T tile_r[SPLIT ? L * (C + 1) : 1];
   __shared__ C2 tile_c[SPLIT ? 1 : TileIdx<T, L, C, false>::SIZE];
   __shared__ C2 tw[L];
@@ -156,14 +224,32 @@ __global__ void __launch_bounds__((XYCfg<T, L>::NT))
   }
 }
 
+template <typename T, int L, int AXIS, int DIR, int V>
+static int launch_xy_v(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st) {
+  using Cfg = XYCfg<T, L, V>;
+  if constexpr (!Cfg::VALID) {
+    return AAR_ERR_UNSUPPORTED;
+  } else {
+    constexpr int C = Cfg::C, NT = Cfg::NT;
+    const int nmain = g.N * (g.H / C);
+    const int nnyq = g.N / C;
+    dim3 grid(nmain + nnyq, nf);
+    hipLaunchKernelGGL((k_xy_pass<T, L, AXIS, DIR, V>), grid, dim3(NT), 0, st, f, g, tw, nmain);
+    return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+  }
+}
+
+// Tile variants other than 0 are built for the sizes where the choice matters.
 template <typename T, int L, int AXIS, int DIR>
-static int launch_xy(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st) {
-  constexpr int C = XYCfg<T, L>::C, NT = XYCfg<T, L>::NT;
-  const int nmain = g.N * (g.H / C);
-  const int nnyq = g.N / C;
-  dim3 grid(nmain + nnyq, nf);
-  hipLaunchKernelGGL((k_xy_pass<T, L, AXIS, DIR>), grid, dim3(NT), 0, st, f, g, tw, nmain);
-  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+static int launch_xy(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, int v) {
+  if constexpr (L >= 256) {
+    switch (v) {
+      case 1: return launch_xy_v<T, L, AXIS, DIR, 1>(f, nf, g, tw, st);
+      case 2: return launch_xy_v<T, L, AXIS, DIR, 2>(f, nf, g, tw, st);
+      default: break;
+    }
+  }
+  return launch_xy_v<T, L, AXIS, DIR, 0>(f, nf, g, tw, st);
 }
 
 // ------------------------------------------------------------------ z passes
@@ -204,8 +290,9 @@ __global__ void __launch_bounds__((ZCfg<T, H>::NT))
     return cmake<C2>(s.x - dw.y, s.y + dw.x);
   };
   auto store = [&](int pos, int c, C2 v) { rows[(long long)c * H + pos] = v; };
-  // true: barrier after the first gather, since the transform is in place in HBM
-  fft_tile<T, H, C, NT, +1, 2, true, true>(tile, tw, load, store);
+  // true: barrier after the first gather, since the transform is in place in
+  // HBM (with wavefront-local rows, each output already depends on every load)
+  fft_tile<T, H, C, NT, +1, 2, true, true, ZCfg<T, H>::WL>(tile, tw, load, store);
 }
 
 template <typename T, int H>
@@ -218,37 +305,16 @@ __global__ void __launch_bounds__((ZCfg<T, H>::NT))
   __shared__ C2 tile[TI::SIZE];
   __shared__ C2 tw[N];
   load_twiddles(tw, twg + N, N, NT);
+  __syncthreads();
   C2* f = base + (long long)blockIdx.y * g.fs;
   const long long r0 = (long long)blockIdx.x * C;
   C2* rows = f + r0 * H;
   auto load = [&](int pos, int c) { return rows[(long long)c * H + pos]; };
   auto store = [&](int pos, int c, C2 v) { tile[TI::at(pos, c)] = v; };
-  fft_tile<T, H, C, NT, -1, 2, false, true>(tile, tw, load, store);
-  __syncthreads();
-  constexpr int HP = H / 2 + 1;
-  for (int e = threadIdx.x; e < C * HP; e += NT) {  // threads along k: coalesced stores
-    const int c = e / HP, k = e % HP;
-    C2* out = rows + (long long)c * H;
-    if (k == 0) {
-      const C2 z = tile[TI::at(0, c)];
-      out[0] = cmake<C2>(z.x + z.y, (T)0);
-      f[g.nq + r0 + c] = cmake<C2>(z.x - z.y, (T)0);
-    } else {
-      const int k2 = H - k;
-      const C2 A = tile[TI::at(k, c)], B = tile[TI::at(k2, c)];
-      // E = (A + B*)/2, O = (A - B*)/(2i), X_k = E + W^k O, W = e^{-2 pi i/N}
-      const C2 E = cscale(cadd(A, cconj(B)), (T)0.5);
-      const C2 dd = csub(A, cconj(B));
-      const C2 O = cmake<C2>(dd.y * (T)0.5, -dd.x * (T)0.5);
-      out[k] = cadd(E, cmul(O, tw[k]));
-      if (k2 != k) {
-        const C2 E2 = cscale(cadd(B, cconj(A)), (T)0.5);
-        const C2 dd2 = csub(B, cconj(A));
-        const C2 O2 = cmake<C2>(dd2.y * (T)0.5, -dd2.x * (T)0.5);
-        out[k2] = cadd(E2, cmul(O2, tw[k2]));
-      }
-    }
-  }
+  constexpr bool WL = ZCfg<T, H>::WL;
+  fft_tile<T, H, C, NT, -1, 2, false, true, WL>(tile, tw, load, store);
+  tile_sync<WL>();
+  r2c_post<T, H, C, NT, WL>(tile, tw, f, r0, g);
 }
 
 template <typename T, int H>
@@ -392,38 +458,7 @@ __global__ void __launch_bounds__((ZNGCfg<T, H, MMAX>::NT))
     auto store = [&](int pos, int c, C2 v) { tile[TI::at(pos, c)] = v; };
     fft_tile<T, H, C, NT, -1, 2, true, true, WL>(tile, tw, load, store);
     tile_sync<WL>();
-    C2* fo = base + (long long)kk * g.fs;
-    C2* rows = fo + r0 * H;
-    // Post-processing units per row: k in [1, H/2) writes bins k and H - k;
-    // unit 0 writes bins 0, H/2 and the Nyquist plane.
-    constexpr int U = H / 2;
-    // rows handled by this thread's group: the whole block, or with
-    // wavefront-local exchanges only the rows its own wavefront owns
-    constexpr int GT = WL ? 64 : NT, GC = WL ? C * 64 / NT : C;
-    const int gt = WL ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
-    const int gc0 = WL ? (int)(threadIdx.x >> 6) * GC : 0;
-    auto bin = [&](int k, const C2 A, const C2 B) {  // X_k from Z_k = A and Z_{H-k} = B
-      const C2 Ev = cscale(cadd(A, cconj(B)), (T)0.5);
-      const C2 dd = csub(A, cconj(B));
-      const C2 O = cmake<C2>(dd.y * (T)0.5, -dd.x * (T)0.5);
-      return cadd(Ev, cmul(O, tw[k]));
-    };
-    for (int e = gt; e < GC * U; e += GT) {
-      const int c = gc0 + e / U, k = e % U;
-      C2* out = rows + (long long)c * H;
-      if (k == 0) {
-        const C2 z = tile[TI::at(0, c)];
-        out[0] = cmake<C2>(z.x + z.y, (T)0);
-        fo[g.nq + r0 + c] = cmake<C2>(z.x - z.y, (T)0);
-        const C2 m = tile[TI::at(U, c)];
-        out[U] = bin(U, m, m);
-      } else {
-        const int k2 = H - k;
-        const C2 A = tile[TI::at(k, c)], B = tile[TI::at(k2, c)];
-        out[k] = bin(k, A, B);
-        out[k2] = bin(k2, B, A);
-      }
-    }
+    r2c_post<T, H, C, NT, WL>(tile, tw, base + (long long)kk * g.fs, r0, g);
   }
 }
 
@@ -461,13 +496,18 @@ int Pipeline<T>::z_ng(cx_t<T>* f, int M, int K, const NgTerm* terms, const int* 
 
 template <typename T>
 int Pipeline<T>::xy_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int axis, int dir,
-                         hipStream_t st) {
-#define XY_CASE(LL)                                                                     \
-  case LL:                                                                              \
-    if (axis == 0)                                                                      \
-      return dir > 0 ? launch_xy<T, LL, 0, 1>(f, nf, g, tw, st)                         \
-                     : launch_xy<T, LL, 0, -1>(f, nf, g, tw, st);                       \
-    return dir > 0 ? launch_xy<T, LL, 1, 1>(f, nf, g, tw, st) : launch_xy<T, LL, 1, -1>(f, nf, g, tw, st);
+                         hipStream_t st, int variant) {
+  const int v = variant >= 0 ? variant : aar_xy_variant(sizeof(T) == 8, g.N, axis);
+  if (variant < 0 && v != 0) {  // a pinned shape that is not built for this size runs as shape 0
+    const int rc = xy_pass(f, nf, g, tw, axis, dir, st, v);
+    return rc == AAR_ERR_UNSUPPORTED ? xy_pass(f, nf, g, tw, axis, dir, st, 0) : rc;
+  }
+#define XY_CASE(LL)                                                                           \
+  case LL:                                                                                    \
+    if (axis == 0)                                                                            \
+      return dir > 0 ? launch_xy<T, LL, 0, 1>(f, nf, g, tw, st, v)                            \
+                     : launch_xy<T, LL, 0, -1>(f, nf, g, tw, st, v);                          \
+    return dir > 0 ? launch_xy<T, LL, 1, 1>(f, nf, g, tw, st, v) : launch_xy<T, LL, 1, -1>(f, nf, g, tw, st, v);
   switch (g.N) {
     XY_CASE(16)
     XY_CASE(32)
@@ -517,70 +557,97 @@ int Pipeline<T>::r2c(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipSt
 // One thread per (i, j) pencil, main.c:382-524.  The stream of pencil (i, j)
 // is reseeded with seedtable[i N + j]; two uniforms (phase, amplitude) are
 // drawn for every k < N/2, also where the mode is cut or Nyquist.
+// A block is one wavefront of 64 pencils (consecutive memory rows).  Each
+// ranlxd update yields 6 modes per pencil; they go to a 14-mode LDS ring per
+// pencil (one 8-mode chunk plus the 6 modes that may run past it), and every
+// completed chunk is written by the whole wavefront as row segments of 8
+// modes, so the stores are coalesced.
+constexpr int kGaussRows = 64;
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kGaussRows)
     k_gauss(cx_t<T>* __restrict__ phi, Geom g, KSpace ks, const uint32_t* __restrict__ seeds,
             double anorm, double ns, double fac, double beta, double phase_shift, int fixed_amp) {
   using C2 = cx_t<T>;
+  constexpr int CH = 8, RING = CH + 6;
+  __shared__ C2 ring[kGaussRows][RING + 1];
   const int N = g.N, H = g.H;
-  const long long mrow = (long long)blockIdx.x * blockDim.x + threadIdx.x;  // memory row
-  if (mrow >= (long long)N * N) return;
+  const int lane = threadIdx.x;
+  const long long row0 = (long long)blockIdx.x * kGaussRows;
+  const long long mrow = row0 + lane;  // memory row of this thread's pencil (N^2 % 64 == 0)
   int i, j;
   row_xy(N, mrow, i, j);
   const long long pid = (long long)i * N + j;  // seed table index
   C2* row = phi + mrow * H;
+  // Which k = 0 entries a pencil owns (main.c:476-520): its own k = 0 mode
+  // (and the conjugate at (-i, -j)), a zero, or nothing (the partner writes it).
+  auto owns0 = [&](int pi, int pj, bool& gen, bool& zero) {
+    const bool nyq = (pi == H) || (pj == H);
+    gen = !nyq && ((pi == 0 && pj > 0 && pj < H) || (pi > 0 && pi < H));
+    zero = nyq || (pi == 0 && pj == 0);
+  };
+  bool gen0, zero0;
+  owns0(i, j, gen0, zero0);
   const bool nyq_pencil = (i == H) || (j == H);
   const int ii = (N - i) % N, jj = (N - j) % N;
-  // Which k = 0 entries this pencil owns (main.c:476-520).
-  const bool gen0 = !nyq_pencil && ((i == 0 && j > 0 && j < H) || (i > 0 && i < H));
-  const bool zero0 = nyq_pencil || (i == 0 && j == 0);
   phi[g.nq + mrow] = cmake<C2>(0, 0);
 
-  auto emit = [&](int k, double u1, double u2) {
+  auto value = [&](int k, double u1, double u2) {
     double phase = u1 * 2 * AAR_PI;
     phase += phase_shift;
     C2 val = cmake<C2>(0, 0);
-    bool live = !nyq_pencil && !(i == 0 && j == 0 && k == 0);
-    if (live) {
+    if (!nyq_pencil && !(i == 0 && j == 0 && k == 0)) {
       const double kmag2 = kmag2_of(ks, i, j, k);
       const double kmag = sqrt(kmag2);
-      if (outside_cut(ks, i, j, k, kmag)) {
-        live = false;
-      } else {
+      if (!outside_cut(ks, i, j, k, kmag)) {
         double phig = anorm * exp(ns * log(kmag));
         if (!fixed_amp) phig *= -log(u2);
         phig = sqrt(phig) * fac * beta / kmag2;
         val = cmake<C2>((T)(phig * cos(phase)), (T)(phig * sin(phase)));
       }
     }
-    if (k > 0) {
-      row[k] = val;
-    } else if (gen0) {
-      row[0] = val;
-      phi[row_of(N, ii, jj) * H] = cmake<C2>(val.x, -val.y);
-    } else if (zero0) {
-      row[0] = cmake<C2>(0, 0);
+    if (k == 0 && gen0) phi[row_of(N, ii, jj) * H] = cmake<C2>(val.x, -val.y);
+    return val;
+  };
+  // write the completed chunk c of every pencil of the block
+  auto flush = [&](int c) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kGaussRows * CH / 64; ++q) {
+      const int e = lane + 64 * q, r = e / CH, kk = e % CH;
+      const int k = c * CH + kk;
+      bool write = true;
+      if (k == 0) {
+        int pi, pj;
+        row_xy(N, row0 + r, pi, pj);
+        bool gn, zr;
+        owns0(pi, pj, gn, zr);
+        write = gn || zr;
+      }
+      if (write) phi[(row0 + r) * H + k] = ring[r][k % RING];
     }
+    __syncthreads();
   };
 
-  Ranlxd r;
-  ranlxd_seed(r, seeds[pid]);
+  Ranlxd rs;
+  ranlxd_seed(rs, seeds[pid]);
   bool redo = false;
   for (int kb = 0; kb < H; kb += 6) {
-    ranlxd_update(r);
+    ranlxd_update(rs);
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
       const int k = kb + q;
       if (k < H) {
-        const double u1 = (double)r.s[2 * q] * kTwoM48;
-        const double u2 = (double)r.s[2 * q + 1] * kTwoM48;
+        const double u1 = (double)rs.s[2 * q] * kTwoM48;
+        const double u2 = (double)rs.s[2 * q + 1] * kTwoM48;
         if (u2 == 0) redo = true;
-        if (!redo) emit(k, u1, u2);
+        ring[lane][k % RING] = redo ? cmake<C2>(0, 0) : value(k, u1, u2);
       }
     }
-    if (redo) break;
+    const int c0 = kb / CH, c1 = (kb + 6 < H ? kb + 6 : H) / CH;
+    if (c1 > c0) flush(c0);  // uniform: kb is the same in every lane
   }
   if (redo) {  // an amplitude draw was exactly 0: replay with GSL's redraw loop
+    __threadfence();
     Ranlxd rr;
     ranlxd_seed(rr, seeds[pid]);
     int pos = 12;
@@ -600,7 +667,9 @@ __global__ void __launch_bounds__(256)
       double u2;
       do u2 = next();
       while (u2 == 0);
-      emit(k, u1, u2);
+      const C2 v = value(k, u1, u2);
+      if (k > 0 || gen0) row[k] = v;
+      else if (zero0) row[0] = cmake<C2>(0, 0);
     }
   }
 }
@@ -610,7 +679,8 @@ int Pipeline<T>::gauss(cx_t<T>* phi, const Geom& g, const KSpace& ks, const uint
                        double ns, double fac, double beta, double phase_shift, int fixed_amp,
                        hipStream_t st) {
   const long long np = (long long)g.N * g.N;
-  hipLaunchKernelGGL((k_gauss<T>), dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, phi, g, ks,
+  if (np % kGaussRows) return AAR_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL((k_gauss<T>), dim3((unsigned)(np / kGaussRows)), dim3(kGaussRows), 0, st, phi, g, ks,
                      seeds, anorm, ns, fac, beta, phase_shift, fixed_amp);
   return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
 }
@@ -1000,14 +1070,26 @@ int Pipeline<T>::disp2_spec(const cx_t<T>* src, cx_t<T>* out3, const Geom& g, co
 // update (main.c:1440-1516, periodic_wrap main.c:1534-1544).
 // Interpolation runs in the field precision (FP32 path: FP32 weights and
 // sums); the position update and periodic wrap are always FP64.
+// fmod(x, b) for b > 0 with the common cases done by one compare: for
+// 0 <= |x| < b the result is x, for b <= x < 2b it is x - b (exact, as fmod's
+// result always is: Sterbenz); anything else takes the library routine.
+__device__ __forceinline__ double fmod_box(double x, double b) {
+  if (x >= 0 ? x < b : x > -b) return x;
+  if (x >= b && x < 2 * b) return x - b;
+  return fmod(x, b);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_cic(const cx_t<T>* __restrict__ disp, const cx_t<T>* __restrict__ disp2, Geom g,
           const float* __restrict__ glass, int nglass, double gbox, int tile, double box, double vp1,
           double vp2, float* __restrict__ pos, float* __restrict__ vel, long long npart) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;  // glass particle
+  // XCD-aware block order: each XCD walks a contiguous run of particles, so
+  // the field rows that neighbouring particle lines share stay in its L2
+  const int lin = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int n = (lin % gridDim.x) * blockDim.x + threadIdx.x;  // glass particle
   if (n >= nglass) return;
-  const int t = blockIdx.y;                              // tile (tile-major order of read_glass.c)
+  const int t = lin / gridDim.x;  // tile (tile-major order of read_glass.c)
   const long long p = (long long)t * nglass + n;
   const int N = g.N;
   const int ti = t / (tile * tile), tj = (t / tile) % tile, tk = t % tile;
@@ -1037,7 +1119,8 @@ __global__ void __launch_bounds__(256)
   const W f7 = (uu) * (vv) * (1 - ww), f8 = (uu) * (vv) * (ww);
   const long long c1 = row_of(N, i, j) * N, c2 = row_of(N, i, jj) * N;
   const long long c3 = row_of(N, ii, j) * N, c4 = row_of(N, ii, jj) * N;
-  const float n3f = (float)(N * N * N);
+  // 1 / N^3 is a power of two: multiplying by it equals dividing by (float) N^3
+  const float rn3f = 1.0f / (float)((long long)N * N * N);
   float V[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -1047,10 +1130,10 @@ __global__ void __launch_bounds__(256)
                           D[c3 + k] * f5 + D[c3 + kk] * f6 + D[c4 + k] * f7 + D[c4 + kk] * f8);
     double dis2 = (double)(D2[c1 + k] * f1 + D2[c1 + kk] * f2 + D2[c2 + k] * f3 + D2[c2 + kk] * f4 +
                            D2[c3 + k] * f5 + D2[c3 + kk] * f6 + D2[c4 + k] * f7 + D2[c4 + kk] * f8);
-    dis2 /= n3f;
+    dis2 *= rn3f;
     P[a] = (float)((double)P[a] + (dis - 3. / 7. * dis2));
     V[a] = (float)(dis * vp1 - 3. / 7. * dis2 * vp2);
-    P[a] = (float)fmod(fmod((double)P[a], box) + box, box);
+    P[a] = (float)fmod_box(fmod_box((double)P[a], box) + box, box);
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a) {

@@ -234,6 +234,8 @@ def main():
     N = args.nmesh
     cfg, alpha, x0, glass, gbox = workload(N, args.box, args.template)
     ctx = aar.Context(0)
+    # x / y tile shapes for this mesh, picked by the autotuner before the timed region
+    tsec, tvar = ctx.autotune(N, aar.AAR_FP32, nf=8)
     # timed region: eager launches, each bracketed by HIP events (per-kernel roofline)
     r32 = time_job(aar, ctx, cfg, alpha, x0, glass, gbox, aar.AAR_FP32, aar.AAR_SCHED_BATCHED, args.steps,
                    args.warmup, True)
@@ -276,6 +278,10 @@ def main():
         out["fp64_plain"] = {"value": round(v64, 3), "unit": "Mcells/s",
                              "ms_per_step": round(r64["wall"] / args.fp64_steps * 1e3, 3),
                              "fp32_over_fp64": round(value / v64, 3)}
+    out["autotune"] = {"xy_tile_variant": {"x": tvar[0], "y": tvar[1]},
+                       "us_per_launch_8_fields": {ax: [round(v * 1e6, 1) if v > 0 else None for v in tsec[i]]
+                                                  for i, ax in enumerate(("x", "y"))},
+                       "variants": "0: 128-B tile columns, 1: 64-B, 2: 256-B (same radix plan)"}
     out["copy_peak_gbs"] = round(copy_peak(ctx, min(N, 512)), 1)
     if not args.no_serve:
         out["serving"] = serving(aar, ctx, args.serve_jobs)

@@ -177,6 +177,20 @@ int aar_server_stats(aar_server *s, long long *submitted, long long *completed, 
 /* Cancels jobs not yet admitted, waits for running ones, frees the arena. */
 int aar_server_destroy(aar_server *s);
 
+/* ---- autotuner of the FFT tile shapes ----
+ * Counterpart of the reference's FFT planning (rfftw3d_mpi_create_plan,
+ * main.c:1556-1599): times each tile shape of the x and y passes on nf fields
+ * of an n^3 grid and keeps the fastest per axis for later jobs.  Every shape
+ * uses the same radix plan, so the choice never changes results.  A job's
+ * first aar_job_prepare at n >= 256 tunes by itself on its own buffers.
+ * seconds: optional [2][3] mean seconds per launch (-1: shape not built for n);
+ * variant: optional [2] chosen shape per axis. */
+int aar_autotune(aar_ctx *ctx, int n, int precision, int nf, double *seconds, int *variant);
+/* tile shape in use for (n, precision, axis 0 = x, 1 = y) */
+int aar_xy_tile_variant(int n, int precision, int axis);
+/* pin a tile shape (0, 1, 2); a shape not built for n falls back to 0 at launch */
+int aar_set_xy_tile_variant(int n, int precision, int axis, int variant);
+
 /* ---- building blocks (tests, tools) ---- */
 /* 3-D transforms with FFTW 2 rfftwnd conventions on a host array in the
  * reference layout: dir=+1 C2R (in: N*N*(N/2+1) complex, out: N^3 real),

@@ -165,3 +165,24 @@ def test_graph_replays_match_eager(ctx):
         np.testing.assert_array_equal(g.field("phi_spec"), phi0)
         eager.run()  # unrelated eager work between replays
         eager.sync()
+
+
+def test_xy_tile_shapes_give_identical_transforms(ctx):
+    """Every x / y tile shape shares the radix plan: bit-identical FFTs."""
+    lib = aar.lib()
+    N = 256
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((N, N, N))
+    saved = [lib.aar_xy_tile_variant(N, aar.AAR_FP32, a) for a in (0, 1)]
+    outs = []
+    for v in range(3):
+        for a in (0, 1):
+            assert lib.aar_set_xy_tile_variant(N, aar.AAR_FP32, a, v) == 0
+        outs.append(ctx.fft3d(x, -1, aar.AAR_FP32))
+    for a in (0, 1):
+        lib.aar_set_xy_tile_variant(N, aar.AAR_FP32, a, saved[a])
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
+    sec, chosen = ctx.autotune(N, aar.AAR_FP32, nf=2)
+    assert all(0 <= c < 3 for c in chosen)
+    assert (sec[:, 0] > 0).all()
