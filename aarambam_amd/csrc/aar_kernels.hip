@@ -591,18 +591,31 @@ __global__ void __launch_bounds__(kGaussRows)
   const int ii = (N - i) % N, jj = (N - j) % N;
   phi[g.nq + mrow] = cmake<C2>(0, 0);
 
+  // FP32 path: amplitude and phase in FP32 from the FP64 draws, with the
+  // constant factors folded: sqrt(anorm) fac beta sqrt(k^ns (-ln u2)) / k^2
+  const float amp32 = (float)(sqrt(anorm) * fac * beta);
   auto value = [&](int k, double u1, double u2) {
-    double phase = u1 * 2 * AAR_PI;
-    phase += phase_shift;
     C2 val = cmake<C2>(0, 0);
     if (!nyq_pencil && !(i == 0 && j == 0 && k == 0)) {
       const double kmag2 = kmag2_of(ks, i, j, k);
-      const double kmag = sqrt(kmag2);
+      const double kmag = ks.sphere == 1 ? sqrt(kmag2) : 0.0;  // the cube cut uses per-axis flags
       if (!outside_cut(ks, i, j, k, kmag)) {
-        double phig = anorm * exp(ns * log(kmag));
-        if (!fixed_amp) phig *= -log(u2);
-        phig = sqrt(phig) * fac * beta / kmag2;
-        val = cmake<C2>((T)(phig * cos(phase)), (T)(phig * sin(phase)));
+        if constexpr (sizeof(T) == 4) {
+          const float k2 = (float)kmag2;
+          float a = expf(0.5f * (float)ns * logf(k2));
+          if (!fixed_amp) a *= -logf((float)u2);
+          const float phig = amp32 * sqrtf(a) / k2;
+          float sn, cs;
+          sincosf((float)(u1 * 2 * AAR_PI + phase_shift), &sn, &cs);
+          val = cmake<C2>(phig * cs, phig * sn);
+        } else {
+          double phase = u1 * 2 * AAR_PI;
+          phase += phase_shift;
+          double phig = anorm * exp(ns * log(sqrt(kmag2)));
+          if (!fixed_amp) phig *= -log(u2);
+          phig = sqrt(phig) * fac * beta / kmag2;
+          val = cmake<C2>((T)(phig * cos(phase)), (T)(phig * sin(phase)));
+        }
       }
     }
     if (k == 0 && gen0) phi[row_of(N, ii, jj) * H] = cmake<C2>(val.x, -val.y);
