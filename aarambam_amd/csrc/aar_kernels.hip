@@ -710,7 +710,7 @@ __global__ void __launch_bounds__(256)
   int x, y, z;
   decode_k(g, e, x, y, z);
   const double kmag2 = kmag2_of(ks, x, y, z);
-  const double kmag1 = sqrt(kmag2);
+  const double kmag1 = (sizeof(T) == 8 || ks.sphere == 1) ? sqrt(kmag2) : 0.0;
   const bool dead = (x + y + z == 0) || outside_cut(ks, x, y, z, kmag1);
   if (dead) {
     for (int m = 0; m < M; ++m) out[(long long)m * g.fs + e] = cmake<C2>(0, 0);
@@ -718,8 +718,10 @@ __global__ void __launch_bounds__(256)
   }
   const C2 p = phi[e];
   using RT = T;  // radial functions in the path's precision
-  const RT k1 = (RT)kmag1, k2 = (RT)kmag2;
-  const RT kss = pow(k1, (RT)ks.s);
+  RT kss;
+  const RT k2 = (RT)kmag2;
+  if constexpr (sizeof(T) == 4) kss = exp2f(0.5f * (float)ks.s * log2f(k2));  // k^s = 2^(s/2 log2 k^2)
+  else kss = pow(kmag1, ks.s);
   const RT xb = -1 + 2 * (kss - (RT)ks.kmin_s) / (RT)(ks.kmax_s - ks.kmin_s);
   const RT k3s = kss * kss * kss;  // k^{4-ns}
   LegRecT<RT> leg;
@@ -1054,20 +1056,15 @@ __global__ void __launch_bounds__(256)
   if (e >= g.fs) return;
   int x, y, z;
   decode_k(g, e, x, y, z);
-  const double kv[3] = {ks.kv[x], ks.kv[y], ks.kv[z]};
-  const double kmag2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
-  const double ff = 1 / (ks.sinc[x] * ks.sinc[y] * ks.sinc[z]);
-  const double smth = ff * ff;
+  // the path's precision throughout; one reciprocal replaces the divisions
+  using RT = T;
+  const RT kv[3] = {(RT)ks.kv[x], (RT)ks.kv[y], (RT)ks.kv[z]};
+  const RT kmag2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+  const RT ff = (RT)1 / ((RT)ks.sinc[x] * (RT)ks.sinc[y] * (RT)ks.sinc[z]);
+  const RT f = kmag2 > (RT)0 ? ff * ff / kmag2 : (RT)0;  // CIC deconvolution / k^2
   const C2 s = src[e];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    double re = 0, im = 0;
-    if (kmag2 > 0.0) {
-      re = (double)s.y * kv[a] / kmag2;
-      im = -(double)s.x * kv[a] / kmag2;
-    }
-    out[(long long)a * g.fs + e] = cmake<C2>((T)(re * smth), (T)(im * smth));
-  }
+  for (int a = 0; a < 3; ++a) out[(long long)a * g.fs + e] = cmake<C2>(s.y * kv[a] * f, -s.x * kv[a] * f);
 }
 
 template <typename T>
