@@ -51,7 +51,7 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_load_alpha", "aar_load_x
            "aar_gaussian_field", "aar_ranlxd_device", "aar_bench_pass", "aar_job_set_profiling", "aar_job_kernel_stats", "aar_job_set_graph",
            "aar_server_create", "aar_server_submit", "aar_server_wait", "aar_server_fetch",
            "aar_server_device_output", "aar_server_release", "aar_server_stats", "aar_server_destroy",
-           "aar_autotune", "aar_xy_tile_variant", "aar_set_xy_tile_variant"]
+           "aar_autotune", "aar_xy_tile_variant", "aar_set_xy_tile_variant", "aar_job_fetch_field_box"]
 KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
 
 
@@ -96,6 +96,7 @@ def lib():
             "aar_job_device_output": (C.c_int, [vp, C.POINTER(fp), C.POINTER(fp), C.POINTER(C.c_longlong)]),
             "aar_job_fetch": (C.c_int, [vp, fp, fp]),
             "aar_job_fetch_field": (C.c_int, [vp, C.c_int, dp]),
+            "aar_job_fetch_field_box": (C.c_int, [vp, C.c_int] + [C.c_int] * 6 + [dp]),
             "aar_job_last_seconds": (C.c_double, [vp]),
             "aar_job_destroy": (C.c_int, [vp]),
             "aar_fft3d": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp]),
@@ -298,6 +299,13 @@ class Job:
         if which <= 1:
             return out.view(np.complex128).reshape(N, N, N // 2 + 1)
         return out.reshape(N, N, N)
+
+    def field_box(self, name, lo, shape):
+        """Sub-box [lo, lo + shape) of a real field (displacements or potentials)."""
+        out = np.empty(int(np.prod(shape)), np.float64)
+        _check(lib().aar_job_fetch_field_box(self.h, FIELDS[name], *[int(v) for v in lo],
+                                             *[int(v) for v in shape], _dptr(out)), "aar_job_fetch_field_box")
+        return out.reshape(shape)
 
     def set_graph(self, on=True):
         _check(lib().aar_job_set_graph(self.h, int(on)), "aar_job_set_graph")

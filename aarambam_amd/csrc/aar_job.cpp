@@ -907,6 +907,63 @@ template <typename T> static int fetch_field_t(aar_job* j, int which, double* ou
   return AAR_OK;
 }
 
+// Sub-box of a real field: the rows of one x-block (8 consecutive x) for
+// consecutive y are contiguous in the x-blocked layout, so each x-block of
+// the box is one copy.
+template <typename T>
+static int fetch_box_t(aar_job* j, int which, int x0, int y0, int z0, int nx, int ny, int nz, double* out) {
+  using C2 = cx_t<T>;
+  const Geom& g = j->g;
+  const int N = g.N;
+  const T* src = nullptr;
+  C2* tmp = nullptr;
+  if (which >= AAR_FIELD_DISP_X && which <= AAR_FIELD_DISP2_Z) {
+    const int a = which - AAR_FIELD_DISP_X;
+    src = (const T*)j->slot(j->s_reg + (a < 3 ? 6 + a : a - 3));
+  } else {
+    if (device_alloc((void**)&tmp, (size_t)g.fs * sizeof(C2))) return AAR_ERR_ALLOC;
+    const void* s = j->slot(which == AAR_FIELD_PHIG_REAL ? j->s_phig : j->s_phi);
+    AAR_HIP(hipMemcpyAsync(tmp, s, (size_t)g.fs * sizeof(C2), hipMemcpyDeviceToDevice, j->ctx->stream));
+    const C2* tw = sizeof(T) == 4 ? (const C2*)j->ctx->tw32 : (const C2*)j->ctx->tw64;
+    int rc = Pipeline<T>::c2r(tmp, 1, g, tw, j->ctx->stream);
+    if (rc) {
+      hipFree(tmp);
+      return rc;
+    }
+    AAR_HIP(hipStreamSynchronize(j->ctx->stream));
+    src = (const T*)tmp;
+  }
+  std::vector<T> buf((size_t)ny * 8 * N);
+  int rc = AAR_OK;
+  for (int xb = x0 / 8; xb <= (x0 + nx - 1) / 8 && !rc; ++xb) {
+    const long long r0 = ((long long)xb * N + y0) * 8;
+    if (hipMemcpy(buf.data(), src + r0 * N, buf.size() * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = AAR_ERR_HIP;
+      break;
+    }
+    for (int x = std::max(x0, xb * 8); x < std::min(x0 + nx, xb * 8 + 8); ++x)
+      for (int y = y0; y < y0 + ny; ++y) {
+        const T* row = buf.data() + ((size_t)(y - y0) * 8 + (x & 7)) * N + z0;
+        double* o = out + ((size_t)(x - x0) * ny + (y - y0)) * nz;
+        for (int z = 0; z < nz; ++z) o[z] = row[z];
+      }
+  }
+  if (tmp) hipFree(tmp);
+  return rc;
+}
+
+extern "C" int aar_job_fetch_field_box(aar_job* j, int which, int x0, int y0, int z0, int nx, int ny, int nz,
+                                       double* out) {
+  if (!j || !out || !j->ran || which < AAR_FIELD_DISP_X || which > AAR_FIELD_PHING_REAL) return AAR_ERR_BAD_ARG;
+  const int N = j->g.N;
+  if (x0 < 0 || y0 < 0 || z0 < 0 || nx < 1 || ny < 1 || nz < 1 || x0 + nx > N || y0 + ny > N || z0 + nz > N)
+    return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(j->ctx->device));
+  AAR_HIP(hipEventSynchronize(j->ev1));
+  return j->prec == AAR_FP32 ? fetch_box_t<float>(j, which, x0, y0, z0, nx, ny, nz, out)
+                             : fetch_box_t<double>(j, which, x0, y0, z0, nx, ny, nz, out);
+}
+
 extern "C" int aar_job_fetch_field(aar_job* j, int which, double* out) {
   if (!j || !out || !j->ran) return AAR_ERR_BAD_ARG;
   AAR_HIP(hipSetDevice(j->ctx->device));

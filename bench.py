@@ -28,29 +28,24 @@ TEMPLATES = {
     "le": "local+equilateral closed-form table, 12 modes",
     "local": "local closed-form table, 8 modes",
     "equil": "equilateral closed-form table, 12 modes",
+    "dense12": "synthetic dense 12-mode table (60 triples, alpha ~ N(0,1), 39 product pairs)",
 }
 
 
 def workload(N=512, box=2000.0, template="scalari", seed=1234, fnl=100.0):
     from tests import cases
-    nm = {"scalari": 15, "le": 12, "local": 8, "equil": 12}[template]
+    nm = {"scalari": 15, "le": 12, "local": 8, "equil": 12, "dense12": 12}[template]
     cfg = cases.base_config(Nmesh=N, Nsample=N, Nmax=N, N_modes=nm, Box=box, GlassTileFac=max(1, N // 64),
                             Fnl=fnl, Seed=seed)
     x0 = cases.x0_closed_form(nm, cfg["PrimordialIndex"], box, N)
     if template == "scalari":
         alpha = load_alpha(os.path.join(cases.GOLDEN, "AlphaTable.dat"), 15)
     else:
-        rows = []
-        if template in ("local", "le"):
-            rows += list(cases.local_rows())
-        if template in ("equil", "le"):
-            rows += list(cases.equilateral_rows())
-        merged = {}
-        for n1, n2, n3, a in rows:  # a sum of templates adds coefficients of the same triple
-            key = tuple(sorted((int(n1), int(n2), int(n3)), reverse=True))
-            merged[key] = merged.get(key, 0.0) + a
+        rows = {"local": cases.local_rows, "equil": cases.equilateral_rows, "le": cases.le_rows,
+                "dense12": cases.dense12_rows}[template]()
         alpha = np.zeros((nm, nm, nm))
-        for (n1, n2, n3), a in merged.items():  # main.c:231-241 fills all six permutations
+        for n1, n2, n3, a in rows:  # main.c:231-241 fills all six permutations
+            n1, n2, n3 = int(n1), int(n2), int(n3)
             for p in [(n1, n2, n3), (n1, n3, n2), (n2, n1, n3), (n2, n3, n1), (n3, n1, n2), (n3, n2, n1)]:
                 alpha[p] = a
     glass = cases.lattice_glass(64 if N >= 64 else N, 100.0)

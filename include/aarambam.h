@@ -133,6 +133,10 @@ int aar_job_sync(aar_job *job);
 int aar_job_device_output(aar_job *job, float **pos, float **vel, long long *npart);
 int aar_job_fetch(aar_job *job, float *pos, float *vel);
 int aar_job_fetch_field(aar_job *job, int which, double *out);
+/* A sub-box of a real field (which >= AAR_FIELD_DISP_X): out[(x*ny + y)*nz + z]
+ * for x in [x0, x0+nx) etc., without copying the whole grid back. */
+int aar_job_fetch_field_box(aar_job *job, int which, int x0, int y0, int z0, int nx, int ny, int nz,
+                            double *out);
 /* seconds spent in the last run (HIP events around the enqueued work) */
 double aar_job_last_seconds(aar_job *job);
 /* Per-kernel-kind timing: when on, every launch of the next runs is bracketed

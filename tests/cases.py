@@ -56,6 +56,25 @@ def equilateral_rows():
     return np.array([[3, 0, 0, -3.0], [1, 1, 1, -2.0], [2, 1, 0, 6.0]])
 
 
+def le_rows():
+    """Local + equilateral: the sum of the two templates, so coefficients of the
+    same triple add (the (3,0,0) entry becomes 1 - 3 = -2)."""
+    merged = {}
+    for n1, n2, n3, a in list(local_rows()) + list(equilateral_rows()):
+        key = tuple(sorted((int(n1), int(n2), int(n3)), reverse=True))
+        merged[key] = merged.get(key, 0.0) + float(a)
+    return np.array([[*k, v] for k, v in sorted(merged.items())])
+
+
+def dense12_rows(n_triples=60, seed=12):
+    """Synthetic dense 12-mode table: 60 distinct triples p1 >= p2 >= p3 < 12 with
+    alpha ~ N(0, 1) from PCG64(seed): 39 distinct (p2, p3) product pairs."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    triples = [(a, b, c) for a in range(12) for b in range(a + 1) for c in range(b + 1)]
+    pick = rng.choice(len(triples), size=n_triples, replace=False)
+    return np.array([[*triples[i], rng.standard_normal()] for i in sorted(pick)])
+
+
 def x0_closed_form(Nm, ns, Box, Nmax):
     """AiTable x0[p] = P_p(-1 - 2 kmin^s/(kmax^s - kmin^s)), decompose.py:176-216."""
     from scipy import special
