@@ -51,7 +51,8 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_load_alpha", "aar_load_x
            "aar_gaussian_field", "aar_ranlxd_device", "aar_bench_pass", "aar_job_set_profiling", "aar_job_kernel_stats", "aar_job_set_graph",
            "aar_server_create", "aar_server_submit", "aar_server_wait", "aar_server_fetch",
            "aar_server_device_output", "aar_server_release", "aar_server_stats", "aar_server_destroy",
-           "aar_autotune", "aar_xy_tile_variant", "aar_set_xy_tile_variant", "aar_job_fetch_field_box"]
+           "aar_autotune", "aar_xy_tile_variant", "aar_set_xy_tile_variant", "aar_job_fetch_field_box",
+           "aar_job_ng_statistics"]
 KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
 
 
@@ -97,6 +98,7 @@ def lib():
             "aar_job_fetch": (C.c_int, [vp, fp, fp]),
             "aar_job_fetch_field": (C.c_int, [vp, C.c_int, dp]),
             "aar_job_fetch_field_box": (C.c_int, [vp, C.c_int] + [C.c_int] * 6 + [dp]),
+            "aar_job_ng_statistics": (C.c_int, [vp, C.c_int] + [dp] * 8),
             "aar_job_last_seconds": (C.c_double, [vp]),
             "aar_job_destroy": (C.c_int, [vp]),
             "aar_fft3d": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp]),
@@ -306,6 +308,20 @@ class Job:
         _check(lib().aar_job_fetch_field_box(self.h, FIELDS[name], *[int(v) for v in lo],
                                              *[int(v) for v in shape], _dptr(out)), "aar_job_fetch_field_box")
         return out.reshape(shape)
+
+    def ng_statistics(self, edges):
+        """Binned P(k) of phi_G and cross bispectrum <phi_G phi_G psi> with their expectations
+        (aar_job_ng_statistics); bispectra indexed [b1, b2, b3]."""
+        edges = np.ascontiguousarray(edges, np.float64)
+        nb = len(edges) - 1
+        r = {k: np.zeros(nb) for k in ("pk_meas", "pk_theory", "kmean", "nmodes")}
+        r.update({k: np.zeros(nb ** 3) for k in ("b_meas", "b_theory", "ntri")})
+        _check(lib().aar_job_ng_statistics(self.h, nb, _dptr(edges), *[_dptr(r[k]) for k in
+                                           ("pk_meas", "pk_theory", "kmean", "nmodes", "b_meas", "b_theory",
+                                            "ntri")]), "aar_job_ng_statistics")
+        for k in ("b_meas", "b_theory", "ntri"):
+            r[k] = r[k].reshape(nb, nb, nb)
+        return r
 
     def set_graph(self, on=True):
         _check(lib().aar_job_set_graph(self.h, int(on)), "aar_job_set_graph")

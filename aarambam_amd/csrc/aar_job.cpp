@@ -74,6 +74,8 @@ struct aar_job {
   // schedule
   int M = 0, K = 0;
   std::vector<int> modes_h, kmodes_h;
+  std::vector<NgTerm> terms_h;  // regrouped terms (mode slots i, j; coefficient) by outer index
+  std::vector<int> groups_h;
   std::vector<PairWork> pairs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_seconds = 0;
@@ -664,6 +666,8 @@ int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, i
   Schedule s = build_schedule(*p, t->alpha);
   j->modes_h = s.modes;
   j->kmodes_h = s.kmodes;
+  j->terms_h = s.terms;
+  j->groups_h = s.groups;
   j->M = (int)s.modes.size();
   j->K = (int)s.kmodes.size();
   j->pairs = s.pairs;
@@ -990,6 +994,125 @@ extern "C" int aar_job_destroy(aar_job* j) {
   if (j->ev1) hipEventDestroy(j->ev1);
   delete j;
   return AAR_OK;
+}
+
+// ------------------------------------------------------------------ template check
+// P(k) of phi_G and the cross bispectrum <phi_G phi_G psi> of the non-Gaussian
+// part psi = (Phi - phi_G) N^3 / Fnl, with their expectations from the input
+// P(k) and modal table.  With P(k) = E|phi_G(k)|^2 and Phi = phi_G + Fnl/N^3 psi,
+// psi(k) = sum_K g_K(k) R2C[sum_t c_t f_i f_j], f_i = C2R[q_i phi_G]:
+//   E[phi(k1) phi(k2) psi(k3)] = N^3 P(k1) P(k2) sum_K g_K(k3)
+//                                 sum_{t in K} c_t (q_i(k1) q_j(k2) + q_j(k1) q_i(k2)),
+// a separable sum, so its bin average is a product sum of filtered fields.
+static void stat_reduce(const std::vector<double>& part, int nout, int nchunks, std::vector<double>& out) {
+  out.assign(nout, 0.0);
+  for (int o = 0; o < nout; ++o)
+    for (int c = 0; c < nchunks; ++c) out[o] += part[(size_t)o * nchunks + c];
+}
+
+template <typename T>
+static int ng_statistics_t(aar_job* j, int nb, const double* edges_h, double* pk_meas, double* pk_theory,
+                           double* kmean, double* nmodes, double* b_meas, double* b_theory, double* ntri) {
+  using C2 = cx_t<T>;
+  const Geom& g = j->g;
+  const int M = (int)j->modes_h.size(), K = (int)j->kmodes_h.size();
+  const double N3 = (double)g.N * g.N * g.N;
+  // field ids: I_phi, I_psi, J (ones), P, |k|, Q_m (m < M), G_K (K < K): nb fields each
+  const int fIp = 0, fIs = nb, fJ = 2 * nb, fP = 3 * nb, fK = 4 * nb, fQ = 5 * nb, fG = (5 + M) * nb;
+  const int nf = (5 + M + K) * nb;
+  double2* F = nullptr;
+  double* ed = nullptr;
+  int* ofs_d = nullptr;
+  StatTerm* terms_d = nullptr;
+  double* part_d = nullptr;
+  int rc = device_alloc((void**)&F, (size_t)nf * g.fs * sizeof(double2));
+  if (!rc) rc = device_alloc((void**)&ed, sizeof(double) * (nb + 1));
+  hipStream_t st = j->ctx->stream;
+  const C2* phig = (const C2*)j->slot(j->s_phig);
+  const C2* phi = (const C2*)j->slot(j->s_phi);
+  const double pamp = j->c.Anorm * j->c.fac * j->c.fac * j->c.Beta * j->c.Beta, ns = j->p.PrimordialIndex;
+  if (!rc && hipMemcpy(ed, edges_h, sizeof(double) * (nb + 1), hipMemcpyHostToDevice) != hipSuccess) rc = AAR_ERR_HIP;
+  auto bins = [&](int fid, const C2* a, const C2* b, double scale, int kind, int n) {
+    if (!rc) rc = stat_bin_fields<T>(a, b, scale, kind, n, pamp, ns, F + (size_t)fid * g.fs, nb, ed, g, j->ks, st);
+  };
+  bins(fIp, phig, nullptr, 1.0, 0, 0);
+  bins(fIs, phi, phig, j->p.Fnl != 0 ? N3 / j->p.Fnl : 0.0, 0, 0);
+  bins(fJ, nullptr, nullptr, 0, 1, 0);
+  bins(fP, nullptr, nullptr, 0, 2, -1);
+  bins(fK, nullptr, nullptr, 0, 4, 0);
+  for (int m = 0; m < M; ++m) bins(fQ + m * nb, nullptr, nullptr, 0, 2, j->modes_h[m]);
+  for (int k = 0; k < K; ++k) bins(fG + k * nb, nullptr, nullptr, 0, 3, j->kmodes_h[k]);
+  if (!rc) rc = Pipeline<double>::c2r(F, nf, g, j->ctx->tw64, st);
+  // outputs: 4 pair sums per bin, then nb^3 measured, theory and triangle-count triples
+  std::vector<int> ofs{0};
+  std::vector<StatTerm> tl;
+  auto out = [&](std::initializer_list<StatTerm> ts) {
+    for (auto& t : ts) tl.push_back(t);
+    ofs.push_back((int)tl.size());
+  };
+  for (int b = 0; b < nb; ++b) {
+    out({{fIp + b, fIp + b, -1, 1.0}});  // N^3 sum |phi|^2
+    out({{fJ + b, fP + b, -1, 1.0}});    // N^3 sum P
+    out({{fJ + b, fK + b, -1, 1.0}});    // N^3 sum |k|
+    out({{fJ + b, fJ + b, -1, 1.0}});    // N^3 n_b
+  }
+  for (int b1 = 0; b1 < nb; ++b1)
+    for (int b2 = 0; b2 < nb; ++b2)
+      for (int b3 = 0; b3 < nb; ++b3) {
+        out({{fIp + b1, fIp + b2, fIs + b3, 1.0}});
+        for (int kk = 0; kk < K; ++kk)
+          for (int t = j->groups_h[kk]; t < j->groups_h[kk + 1]; ++t) {
+            const NgTerm& tm = j->terms_h[t];
+            tl.push_back({fQ + tm.i * nb + b1, fQ + tm.j * nb + b2, fG + kk * nb + b3, tm.c * N3});
+            tl.push_back({fQ + tm.j * nb + b1, fQ + tm.i * nb + b2, fG + kk * nb + b3, tm.c * N3});
+          }
+        ofs.push_back((int)tl.size());
+        out({{fJ + b1, fJ + b2, fJ + b3, 1.0 / N3}});
+      }
+  const int nout = (int)ofs.size() - 1, nchunks = 64;
+  if (!rc) rc = upload(&ofs_d, ofs);
+  if (!rc) rc = upload(&terms_d, tl);
+  if (!rc) rc = device_alloc((void**)&part_d, sizeof(double) * (size_t)nout * nchunks);
+  std::vector<double> part((size_t)nout * nchunks), sums;
+  for (int o0 = 0; o0 < nout && !rc; o0 += 65535) {
+    const int n = std::min(nout - o0, 65535);
+    rc = stat_sums(F, g, ofs_d + o0, terms_d, n, nchunks, part_d + (size_t)o0 * nchunks, st);
+  }
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = AAR_ERR_HIP;
+  if (!rc && hipMemcpy(part.data(), part_d, sizeof(double) * part.size(), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = AAR_ERR_HIP;
+  if (!rc) {
+    stat_reduce(part, nout, nchunks, sums);
+    for (int b = 0; b < nb; ++b) {
+      const double n = sums[4 * b + 3] / N3;
+      if (nmodes) nmodes[b] = n;
+      if (pk_meas) pk_meas[b] = n > 0 ? sums[4 * b] / N3 / n : 0;
+      if (pk_theory) pk_theory[b] = n > 0 ? sums[4 * b + 1] / N3 / n : 0;
+      if (kmean) kmean[b] = n > 0 ? sums[4 * b + 2] / N3 / n : 0;
+    }
+    const size_t t0 = 4 * (size_t)nb, nt = (size_t)nb * nb * nb;
+    for (size_t q = 0; q < nt; ++q) {
+      if (b_meas) b_meas[q] = sums[t0 + 3 * q];
+      if (b_theory) b_theory[q] = sums[t0 + 3 * q + 1];
+      if (ntri) ntri[q] = sums[t0 + 3 * q + 2];
+    }
+  }
+  void* bufs[] = {F, ed, ofs_d, terms_d, part_d};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  return rc;
+}
+
+extern "C" int aar_job_ng_statistics(aar_job* j, int nb, const double* edges, double* pk_meas, double* pk_theory,
+                                     double* kmean, double* nmodes, double* b_meas, double* b_theory, double* ntri) {
+  if (!j || !j->ran || nb < 1 || nb > 32 || !edges) return AAR_ERR_BAD_ARG;
+  for (int b = 0; b < nb; ++b)
+    if (!(edges[b] < edges[b + 1])) return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(j->ctx->device));
+  AAR_HIP(hipEventSynchronize(j->ev1));
+  return j->prec == AAR_FP32
+             ? ng_statistics_t<float>(j, nb, edges, pk_meas, pk_theory, kmean, nmodes, b_meas, b_theory, ntri)
+             : ng_statistics_t<double>(j, nb, edges, pk_meas, pk_theory, kmean, nmodes, b_meas, b_theory, ntri);
 }
 
 // ------------------------------------------------------------------ building blocks

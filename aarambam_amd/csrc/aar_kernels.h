@@ -87,6 +87,19 @@ template <typename T> struct Pipeline {
 // zero n doubles (graph-capturable; no memset nodes)
 int aar_zero_fill(double* p, long long n, hipStream_t st);
 int aar_copy16(const void* src, void* dst, long long bytes, hipStream_t st);
+
+// ---- binned statistics (aar_stats.hip)
+struct StatTerm {  // one weighted product of real fields a, b, c (c < 0: none)
+  int a, b, c;
+  double w;
+};
+// nb band-filtered FP64 spectra of one kind (see aar_stats.hip) at out + b fs
+template <typename TS>
+int stat_bin_fields(const cx_t<TS>* a, const cx_t<TS>* b, double scale, int kind, int n, double pamp, double ns,
+                    double2* out, int nb, const double* edges, const Geom& g, const KSpace& ks, hipStream_t st);
+// per-chunk partial sums of nout outputs, each a weighted sum of field products
+int stat_sums(const double2* fields, const Geom& g, const int* ofs, const StatTerm* terms, int nout, int nchunks,
+              double* partial, hipStream_t st);
 // x / y pass tile-shape variants (0 .. kXYVariantCount-1) per precision, mesh size and axis
 constexpr int kXYVariantCount = 3;
 int aar_xy_variant(int fp64, int n, int axis);

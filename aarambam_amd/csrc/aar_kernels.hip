@@ -8,69 +8,8 @@
 
 #include "aar_fft.h"
 #include "aar_kernels.h"
+#include "aar_kspace.h"
 #include "aar_ranlxd.h"
-
-// ------------------------------------------------------------------ helpers
-__device__ __forceinline__ void decode_k(const Geom& g, long long e, int& x, int& y, int& kz) {
-  // N*N rows fit 32 bits for N <= 1024 (2^20 rows)
-  unsigned r;
-  if (e < g.nq) {
-    kz = (int)(e & (g.H - 1));
-    r = (unsigned)(e >> g.hlog);
-  } else {
-    kz = g.H;
-    r = (unsigned)(e - g.nq);
-  }
-  row_xy_log(g.nlog, r, x, y);
-}
-
-// main.c:452-462: SphereMode 1 keeps |k| L/2pi <= Nsample/2, else the cube.
-__device__ __forceinline__ bool outside_cut(const KSpace& ks, int x, int y, int z, double kmag1) {
-  if (ks.sphere == 1) return kmag1 * ks.box / (2 * AAR_PI) > ks.nsh;
-  return (ks.outc[x] | ks.outc[y] | ks.outc[z]) != 0;
-}
-
-__device__ __forceinline__ double kmag2_of(const KSpace& ks, int x, int y, int z) {
-  const double a = ks.kv[x], b = ks.kv[y], c = ks.kv[z];
-  return a * a + b * b + c * c;
-}
-
-// Upward Legendre recurrence, GSL gsl_sf_legendre_Pl (legendre_poly.c).
-// RT is the precision of the radial functions: FP64 on the FP64 path, FP32 on
-// the FP32 path (|x| <= 1 and orders <= 31 keep the recurrence stable).
-template <typename RT> struct LegRecT {
-  RT x, pm2, pm1;
-  int l;
-  __device__ void init(RT xx) {
-    x = xx;
-    pm2 = 1;
-    pm1 = xx;
-    l = 1;
-  }
-  __device__ RT at(int L) {
-    if (L == 0) return 1;
-    while (l < L) {
-      ++l;
-      RT p = (x * (RT)(2 * l - 1) * pm1 - (RT)(l - 1) * pm2) / (RT)l;
-      pm2 = pm1;
-      pm1 = p;
-    }
-    return pm1;
-  }
-};
-using LegRec = LegRecT<double>;
-
-// Radial weight g_n(k) of the modal basis (main.c:835-862):
-// n <= 3: k^{s(n-3)};  n >= 4: (P_{n-1}(xbar) - X0[n-1]) / k^2.
-template <typename RT>
-__device__ __forceinline__ RT weight_g(int n, RT ks, RT kmag2, LegRecT<RT>& leg, const double* x0) {
-  if (n <= 3) {
-    RT r = 1;
-    for (int q = n; q < 3; ++q) r *= ks;
-    return (RT)1 / r;
-  }
-  return (leg.at(n - 1) - (RT)x0[n - 1]) / kmag2;
-}
 
 // ------------------------------------------------------------------ twiddles
 template <typename T> void make_twiddles(cx_t<T>* tw) {

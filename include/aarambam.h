@@ -153,6 +153,22 @@ int aar_job_set_graph(aar_job *job, int on);
 int aar_job_kernel_stats(aar_job *job, int kind, int *launches, double *seconds, double *bytes);
 int aar_job_destroy(aar_job *job);
 
+/* ---- template check of a finished job (no counterpart in the reference) ----
+ * Binned statistics of the potentials on the device.  Bins are |k| in
+ * [edges[b], edges[b+1]) on the job's k grid (2 pi / Box units), excluding the
+ * origin, the Nyquist planes and the modes outside the Nsample cut.
+ *   pk_meas[b]:   mean |phi_G(k)|^2 over the bin (all k, both halves);
+ *   pk_theory[b]: mean of its expectation P(k) = Anorm k^ns (fac Beta / k^2)^2;
+ *   kmean[b], nmodes[b]: mean |k| and number of modes in the bin;
+ *   b_meas[(b1 nb + b2) nb + b3]: N^3 sum over closed triangles (k1 in b1, ...)
+ *     of phi_G(k1) phi_G(k2) psi(k3), psi = (Phi - phi_G) N^3 / Fnl the
+ *     non-Gaussian part built from the modal table;
+ *   b_theory: its expectation from P(k) and the table (tree level);
+ *   ntri: number of closed triangles of the bin triple.
+ * Any output pointer may be NULL.  nb <= 32. */
+int aar_job_ng_statistics(aar_job *job, int nb, const double *edges, double *pk_meas, double *pk_theory,
+                          double *kmean, double *nmodes, double *b_meas, double *b_theory, double *ntri);
+
 /* ---- serving: continuous batching of IC jobs on one device ----
  * No counterpart in the reference, which runs one IC per process
  * (main.c:53-102); every job keeps the semantics of aar_job_prepare/run.
