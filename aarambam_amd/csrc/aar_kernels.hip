@@ -1033,12 +1033,9 @@ __global__ void __launch_bounds__(256)
     k_cic(const cx_t<T>* __restrict__ disp, const cx_t<T>* __restrict__ disp2, Geom g,
           const float* __restrict__ glass, int nglass, double gbox, int tile, double box, double vp1,
           double vp2, float* __restrict__ pos, float* __restrict__ vel, long long npart) {
-  // XCD-aware block order: each XCD walks a contiguous run of particles, so
-  // the field rows that neighbouring particle lines share stay in its L2
-  const int lin = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-  const int n = (lin % gridDim.x) * blockDim.x + threadIdx.x;  // glass particle
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;  // glass particle
   if (n >= nglass) return;
-  const int t = lin / gridDim.x;  // tile (tile-major order of read_glass.c)
+  const int t = blockIdx.y;  // tile (tile-major order of read_glass.c)
   const long long p = (long long)t * nglass + n;
   const int N = g.N;
   const int ti = t / (tile * tile), tj = (t / tile) % tile, tk = t % tile;
