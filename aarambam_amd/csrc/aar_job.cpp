@@ -104,8 +104,12 @@ struct aar_job {
   size_t ws_size = 0, ws_used = 0;
   hipStream_t up = nullptr;
   std::vector<std::vector<char>> staged;
-  // slot map
-  int s_phig = 0, s_phi = 1, s_reg = 2;
+  // 2LPT source as a product sum of the six gradient fields, for the fused z pass
+  NgTerm* src_terms = nullptr;
+  int* src_groups = nullptr;
+  bool fused_lpt = false;
+  // slot map: phi_G, Phi, the work region; ZA and 2LPT displacement fields
+  int s_phig = 0, s_phi = 1, s_reg = 2, s_disp = 8, s_disp2 = 2;
   void* slot(int s) const { return (char*)pool + (size_t)s * g.fs * cxsize; }
 };
 
@@ -279,14 +283,36 @@ template <typename T> struct Exec {
                               c.Dplus, c.Beta, st);
            })))
         return rc;
-      if ((rc = c2r(reg, 9))) return rc;
-      if ((rc = timed(j, st, AAR_K_PRODUCTS, 7 * R, [&] { return P::source(reg, g, st); }))) return rc;
-      if ((rc = r2c(reg + 3 * g.fs, 1))) return rc;
-      if ((rc = timed(j, st, AAR_K_LPT, 4 * S, [&] { return P::disp2_spec(reg + 3 * g.fs, reg, g, j->ks, st); })))
-        return rc;
-      if ((rc = c2r(reg, 3))) return rc;
+      C2* disp = F(j, j->s_disp);
+      C2* disp2 = F(j, j->s_disp2);
+      if (j->fused_lpt) {
+        // x, y inverse passes of all nine spectra; z C2R of the three ZA fields;
+        // the six gradient fields go through the fused z pass, whose one
+        // product is the 2LPT source (never stored in real space), then y, x
+        // forward passes of the source
+        if ((rc = timed(j, st, AAR_K_XY, 2 * 9 * S, [&] { return P::xy_pass(reg, 9, g, tw(j), 0, +1, st); })) ||
+            (rc = timed(j, st, AAR_K_XY, 2 * 9 * S, [&] { return P::xy_pass(reg, 9, g, tw(j), 1, +1, st); })))
+          return rc;
+        if ((rc = timed(j, st, AAR_K_Z, 3 * (S + R), [&] { return P::z_pass(disp, 3, g, tw(j), +1, st); })))
+          return rc;
+        if ((rc = timed(j, st, AAR_K_PRODUCTS, 7 * S,
+                        [&] { return P::z_ng(reg, 6, 1, j->src_terms, j->src_groups, g, tw(j), st); })))
+          return rc;
+        if ((rc = timed(j, st, AAR_K_XY, 2 * S, [&] { return P::xy_pass(reg, 1, g, tw(j), 1, -1, st); })) ||
+            (rc = timed(j, st, AAR_K_XY, 2 * S, [&] { return P::xy_pass(reg, 1, g, tw(j), 0, -1, st); })))
+          return rc;
+        if ((rc = timed(j, st, AAR_K_LPT, 4 * S, [&] { return P::disp2_spec(reg, disp2, g, j->ks, st); })))
+          return rc;
+      } else {
+        if ((rc = c2r(reg, 9))) return rc;
+        if ((rc = timed(j, st, AAR_K_PRODUCTS, 7 * R, [&] { return P::source(reg, g, st); }))) return rc;
+        if ((rc = r2c(reg + 3 * g.fs, 1))) return rc;
+        if ((rc = timed(j, st, AAR_K_LPT, 4 * S, [&] { return P::disp2_spec(reg + 3 * g.fs, disp2, g, j->ks, st); })))
+          return rc;
+      }
+      if ((rc = c2r(disp2, 3))) return rc;
       return timed(j, st, AAR_K_CIC, 6 * R + 24.0 * j->npart, [&] {
-        return P::cic(reg + 6 * g.fs, reg, g, j->glass, j->nglass, j->glass_box, j->p.GlassTileFac, p.Box,
+        return P::cic(disp, disp2, g, j->glass, j->nglass, j->glass_box, j->p.GlassTileFac, p.Box,
                       c.vel_prefac, c.vel_prefac2, j->pos, j->vel, j->npart, st);
       });
     } else {
@@ -615,6 +641,7 @@ static long long job_footprint(const aar_params* p, const aar_tables* t, int pre
   add(4 * (plain ? 2 : (long long)s.groups.size()));
   add((long long)sizeof(NgTerm) * (plain ? 1 : (long long)s.terms.size()));
   add((long long)sizeof(NgWeight) * (long long)s.weights.size());
+  add((long long)sizeof(NgTerm) * 6), add(4 * 2);  // 2LPT source terms and groups
   if (p->WhichTransfer == 2) {
     std::vector<double> logk, logT;
     if (aar_host_transfer_tables(p, t, logk, logT)) return -1;
@@ -721,6 +748,16 @@ int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, i
       (rc = job_upload(j, &j->kmodes, s.kmodes)) || (rc = job_upload(j, &j->groups, groups)) ||
       (rc = job_upload(j, &j->terms, terms)) || (rc = job_upload(j, &j->weights, s.weights)))
     return fail(rc);
+  // src = f00 (f11 + f22) + f11 f22 - f01^2 - f02^2 - f12^2 over the gradient slots
+  // 00, 01, 02, 11, 12, 22 (main.c:1286-1290)
+  const std::vector<NgTerm> src_terms = {{0, 3, 1.0}, {0, 5, 1.0}, {3, 5, 1.0},
+                                         {1, 1, -1.0}, {2, 2, -1.0}, {4, 4, -1.0}};
+  const std::vector<int> src_groups = {0, 6};
+  if ((rc = job_upload(j, &j->src_terms, src_terms)) || (rc = job_upload(j, &j->src_groups, src_groups)))
+    return fail(rc);
+  j->fused_lpt = schedule == AAR_SCHED_BATCHED && N >= 64;
+  j->s_disp = j->s_reg + 6;
+  j->s_disp2 = j->s_reg + (j->fused_lpt ? 1 : 0);
   if (p->WhichTransfer == 2 && ((rc = job_upload(j, &j->logk, logk)) || (rc = job_upload(j, &j->logT, logT))))
     return fail(rc);
   j->nglass = t->nglass;
@@ -890,7 +927,7 @@ template <typename T> static int fetch_field_t(aar_job* j, int which, double* ou
     src = (const C2*)j->slot(which == AAR_FIELD_PHIG_SPEC ? j->s_phig : j->s_phi);
   } else if (which >= AAR_FIELD_DISP_X && which <= AAR_FIELD_DISP2_Z) {
     const int a = which - AAR_FIELD_DISP_X;  // 0..5
-    src = (const C2*)j->slot(j->s_reg + (a < 3 ? 6 + a : a - 3));
+    src = (const C2*)j->slot(a < 3 ? j->s_disp + a : j->s_disp2 + a - 3);
   } else if (which == AAR_FIELD_PHIG_REAL || which == AAR_FIELD_PHING_REAL) {
     if (device_alloc((void**)&tmp, (size_t)g.fs * sizeof(C2))) return AAR_ERR_ALLOC;
     const void* s = j->slot(which == AAR_FIELD_PHIG_REAL ? j->s_phig : j->s_phi);
@@ -923,7 +960,7 @@ static int fetch_box_t(aar_job* j, int which, int x0, int y0, int z0, int nx, in
   C2* tmp = nullptr;
   if (which >= AAR_FIELD_DISP_X && which <= AAR_FIELD_DISP2_Z) {
     const int a = which - AAR_FIELD_DISP_X;
-    src = (const T*)j->slot(j->s_reg + (a < 3 ? 6 + a : a - 3));
+    src = (const T*)j->slot(a < 3 ? j->s_disp + a : j->s_disp2 + a - 3);
   } else {
     if (device_alloc((void**)&tmp, (size_t)g.fs * sizeof(C2))) return AAR_ERR_ALLOC;
     const void* s = j->slot(which == AAR_FIELD_PHIG_REAL ? j->s_phig : j->s_phi);
@@ -981,7 +1018,8 @@ extern "C" int aar_job_destroy(aar_job* j) {
   if (j->ran) hipEventSynchronize(j->ev1);
   if (!j->ws) {
     void* bufs[] = {j->pool, j->seeds, j->kv, j->sinc, j->x0, j->logk, j->logT, j->outc, j->modes,
-                    j->kmodes, j->groups, j->terms, j->weights, j->acc, j->glass, j->pos, j->vel};
+                    j->kmodes, j->groups, j->terms, j->weights, j->src_terms, j->src_groups, j->acc, j->glass,
+                    j->pos, j->vel};
     for (void* b : bufs)
       if (b) hipFree(b);
   } else if (j->up && !j->ran) {

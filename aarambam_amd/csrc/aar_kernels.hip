@@ -276,8 +276,8 @@ static int launch_z(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int di
 // 3(M + K)).
 template <typename T, int H, int MMAX> struct ZNGCfg {
   // complex elements per thread: 8 keeps the 1024-point rows in one
-  // wavefront each when at most 4 mode fields sit in registers
-  static constexpr int E = sizeof(T) == 4 ? (H == 512 && MMAX <= 4 ? 8 : 4) : 2;
+  // wavefront each when at most 6 mode fields sit in registers
+  static constexpr int E = sizeof(T) == 4 ? (H == 512 && MMAX <= 6 ? 8 : 4) : 2;
   static constexpr int C = sizeof(T) == 4 && (H < 512 || E == 8) ? 4 : 2;  // rows per block
   static constexpr int NT = C * H / E;
   // every row's exchanges inside one wavefront: no workgroup barriers
@@ -414,6 +414,7 @@ template <typename T, int H>
 static int launch_z_ng(cx_t<T>* base, int M, int K, const NgTerm* terms, const int* gs, const Geom& g,
                        const cx_t<T>* tw, hipStream_t st) {
   if (M <= 4) return launch_z_ng_m<T, H, 4>(base, M, K, terms, gs, g, tw, st);
+  if (M <= 6) return launch_z_ng_m<T, H, 6>(base, M, K, terms, gs, g, tw, st);
   if (M <= 8) return launch_z_ng_m<T, H, 8>(base, M, K, terms, gs, g, tw, st);
   if (M <= 16) return launch_z_ng_m<T, H, 16>(base, M, K, terms, gs, g, tw, st);
   return AAR_ERR_UNSUPPORTED;
