@@ -52,7 +52,7 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_load_alpha", "aar_load_x
            "aar_server_create", "aar_server_submit", "aar_server_wait", "aar_server_fetch",
            "aar_server_device_output", "aar_server_release", "aar_server_stats", "aar_server_destroy",
            "aar_autotune", "aar_xy_tile_variant", "aar_set_xy_tile_variant", "aar_job_fetch_field_box",
-           "aar_job_ng_statistics"]
+           "aar_job_ng_statistics", "aar_mode_bispectra_covariance", "aar_mode_QS"]
 KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
 
 
@@ -73,6 +73,7 @@ def lib():
             raise RuntimeError(f"{LIB_PATH} is not built; run `python -m aarambam_amd.build`")
         L = C.CDLL(LIB_PATH)
         vp, dp, fp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_float)
+        ip = C.POINTER(C.c_int)
         PP, TP, CP = C.POINTER(AarParams), C.POINTER(AarTables), C.POINTER(AarCosmo)
         sig = {
             "aar_status_string": (C.c_char_p, [C.c_int]),
@@ -99,6 +100,10 @@ def lib():
             "aar_job_fetch_field": (C.c_int, [vp, C.c_int, dp]),
             "aar_job_fetch_field_box": (C.c_int, [vp, C.c_int] + [C.c_int] * 6 + [dp]),
             "aar_job_ng_statistics": (C.c_int, [vp, C.c_int] + [dp] * 8),
+            "aar_mode_bispectra_covariance": (C.c_int, [vp, dp, dp, ip, ip, ip, C.c_int, ip, ip, ip, C.c_int, dp,
+                                                        C.c_int, C.c_int]),
+            "aar_mode_QS": (C.c_int, [vp, dp, dp, C.c_int, dp, ip, ip, ip, C.c_int, ip, ip, ip, C.c_int, dp, C.c_int,
+                                      C.c_int]),
             "aar_job_last_seconds": (C.c_double, [vp]),
             "aar_job_destroy": (C.c_int, [vp]),
             "aar_fft3d": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp]),
@@ -258,6 +263,35 @@ class Context:
         ch = (C.c_int * 2)()
         _check(lib().aar_autotune(self.h, n, precision, nf, _dptr(sec), ch), "aar_autotune")
         return sec.reshape(2, 3), [ch[0], ch[1]]
+
+    def _modal(self, grid):
+        c = lambda a, t: np.ascontiguousarray(a, dtype=t)
+        ev = c(grid["evals"], np.float64)
+        arrs = dict(w=c(grid["w"], np.float64), i1=c(grid["i1"], np.int32), i2=c(grid["i2"], np.int32),
+                    i3=c(grid["i3"], np.int32), p1=c(grid["p1"], np.int32), p2=c(grid["p2"], np.int32),
+                    p3=c(grid["p3"], np.int32), ev=ev)
+        ipt = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+        args = [_dptr(arrs["w"]), ipt(arrs["i1"]), ipt(arrs["i2"]), ipt(arrs["i3"]), len(arrs["w"]),
+                ipt(arrs["p1"]), ipt(arrs["p2"]), ipt(arrs["p3"]), len(arrs["p1"]), _dptr(ev), ev.shape[0],
+                ev.shape[1]]
+        return arrs, args
+
+    def mode_covariance(self, grid):
+        """Mode-bispectrum covariance on the device (tetrapyd.c:111-208); grid: dict of
+        w, i1, i2, i3 (tetrapyd points), p1, p2, p3 (3-D modes), evals (pmax x nk)."""
+        keep, args = self._modal(grid)
+        n = args[8]
+        out = np.zeros(n * n)
+        _check(lib().aar_mode_bispectra_covariance(self.h, _dptr(out), *args), "aar_mode_bispectra_covariance")
+        return out.reshape(n, n)
+
+    def mode_qs(self, grid, S):
+        """Projections of shapes S (n_shapes x npts) on the modes (tetrapyd.c:210-281)."""
+        keep, args = self._modal(grid)
+        S = np.ascontiguousarray(S, dtype=np.float64).reshape(-1, args[4])
+        out = np.zeros(S.shape[0] * args[8])
+        _check(lib().aar_mode_QS(self.h, _dptr(out), _dptr(S), S.shape[0], *args), "aar_mode_QS")
+        return out.reshape(S.shape[0], args[8])
 
     def ranlxd(self, seed, n):
         out = np.empty(n, np.float64)

@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libaarambam_hip.so")
 EXE = os.path.join(HERE, "bin", "2LPTBasis")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["aar_kernels.hip", "aar_stats.hip", "aar_job.cpp", "aar_host.cpp", "aar_serve.cpp"]
+SOURCES = ["aar_kernels.hip", "aar_stats.hip", "aar_modal.hip", "aar_job.cpp", "aar_host.cpp", "aar_serve.cpp"]
 # -ffp-contract=off keeps the reference's operation order (no fused multiply-add
 # contraction); packed FP32 instructions are disabled by target feature.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -88,6 +88,14 @@ template <typename T> struct Pipeline {
 int aar_zero_fill(double* p, long long n, hipStream_t st);
 int aar_copy16(const void* src, void* dst, long long bytes, hipStream_t st);
 
+// ---- modal decomposition (aar_modal.hip): QQ = (w Q) Q^T and QS = S (w Q)^T over tetrapyd points
+int modal_cov_launch(const double* w, const int* i1, const int* i2, const int* i3, int npts, const int* p1,
+                     const int* p2, const int* p3, int nmodes, const double* evals, int pmax, int nk, int nsplit,
+                     double* partial, double* cov, hipStream_t st);
+int modal_qs_launch(const double* S, int nshapes, const double* w, const int* i1, const int* i2, const int* i3,
+                    int npts, const int* p1, const int* p2, const int* p3, int nmodes, const double* evals, int pmax,
+                    int nk, int nsplit, double* partial, double* qs, hipStream_t st);
+
 // ---- binned statistics (aar_stats.hip)
 struct StatTerm {  // one weighted product of real fields a, b, c (c < 0: none)
   int a, b, c;

@@ -1,0 +1,209 @@
+// Modal decomposition on the device: the mode-bispectrum covariance
+// QQ[n][m] = sum_t w_t Q_n(t) Q_m(t) and the shape projections
+// QS[s][n] = sum_t S[s][t] w_t Q_n(t) of Aarambam/basis/tetrapyd.c:111-281,
+// with Q_n(t) the symmetrised product of three 1-D mode functions at the
+// tetrapyd point t (k1 = k[i1], k2 = k[i2], k3 = k[i3]):
+//   Q_n = (q1(k1) q2(k2) q3(k3) + q1(k1) q2(k3) q3(k2) + q1(k2) q2(k1) q3(k3)
+//          + q1(k2) q2(k3) q3(k1) + q1(k3) q2(k1) q3(k2) + q1(k3) q2(k2) q3(k1)) / 6.
+//
+// QQ is a dense contraction over the tetrapyd points, so it runs on the FP64
+// matrix cores (v_mfma_f64_16x16x4f64).  Q is never stored: each block keeps
+// the 1-D mode table in LDS and builds the 64 x 32 slices of w Q (rows n) and
+// Q (rows m) it needs for each chunk of points, then feeds them to the MFMA.
+// The point range is split over blocks (split-K); the per-split partial
+// matrices are summed in a fixed order, so results are deterministic.
+#include <hip/hip_runtime.h>
+
+#include "aar_common.h"
+#include "aar_kernels.h"
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int kTile = 64;   // output tile (modes n) x (modes m)
+constexpr int kChunk = 32;  // tetrapyd points per LDS slice
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ double sym_q(const double* ev, int nk, int p1, int p2, int p3, int i1, int i2,
+                                        int i3) {
+  const double m11 = ev[p1 * nk + i1], m12 = ev[p1 * nk + i2], m13 = ev[p1 * nk + i3];
+  const double m21 = ev[p2 * nk + i1], m22 = ev[p2 * nk + i2], m23 = ev[p2 * nk + i3];
+  const double m31 = ev[p3 * nk + i1], m32 = ev[p3 * nk + i2], m33 = ev[p3 * nk + i3];
+  return (m11 * m22 * m33 + m11 * m23 * m32 + m12 * m21 * m33 + m12 * m23 * m31 + m13 * m21 * m32 +
+          m13 * m22 * m31) /
+         6.0;
+}
+
+// One 64 x 64 tile (tn <= tm, upper triangle of tiles) over the point range
+// [t0, t1) of split `sp`: partial[sp][n][m] for the tile.  Four wavefronts,
+// each a 32 x 32 quadrant = 2 x 2 MFMA tiles of 16 x 16.
+__global__ void __launch_bounds__(kThreads)
+    k_mode_cov(const double* __restrict__ w, const int* __restrict__ ti1, const int* __restrict__ ti2,
+               const int* __restrict__ ti3, int npts, const int* __restrict__ mp1, const int* __restrict__ mp2,
+               const int* __restrict__ mp3, int nmodes, const double* __restrict__ evals, int pmax, int nk,
+               int ntiles, int nsplit, double* __restrict__ partial) {
+  extern __shared__ double smem[];
+  double* ev = smem;                         // pmax * nk
+  double* As = ev + pmax * nk;               // [kTile][kChunk + 1]: w Q for rows n
+  double* Bs = As + kTile * (kChunk + 1);    // [kTile][kChunk + 1]: Q for rows m
+  int* pn = (int*)(Bs + kTile * (kChunk + 1));  // [2][3][kTile] mode indices of the tile's rows
+  // upper-triangle tile pair from the linear block index
+  int tn = 0, rem = blockIdx.x;
+  while (rem >= ntiles - tn) {
+    rem -= ntiles - tn;
+    ++tn;
+  }
+  const int tm = tn + rem;
+  const int sp = blockIdx.y;
+  const long long per = ((long long)npts + nsplit - 1) / nsplit;
+  const int t0 = (int)(sp * per), t1 = (int)((sp + 1) * per < npts ? (sp + 1) * per : npts);
+  for (int q = threadIdx.x; q < pmax * nk; q += kThreads) ev[q] = evals[q];
+  for (int q = threadIdx.x; q < 2 * kTile; q += kThreads) {
+    const int side = q / kTile, r = q % kTile;
+    const int n = (side ? tm : tn) * kTile + r;
+    const bool ok = n < nmodes;
+    pn[(side * 3 + 0) * kTile + r] = ok ? mp1[n] : -1;
+    pn[(side * 3 + 1) * kTile + r] = ok ? mp2[n] : 0;
+    pn[(side * 3 + 2) * kTile + r] = ok ? mp3[n] : 0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int qr = (wv >> 1) * 32, qc = (wv & 1) * 32;  // this wavefront's quadrant
+  v4d acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = (v4d){0, 0, 0, 0};
+  for (int tc = t0; tc < t1; tc += kChunk) {
+    // build the slices: 2 x 64 x 32 values, 16 per thread; points past t1 are zero
+    for (int q = threadIdx.x; q < 2 * kTile * kChunk; q += kThreads) {
+      const int side = q / (kTile * kChunk), rr = (q / kChunk) % kTile, c = q % kChunk;
+      const int t = tc + c;
+      const int p1 = pn[(side * 3 + 0) * kTile + rr];
+      double v = 0;
+      if (t < t1 && p1 >= 0) {
+        v = sym_q(ev, nk, p1, pn[(side * 3 + 1) * kTile + rr], pn[(side * 3 + 2) * kTile + rr], ti1[t], ti2[t],
+                  ti3[t]);
+        if (side == 0) v *= w[t];
+      }
+      (side ? Bs : As)[rr * (kChunk + 1) + c] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kChunk; ks += 4) {
+      const int kk = ks + lane / 16;
+      double av[2], bv[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) av[a] = As[(qr + a * 16 + lane % 16) * (kChunk + 1) + kk];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bv[b] = Bs[(qc + b * 16 + lane % 16) * (kChunk + 1) + kk];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // D[i][j] of a 16 x 16 MFMA tile: lane l, element r -> i = l / 16 + 4 r, j = l % 16
+  double* out = partial + (long long)sp * nmodes * nmodes;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = tn * kTile + qr + a * 16 + lane / 16 + 4 * r;
+        const int m = tm * kTile + qc + b * 16 + lane % 16;
+        if (n < nmodes && m < nmodes) out[(long long)n * nmodes + m] = acc[a][b][r];
+      }
+}
+
+// cov[n][m] = sum over splits of partial[sp][min(n,m) tile rows...]: the tiles
+// hold n-tile <= m-tile; the lower triangle mirrors the upper.
+__global__ void k_mode_cov_reduce(const double* __restrict__ partial, int nsplit, int nmodes,
+                                  double* __restrict__ cov) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)nmodes * nmodes) return;
+  int n = (int)(e / nmodes), m = (int)(e % nmodes);
+  if (n / kTile > m / kTile) {
+    const int tmp = n;
+    n = m;
+    m = tmp;
+  }
+  double s = 0;
+  for (int sp = 0; sp < nsplit; ++sp) s += partial[(long long)sp * nmodes * nmodes + (long long)n * nmodes + m];
+  cov[e] = s;
+}
+
+// QS partials: block = (64 modes, one point split); thread pairs loop over
+// points; each w Q_n(t) is used for every shape s.
+__global__ void __launch_bounds__(kThreads)
+    k_mode_qs(const double* __restrict__ S, int nshapes, const double* __restrict__ w, const int* __restrict__ ti1,
+              const int* __restrict__ ti2, const int* __restrict__ ti3, int npts, const int* __restrict__ mp1,
+              const int* __restrict__ mp2, const int* __restrict__ mp3, int nmodes, const double* __restrict__ evals,
+              int pmax, int nk, int nsplit, double* __restrict__ partial) {
+  extern __shared__ double smem[];
+  double* ev = smem;
+  double* red = ev + pmax * nk;  // [kThreads]
+  for (int q = threadIdx.x; q < pmax * nk; q += kThreads) ev[q] = evals[q];
+  __syncthreads();
+  const int n = blockIdx.x;  // one mode per block
+  const int sp = blockIdx.y;
+  const long long per = ((long long)npts + nsplit - 1) / nsplit;
+  const int t0 = (int)(sp * per), t1 = (int)((sp + 1) * per < npts ? (sp + 1) * per : npts);
+  const int p1 = mp1[n], p2 = mp2[n], p3 = mp3[n];
+  for (int s = 0; s < nshapes; ++s) {
+    double acc = 0;
+    for (int t = t0 + threadIdx.x; t < t1; t += kThreads)
+      acc += S[(long long)s * npts + t] * (w[t] * sym_q(ev, nk, p1, p2, p3, ti1[t], ti2[t], ti3[t]));
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = kThreads / 2; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[((long long)sp * nshapes + s) * nmodes + n] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_mode_qs_reduce(const double* __restrict__ partial, int nsplit, int nshapes, int nmodes,
+                                 double* __restrict__ qs) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nshapes * nmodes) return;
+  double s = 0;
+  for (int sp = 0; sp < nsplit; ++sp) s += partial[(long long)sp * nshapes * nmodes + e];
+  qs[e] = s;
+}
+
+}  // namespace
+
+int modal_cov_launch(const double* w, const int* i1, const int* i2, const int* i3, int npts, const int* p1,
+                     const int* p2, const int* p3, int nmodes, const double* evals, int pmax, int nk, int nsplit,
+                     double* partial, double* cov, hipStream_t st) {
+  const int ntiles = (nmodes + kTile - 1) / kTile;
+  const int npairs = ntiles * (ntiles + 1) / 2;
+  const size_t lds = sizeof(double) * ((size_t)pmax * nk + 2 * kTile * (kChunk + 1)) + sizeof(int) * 6 * kTile;
+  if (lds > 64 * 1024) return AAR_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(k_mode_cov, dim3(npairs, nsplit), dim3(kThreads), lds, st, w, i1, i2, i3, npts, p1, p2, p3,
+                     nmodes, evals, pmax, nk, ntiles, nsplit, partial);
+  if (hipGetLastError() != hipSuccess) return AAR_ERR_HIP;
+  const long long ne = (long long)nmodes * nmodes;
+  hipLaunchKernelGGL(k_mode_cov_reduce, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, partial, nsplit,
+                     nmodes, cov);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+int modal_qs_launch(const double* S, int nshapes, const double* w, const int* i1, const int* i2, const int* i3,
+                    int npts, const int* p1, const int* p2, const int* p3, int nmodes, const double* evals, int pmax,
+                    int nk, int nsplit, double* partial, double* qs, hipStream_t st) {
+  const size_t lds = sizeof(double) * ((size_t)pmax * nk + kThreads);
+  if (lds > 64 * 1024) return AAR_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(k_mode_qs, dim3(nmodes, nsplit), dim3(kThreads), lds, st, S, nshapes, w, i1, i2, i3, npts, p1,
+                     p2, p3, nmodes, evals, pmax, nk, nsplit, partial);
+  if (hipGetLastError() != hipSuccess) return AAR_ERR_HIP;
+  hipLaunchKernelGGL(k_mode_qs_reduce, dim3((unsigned)((nshapes * nmodes + 255) / 256)), dim3(256), 0, st, partial,
+                     nsplit, nshapes, nmodes, qs);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}

@@ -169,6 +169,27 @@ int aar_job_destroy(aar_job *job);
 int aar_job_ng_statistics(aar_job *job, int nb, const double *edges, double *pk_meas, double *pk_theory,
                           double *kmean, double *nmodes, double *b_meas, double *b_theory, double *ntri);
 
+/* ---- modal decomposition on the device ----
+ * The two dense contractions of the decomposition that produces the alpha
+ * tables, with the reference's argument lists (host arrays):
+ *   aar_mode_bispectra_covariance  Aarambam/basis/tetrapyd.c:111-208
+ *     (compute_mode_bispectra_covariance): cov[n*n_modes + m] =
+ *     sum_t w_t Q_n(t) Q_m(t), Q_n the symmetrised product of the 1-D mode
+ *     functions mode_evals[p*k_npts + i] at the tetrapyd point
+ *     (tetra_i1, i2, i3)[t]; runs on the FP64 matrix cores.
+ *   aar_mode_QS                    Aarambam/basis/tetrapyd.c:210-281 (compute_QS):
+ *     QS[s*n_modes + n] = sum_t S[s*tetra_npts + t] w_t Q_n(t).
+ * Indices outside [0, k_npts) or [0, mode_p_max) are rejected
+ * (AAR_ERR_BAD_ARG); mode_p_max * k_npts <= about 6000 (the table sits in LDS). */
+int aar_mode_bispectra_covariance(aar_ctx *ctx, double *mode_bispectra_covariance, const double *tetra_weights,
+                                  const int *tetra_i1, const int *tetra_i2, const int *tetra_i3, int tetra_npts,
+                                  const int *mode_p1, const int *mode_p2, const int *mode_p3, int n_modes,
+                                  const double *mode_evals, int mode_p_max, int k_npts);
+int aar_mode_QS(aar_ctx *ctx, double *QS, const double *S, int n_shapes, const double *tetra_weights,
+                const int *tetra_i1, const int *tetra_i2, const int *tetra_i3, int tetra_npts, const int *mode_p1,
+                const int *mode_p2, const int *mode_p3, int n_modes, const double *mode_evals, int mode_p_max,
+                int k_npts);
+
 /* ---- serving: continuous batching of IC jobs on one device ----
  * No counterpart in the reference, which runs one IC per process
  * (main.c:53-102); every job keeps the semantics of aar_job_prepare/run.

@@ -1021,3 +1021,57 @@ int orc_run(const orc_params *p, const double *alpha, const double *x0, const fl
   for (int a = 0; a < 6; a++) free(f[a]);
   return 0;
 }
+
+
+/* ------------------------------------------------------------ modal decomposition
+ * Restatement of compute_mode_bispectra_covariance and compute_QS
+ * (Aarambam/basis/tetrapyd.c:111-208, 210-281): the symmetrised mode product
+ * with the same six-term order and the 1/6, weights applied to the first
+ * factor, sums over points in increasing t. */
+static double orc_sym_q(const double *ev, int nk, int p1, int p2, int p3, int i1, int i2, int i3) {
+  const double m11 = ev[p1 * nk + i1], m12 = ev[p1 * nk + i2], m13 = ev[p1 * nk + i3];
+  const double m21 = ev[p2 * nk + i1], m22 = ev[p2 * nk + i2], m23 = ev[p2 * nk + i3];
+  const double m31 = ev[p3 * nk + i1], m32 = ev[p3 * nk + i2], m33 = ev[p3 * nk + i3];
+  return (m11 * m22 * m33 + m11 * m23 * m32 + m12 * m21 * m33 + m12 * m23 * m31 + m13 * m21 * m32 +
+          m13 * m22 * m31) / 6.0;
+}
+
+void orc_mode_covariance(double *cov, const double *w, const int *i1, const int *i2, const int *i3, int npts,
+                         const int *p1, const int *p2, const int *p3, int n_modes, const double *evals, int pmax,
+                         int nk) {
+  (void)pmax;
+#pragma omp parallel
+  {
+    double *qv = (double *)malloc(sizeof(double) * (size_t)npts);
+#pragma omp for schedule(dynamic)
+    for (int n = 0; n < n_modes; n++) {
+      for (int t = 0; t < npts; t++) qv[t] = w[t] * orc_sym_q(evals, nk, p1[n], p2[n], p3[n], i1[t], i2[t], i3[t]);
+      for (int m = 0; m < n_modes; m++) {
+        double sum = 0.0;
+        for (int t = 0; t < npts; t++) sum += orc_sym_q(evals, nk, p1[m], p2[m], p3[m], i1[t], i2[t], i3[t]) * qv[t];
+        cov[(size_t)n * n_modes + m] = sum;
+      }
+    }
+    free(qv);
+  }
+}
+
+void orc_mode_qs(double *qs, const double *S, int n_shapes, const double *w, const int *i1, const int *i2,
+                 const int *i3, int npts, const int *p1, const int *p2, const int *p3, int n_modes,
+                 const double *evals, int pmax, int nk) {
+  (void)pmax;
+#pragma omp parallel
+  {
+    double *qv = (double *)malloc(sizeof(double) * (size_t)npts);
+#pragma omp for schedule(dynamic)
+    for (int n = 0; n < n_modes; n++) {
+      for (int t = 0; t < npts; t++) qv[t] = w[t] * orc_sym_q(evals, nk, p1[n], p2[n], p3[n], i1[t], i2[t], i3[t]);
+      for (int s = 0; s < n_shapes; s++) {
+        double sum = 0.0;
+        for (int t = 0; t < npts; t++) sum += S[(size_t)s * npts + t] * qv[t];
+        qs[(size_t)s * n_modes + n] = sum;
+      }
+    }
+    free(qv);
+  }
+}

@@ -112,6 +112,17 @@ int orc_run(const orc_params *p, const double *alpha, const double *x0, const fl
             int nglass, double glass_box, float *pos, float *vel, int *ids,
             double *phig_spec, double *phi_spec, double *phig_real, double *phing_real);
 
+/* Modal decomposition contractions (Aarambam/basis/tetrapyd.c:111-208, 210-281):
+ * cov[n*n_modes + m] = sum_t w_t Q_n(t) Q_m(t) and QS[s*n_modes + n] =
+ * sum_t S[s*npts + t] w_t Q_n(t), Q_n the symmetrised product of three 1-D
+ * mode functions, accumulated over t in order in double. */
+void orc_mode_covariance(double *cov, const double *w, const int *i1, const int *i2, const int *i3, int npts,
+                         const int *p1, const int *p2, const int *p3, int n_modes, const double *evals, int pmax,
+                         int nk);
+void orc_mode_qs(double *qs, const double *S, int n_shapes, const double *w, const int *i1, const int *i2,
+                 const int *i3, int npts, const int *p1, const int *p2, const int *p3, int n_modes,
+                 const double *evals, int pmax, int nk);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,8 +76,54 @@ def lib():
                                     fp, fp]
         L.orc_run.argtypes = [P, dp, dp, fp, C.c_int, C.c_double, fp, fp, C.c_void_p,
                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        modal = [dp, ip, ip, ip, C.c_int, ip, ip, ip, C.c_int, dp, C.c_int, C.c_int]
+        L.orc_mode_covariance.argtypes = [dp] + modal
+        L.orc_mode_qs.argtypes = [dp, dp, C.c_int] + modal
         _lib = L
     return _lib
+
+
+REF_PATH = os.path.join(HERE, "_ref", "libtetrapyd_ref.so")
+_ref = None
+
+
+def ref_lib():
+    """The reference's own tetrapyd.c/arrays.c compiled by `make -C oracle ref` (None if not built)."""
+    global _ref
+    if _ref is None and os.path.exists(REF_PATH):
+        L = C.CDLL(REF_PATH)
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        modal = [dp, ip, ip, ip, C.c_int, ip, ip, ip, C.c_int, dp, C.c_int, C.c_int]
+        L.compute_mode_bispectra_covariance.argtypes = [dp] + modal
+        L.compute_QS.argtypes = [dp, dp, C.c_int] + modal
+        _ref = L
+    return _ref
+
+
+def modal_args(grid):
+    """Argument tuple (w, i1, i2, i3, npts, p1, p2, p3, n_modes, evals, pmax, nk) of a modal grid dict."""
+    c = lambda a, t: np.ascontiguousarray(a, dtype=t)
+    return (c(grid["w"], np.float64), c(grid["i1"], np.int32), c(grid["i2"], np.int32), c(grid["i3"], np.int32),
+            len(grid["w"]), c(grid["p1"], np.int32), c(grid["p2"], np.int32), c(grid["p3"], np.int32),
+            len(grid["p1"]), c(grid["evals"], np.float64).reshape(-1), grid["evals"].shape[0], grid["evals"].shape[1])
+
+
+def mode_covariance(grid, use_ref=False):
+    a = modal_args(grid)
+    out = np.zeros(a[8] * a[8])
+    L = ref_lib() if use_ref else lib()
+    (L.compute_mode_bispectra_covariance if use_ref else L.orc_mode_covariance)(out, *a)
+    return out.reshape(a[8], a[8])
+
+
+def mode_qs(grid, S, use_ref=False):
+    a = modal_args(grid)
+    S = np.ascontiguousarray(S, dtype=np.float64).reshape(-1, a[4])
+    out = np.zeros(S.shape[0] * a[8])
+    L = ref_lib() if use_ref else lib()
+    (L.compute_QS if use_ref else L.orc_mode_qs)(out, S.reshape(-1), S.shape[0], *a)
+    return out.reshape(S.shape[0], a[8])
 
 
 def make_params(cfg):
