@@ -170,6 +170,40 @@ def serving(aar, ctx, n_jobs=64, load=0.75, budget_gib=96, max_active=4, seed=20
                    "from PCG64(2024), FP32 batched, Lbox = 2000 Mpc/h * Nmesh/512"}
 
 
+def decomposition(aar, ctx, p_max=16, nk=150, reps=3):
+    """The decomposition's mode-bispectrum covariance (Aarambam/basis/tetrapyd.c:111-208) on the
+    FP64 matrix cores, at the Basis sizes of the reference's decompose.py (k_grid_size 150) with 16
+    modes (816 3-D modes); the reference's own C (oracle/_ref) timed beside it on a 40-point grid."""
+    from oracle import pyoracle as orc
+    from tests import modal
+    g = modal.basis_grid(p_max=p_max, k_grid_size=nk)
+    n, T = len(g["p1"]), len(g["w"])
+    ctx.mode_covariance(g)  # warm
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ctx.mode_covariance(g)
+        ts.append(time.perf_counter() - t0)
+    dt = min(ts)
+    nt = (n + 63) // 64
+    flops = 2.0 * (nt * (nt + 1) // 2) * 64 * 64 * T  # MFMA work of the upper-triangle tiles
+    out = {"modes_3d": n, "tetrapyd_points": T, "ms": round(dt * 1e3, 2),
+           "mfma_tflops": round(flops / dt / 1e12, 2), "fp64_mfma_peak_tflops": 78.6,
+           "note": "wall time of aar_mode_bispectra_covariance incl. input upload and result copy"}
+    small = modal.basis_grid(p_max=p_max, k_grid_size=40)
+    t0 = time.perf_counter()
+    ctx.mode_covariance(small)
+    gdt = time.perf_counter() - t0
+    lib = orc.ref_lib()
+    t0 = time.perf_counter()
+    orc.mode_covariance(small, use_ref=lib is not None)
+    cdt = time.perf_counter() - t0
+    out["cpu_sample"] = {"kind": "reference" if lib is not None else "port", "k_grid": 40,
+                         "points": len(small["w"]), "cpu_s": round(cdt, 3), "gpu_s": round(gdt, 4),
+                         "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+    return out
+
+
 def copy_peak(ctx, N=512, nf=8, iters=10):
     """HBM bandwidth of a plain streaming copy kernel (reads + writes nf spectra)."""
     sec = ctx.bench_pass(N, 4, nf, iters)
@@ -220,6 +254,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=64)
+    ap.add_argument("--no-decomp", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs one IC job per GPU on a single GPU; multi-GPU runs are not "
@@ -280,6 +315,8 @@ def main():
     out["copy_peak_gbs"] = round(copy_peak(ctx, min(N, 512)), 1)
     if not args.no_serve:
         out["serving"] = serving(aar, ctx, args.serve_jobs)
+    if not args.no_decomp:
+        out["decomposition"] = decomposition(aar, ctx)
     ctx.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
