@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def test_library_exports_every_declared_symbol():
     hdr = open(os.path.join(ROOT, "include", "aarambam.h")).read()
-    names = set(re.findall(r"\b(aar_[a-z0-9_]+)\s*\(", hdr))
+    names = set(re.findall(r"\b(aar_[A-Za-z0-9_]+)\s*\(", hdr))
     assert len(names) >= 25
     L = aar.lib()
     missing = [n for n in names if not hasattr(L, n)]
