@@ -7,9 +7,10 @@
 //          + q1(k2) q2(k3) q3(k1) + q1(k3) q2(k1) q3(k2) + q1(k3) q2(k2) q3(k1)) / 6.
 //
 // QQ is a dense contraction over the tetrapyd points, so it runs on the FP64
-// matrix cores (v_mfma_f64_16x16x4f64).  Q is never stored: each block keeps
-// the 1-D mode table in LDS and builds the 64 x 32 slices of w Q (rows n) and
-// Q (rows m) it needs for each chunk of points, then feeds them to the MFMA.
+// matrix cores (v_mfma_f64_16x16x4f64).  Q is never stored: for each chunk of
+// points a block gathers the 1-D modes at the chunk's k1, k2, k3 into LDS and
+// builds the 64 x 32 slices of w Q (rows n) and Q (rows m) from them, then
+// feeds them to the MFMA.
 // The point range is split over blocks (split-K); the per-split partial
 // matrices are summed in a fixed order, so results are deterministic.
 #include <hip/hip_runtime.h>
@@ -38,17 +39,20 @@ __device__ __forceinline__ double sym_q(const double* ev, int nk, int p1, int p2
 
 // One 64 x 64 tile (tn <= tm, upper triangle of tiles) over the point range
 // [t0, t1) of split `sp`: partial[sp][n][m] for the tile.  Four wavefronts,
-// each a 32 x 32 quadrant = 2 x 2 MFMA tiles of 16 x 16.
+// each a 32 x 32 quadrant = 2 x 2 MFMA tiles of 16 x 16.  Per chunk of points
+// the block first gathers every 1-D mode at the chunk's k1, k2, k3 into LDS
+// (E[j][p][c]), so building a slice value is nine LDS reads and the products.
 __global__ void __launch_bounds__(kThreads)
     k_mode_cov(const double* __restrict__ w, const int* __restrict__ ti1, const int* __restrict__ ti2,
                const int* __restrict__ ti3, int npts, const int* __restrict__ mp1, const int* __restrict__ mp2,
                const int* __restrict__ mp3, int nmodes, const double* __restrict__ evals, int pmax, int nk,
                int ntiles, int nsplit, double* __restrict__ partial) {
   extern __shared__ double smem[];
-  double* ev = smem;                         // pmax * nk
-  double* As = ev + pmax * nk;               // [kTile][kChunk + 1]: w Q for rows n
-  double* Bs = As + kTile * (kChunk + 1);    // [kTile][kChunk + 1]: Q for rows m
-  int* pn = (int*)(Bs + kTile * (kChunk + 1));  // [2][3][kTile] mode indices of the tile's rows
+  double* As = smem;                          // [kTile][kChunk + 1]: w Q for rows n
+  double* Bs = As + kTile * (kChunk + 1);     // [kTile][kChunk + 1]: Q for rows m
+  double* Ev = Bs + kTile * (kChunk + 1);     // [3][pmax][kChunk]: mode p at k_j of point c
+  double* wc = Ev + 3 * pmax * kChunk;        // [kChunk]
+  int* pn = (int*)(wc + kChunk);              // [2][3][kTile] mode indices of the tile's rows
   // upper-triangle tile pair from the linear block index
   int tn = 0, rem = blockIdx.x;
   while (rem >= ntiles - tn) {
@@ -59,7 +63,6 @@ __global__ void __launch_bounds__(kThreads)
   const int sp = blockIdx.y;
   const long long per = ((long long)npts + nsplit - 1) / nsplit;
   const int t0 = (int)(sp * per), t1 = (int)((sp + 1) * per < npts ? (sp + 1) * per : npts);
-  for (int q = threadIdx.x; q < pmax * nk; q += kThreads) ev[q] = evals[q];
   for (int q = threadIdx.x; q < 2 * kTile; q += kThreads) {
     const int side = q / kTile, r = q % kTile;
     const int n = (side ? tm : tn) * kTile + r;
@@ -68,7 +71,6 @@ __global__ void __launch_bounds__(kThreads)
     pn[(side * 3 + 1) * kTile + r] = ok ? mp2[n] : 0;
     pn[(side * 3 + 2) * kTile + r] = ok ? mp3[n] : 0;
   }
-  __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int qr = (wv >> 1) * 32, qc = (wv & 1) * 32;  // this wavefront's quadrant
   v4d acc[2][2];
@@ -77,16 +79,33 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = (v4d){0, 0, 0, 0};
   for (int tc = t0; tc < t1; tc += kChunk) {
-    // build the slices: 2 x 64 x 32 values, 16 per thread; points past t1 are zero
+    // the modes at the chunk's points (zero past t1)
+    for (int q = threadIdx.x; q < 3 * pmax * kChunk; q += kThreads) {
+      const int j = q / (pmax * kChunk), p = (q / kChunk) % pmax, c = q % kChunk;
+      const int t = tc + c;
+      double v = 0;
+      if (t < t1) v = evals[p * nk + (j == 0 ? ti1[t] : (j == 1 ? ti2[t] : ti3[t]))];
+      Ev[q] = v;
+    }
+    for (int c = threadIdx.x; c < kChunk; c += kThreads) wc[c] = tc + c < t1 ? w[tc + c] : 0.0;
+    __syncthreads();
+    // the slices: 2 x 64 x 32 values, 16 per thread
     for (int q = threadIdx.x; q < 2 * kTile * kChunk; q += kThreads) {
       const int side = q / (kTile * kChunk), rr = (q / kChunk) % kTile, c = q % kChunk;
-      const int t = tc + c;
       const int p1 = pn[(side * 3 + 0) * kTile + rr];
       double v = 0;
-      if (t < t1 && p1 >= 0) {
-        v = sym_q(ev, nk, p1, pn[(side * 3 + 1) * kTile + rr], pn[(side * 3 + 2) * kTile + rr], ti1[t], ti2[t],
-                  ti3[t]);
-        if (side == 0) v *= w[t];
+      if (p1 >= 0) {
+        const int p2 = pn[(side * 3 + 1) * kTile + rr], p3 = pn[(side * 3 + 2) * kTile + rr];
+        const double* E1 = Ev + c;
+        const double* E2 = Ev + pmax * kChunk + c;
+        const double* E3 = Ev + 2 * pmax * kChunk + c;
+        const double m11 = E1[p1 * kChunk], m12 = E2[p1 * kChunk], m13 = E3[p1 * kChunk];
+        const double m21 = E1[p2 * kChunk], m22 = E2[p2 * kChunk], m23 = E3[p2 * kChunk];
+        const double m31 = E1[p3 * kChunk], m32 = E2[p3 * kChunk], m33 = E3[p3 * kChunk];
+        v = (m11 * m22 * m33 + m11 * m23 * m32 + m12 * m21 * m33 + m12 * m23 * m31 + m13 * m21 * m32 +
+             m13 * m22 * m31) *
+            (1.0 / 6.0);
+        if (side == 0) v *= wc[c];
       }
       (side ? Bs : As)[rr * (kChunk + 1) + c] = v;
     }
@@ -185,7 +204,8 @@ int modal_cov_launch(const double* w, const int* i1, const int* i2, const int* i
                      double* partial, double* cov, hipStream_t st) {
   const int ntiles = (nmodes + kTile - 1) / kTile;
   const int npairs = ntiles * (ntiles + 1) / 2;
-  const size_t lds = sizeof(double) * ((size_t)pmax * nk + 2 * kTile * (kChunk + 1)) + sizeof(int) * 6 * kTile;
+  const size_t lds = sizeof(double) * (2 * kTile * (kChunk + 1) + 3 * (size_t)pmax * kChunk + kChunk) +
+                     sizeof(int) * 6 * kTile;
   if (lds > 64 * 1024) return AAR_ERR_UNSUPPORTED;
   hipLaunchKernelGGL(k_mode_cov, dim3(npairs, nsplit), dim3(kThreads), lds, st, w, i1, i2, i3, npts, p1, p2, p3,
                      nmodes, evals, pmax, nk, ntiles, nsplit, partial);

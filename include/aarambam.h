@@ -180,7 +180,8 @@ int aar_job_ng_statistics(aar_job *job, int nb, const double *edges, double *pk_
  *   aar_mode_QS                    Aarambam/basis/tetrapyd.c:210-281 (compute_QS):
  *     QS[s*n_modes + n] = sum_t S[s*tetra_npts + t] w_t Q_n(t).
  * Indices outside [0, k_npts) or [0, mode_p_max) are rejected
- * (AAR_ERR_BAD_ARG); mode_p_max * k_npts <= about 6000 (the table sits in LDS). */
+ * (AAR_ERR_BAD_ARG).  Limits from the LDS the kernels use: mode_p_max <= 39 for
+ * the covariance, mode_p_max * k_npts <= 7936 for QS (AAR_ERR_UNSUPPORTED). */
 int aar_mode_bispectra_covariance(aar_ctx *ctx, double *mode_bispectra_covariance, const double *tetra_weights,
                                   const int *tetra_i1, const int *tetra_i2, const int *tetra_i3, int tetra_npts,
                                   const int *mode_p1, const int *mode_p2, const int *mode_p3, int n_modes,
