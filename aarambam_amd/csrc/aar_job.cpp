@@ -1207,7 +1207,7 @@ extern "C" int aar_mode_bispectra_covariance(aar_ctx* ctx, double* cov, const do
   ModalBufs b;
   if ((rc = modal_upload(b, w, i1, i2, i3, npts, p1, p2, p3, nmodes, evals, pmax, nk, st))) return rc;
   // split the points so that the launch has about 2048 blocks, at least 256 points each
-  const int ntiles = (nmodes + 63) / 64, npairs = ntiles * (ntiles + 1) / 2;
+  const int ntiles = (nmodes + 127) / 128, npairs = ntiles * (ntiles + 1) / 2;
   int nsplit = std::max(1, std::min((2048 + npairs - 1) / npairs, 64));
   nsplit = std::max(1, std::min(nsplit, npts / 256));
   if (device_alloc((void**)&b.part, sizeof(double) * (size_t)nsplit * nmodes * nmodes) ||

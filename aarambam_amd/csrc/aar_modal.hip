@@ -22,8 +22,10 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int kTile = 64;   // output tile (modes n) x (modes m)
-constexpr int kChunk = 32;  // tetrapyd points per LDS slice
+constexpr int kTile = 128;  // output tile (modes n) x (modes m)
+constexpr int kChunk = 16;  // tetrapyd points per LDS slice
+constexpr int kQuad = kTile / 2;    // each wavefront owns a kQuad x kQuad quadrant
+constexpr int kSub = kQuad / 16;    // 16 x 16 MFMA tiles per quadrant side
 constexpr int kThreads = 256;
 
 __device__ __forceinline__ double sym_q(const double* ev, int nk, int p1, int p2, int p3, int i1, int i2,
@@ -37,9 +39,9 @@ __device__ __forceinline__ double sym_q(const double* ev, int nk, int p1, int p2
          (1.0 / 6.0);
 }
 
-// One 64 x 64 tile (tn <= tm, upper triangle of tiles) over the point range
-// [t0, t1) of split `sp`: partial[sp][n][m] for the tile.  Four wavefronts,
-// each a 32 x 32 quadrant = 2 x 2 MFMA tiles of 16 x 16.  Per chunk of points
+// One kTile x kTile tile (tn <= tm, upper triangle of tiles) over the point
+// range [t0, t1) of split `sp`: partial[sp][n][m] for the tile.  Four
+// wavefronts, each a quadrant of kSub x kSub MFMA tiles of 16 x 16.  Per chunk of points
 // the block first gathers every 1-D mode at the chunk's k1, k2, k3 into LDS
 // (E[j][p][c]), so building a slice value is nine LDS reads and the products.
 __global__ void __launch_bounds__(kThreads)
@@ -72,12 +74,12 @@ __global__ void __launch_bounds__(kThreads)
     pn[(side * 3 + 2) * kTile + r] = ok ? mp3[n] : 0;
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int qr = (wv >> 1) * 32, qc = (wv & 1) * 32;  // this wavefront's quadrant
-  v4d acc[2][2];
+  const int qr = (wv >> 1) * kQuad, qc = (wv & 1) * kQuad;  // this wavefront's quadrant
+  v4d acc[kSub][kSub];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < kSub; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = (v4d){0, 0, 0, 0};
+    for (int b = 0; b < kSub; ++b) acc[a][b] = (v4d){0, 0, 0, 0};
   for (int tc = t0; tc < t1; tc += kChunk) {
     // the modes at the chunk's points (zero past t1)
     for (int q = threadIdx.x; q < 3 * pmax * kChunk; q += kThreads) {
@@ -89,7 +91,7 @@ __global__ void __launch_bounds__(kThreads)
     }
     for (int c = threadIdx.x; c < kChunk; c += kThreads) wc[c] = tc + c < t1 ? w[tc + c] : 0.0;
     __syncthreads();
-    // the slices: 2 x 64 x 32 values, 16 per thread
+    // the slices: 2 x kTile x kChunk values
     for (int q = threadIdx.x; q < 2 * kTile * kChunk; q += kThreads) {
       const int side = q / (kTile * kChunk), rr = (q / kChunk) % kTile, c = q % kChunk;
       const int p1 = pn[(side * 3 + 0) * kTile + rr];
@@ -113,24 +115,25 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
     for (int ks = 0; ks < kChunk; ks += 4) {
       const int kk = ks + lane / 16;
-      double av[2], bv[2];
+      double av[kSub], bv[kSub];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) av[a] = As[(qr + a * 16 + lane % 16) * (kChunk + 1) + kk];
+      for (int a = 0; a < kSub; ++a) av[a] = As[(qr + a * 16 + lane % 16) * (kChunk + 1) + kk];
 #pragma unroll
-      for (int b = 0; b < 2; ++b) bv[b] = Bs[(qc + b * 16 + lane % 16) * (kChunk + 1) + kk];
+      for (int b = 0; b < kSub; ++b) bv[b] = Bs[(qc + b * 16 + lane % 16) * (kChunk + 1) + kk];
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < kSub; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+        for (int b = 0; b < kSub; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
     __syncthreads();
   }
   // D[i][j] of a 16 x 16 MFMA tile: lane l, element r -> i = l / 16 + 4 r, j = l % 16
   double* out = partial + (long long)sp * nmodes * nmodes;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < kSub; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < kSub; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int n = tn * kTile + qr + a * 16 + lane / 16 + 4 * r;
