@@ -25,9 +25,14 @@ _STRS = ["GlassFile", "FileWithInputSpectrum", "FileWithInputTransfer", "FileWit
          "FileWithX0Coeff", "OutputDir", "FileBase"]
 
 
+_RES_INT = ["Variant"]  # 0 Basis, 1 ResBasis
+_RES_DBL = ["A_res", "w_res", "phi_res"]
+
+
 class AarParams(C.Structure):
     _fields_ = ([(n, C.c_int) for n in _INTS] + [(n, C.c_double) for n in _DBLS] +
-                [(n, C.c_char * 500) for n in _STRS])
+                [(n, C.c_char * 500) for n in _STRS] + [(n, C.c_int) for n in _RES_INT] +
+                [(n, C.c_double) for n in _RES_DBL])
 
 
 class AarTables(C.Structure):
@@ -43,7 +48,7 @@ class AarCosmo(C.Structure):
                  "Beta", "hubble_a", "vel_prefac", "vel_prefac2", "fac"]]
 
 
-EXPORTS = ["aar_status_string", "aar_read_params", "aar_load_alpha", "aar_load_x0", "aar_load_transfer",
+EXPORTS = ["aar_status_string", "aar_read_params", "aar_read_params_res", "aar_load_alpha", "aar_load_x0", "aar_load_transfer",
            "aar_read_glass", "aar_cosmology", "aar_seedtable", "aar_ranlxd_nth", "aar_write_gadget",
            "aar_write_phi", "aar_write_inputspec", "aar_ctx_create", "aar_ctx_destroy", "aar_ctx_sync", "aar_job_bytes",
            "aar_job_prepare", "aar_job_run", "aar_job_sync", "aar_job_device_output", "aar_job_fetch",
@@ -78,6 +83,7 @@ def lib():
         sig = {
             "aar_status_string": (C.c_char_p, [C.c_int]),
             "aar_read_params": (C.c_int, [C.c_char_p, PP, C.c_char_p, C.c_size_t]),
+            "aar_read_params_res": (C.c_int, [C.c_char_p, PP, C.c_char_p, C.c_size_t]),
             "aar_load_alpha": (C.c_int, [C.c_char_p, C.c_int, dp]),
             "aar_load_x0": (C.c_int, [C.c_char_p, C.c_int, dp]),
             "aar_load_transfer": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), dp, dp]),
@@ -146,9 +152,9 @@ def _dptr(a):
 
 def params_from_dict(cfg):
     p = AarParams()
-    for n in _INTS:
+    for n in _INTS + _RES_INT:
         setattr(p, n, int(cfg.get(n, 0)))
-    for n in _DBLS:
+    for n in _DBLS + _RES_DBL:
         setattr(p, n, float(cfg.get(n, 0.0)))
     for n in _STRS:
         setattr(p, n, str(cfg.get(n, "")).encode())
@@ -156,15 +162,17 @@ def params_from_dict(cfg):
 
 
 def params_to_dict(p):
-    d = {n: getattr(p, n) for n in _INTS + _DBLS}
+    d = {n: getattr(p, n) for n in _INTS + _DBLS + _RES_INT + _RES_DBL}
     d.update({n: getattr(p, n).decode() for n in _STRS})
     return d
 
 
-def read_params(path):
+def read_params(path, res=False):
+    """Parse a parameter file: the Basis tag set, or with res=True the ResBasis one."""
     p = AarParams()
     msg = C.create_string_buffer(4096)
-    rc = lib().aar_read_params(path.encode(), C.byref(p), msg, 4096)
+    fn = lib().aar_read_params_res if res else lib().aar_read_params
+    rc = fn(path.encode(), C.byref(p), msg, 4096)
     if rc:
         raise AarError(rc, msg.value.decode())
     return p

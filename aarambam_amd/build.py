@@ -1,4 +1,5 @@
-"""Build the HIP library (libaarambam_hip.so) and the 2LPTBasis executable in-tree for gfx950.
+"""Build the HIP library (libaarambam_hip.so) and the 2LPTBasis / 2LPTResBasis executables
+in-tree for gfx950.
 
 Usage: python -m aarambam_amd.build   (or aarambam_amd.build.build())
 """
@@ -12,6 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libaarambam_hip.so")
 EXE = os.path.join(HERE, "bin", "2LPTBasis")
+EXE_RES = os.path.join(HERE, "bin", "2LPTResBasis")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["aar_kernels.hip", "aar_stats.hip", "aar_modal.hip", "aar_job.cpp", "aar_host.cpp", "aar_serve.cpp"]
@@ -49,8 +51,9 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     main_src = os.path.join(CSRC, "aar_main.cpp")
     if os.path.exists(main_src):
-        _run([HIPCC] + FLAGS + ["-x", "hip", main_src, "-o", EXE, "-L" + HERE, "-laarambam_hip",
-                                "-Wl,-rpath,$ORIGIN/.."])
+        for exe, defs in [(EXE, []), (EXE_RES, ["-DAAR_RESBASIS"])]:
+            _run([HIPCC] + FLAGS + defs + ["-x", "hip", main_src, "-o", exe, "-L" + HERE, "-laarambam_hip",
+                                           "-Wl,-rpath,$ORIGIN/.."])
     if verbose:
         print("built", LIB)
     return LIB

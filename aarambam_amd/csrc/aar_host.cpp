@@ -45,7 +45,10 @@ extern "C" const char* aar_status_string(int s) {
 }
 
 // ------------------------------------------------------------------ parameter file
-extern "C" int aar_read_params(const char* path, aar_params* out, char* msg, size_t msglen) {
+// read_parameterfile of Aarambam/LPT/Basis/read_param.c:8-262; with res the
+// tag set of Aarambam/LPT/ResBasis/read_param.c:8-274 (A_res, w_res, phi_res
+// follow SavePotentialField, which fixes the order of the missing-tag report).
+static int read_params(const char* path, aar_params* out, char* msg, size_t msglen, bool res) {
   if (!path || !out) return AAR_ERR_BAD_ARG;
   enum { F, S, I };
   struct Tag {
@@ -72,6 +75,9 @@ extern "C" int aar_read_params(const char* path, aar_params* out, char* msg, siz
       {"FileWithAlphaCoeff", S, p.FileWithAlphaCoeff},
       {"FileWithX0Coeff", S, p.FileWithX0Coeff},
       {"SavePotentialField", I, &p.SavePotentialField},
+      {"A_res", F, &p.A_res},
+      {"w_res", F, &p.w_res},
+      {"phi_res", F, &p.phi_res},
       {"FixedAmplitude", I, &p.FixedAmplitude},
       {"PhaseFlip", I, &p.PhaseFlip},
       {"Nmesh", I, &p.Nmesh},
@@ -96,6 +102,10 @@ extern "C" int aar_read_params(const char* path, aar_params* out, char* msg, siz
       {"WDM_Vtherm_On", I, &p.WDM_Vtherm_On},
       {"WDM_PartMass_in_kev", F, &p.WDM_PartMass_in_kev},
   };
+  if (!res)  // Basis: the resonance tags are "not allowed"
+    tags.erase(std::remove_if(tags.begin(), tags.end(), [&](const Tag& t) { return t.addr == &p.A_res ||
+                                t.addr == &p.w_res || t.addr == &p.phi_res; }),
+               tags.end());
   std::string report;
   bool error = false;
   FILE* fd = fopen(path, "r");
@@ -143,8 +153,17 @@ extern "C" int aar_read_params(const char* path, aar_params* out, char* msg, siz
     strncpy(msg, report.c_str(), msglen - 1);
     msg[msglen - 1] = 0;
   }
+  p.Variant = res ? 1 : 0;
   *out = p;
   return error ? 100010 : AAR_OK;
+}
+
+extern "C" int aar_read_params(const char* path, aar_params* out, char* msg, size_t msglen) {
+  return read_params(path, out, msg, msglen, false);
+}
+
+extern "C" int aar_read_params_res(const char* path, aar_params* out, char* msg, size_t msglen) {
+  return read_params(path, out, msg, msglen, true);
 }
 
 // ------------------------------------------------------------------ tables
@@ -363,7 +382,12 @@ struct Cosmo {
 
   double powerspec(double k) const {  // power.c:240-288
     double power;
-    if (p.WhichSpectrum == 0) {
+    if (p.Variant == 1) {  // ResBasis/power.c:240-260: oscillating primordial P(k) times T(k)^2
+      power = norm * exp(p.PrimordialIndex * log(k));
+      if (p.WhichSpectrum == 1) power *= 1 + p.A_res * cos(p.w_res * k + p.phi_res);
+      if (p.WhichSpectrum == 2) power *= 1 + p.A_res * cos(p.w_res * log(k) + p.phi_res);
+      power *= transfer(k) * transfer(k);
+    } else if (p.WhichSpectrum == 0) {
       power = norm * exp(p.PrimordialIndex * log(k)) * transfer(k) * transfer(k);
     } else {  // 1: Eisenstein & Hu spectrum
       power = norm * k * pow(tk_eh(k), 2);
@@ -462,7 +486,11 @@ struct Cosmo {
 
 extern "C" int aar_cosmology(const aar_params* p, const aar_tables* t, aar_cosmo* c) {
   if (!p || !c) return AAR_ERR_BAD_ARG;
-  if (p->WhichSpectrum != 0 && p->WhichSpectrum != 1) return AAR_ERR_UNSUPPORTED;
+  // Basis: 0 (T(k) from WhichTransfer) and 1 (Eisenstein & Hu); ResBasis: 0, 1, 2
+  // (its PowerSpec leaves every other value undefined, ResBasis/power.c:244-258).
+  const int ws_max = p->Variant == 1 ? 2 : 1;
+  if (p->Variant != 0 && p->Variant != 1) return AAR_ERR_BAD_ARG;
+  if (p->WhichSpectrum < 0 || p->WhichSpectrum > ws_max) return AAR_ERR_UNSUPPORTED;
   if (p->WhichTransfer == 3) return 117;  // power.c:45-50
   Cosmo cm(*p, t);
   if (cm.status) return cm.status;
@@ -479,7 +507,8 @@ extern "C" int aar_cosmology(const aar_params* p, const aar_tables* t, aar_cosmo
   c->Norm = p->Sigma8 * p->Sigma8 / res;
   c->D0 = cm.growth_factor(1.e-8, 1.0) * 1.e-8;
   c->Dplus = cm.growth_factor(c->InitTime, 1.0);
-  c->Anorm = p->WhichSpectrum == 0 ? c->Norm : 0.0;
+  // Basis sets Anorm for WhichSpectrum 0 only (power.c:426-446); ResBasis always (power.c:418)
+  c->Anorm = (p->Variant == 1 || p->WhichSpectrum == 0) ? c->Norm : 0.0;
   const double a = c->InitTime;
   c->hubble_a = c->Hubble * sqrt(p->Omega / pow(a, 3) + (1 - p->Omega - p->OmegaLambda) / pow(a, 2) +
                                  p->OmegaLambda);

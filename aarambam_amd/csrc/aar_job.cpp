@@ -583,8 +583,8 @@ static int ctx_cosmology(aar_ctx* ctx, const aar_params* p, const aar_tables* t,
   const double d[] = {p->Box, p->Omega, p->OmegaLambda, p->OmegaBaryon, p->OmegaDM_2ndSpecies, p->HubbleParam,
                       p->Sigma8, p->PrimordialIndex, p->Redshift, p->ShapeGamma, p->UnitLength_in_cm,
                       p->UnitMass_in_g, p->UnitVelocity_in_cm_per_s, p->InputSpectrum_UnitLength_in_cm,
-                      p->WDM_PartMass_in_kev};
-  const int i[] = {p->WhichSpectrum, p->WhichTransfer, p->WDM_On, p->WDM_Vtherm_On};
+                      p->WDM_PartMass_in_kev, p->A_res, p->w_res, p->phi_res};
+  const int i[] = {p->WhichSpectrum, p->WhichTransfer, p->WDM_On, p->WDM_Vtherm_On, p->Variant};
   put(d, sizeof d);
   put(i, sizeof i);
   if (p->WhichTransfer == 2 && t->ntransfer > 0 && t->transfer_k && t->transfer_t) {
@@ -781,6 +781,10 @@ int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, i
   ks.kmin_s = pow(kmin, (4 - p->PrimordialIndex) / 3);
   ks.kmax_s = pow(kmax, (4 - p->PrimordialIndex) / 3);
   ks.x0 = j->x0;
+  ks.res = p->Variant != 1 ? 0 : (p->WhichSpectrum == 2 ? 2 : 3);
+  ks.res_A = p->A_res;
+  ks.res_w = p->w_res;
+  ks.res_phi = p->phi_res;
   TransferDev& tf = j->tf;
   tf.which = p->WhichTransfer;
   if (p->WhichTransfer == 1) {  // k-independent part of tk_eh (power.c:526-540)

@@ -14,6 +14,8 @@ struct KSpace {
   double s;                   // (4 - ns)/3
   double kmin_s, kmax_s;      // kmin^s, kmax^s (Legendre argument map)
   const double* x0;           // [N_modes] X0Table[n][0]
+  int res;                    // ResBasis oscillations of phi_G: bit 0 linear in k, bit 1 in ln k
+  double res_A, res_w, res_phi;
 };
 
 // Transfer function parameters (power.c:32-63, 181-217, 519-548).

@@ -543,6 +543,8 @@ __global__ void __launch_bounds__(kGaussRows)
         if constexpr (sizeof(T) == 4) {
           const float k2 = (float)kmag2;
           float a = expf(0.5f * (float)ns * logf(k2));
+          // the oscillation phase w k can be large: its factor is formed in FP64
+          if (ks.res) a *= (float)res_modulate(ks, 1.0, sqrt(kmag2));
           if (!fixed_amp) a *= -logf((float)u2);
           const float phig = amp32 * sqrtf(a) / k2;
           float sn, cs;
@@ -551,7 +553,9 @@ __global__ void __launch_bounds__(kGaussRows)
         } else {
           double phase = u1 * 2 * AAR_PI;
           phase += phase_shift;
-          double phig = anorm * exp(ns * log(sqrt(kmag2)));
+          const double km = sqrt(kmag2);
+          double phig = anorm * exp(ns * log(km));
+          if (ks.res) phig = res_modulate(ks, phig, km);
           if (!fixed_amp) phig *= -log(u2);
           phig = sqrt(phig) * fac * beta / kmag2;
           val = cmake<C2>((T)(phig * cos(phase)), (T)(phig * sin(phase)));

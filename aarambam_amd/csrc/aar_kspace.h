@@ -28,6 +28,16 @@ __device__ __forceinline__ double kmag2_of(const KSpace& ks, int x, int y, int z
   return a * a + b * b + c * c;
 }
 
+// Oscillation chain of the ResBasis Gaussian potential (ResBasis/main.c:445-455),
+// applied to Anorm k^ns in this order.  Its switch on WhichSpectrum has no
+// breaks: 0 and 1 apply the factor in k and the factor in ln k, 2 the one in
+// ln k only (res bits 1|2, 1|2 and 2; 0 for Basis).
+__device__ __forceinline__ double res_modulate(const KSpace& ks, double v, double kmag) {
+  if (ks.res & 1) v *= (1.0 + ks.res_A * cos(ks.res_w * kmag + ks.res_phi));
+  if (ks.res & 2) v *= (1.0 + ks.res_A * cos(ks.res_w * log(kmag) + ks.res_phi));
+  return v;
+}
+
 // Upward Legendre recurrence, GSL gsl_sf_legendre_Pl (legendre_poly.c).
 // RT is the precision of the radial functions: FP64 on the FP64 path, FP32 on
 // the FP32 path (|x| <= 1 and orders <= 31 keep the recurrence stable).

@@ -1,10 +1,12 @@
 // 2LPTBasis <ParameterFile> [--fp32] [--plain] [--device N]
+// 2LPTResBasis <ParameterFile> [...]   (this file built with -DAAR_RESBASIS)
 //
-// Drop-in for the reference executable Aarambam/LPT/Basis (main.c:53-102)
-// running as a single task on one GPU: same parameter file, tables, glass,
-// stdout stage banners, Gadget snapshot, optional potential dumps and
-// inputspec file.  The default is the FP64 pairwise schedule, the closest
-// to the reference; --fp32 selects the FP32 batched schedule.
+// Drop-in for the reference executables Aarambam/LPT/Basis and
+// Aarambam/LPT/ResBasis (main.c:53-102) running as a single task on one GPU:
+// same parameter file, tables, glass, stdout stage banners, Gadget snapshot,
+// optional potential dumps and inputspec file.  The default is the FP64
+// pairwise schedule, the closest to the reference; --fp32 selects the FP32
+// batched schedule.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -26,12 +28,21 @@ static void die(int code, const char* what) {
   exit(code > 255 ? 1 : (code ? code : 1));
 }
 
-static void banner(const aar_params& p, int nglass) {  // main.c:104-139
+#ifdef AAR_RESBASIS
+static const bool kRes = true;
+static const char kExec[] = "Aarambam-2LPT-ResBasis";
+#else
+static const bool kRes = false;
+static const char kExec[] = "Aarambam-2LPT-Basis";
+#endif
+
+static void banner(const aar_params& p, int nglass) {  // main.c:104-139 (ResBasis main.c:104-137)
   char pstr[80];
   memset(pstr, '*', 79);
   pstr[79] = 0;
   printf("%s\n", pstr);
-  printf("*%*s%s%*s*\n", 28, "", "Aarambam-2LPT-Basis", 29, "");
+  // the reference's title line ends at the terminator its sprintf leaves after the name
+  printf("*%*s%s\n", 28, "", kExec);
   printf("%s\n\n", pstr);
   printf("      Box = %.2e   Nmesh = %d   Nsample = %d   Nmax = %d\n", p.Box, p.Nmesh, p.Nsample, p.Nmax);
   printf("                       Nglass = %d    GlassTileFac = %d\n\n", nglass, p.GlassTileFac);
@@ -39,8 +50,14 @@ static void banner(const aar_params& p, int nglass) {  // main.c:104-139
          p.OmegaBaryon);
   printf("       sigma8 = %.4f     PrimoridalIndex = %.4f       Redshift = %.2e\n", p.Sigma8,
          p.PrimordialIndex, p.Redshift);
-  printf("   HubbleParam = %.4f  OmegaDM_2ndSpecies = %.2e          fNL = %+.2e\n\n", p.HubbleParam,
-         p.OmegaDM_2ndSpecies, p.Fnl);
+  if (kRes) {
+    printf("       A_Pk = %.4f     w_Pk = %.4f       phi_Pk = %.4f\n", p.A_res, p.w_res, p.phi_res);
+    printf("  HubbleParam = %.4f  OmegaDM_2ndSpecies = %.2e          fNL = %+.2e\n\n", p.HubbleParam,
+           p.OmegaDM_2ndSpecies, p.Fnl);
+  } else {
+    printf("   HubbleParam = %.4f  OmegaDM_2ndSpecies = %.2e          fNL = %+.2e\n\n", p.HubbleParam,
+           p.OmegaDM_2ndSpecies, p.Fnl);
+  }
   printf("   FixedAmplitude = %d    PhaseFlip = % d   SphereMode = %d    Seed = %d\n", p.FixedAmplitude,
          p.PhaseFlip, p.SphereMode, p.Seed);
   printf("\n%s\n\n", pstr);
@@ -69,7 +86,9 @@ int main(int argc, char** argv) {
   printf("RUNNING LPTPNG WITH %d TASKS", 1);
   aar_params p;
   char msg[8192];
-  if (aar_read_params(argv[1], &p, msg, sizeof(msg))) {  // read_param.c:241-255: report, exit(0)
+  const int prc = kRes ? aar_read_params_res(argv[1], &p, msg, sizeof(msg))
+                       : aar_read_params(argv[1], &p, msg, sizeof(msg));
+  if (prc) {  // read_param.c:241-255: report, exit(0)
     fputs(msg, stdout);
     return 0;
   }

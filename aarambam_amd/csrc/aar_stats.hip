@@ -13,7 +13,8 @@
 #include "aar_kspace.h"
 
 // kind 0: (a - b) * scale;  1: 1;  2: P(k) q_n(k) (n < 0: P(k));  3: g_n(k);  4: |k|
-// with P(k) = pamp k^ns / k^4, the variance of phi_G(k).  Bins are
+// with P(k) = pamp k^ns / k^4 (times the ResBasis oscillations), the variance
+// of phi_G(k).  Bins are
 // |k| in [edges[b], edges[b+1]); the origin, the Nyquist planes and the modes
 // outside the Nsample cut belong to no bin.
 template <typename TS>
@@ -56,7 +57,7 @@ __global__ void __launch_bounds__(256)
         if (n >= 4) q = (leg.at(n - 1) - ks.x0[n - 1]) * (kss * kss * kss / k2);
         else
           for (int t = 0; t < n; ++t) q *= kss;
-        r = pamp * pow(k1, ns) / (k2 * k2) * q;
+        r = res_modulate(ks, pamp * pow(k1, ns), k1) / (k2 * k2) * q;
       } else {
         r = weight_g<double>(n, kss, k2, leg, ks.x0);
       }

@@ -11,6 +11,7 @@
  * Reference interfaces each entry point replaces (paths relative to the
  * reference repository root):
  *   aar_read_params      Aarambam/LPT/Basis/read_param.c:8-262 (read_parameterfile)
+ *   aar_read_params_res  Aarambam/LPT/ResBasis/read_param.c:8-274 (read_parameterfile)
  *   aar_load_alpha       Aarambam/LPT/Basis/main.c:215-247 (AlphaTable load + symmetrisation)
  *   aar_load_x0          Aarambam/LPT/Basis/main.c:250-289 (X0Table load)
  *   aar_load_transfer    Aarambam/LPT/Basis/power.c:66-153 (read_transfer_table)
@@ -38,7 +39,13 @@ extern "C" {
 
 #define AAR_STR 500
 
-/* The 38 tags of the reference parameter file (read_param.c:27-184). */
+/* The 38 tags of the reference parameter file (read_param.c:27-184), and the
+ * three resonance tags of the ResBasis variant (Aarambam/LPT/ResBasis/
+ * read_param.c:88-98).  Variant 0 is Aarambam/LPT/Basis, variant 1
+ * Aarambam/LPT/ResBasis: its P(k) carries the oscillation
+ * 1 + A_res cos(w_res k + phi_res) (WhichSpectrum 1) or
+ * 1 + A_res cos(w_res ln k + phi_res) (WhichSpectrum 2), ResBasis/power.c:240-266,
+ * and its Gaussian potential the modulation chain of ResBasis/main.c:445-455. */
 typedef struct aar_params {
   int Nmesh, Nsample, Nmax, N_modes, Seed;
   int SphereMode, FixedAmplitude, PhaseFlip, SavePotentialField;
@@ -50,6 +57,8 @@ typedef struct aar_params {
   double InputSpectrum_UnitLength_in_cm, WDM_PartMass_in_kev;
   char GlassFile[AAR_STR], FileWithInputSpectrum[AAR_STR], FileWithInputTransfer[AAR_STR];
   char FileWithAlphaCoeff[AAR_STR], FileWithX0Coeff[AAR_STR], OutputDir[AAR_STR], FileBase[AAR_STR];
+  int Variant;                   /* 0 Basis, 1 ResBasis */
+  double A_res, w_res, phi_res;  /* ResBasis only (0 for Basis) */
 } aar_params;
 
 /* Host-side inputs of one job.  Arrays are owned by the caller. */
@@ -101,6 +110,8 @@ const char *aar_status_string(int status);
 
 /* ---- host side (no GPU needed) ---- */
 int aar_read_params(const char *path, aar_params *out, char *msg, size_t msglen);
+/* The ResBasis tag set (41 tags); sets out->Variant = 1. */
+int aar_read_params_res(const char *path, aar_params *out, char *msg, size_t msglen);
 int aar_load_alpha(const char *path, int n_modes, double *alpha_out /* Nm^3 */);
 int aar_load_x0(const char *path, int n_modes, double *x0_out /* Nm*2 */);
 /* count first (k, t NULL), then fill */

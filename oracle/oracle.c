@@ -243,6 +243,22 @@ double orc_transfer(const orc_params *p, double k) {
 
 static double powerspec(const orc_ctx *cx, double k) { /* power.c:240-288 */
   const orc_params *p = cx->p;
+  if (p->Variant == 1) { /* ResBasis/power.c:240-260 */
+    double power = 0;
+    switch (p->WhichSpectrum) {
+      case 0: power = cx->Norm * exp(p->PrimordialIndex * log(k)); break;
+      case 1:
+        power = cx->Norm * exp(p->PrimordialIndex * log(k)) * (1 + p->A_res * cos(p->w_res * k + p->phi_res));
+        break;
+      case 2:
+        power = cx->Norm * exp(p->PrimordialIndex * log(k)) *
+                (1 + p->A_res * cos(p->w_res * log(k) + p->phi_res));
+        break;
+    }
+    double t = transfer_ctx(cx, k);
+    power *= t * t;
+    return power;
+  }
   if (p->WhichSpectrum == 0) {
     double t = transfer_ctx(cx, k);
     return cx->Norm * exp(p->PrimordialIndex * log(k)) * t * t;
@@ -377,7 +393,8 @@ int orc_cosmology(const orc_params *p, orc_cosmo *c) {
   c->Norm = p->Sigma8 * p->Sigma8 / res;
   c->D0 = growth(&cx, 1.0) / growth(&cx, 1.e-8) * 1.e-8;
   c->Dplus = growth(&cx, 1.0) / growth(&cx, c->InitTime);
-  c->Anorm = (p->WhichSpectrum == 0) ? c->Norm : 0.0;
+  /* Basis: power.c:426-446; ResBasis sets it for every spectrum, ResBasis/power.c:418 */
+  c->Anorm = (p->Variant == 1 || p->WhichSpectrum == 0) ? c->Norm : 0.0;
   /* displacement_fields prefactors, main.c:305-313, 372 */
   double a = c->InitTime;
   c->hubble_a = c->Hubble * sqrt(p->Omega / pow(a, 3) + (1 - p->Omega - p->OmegaLambda) / pow(a, 2) +
@@ -555,6 +572,11 @@ void orc_gaussian(const orc_params *p, const orc_cosmo *c, const uint32_t *seedt
         double kmag = sqrt(kmag2);
         if (outside_cut(p, kv, kmag)) continue;
         double phig = c->Anorm * exp(p->PrimordialIndex * log(kmag));
+        if (p->Variant == 1) switch (p->WhichSpectrum) { /* ResBasis/main.c:445-455, no breaks */
+            case 0: phig *= 1.0; /* fall through */
+            case 1: phig *= (1.0 + p->A_res * cos(p->w_res * kmag + p->phi_res)); /* fall through */
+            case 2: phig *= (1.0 + p->A_res * cos(p->w_res * log(kmag) + p->phi_res));
+          }
         if (!p->FixedAmplitude) phig *= -log(ampl);
         phig = sqrt(phig) * c->fac * c->Beta / kmag2;
         double re = phig * cos(phase), im = phig * sin(phase);

@@ -46,6 +46,10 @@ typedef struct {
   /* transfer table rows (WhichTransfer 2), raw k and T as read from file */
   int ntransfer;
   const double *transfer_k, *transfer_t;
+  /* ResBasis variant (Aarambam/LPT/ResBasis): Variant 1 and its three tags
+   * (ResBasis/read_param.c:88-98); Variant 0 is Basis */
+  int Variant;
+  double A_res, w_res, phi_res;
 } orc_params;
 
 typedef struct {

@@ -25,7 +25,8 @@ _DBL_FIELDS = ["Box", "Omega", "OmegaLambda", "OmegaBaryon", "HubbleParam", "Sig
 class OrcParams(C.Structure):
     _fields_ = ([(n, C.c_int) for n in _INT_FIELDS] + [(n, C.c_double) for n in _DBL_FIELDS] +
                 [("ntransfer", C.c_int), ("transfer_k", C.POINTER(C.c_double)),
-                 ("transfer_t", C.POINTER(C.c_double))])
+                 ("transfer_t", C.POINTER(C.c_double)), ("Variant", C.c_int)] +
+                [(n, C.c_double) for n in ["A_res", "w_res", "phi_res"]])
 
 
 class OrcCosmo(C.Structure):
@@ -131,8 +132,9 @@ def make_params(cfg):
     p = OrcParams()
     for n in _INT_FIELDS:
         setattr(p, n, int(cfg.get(n, 0)))
-    for n in _DBL_FIELDS:
+    for n in _DBL_FIELDS + ["A_res", "w_res", "phi_res"]:
         setattr(p, n, float(cfg.get(n, 0.0)))
+    p.Variant = int(cfg.get("Variant", 0))
     keep = []
     if cfg.get("WhichTransfer", 0) == 2:
         tk = np.ascontiguousarray(cfg["transfer_k"], dtype=np.float64)

@@ -164,7 +164,15 @@ WDM_PartMass_in_kev                 10.0
 """
 
 
-def write_param_file(path, cfg, **files):
+# the three extra tags of the ResBasis parameter file (Aarambam/utils.py:72-78)
+RES_TEMPLATE = """
+A_res               %(A_res)f
+w_res               %(w_res)f
+phi_res             %(phi_res)f
+"""
+
+
+def write_param_file(path, cfg, res=False, **files):
     d = dict(cfg)
     d.setdefault("FileBase", "ics")
     d.setdefault("OutputDir", os.path.dirname(path))
@@ -172,7 +180,13 @@ def write_param_file(path, cfg, **files):
     d.setdefault("NumFilesWrittenInParallel", 1)
     d.update(files)
     with open(path, "w") as f:
-        f.write(PARAM_TEMPLATE % d)
+        f.write(PARAM_TEMPLATE % d + (RES_TEMPLATE % d if res else ""))
+
+
+def res_config(which, A=0.5, w=40.0, phi=0.3, **kw):
+    """ResBasis case: WhichSpectrum 0/1/2 with the oscillation A cos(w k + phi) or
+    A cos(w ln k + phi) (Aarambam/LPT/ResBasis/power.c:240-260)."""
+    return base_config(Variant=1, WhichSpectrum=which, A_res=A, w_res=w, phi_res=phi, **kw)
 
 
 def write_lattice_glass(path, n=64, box=100.0):

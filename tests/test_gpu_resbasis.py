@@ -1,0 +1,141 @@
+"""GPU parity of the ResBasis variant (Aarambam/LPT/ResBasis) and of the two
+drop-in executables.
+
+* The Gaussian potential and whole jobs through the C-ABI against the oracle
+  for WhichSpectrum 0, 1 and 2 (FP64: 1e-13 on the Gaussian potential, 1e-10
+  relative L2 on fields, 1 ulp on particles; FP32: <= 1e-5 relative L2, with
+  an oscillation phase w k of up to ~100 rad).
+* The measured P(k) of phi_G against the modulated input spectrum.
+* 2LPTBasis and 2LPTResBasis run on parameter files in the reference's
+  format; their Gadget snapshots hold the oracle's particles to 1 ulp.
+"""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as orc
+from tests import cases
+from tests.test_gpu_parity import _compare_job, tables_for
+
+pytestmark = pytest.mark.gpu
+
+aar = pytest.importorskip("aarambam_amd")
+
+BIN = os.path.join(os.path.dirname(aar.api.__file__), "bin")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = aar.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_res_gaussian_field_vs_oracle(ctx, which):
+    cfg = cases.res_config(which, A=0.6, w=40.0, phi=0.3, Nmesh=32, Nmax=32, Nsample=32)
+    alpha, x0, glass, gbox = cases.job_inputs(cfg)
+    dev = ctx.gaussian_field(aar.params_from_dict(cfg), tables_for(cfg, alpha, x0, glass, gbox), aar.AAR_FP64)
+    ref = orc.gaussian(cfg)
+    np.testing.assert_array_equal(dev == 0, ref == 0)
+    assert np.max(np.abs(dev - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("which,sched", [(0, aar.AAR_SCHED_BATCHED), (1, aar.AAR_SCHED_PLAIN),
+                                         (2, aar.AAR_SCHED_BATCHED)])
+def test_res_job_fp64_vs_oracle(ctx, which, sched):
+    tk, tt = cases.load_transfer()
+    cfg = cases.res_config(which, A=0.9, w=2.0 if which != 2 else 8.0, phi=0.0, Nmesh=32, Nsample=32, Nmax=32,
+                           N_modes=4, WhichTransfer=2, transfer_k=tk, transfer_t=tt, Seed=42)
+    _compare_job(ctx, cfg, cases.equilateral_rows(), None, aar.AAR_FP64, sched, 1e-10)
+
+
+def test_res_job_fp32_vs_oracle(ctx):
+    # |k| up to ~0.35 h/Mpc: the phase w k runs to ~100 rad
+    cfg = cases.res_config(1, A=0.5, w=300.0, phi=1.0, N_modes=4)
+    res = _compare_job(ctx, cfg, cases.equilateral_rows(), None, aar.AAR_FP32, aar.AAR_SCHED_BATCHED, 1e-5)
+    print("fp32 rel L2 (ResBasis)", res)
+
+
+@pytest.mark.parametrize("which", [1, 2])
+def test_res_power_spectrum_follows_modulated_input(ctx, which):
+    N, L = 128, 1000.0
+    kf = 2 * np.pi / L
+    edges = kf * np.array([1, 4, 8, 14, 22, 32, 44, 60.5])
+    ratios = []
+    for seed in [1, 2]:
+        cfg = cases.res_config(which, A=0.8, w=20.0 if which == 1 else 6.0, phi=0.0, Nmesh=N, Nsample=N, Nmax=N,
+                               N_modes=4, Box=L, GlassTileFac=2, Seed=seed)
+        alpha, x0, glass, gbox = cases.job_inputs(cfg, cases.local_rows())
+        job = aar.Job(ctx, aar.params_from_dict(cfg), aar.tables(alpha, x0, glass, gbox), aar.AAR_FP32,
+                      aar.AAR_SCHED_BATCHED)
+        job.run()
+        job.sync()
+        r = job.ng_statistics(edges)
+        job.close()
+        big = r["nmodes"] > 3e4
+        ratios.append(r["pk_meas"][big] / r["pk_theory"][big])
+        # the oscillation moves the bin expectations by far more than the scatter allowed below
+        base = cases.base_config(**{k: v for k, v in cfg.items() if k not in ("Variant", "A_res", "w_res",
+                                                                                  "phi_res", "WhichSpectrum")})
+        job0 = aar.Job(ctx, aar.params_from_dict(base), aar.tables(alpha, x0, glass, gbox), aar.AAR_FP32,
+                       aar.AAR_SCHED_BATCHED)
+        job0.run()
+        job0.sync()
+        r0 = job0.ng_statistics(edges)
+        job0.close()
+        shape = (r["pk_theory"] / r0["pk_theory"])[big]
+        assert np.ptp(shape / shape.mean()) > 0.1, shape
+    ratio = np.mean(ratios, axis=0)
+    assert np.all(np.abs(ratio - 1) < 0.03), ratio
+
+
+def _read_gadget(path):
+    b = open(path, "rb").read()
+    off = 0
+    blocks = []
+    while off < len(b):
+        (m,) = struct.unpack_from("i", b, off)
+        blocks.append(b[off + 4: off + 4 + m])
+        off += 8 + m
+    pos = np.frombuffer(blocks[1], np.float32).reshape(-1, 3)
+    vel = np.frombuffer(blocks[2], np.float32).reshape(-1, 3)
+    return pos, vel
+
+
+@pytest.mark.parametrize("exe", ["2LPTBasis", "2LPTResBasis"])
+def test_executable_snapshot_vs_oracle(tmp_path, exe):
+    res = exe == "2LPTResBasis"
+    N = 32
+    tk, tt = cases.load_transfer()
+    kw = dict(Nmesh=N, Nsample=N, Nmax=N, N_modes=4, WhichTransfer=2, Seed=7)
+    cfg = cases.res_config(2, A=0.9, w=8.0, phi=0.5, **kw) if res else cases.base_config(**kw)
+    rows = cases.equilateral_rows()
+    np.savetxt(tmp_path / "alpha.dat", rows, fmt=["%d", "%d", "%d", "%.17g"])
+    x0 = cases.x0_closed_form(cfg["N_modes"], cfg["PrimordialIndex"], cfg["Box"], cfg["Nmax"])
+    np.savetxt(tmp_path / "ai.dat", np.column_stack([np.arange(len(x0)), x0]), fmt=["%d", "%.17g", "%.17g"])
+    np.savetxt(tmp_path / "tk.dat", np.column_stack([tk, tt]), fmt="%.17g")
+    glass = cases.write_lattice_glass(str(tmp_path / "glass.dat"), n=N, box=100.0)
+    pf = tmp_path / "LPTConfig"
+    cases.write_param_file(str(pf), cfg, res=res, GlassFile=str(tmp_path / "glass.dat"),
+                           TransferPath=str(tmp_path / "tk.dat"), AlphaPath=str(tmp_path / "alpha.dat"),
+                           AiPath=str(tmp_path / "ai.dat"), FileBase="snap", OutputDir=str(tmp_path))
+    print("running", exe, flush=True)
+    out = subprocess.run([os.path.join(BIN, exe), str(pf)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    title = "Aarambam-2LPT-ResBasis" if res else "Aarambam-2LPT-Basis"
+    assert title in out.stdout and ("A_Pk = 0.9000" in out.stdout) == res
+    pos, vel = _read_gadget(tmp_path / "snap")
+    ref = orc.run(dict(cfg, transfer_k=tk, transfer_t=tt), orc.alpha_table(cfg["N_modes"], rows), x0, glass,
+                  100.0, want_fields=False)
+    box = cfg["Box"]
+    dpos = np.abs(pos.astype(np.float64) - ref["pos"])
+    dpos = np.minimum(dpos, box - dpos)
+    ulp = np.spacing(np.abs(ref["pos"]).astype(np.float32)).astype(np.float64)
+    assert np.all(dpos <= ulp), float((dpos / ulp).max())
+    vulp = np.spacing(np.abs(ref["vel"]).astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(vel - ref["vel"]) <= vulp)
+    assert os.path.exists(tmp_path / "inputspec_snap.txt")
