@@ -50,6 +50,7 @@ class AarCosmo(C.Structure):
 
 EXPORTS = ["aar_status_string", "aar_read_params", "aar_read_params_res", "aar_load_alpha", "aar_load_x0", "aar_load_transfer",
            "aar_read_glass", "aar_cosmology", "aar_seedtable", "aar_ranlxd_nth", "aar_write_gadget",
+           "aar_write_gadget_files",
            "aar_write_phi", "aar_write_inputspec", "aar_ctx_create", "aar_ctx_destroy", "aar_ctx_sync", "aar_job_bytes",
            "aar_job_prepare", "aar_job_run", "aar_job_sync", "aar_job_device_output", "aar_job_fetch",
            "aar_job_fetch_field", "aar_job_last_seconds", "aar_job_destroy", "aar_fft3d",
@@ -92,6 +93,8 @@ def lib():
             "aar_seedtable": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_uint32)]),
             "aar_ranlxd_nth": (C.c_ulong, [C.c_ulong, C.c_int, C.c_int]),
             "aar_write_gadget": (C.c_int, [PP, CP, fp, fp, C.c_longlong, C.POINTER(C.c_uint8)]),
+            "aar_write_gadget_files": (C.c_int, [PP, CP, fp, fp, C.c_longlong, C.POINTER(C.c_uint8), C.c_int,
+                                                 C.c_longlong, C.POINTER(C.c_int)]),
             "aar_write_phi": (C.c_int, [PP, dp, C.c_int]),
             "aar_write_inputspec": (C.c_int, [PP, TP, CP]),
             "aar_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),

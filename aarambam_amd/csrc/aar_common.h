@@ -6,21 +6,8 @@
 
 #define AAR_PI 3.14159265358979323846  // Aarambam/LPT/Basis/allvars.h:4
 
-// Status codes.  Numbers below 100000 are the reference's FatalError codes
-// (Aarambam/LPT/Basis/main.c:1612-1618 and the call sites listed there).
-enum {
-  AAR_OK = 0,
-  AAR_ERR_ALLOC = 1,          // main.c ASSERT_ALLOC -> FatalError(1)
-  AAR_ERR_TABLE_FILE = 17,    // missing alpha / X0 / transfer file (main.c:230,270; power.c:81)
-  AAR_ERR_KLOWER = 1111,      // transfer table does not reach k < 10^-4.6 (power.c:117-121)
-  AAR_ERR_WRITE_PARALLEL = 24131,  // NumFilesWrittenInParallel > NTask (save.c:112-117)
-  AAR_ERR_BAD_ARG = 100001,   // invalid argument at the C-ABI
-  AAR_ERR_HIP = 100002,       // a HIP runtime call failed
-  AAR_ERR_HBM_BUDGET = 100003,  // job does not fit the admitted HBM budget
-  AAR_ERR_NMAX = 100004,      // Nmax < Nsample (main.c:200 assert)
-  AAR_ERR_UNSUPPORTED = 100005,  // a configuration the reference does not run either
-  AAR_ERR_NO_DEVICE = 100006,
-};
+// Status codes (AAR_OK, AAR_ERR_*) are part of the C-ABI header.
+#include "../../include/aarambam.h"
 
 #define AAR_HIP(call)                                                              \
   do {                                                                             \

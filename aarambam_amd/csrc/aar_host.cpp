@@ -582,13 +582,11 @@ static int fwrite_all(const void* p, size_t sz, size_t n, FILE* fd) {
   return fwrite(p, sz, n, fd) == n ? AAR_OK : 777;
 }
 
-// Gadget-1 snapshot of one task (save.c:162-520, NO64BITID).
-extern "C" int aar_write_gadget(const aar_params* p, const aar_cosmo* c, const float* pos,
-                                const float* vel, long long npart, const uint8_t* glass_header) {
-  if (!p || !c || !pos || !vel) return AAR_ERR_BAD_ARG;
-  if (p->NumFilesWrittenInParallel > 1) return AAR_ERR_WRITE_PARALLEL;  // save.c:112-117, NTask = 1
-  if (npart == 0) return AAR_OK;
-  std::string fn = std::string(p->OutputDir) + "/" + p->FileBase;
+// One Gadget-1 file of save_local_data (save.c:162-520, NO64BITID): n particles
+// starting at global index first, of ntot in numfiles files.
+static int write_gadget_file(const std::string& fn, const aar_params* p, const aar_cosmo* c, const float* pos,
+                             const float* vel, long long first, long long n, long long ntot, int numfiles,
+                             const uint8_t* glass_header) {
   FILE* fd = fopen(fn.c_str(), "wb");
   if (!fd) return 10;
   uint8_t hdr[256];
@@ -596,12 +594,12 @@ extern "C" int aar_write_gadget(const aar_params* p, const aar_cosmo* c, const f
   else memset(hdr, 0, 256);
   unsigned npart6[6] = {0, 0, 0, 0, 0, 0}, ntot6[6] = {0, 0, 0, 0, 0, 0};
   double mass6[6] = {0, 0, 0, 0, 0, 0};
-  npart6[1] = (unsigned)npart;
-  ntot6[1] = (unsigned)npart;
-  ntot6[2] = (unsigned)(npart >> 32);
-  mass6[1] = (p->Omega) * 3 * c->Hubble * c->Hubble / (8 * kPi * c->G) * pow(p->Box, 3) / npart;
+  npart6[1] = (unsigned)n;
+  ntot6[1] = (unsigned)ntot;
+  ntot6[2] = (unsigned)(ntot >> 32);
+  mass6[1] = (p->Omega) * 3 * c->Hubble * c->Hubble / (8 * kPi * c->G) * pow(p->Box, 3) / ntot;
   double time = c->InitTime, redshift = 1.0 / c->InitTime - 1;
-  int zero = 0, numfiles = 1;
+  int zero = 0;
   memcpy(hdr + 0, npart6, 24);
   memcpy(hdr + 24, mass6, 48);
   memcpy(hdr + 72, &time, 8);
@@ -624,24 +622,65 @@ extern "C" int aar_write_gadget(const aar_params* p, const aar_cosmo* c, const f
   rc |= fwrite_all(hdr, 256, 1, fd);
   rc |= fwrite_all(&dummy, 4, 1, fd);
   // record markers are 4-byte ints, truncated as in the reference for > 2^31 bytes
-  dummy = (int)(uint32_t)(sizeof(float) * 3 * (size_t)npart);
+  dummy = (int)(uint32_t)(sizeof(float) * 3 * (size_t)n);
   rc |= fwrite_all(&dummy, 4, 1, fd);
-  rc |= fwrite_all(pos, sizeof(float), 3 * (size_t)npart, fd);
+  rc |= fwrite_all(pos + 3 * first, sizeof(float), 3 * (size_t)n, fd);
   rc |= fwrite_all(&dummy, 4, 1, fd);
   rc |= fwrite_all(&dummy, 4, 1, fd);
-  rc |= fwrite_all(vel, sizeof(float), 3 * (size_t)npart, fd);
+  rc |= fwrite_all(vel + 3 * first, sizeof(float), 3 * (size_t)n, fd);
   rc |= fwrite_all(&dummy, 4, 1, fd);
-  dummy = (int)(uint32_t)(sizeof(int) * (size_t)npart);
+  dummy = (int)(uint32_t)(sizeof(int) * (size_t)n);
   rc |= fwrite_all(&dummy, 4, 1, fd);
   std::vector<int> ids(1 << 20);
-  for (long long s = 0; s < npart; s += (long long)ids.size()) {
-    const long long n = std::min<long long>((long long)ids.size(), npart - s);
-    for (long long q = 0; q < n; ++q) ids[q] = (int)(s + q + 1);  // IDs from 1 (read_glass.c:176)
-    rc |= fwrite_all(ids.data(), sizeof(int), (size_t)n, fd);
+  for (long long s = 0; s < n; s += (long long)ids.size()) {
+    const long long m = std::min<long long>((long long)ids.size(), n - s);
+    // IDs count from 1 over the whole run (read_glass.c:176)
+    for (long long q = 0; q < m; ++q) ids[q] = (int)(first + s + q + 1);
+    rc |= fwrite_all(ids.data(), sizeof(int), (size_t)m, fd);
   }
   rc |= fwrite_all(&dummy, 4, 1, fd);
-  fclose(fd);
+  if (fclose(fd)) rc = 777;
   return rc ? 777 : AAR_OK;
+}
+
+// Gadget-1 snapshot of one task (save.c:162-520, NO64BITID).
+extern "C" int aar_write_gadget(const aar_params* p, const aar_cosmo* c, const float* pos,
+                                const float* vel, long long npart, const uint8_t* glass_header) {
+  if (!p || !c || !pos || !vel || npart < 0) return AAR_ERR_BAD_ARG;
+  if (p->NumFilesWrittenInParallel > 1) return AAR_ERR_WRITE_PARALLEL;  // save.c:112-117, NTask = 1
+  if (npart == 0) return AAR_OK;
+  return write_gadget_file(std::string(p->OutputDir) + "/" + p->FileBase, p, c, pos, vel, 0, npart, npart, 1,
+                           glass_header);
+}
+
+// The snapshot split over files the way several tasks write it (save.c:184-187, 252:
+// OutputDir/FileBase.<i>, num_files = the file count, npart per file,
+// npartTotal of the run), so that every record stays below max_record bytes.
+extern "C" int aar_write_gadget_files(const aar_params* p, const aar_cosmo* c, const float* pos,
+                                      const float* vel, long long npart, const uint8_t* glass_header, int nfiles,
+                                      long long max_record, int* nfiles_out) {
+  if (!p || !c || !pos || !vel || npart < 0 || nfiles < 0 || max_record < 0) return AAR_ERR_BAD_ARG;
+  if (npart == 0) {  // save.c:181-182: a task without particles writes no file
+    if (nfiles_out) *nfiles_out = 0;
+    return AAR_OK;
+  }
+  if (max_record == 0 || max_record > 2147483647LL) max_record = 2147483647LL;
+  const long long per_max = max_record / (3 * (long long)sizeof(float));  // position block is the largest
+  if (per_max < 1) return AAR_ERR_BAD_ARG;
+  if (nfiles == 0) {
+    nfiles = (int)std::max<long long>(1, (npart + per_max - 1) / per_max);
+    while ((npart + nfiles - 1) / nfiles > per_max) ++nfiles;
+  }
+  if ((npart + nfiles - 1) / nfiles > per_max || nfiles > npart) return AAR_ERR_BAD_ARG;
+  if (nfiles_out) *nfiles_out = nfiles;
+  const std::string base = std::string(p->OutputDir) + "/" + p->FileBase;
+  for (int f = 0; f < nfiles; ++f) {
+    const long long a = npart * f / nfiles, b = npart * (f + 1) / nfiles;
+    const std::string fn = nfiles > 1 ? base + "." + std::to_string(f) : base;
+    const int rc = write_gadget_file(fn, p, c, pos, vel, a, b - a, npart, nfiles, glass_header);
+    if (rc) return rc;
+  }
+  return AAR_OK;
 }
 
 // Real-space potential dump (save.c:14-105), one task.
@@ -657,18 +696,22 @@ extern "C" int aar_write_phi(const aar_params* p, const double* real, int non_ga
   int rc = fwrite_all(&total, 4, 1, fd);
   rc |= fwrite_all(&N, 4, 1, fd);
   rc |= fwrite_all(&p->Box, 8, 1, fd);
-  std::vector<unsigned char> rec((size_t)N * 12);
-  for (int i = 0; i < N; ++i)
+  // the reference issues two fwrite calls per element; here one x-plane of
+  // {int coord, double value} records is packed and written at a time
+  std::vector<unsigned char> rec((size_t)N * N * 12);
+  for (int i = 0; i < N && !rc; ++i) {
+    unsigned char* o = rec.data();
     for (int j = 0; j < N; ++j) {
-      for (int k = 0; k < N; ++k) {
-        int coord = (i * N + j) * W + k;
-        double v = real[((size_t)i * N + j) * N + k];
-        memcpy(&rec[(size_t)k * 12], &coord, 4);
-        memcpy(&rec[(size_t)k * 12 + 4], &v, 8);
+      const double* src = real + ((size_t)i * N + j) * N;
+      for (int k = 0; k < N; ++k, o += 12) {
+        const int coord = (i * N + j) * W + k;
+        memcpy(o, &coord, 4);
+        memcpy(o + 4, &src[k], 8);
       }
-      rc |= fwrite_all(rec.data(), 1, rec.size(), fd);
     }
-  fclose(fd);
+    rc |= fwrite_all(rec.data(), 1, rec.size(), fd);
+  }
+  if (fclose(fd)) rc = 777;
   return rc ? 777 : AAR_OK;
 }
 

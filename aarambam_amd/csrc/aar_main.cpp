@@ -1,4 +1,4 @@
-// 2LPTBasis <ParameterFile> [--fp32] [--plain] [--device N]
+// 2LPTBasis <ParameterFile> [--fp32] [--plain] [--device N] [--max-record BYTES]
 // 2LPTResBasis <ParameterFile> [...]   (this file built with -DAAR_RESBASIS)
 //
 // Drop-in for the reference executables Aarambam/LPT/Basis and
@@ -6,7 +6,8 @@
 // same parameter file, tables, glass, stdout stage banners, Gadget snapshot,
 // optional potential dumps and inputspec file.  The default is the FP64
 // pairwise schedule, the closest to the reference; --fp32 selects the FP32
-// batched schedule.
+// batched schedule.  The snapshot is one file unless a record would exceed
+// 2^31 - 1 bytes (or --max-record BYTES): then FileBase.<i> files.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -70,6 +71,7 @@ int main(int argc, char** argv) {
     return 0;
   }
   int precision = AAR_FP64, schedule = AAR_SCHED_PLAIN, device = 0;
+  long long max_record = 2147483647LL;
   for (int a = 2; a < argc; ++a) {
     if (!strcmp(argv[a], "--fp32")) {
       precision = AAR_FP32;
@@ -80,6 +82,12 @@ int main(int argc, char** argv) {
       schedule = AAR_SCHED_PLAIN;
     } else if (!strcmp(argv[a], "--device") && a + 1 < argc) {
       device = atoi(argv[++a]);
+    } else if (!strcmp(argv[a], "--max-record") && a + 1 < argc) {
+      max_record = atoll(argv[++a]);
+      if (max_record < 12 || max_record > 2147483647LL) {
+        printf("--max-record takes 12 to 2147483647 bytes\n");
+        return 1;
+      }
     }
   }
   const double t0 = now();
@@ -152,8 +160,17 @@ int main(int argc, char** argv) {
   printf("Writing initial conditions snapshot...");
   fflush(stdout);
   const double tw = now();
-  if ((rc = aar_write_gadget(&p, &c, pos.data(), vel.data(), npart, ghdr))) die(rc, "write_particle_data");
-  printf(" Done [%.3f s]\n", now() - tw);
+  // One file as the reference writes it, unless its position block would overflow Gadget-1's
+  // 4-byte record markers (12 npart >= 2^31, e.g. 1024^3) or --max-record: then FileBase.<i>
+  // files as several tasks write them.
+  int nfiles = 1;
+  if (12LL * npart <= max_record) {
+    if ((rc = aar_write_gadget(&p, &c, pos.data(), vel.data(), npart, ghdr))) die(rc, "write_particle_data");
+  } else if ((rc = aar_write_gadget_files(&p, &c, pos.data(), vel.data(), npart, ghdr, 0, max_record,
+                                          &nfiles))) {
+    die(rc, "write_particle_data");
+  }
+  printf(" Done [%.3f s, %d file%s]\n", now() - tw, nfiles, nfiles > 1 ? "s" : "");
   aar_job_destroy(job);
   aar_ctx_destroy(ctx);
   if ((rc = aar_write_inputspec(&p, &t, &c))) die(rc, "print_spec");

@@ -21,6 +21,8 @@
  *   aar_seedtable        Aarambam/LPT/Basis/main.c:315-349
  *   aar_job_prepare/run  Aarambam/LPT/Basis/main.c:149-1532 (displacement_fields)
  *   aar_write_gadget     Aarambam/LPT/Basis/save.c:108-520 (write_particle_data, save_local_data)
+ *   aar_write_gadget_files  Aarambam/LPT/Basis/save.c:108-520 with several writer
+ *                        tasks (FileBase.<i>, num_files), save.c:184-187, 252
  *   aar_write_phi        Aarambam/LPT/Basis/save.c:14-105 (write_phi)
  *   aar_write_inputspec  Aarambam/LPT/Basis/main.c:1631-1694 (print_spec)
  *
@@ -38,6 +40,22 @@ extern "C" {
 #endif
 
 #define AAR_STR 500
+
+/* Status codes.  Numbers below 100000 are the reference's FatalError codes
+ * (Aarambam/LPT/Basis/main.c:1612-1618 and the call sites listed there). */
+enum {
+  AAR_OK = 0,
+  AAR_ERR_ALLOC = 1,               /* main.c ASSERT_ALLOC -> FatalError(1) */
+  AAR_ERR_TABLE_FILE = 17,         /* missing alpha / X0 / transfer file (main.c:230,270; power.c:81) */
+  AAR_ERR_KLOWER = 1111,           /* transfer table does not reach k < 10^-4.6 (power.c:117-121) */
+  AAR_ERR_WRITE_PARALLEL = 24131,  /* NumFilesWrittenInParallel > NTask (save.c:112-117) */
+  AAR_ERR_BAD_ARG = 100001,        /* invalid argument at the C-ABI */
+  AAR_ERR_HIP = 100002,            /* a HIP runtime call failed */
+  AAR_ERR_HBM_BUDGET = 100003,     /* job does not fit the admitted HBM budget */
+  AAR_ERR_NMAX = 100004,           /* Nmax < Nsample (main.c:200 assert) */
+  AAR_ERR_UNSUPPORTED = 100005,    /* a configuration the reference does not run either */
+  AAR_ERR_NO_DEVICE = 100006
+};
 
 /* The 38 tags of the reference parameter file (read_param.c:27-184), and the
  * three resonance tags of the ResBasis variant (Aarambam/LPT/ResBasis/
@@ -124,6 +142,14 @@ int aar_seedtable(int nmesh, int seed, uint32_t *out /* nmesh^2 */);
 unsigned long aar_ranlxd_nth(unsigned long seed, int luxury, int n);
 int aar_write_gadget(const aar_params *p, const aar_cosmo *c, const float *pos, const float *vel,
                      long long npart, const uint8_t *glass_header);
+/* The snapshot in nfiles files OutputDir/FileBase.<i> (one file: OutputDir/FileBase), the
+ * layout the reference writes with several tasks: npart per file, npartTotal and num_files of
+ * the run, IDs counting on across files.  nfiles 0 picks the fewest files whose records stay
+ * within max_record bytes (0: 2^31 - 1, the limit of Gadget-1's 4-byte record markers, which a
+ * single 1024^3 file overflows).  *nfiles_out (optional) receives the count. */
+int aar_write_gadget_files(const aar_params *p, const aar_cosmo *c, const float *pos, const float *vel,
+                           long long npart, const uint8_t *glass_header, int nfiles, long long max_record,
+                           int *nfiles_out);
 int aar_write_phi(const aar_params *p, const double *real_nnn, int non_gaussian);
 int aar_write_inputspec(const aar_params *p, const aar_tables *t, const aar_cosmo *c);
 

@@ -7,10 +7,10 @@ drop-in executables.
   an oscillation phase w k of up to ~100 rad).
 * The measured P(k) of phi_G against the modulated input spectrum.
 * 2LPTBasis and 2LPTResBasis run on parameter files in the reference's
-  format; their Gadget snapshots hold the oracle's particles to 1 ulp.
+  format; their Gadget snapshots (one file, or split over four) hold the
+  oracle's particles to 1 ulp.
 """
 import os
-import struct
 import subprocess
 
 import numpy as np
@@ -19,6 +19,7 @@ import pytest
 from oracle import pyoracle as orc
 from tests import cases
 from tests.test_gpu_parity import _compare_job, tables_for
+from tests.test_outputs import read_gadget_file
 
 pytestmark = pytest.mark.gpu
 
@@ -93,21 +94,18 @@ def test_res_power_spectrum_follows_modulated_input(ctx, which):
     assert np.all(np.abs(ratio - 1) < 0.03), ratio
 
 
-def _read_gadget(path):
-    b = open(path, "rb").read()
-    off = 0
-    blocks = []
-    while off < len(b):
-        (m,) = struct.unpack_from("i", b, off)
-        blocks.append(b[off + 4: off + 4 + m])
-        off += 8 + m
-    pos = np.frombuffer(blocks[1], np.float32).reshape(-1, 3)
-    vel = np.frombuffer(blocks[2], np.float32).reshape(-1, 3)
-    return pos, vel
+def _read_gadget(paths):
+    pos, vel = [], []
+    for path in paths:
+        _, p, v, _ = read_gadget_file(path)
+        pos.append(p)
+        vel.append(v)
+    return np.concatenate(pos), np.concatenate(vel)
 
 
-@pytest.mark.parametrize("exe", ["2LPTBasis", "2LPTResBasis"])
-def test_executable_snapshot_vs_oracle(tmp_path, exe):
+@pytest.mark.parametrize("exe,split", [("2LPTBasis", False), ("2LPTResBasis", False), ("2LPTBasis", True)])
+def test_executable_snapshot_vs_oracle(tmp_path, exe, split):
+    """split: --max-record 120000 puts the 32^3 particles in FileBase.0-3."""
     res = exe == "2LPTResBasis"
     N = 32
     tk, tt = cases.load_transfer()
@@ -124,11 +122,14 @@ def test_executable_snapshot_vs_oracle(tmp_path, exe):
                            TransferPath=str(tmp_path / "tk.dat"), AlphaPath=str(tmp_path / "alpha.dat"),
                            AiPath=str(tmp_path / "ai.dat"), FileBase="snap", OutputDir=str(tmp_path))
     print("running", exe, flush=True)
-    out = subprocess.run([os.path.join(BIN, exe), str(pf)], capture_output=True, text=True, timeout=300)
+    args = [os.path.join(BIN, exe), str(pf)] + (["--max-record", "120000"] if split else [])
+    out = subprocess.run(args, capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     title = "Aarambam-2LPT-ResBasis" if res else "Aarambam-2LPT-Basis"
     assert title in out.stdout and ("A_Pk = 0.9000" in out.stdout) == res
-    pos, vel = _read_gadget(tmp_path / "snap")
+    files = [tmp_path / f"snap.{i}" for i in range(4)] if split else [tmp_path / "snap"]
+    assert (tmp_path / "snap").exists() != split
+    pos, vel = _read_gadget(files)
     ref = orc.run(dict(cfg, transfer_k=tk, transfer_t=tt), orc.alpha_table(cfg["N_modes"], rows), x0, glass,
                   100.0, want_fields=False)
     box = cfg["Box"]
