@@ -56,19 +56,24 @@ struct Geom {
   long long fs;  // field stride (complex elements) = N*N*(H+1)
 };
 
+// x-planes per block of the row order
+#ifndef AAR_XB_LOG
+#define AAR_XB_LOG 3
+#endif
+constexpr int kXBLog = AAR_XB_LOG, kXB = 1 << kXBLog;
 __host__ __device__ __forceinline__ long long row_of(int N, int x, int y) {
-  return ((long long)(x >> 3) * N + y) * 8 + (x & 7);
+  return ((long long)(x >> kXBLog) * N + y) * kXB + (x & (kXB - 1));
 }
 __host__ __device__ __forceinline__ void row_xy(int N, long long r, int& x, int& y) {
-  const long long t = r >> 3;
+  const long long t = r >> kXBLog;
   y = (int)(t % N);
-  x = (int)(t / N) * 8 + (int)(r & 7);
+  x = (int)(t / N) * kXB + (int)(r & (kXB - 1));
 }
 // same, for power-of-two N given as log2 (no 64-bit division)
 __host__ __device__ __forceinline__ void row_xy_log(int nlog, unsigned r, int& x, int& y) {
-  const unsigned t = r >> 3;
+  const unsigned t = r >> kXBLog;
   y = (int)(t & ((1u << nlog) - 1));
-  x = (int)((t >> nlog) << 3) + (int)(r & 7);
+  x = (int)((t >> nlog) << kXBLog) + (int)(r & (kXB - 1));
 }
 
 static inline Geom make_geom(int N) {

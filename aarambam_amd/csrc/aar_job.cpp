@@ -978,17 +978,17 @@ static int fetch_box_t(aar_job* j, int which, int x0, int y0, int z0, int nx, in
     AAR_HIP(hipStreamSynchronize(j->ctx->stream));
     src = (const T*)tmp;
   }
-  std::vector<T> buf((size_t)ny * 8 * N);
+  std::vector<T> buf((size_t)ny * kXB * N);
   int rc = AAR_OK;
-  for (int xb = x0 / 8; xb <= (x0 + nx - 1) / 8 && !rc; ++xb) {
-    const long long r0 = ((long long)xb * N + y0) * 8;
+  for (int xb = x0 / kXB; xb <= (x0 + nx - 1) / kXB && !rc; ++xb) {
+    const long long r0 = ((long long)xb * N + y0) * kXB;
     if (hipMemcpy(buf.data(), src + r0 * N, buf.size() * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) {
       rc = AAR_ERR_HIP;
       break;
     }
-    for (int x = std::max(x0, xb * 8); x < std::min(x0 + nx, xb * 8 + 8); ++x)
+    for (int x = std::max(x0, xb * kXB); x < std::min(x0 + nx, xb * kXB + kXB); ++x)
       for (int y = y0; y < y0 + ny; ++y) {
-        const T* row = buf.data() + ((size_t)(y - y0) * 8 + (x & 7)) * N + z0;
+        const T* row = buf.data() + ((size_t)(y - y0) * kXB + (x & (kXB - 1))) * N + z0;
         double* o = out + ((size_t)(x - x0) * ny + (y - y0)) * nz;
         for (int z = 0; z < nz; ++z) o[z] = row[z];
       }
