@@ -58,7 +58,8 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_read_params_res", "aar_l
            "aar_server_create", "aar_server_submit", "aar_server_wait", "aar_server_fetch",
            "aar_server_device_output", "aar_server_release", "aar_server_stats", "aar_server_destroy",
            "aar_autotune", "aar_xy_tile_variant", "aar_set_xy_tile_variant", "aar_job_fetch_field_box",
-           "aar_job_ng_statistics", "aar_mode_bispectra_covariance", "aar_mode_QS"]
+           "aar_job_ng_statistics", "aar_mode_bispectra_covariance", "aar_mode_QS",
+           "aar_mode_basis_expansion"]
 KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
 
 
@@ -113,6 +114,10 @@ def lib():
                                                         C.c_int, C.c_int]),
             "aar_mode_QS": (C.c_int, [vp, dp, dp, C.c_int, dp, ip, ip, ip, C.c_int, ip, ip, ip, C.c_int, dp, C.c_int,
                                       C.c_int]),
+            "aar_mode_basis_expansion": (C.c_int, [vp, dp, dp, dp, dp, C.POINTER(C.c_uint8), dp, C.c_int, dp, ip, ip,
+                                                   ip, C.c_int, ip, ip, ip, C.c_int, dp, C.c_int, C.c_int,
+                                                   C.POINTER(C.c_uint8), C.c_double, C.c_longlong, C.c_double,
+                                                   C.c_double, C.c_double, C.c_int]),
             "aar_job_last_seconds": (C.c_double, [vp]),
             "aar_job_destroy": (C.c_int, [vp]),
             "aar_fft3d": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp]),
@@ -303,6 +308,27 @@ class Context:
         out = np.zeros(S.shape[0] * args[8])
         _check(lib().aar_mode_QS(self.h, _dptr(out), _dptr(S), S.shape[0], *args), "aar_mode_QS")
         return out.reshape(S.shape[0], args[8])
+
+    def mode_basis_expansion(self, grid, S, regularization=0.0, Niter=1e4, rtol=1e-12, bad_modes=None,
+                             RR_tol=1e-2, cond_tol=1e9, max_modes=None):
+        """Basis.basis_expansion (cmbbest.pyx:320-490) with the contractions on the device.
+        S: n_shapes x npts shape values at the tetrapyd points; max_modes None: every mode.
+        Returns (alpha, shape_cov, corr, mse, selected)."""
+        keep, args = self._modal(grid)
+        S = np.ascontiguousarray(S, dtype=np.float64).reshape(-1, args[4])
+        ns, n = S.shape[0], args[8]
+        alpha, cov = np.zeros(ns * n), np.zeros(ns * ns)
+        corr, mse = np.zeros(ns), np.zeros(ns)
+        sel = np.zeros(ns * n, np.uint8)
+        u8 = C.POINTER(C.c_uint8)
+        bad = None if bad_modes is None else np.ascontiguousarray(bad_modes, dtype=np.uint8).reshape(n)
+        badp = None if bad is None else bad.ctypes.data_as(u8)
+        mm = 0 if max_modes is None else int(max_modes)
+        _check(lib().aar_mode_basis_expansion(self.h, _dptr(alpha), _dptr(cov), _dptr(corr), _dptr(mse),
+                                              sel.ctypes.data_as(u8), _dptr(S), ns, *args, badp,
+                                              float(regularization), int(Niter), float(rtol), float(RR_tol),
+                                              float(cond_tol), mm), "aar_mode_basis_expansion")
+        return alpha.reshape(ns, n), cov.reshape(ns, ns), corr, mse, sel.reshape(ns, n).astype(bool)
 
     def ranlxd(self, seed, n):
         out = np.empty(n, np.float64)

@@ -16,7 +16,8 @@ EXE = os.path.join(HERE, "bin", "2LPTBasis")
 EXE_RES = os.path.join(HERE, "bin", "2LPTResBasis")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["aar_kernels.hip", "aar_stats.hip", "aar_modal.hip", "aar_job.cpp", "aar_host.cpp", "aar_serve.cpp"]
+SOURCES = ["aar_kernels.hip", "aar_stats.hip", "aar_modal.hip", "aar_job.cpp", "aar_host.cpp", "aar_serve.cpp",
+           "aar_modal_host.cpp"]
 # -ffp-contract=off keeps the reference's operation order (no fused multiply-add
 # contraction); packed FP32 instructions are disabled by target feature.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -47,7 +48,8 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs +
+         ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     main_src = os.path.join(CSRC, "aar_main.cpp")
     if os.path.exists(main_src):

@@ -481,6 +481,8 @@ extern "C" int aar_ctx_create(int device, aar_ctx** out) {
 }
 
 int aar_ctx_device(const aar_ctx* c) { return c->device; }
+hipStream_t aar_ctx_stream(const aar_ctx* c) { return c->stream; }
+int aar_device_alloc(void** p, size_t bytes) { return device_alloc(p, bytes); }
 
 extern "C" int aar_ctx_sync(aar_ctx* c) {
   if (!c) return AAR_ERR_BAD_ARG;
@@ -1155,98 +1157,6 @@ extern "C" int aar_job_ng_statistics(aar_job* j, int nb, const double* edges, do
   return j->prec == AAR_FP32
              ? ng_statistics_t<float>(j, nb, edges, pk_meas, pk_theory, kmean, nmodes, b_meas, b_theory, ntri)
              : ng_statistics_t<double>(j, nb, edges, pk_meas, pk_theory, kmean, nmodes, b_meas, b_theory, ntri);
-}
-
-// ------------------------------------------------------------------ modal decomposition
-// Device versions of compute_mode_bispectra_covariance and compute_QS
-// (Aarambam/basis/tetrapyd.c:111-208, 210-281), same arguments.  Every index
-// is checked on the host first: the kernels gather from the mode table.
-struct ModalBufs {
-  double *w = nullptr, *ev = nullptr, *part = nullptr, *out = nullptr, *S = nullptr;
-  int *i1 = nullptr, *i2 = nullptr, *i3 = nullptr, *p1 = nullptr, *p2 = nullptr, *p3 = nullptr;
-  ~ModalBufs() {
-    void* b[] = {w, ev, part, out, S, i1, i2, i3, p1, p2, p3};
-    for (void* x : b)
-      if (x) hipFree(x);
-  }
-};
-
-static int modal_check(const double* w, const int* i1, const int* i2, const int* i3, int npts, const int* p1,
-                       const int* p2, const int* p3, int nmodes, const double* evals, int pmax, int nk) {
-  if (!w || !i1 || !i2 || !i3 || !p1 || !p2 || !p3 || !evals || npts < 1 || nmodes < 1 || pmax < 1 || nk < 1)
-    return AAR_ERR_BAD_ARG;
-  for (int t = 0; t < npts; ++t)
-    if (i1[t] < 0 || i1[t] >= nk || i2[t] < 0 || i2[t] >= nk || i3[t] < 0 || i3[t] >= nk) return AAR_ERR_BAD_ARG;
-  for (int n = 0; n < nmodes; ++n)
-    if (p1[n] < 0 || p1[n] >= pmax || p2[n] < 0 || p2[n] >= pmax || p3[n] < 0 || p3[n] >= pmax)
-      return AAR_ERR_BAD_ARG;
-  return AAR_OK;
-}
-
-static int modal_upload(ModalBufs& b, const double* w, const int* i1, const int* i2, const int* i3, int npts,
-                        const int* p1, const int* p2, const int* p3, int nmodes, const double* evals, int pmax,
-                        int nk, hipStream_t st) {
-  auto up = [&](void** d, const void* h, size_t n) -> int {
-    if (device_alloc(d, n)) return AAR_ERR_ALLOC;
-    return hipMemcpyAsync(*d, h, n, hipMemcpyHostToDevice, st) == hipSuccess ? AAR_OK : AAR_ERR_HIP;
-  };
-  int rc;
-  if ((rc = up((void**)&b.w, w, sizeof(double) * npts)) || (rc = up((void**)&b.i1, i1, sizeof(int) * npts)) ||
-      (rc = up((void**)&b.i2, i2, sizeof(int) * npts)) || (rc = up((void**)&b.i3, i3, sizeof(int) * npts)) ||
-      (rc = up((void**)&b.p1, p1, sizeof(int) * nmodes)) || (rc = up((void**)&b.p2, p2, sizeof(int) * nmodes)) ||
-      (rc = up((void**)&b.p3, p3, sizeof(int) * nmodes)) ||
-      (rc = up((void**)&b.ev, evals, sizeof(double) * (size_t)pmax * nk)))
-    return rc;
-  return AAR_OK;
-}
-
-extern "C" int aar_mode_bispectra_covariance(aar_ctx* ctx, double* cov, const double* w, const int* i1,
-                                             const int* i2, const int* i3, int npts, const int* p1, const int* p2,
-                                             const int* p3, int nmodes, const double* evals, int pmax, int nk) {
-  if (!ctx || !cov) return AAR_ERR_BAD_ARG;
-  int rc = modal_check(w, i1, i2, i3, npts, p1, p2, p3, nmodes, evals, pmax, nk);
-  if (rc) return rc;
-  AAR_HIP(hipSetDevice(ctx->device));
-  hipStream_t st = ctx->stream;
-  ModalBufs b;
-  if ((rc = modal_upload(b, w, i1, i2, i3, npts, p1, p2, p3, nmodes, evals, pmax, nk, st))) return rc;
-  // split the points so that the launch has about 2048 blocks, at least 256 points each
-  const int ntiles = (nmodes + 127) / 128, npairs = ntiles * (ntiles + 1) / 2;
-  int nsplit = std::max(1, std::min((2048 + npairs - 1) / npairs, 64));
-  nsplit = std::max(1, std::min(nsplit, npts / 256));
-  if (device_alloc((void**)&b.part, sizeof(double) * (size_t)nsplit * nmodes * nmodes) ||
-      device_alloc((void**)&b.out, sizeof(double) * (size_t)nmodes * nmodes))
-    return AAR_ERR_ALLOC;
-  if ((rc = modal_cov_launch(b.w, b.i1, b.i2, b.i3, npts, b.p1, b.p2, b.p3, nmodes, b.ev, pmax, nk, nsplit, b.part,
-                             b.out, st)))
-    return rc;
-  AAR_HIP(hipMemcpyAsync(cov, b.out, sizeof(double) * (size_t)nmodes * nmodes, hipMemcpyDeviceToHost, st));
-  AAR_HIP(hipStreamSynchronize(st));
-  return AAR_OK;
-}
-
-extern "C" int aar_mode_QS(aar_ctx* ctx, double* QS, const double* S, int nshapes, const double* w, const int* i1,
-                           const int* i2, const int* i3, int npts, const int* p1, const int* p2, const int* p3,
-                           int nmodes, const double* evals, int pmax, int nk) {
-  if (!ctx || !QS || !S || nshapes < 1) return AAR_ERR_BAD_ARG;
-  int rc = modal_check(w, i1, i2, i3, npts, p1, p2, p3, nmodes, evals, pmax, nk);
-  if (rc) return rc;
-  AAR_HIP(hipSetDevice(ctx->device));
-  hipStream_t st = ctx->stream;
-  ModalBufs b;
-  if ((rc = modal_upload(b, w, i1, i2, i3, npts, p1, p2, p3, nmodes, evals, pmax, nk, st))) return rc;
-  const int nsplit = std::max(1, std::min(64, npts / 4096));
-  if (device_alloc((void**)&b.S, sizeof(double) * (size_t)nshapes * npts) ||
-      device_alloc((void**)&b.part, sizeof(double) * (size_t)nsplit * nshapes * nmodes) ||
-      device_alloc((void**)&b.out, sizeof(double) * (size_t)nshapes * nmodes))
-    return AAR_ERR_ALLOC;
-  AAR_HIP(hipMemcpyAsync(b.S, S, sizeof(double) * (size_t)nshapes * npts, hipMemcpyHostToDevice, st));
-  if ((rc = modal_qs_launch(b.S, nshapes, b.w, b.i1, b.i2, b.i3, npts, b.p1, b.p2, b.p3, nmodes, b.ev, pmax, nk,
-                            nsplit, b.part, b.out, st)))
-    return rc;
-  AAR_HIP(hipMemcpyAsync(QS, b.out, sizeof(double) * (size_t)nshapes * nmodes, hipMemcpyDeviceToHost, st));
-  AAR_HIP(hipStreamSynchronize(st));
-  return AAR_OK;
 }
 
 // ------------------------------------------------------------------ building blocks

@@ -97,6 +97,14 @@ int modal_cov_launch(const double* w, const int* i1, const int* i2, const int* i
 int modal_qs_launch(const double* S, int nshapes, const double* w, const int* i1, const int* i2, const int* i3,
                     int npts, const int* p1, const int* p2, const int* p3, int nmodes, const double* evals, int pmax,
                     int nk, int nsplit, double* partial, double* qs, hipStream_t st);
+// R = S - sum_j coef_j Q_{sel_j} at every point; partial[block] = sum of w R^2 over the block's points
+// (ceil(npts / 256) blocks)
+int modal_residual_launch(const double* S, const double* w, const int* i1, const int* i2, const int* i3, int npts,
+                          const int* sel, int nsel, const double* coef, const int* p1, const int* p2, const int* p3,
+                          const double* evals, int pmax, int nk, double* R, double* partial, hipStream_t st);
+// ncand column-major (nsel+1)^2 matrices qq[idx][idx], idx = sel followed by cand[b]
+int modal_bordered_launch(const double* qq, int nmodes, const int* sel, int nsel, const int* cand, int ncand,
+                          double* out, hipStream_t st);
 
 // ---- binned statistics (aar_stats.hip)
 struct StatTerm {  // one weighted product of real fields a, b, c (c < 0: none)

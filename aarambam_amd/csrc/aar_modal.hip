@@ -200,7 +200,84 @@ __global__ void k_mode_qs_reduce(const double* __restrict__ partial, int nsplit,
   qs[e] = s;
 }
 
+// Residual of the expansion at every tetrapyd point, R[t] = S[t] - sum_j c_j Q_{n_j}(t)
+// (c_j = alpha_j / norm_j, modes n_j given by their index triples), and per block
+// the partial sum of w_t R[t]^2.
+__global__ void __launch_bounds__(kThreads)
+    k_mode_residual(const double* __restrict__ S, const double* __restrict__ w, const int* __restrict__ ti1,
+                    const int* __restrict__ ti2, const int* __restrict__ ti3, int npts,
+                    const int* __restrict__ sel, int nsel, const double* __restrict__ coef,
+                    const int* __restrict__ mp1, const int* __restrict__ mp2, const int* __restrict__ mp3,
+                    const double* __restrict__ evals, int pmax, int nk, double* __restrict__ R,
+                    double* __restrict__ partial) {
+  extern __shared__ double smem[];
+  double* ev = smem;
+  double* cf = ev + pmax * nk;  // [nsel]
+  int* sp = reinterpret_cast<int*>(cf + nsel);  // [3][nsel]
+  double* red = reinterpret_cast<double*>(sp + 3 * ((nsel + 1) & ~1));  // [kThreads]
+  for (int q = threadIdx.x; q < pmax * nk; q += kThreads) ev[q] = evals[q];
+  for (int j = threadIdx.x; j < nsel; j += kThreads) {
+    const int n = sel[j];
+    cf[j] = coef[j];
+    sp[j] = mp1[n];
+    sp[nsel + j] = mp2[n];
+    sp[2 * nsel + j] = mp3[n];
+  }
+  __syncthreads();
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  double acc = 0;
+  if (t < npts) {
+    const int i1 = ti1[t], i2 = ti2[t], i3 = ti3[t];
+    double sum = 0;
+    for (int j = 0; j < nsel; ++j) sum += sym_q(ev, nk, sp[j], sp[nsel + j], sp[2 * nsel + j], i1, i2, i3) * cf[j];
+    const double r = S[t] - sum;
+    R[t] = r;
+    acc = r * w[t] * r;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = kThreads / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// The bordered Gram matrices of the candidate modes: matrix b (column-major,
+// m = nsel + 1 rows) holds qq[idx][idx] for idx = the selected modes followed
+// by candidate cand[b].
+__global__ void k_mode_bordered(const double* __restrict__ qq, int nmodes, const int* __restrict__ sel, int nsel,
+                                const int* __restrict__ cand, double* __restrict__ out) {
+  const int m = nsel + 1;
+  const int b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)m * m) return;
+  const int r = (int)(e % m), c = (int)(e / m);
+  const int ir = r < nsel ? sel[r] : cand[b], ic = c < nsel ? sel[c] : cand[b];
+  out[(long long)b * m * m + e] = qq[(long long)ir * nmodes + ic];
+}
+
 }  // namespace
+
+int modal_residual_launch(const double* S, const double* w, const int* i1, const int* i2, const int* i3, int npts,
+                          const int* sel, int nsel, const double* coef, const int* p1, const int* p2, const int* p3,
+                          const double* evals, int pmax, int nk, double* R, double* partial, hipStream_t st) {
+  const size_t lds = sizeof(double) * ((size_t)pmax * nk + nsel + kThreads) + sizeof(int) * 3 * ((nsel + 1) & ~1);
+  if (lds > 64 * 1024) return AAR_ERR_UNSUPPORTED;
+  const int nb = (npts + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(k_mode_residual, dim3(nb), dim3(kThreads), lds, st, S, w, i1, i2, i3, npts, sel, nsel, coef,
+                     p1, p2, p3, evals, pmax, nk, R, partial);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+int modal_bordered_launch(const double* qq, int nmodes, const int* sel, int nsel, const int* cand, int ncand,
+                          double* out, hipStream_t st) {
+  const long long mm = (long long)(nsel + 1) * (nsel + 1);
+  if (ncand < 1 || ncand > 65535) return AAR_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(k_mode_bordered, dim3((unsigned)((mm + 255) / 256), ncand), dim3(256), 0, st, qq, nmodes, sel,
+                     nsel, cand, out);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
 
 int modal_cov_launch(const double* w, const int* i1, const int* i2, const int* i3, int npts, const int* p1,
                      const int* p2, const int* p3, int nmodes, const double* evals, int pmax, int nk, int nsplit,

@@ -227,6 +227,33 @@ int aar_mode_QS(aar_ctx *ctx, double *QS, const double *S, int n_shapes, const d
                 const int *tetra_i1, const int *tetra_i2, const int *tetra_i3, int tetra_npts, const int *mode_p1,
                 const int *mode_p2, const int *mode_p3, int n_modes, const double *mode_evals, int mode_p_max,
                 int k_npts);
+/* Basis.basis_expansion (Aarambam/basis/cmbbest.pyx:320-490): the expansion
+ * of n_shapes shapes S[s*tetra_npts + t] in the normalised mode bispectra by
+ * orthogonal matching pursuit.  The modes with p1 <= 3, p2 <= 3, p3 <= 3 start
+ * selected (all with max_modes -1); each step solves the normal equations of
+ * the selection by conjugate gradients (scipy's cg: x0 = 0, stop when
+ * |r| < rtol |b|, at most niter iterations, plus regularization on the
+ * diagonal), and adds, of the ten unselected modes most correlated with the
+ * residual, the one that raises the condition number of the selection's Gram
+ * matrix least; it stops when |R|^2 / <S,S> <= rr_tol, when max_modes - 1
+ * modes are selected (max_modes 0: n_modes), or when the condition number
+ * exceeds cond_tol; a final solve with rtol 1e-16 (10^6 iterations) gives
+ * the coefficients.  Modes (p,0,0) with p <= 2, (1,1,0) and those flagged in
+ * bad_modes (n_modes bytes, or NULL) are dropped.  As in the reference, the
+ * residual that drives the selection is that of the first shape for every
+ * shape.  Outputs: alpha[s*n_modes + n], shape_cov (n_shapes^2, may be NULL),
+ * corr and mse per shape (the reference's convergence_correlation and
+ * convergence_MSE, may be NULL), selected[s*n_modes + n] (may be NULL).  QQ,
+ * QS, the residuals and their projections run on the device, the Gram
+ * matrices' eigenvalues through rocSOLVER (syev, batched over the
+ * candidates); the conjugate gradients on the host. */
+int aar_mode_basis_expansion(aar_ctx *ctx, double *alpha, double *shape_cov, double *corr, double *mse,
+                             uint8_t *selected, const double *S, int n_shapes, const double *tetra_weights,
+                             const int *tetra_i1, const int *tetra_i2, const int *tetra_i3, int tetra_npts,
+                             const int *mode_p1, const int *mode_p2, const int *mode_p3, int n_modes,
+                             const double *mode_evals, int mode_p_max, int k_npts, const uint8_t *bad_modes,
+                             double regularization, long long niter, double rtol, double rr_tol, double cond_tol,
+                             int max_modes);
 
 /* ---- serving: continuous batching of IC jobs on one device ----
  * No counterpart in the reference, which runs one IC per process

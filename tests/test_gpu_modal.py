@@ -53,3 +53,57 @@ def test_out_of_range_indices_are_rejected(ctx):
     bad["p3"][0] = -1
     with pytest.raises(aar.AarError):
         ctx.mode_qs(bad, np.ones((1, len(g["w"]))))
+
+
+# ---------------------------------------------------------------- basis expansion
+def _shape(grid, kind):
+    from oracle import pymodal
+    k = grid["k"]
+    k1, k2, k3 = k[grid["i1"]], k[grid["i2"]], k[grid["i3"]]
+    if kind == "local":
+        return ((k1 * k1 / (k2 * k3) + k2 * k2 / (k1 * k3) + k3 * k3 / (k1 * k2)) / 3)[None, :]
+    if kind == "equilateral":
+        return ((k1 + k2 - k3) * (k2 + k3 - k1) * (k3 + k1 - k2) / (k1 * k2 * k3))[None, :]
+    Q = pymodal.mode_bispectra(grid)
+    n = int(np.nonzero((grid["p1"] == 5) & (grid["p2"] == 2) & (grid["p3"] == 1))[0][0])
+    return Q[n][None, :]
+
+
+@pytest.mark.parametrize("kind,kw", [
+    ("local", dict(RR_tol=1e-8, max_modes=40)),
+    ("equilateral", dict(RR_tol=1e-8, max_modes=40, rtol=1e-10, Niter=1e3)),
+    ("mode", dict(RR_tol=1e-12, cond_tol=1e14, max_modes=60)),
+    ("local", dict(max_modes=-1)),
+])
+def test_basis_expansion_matches_oracle(ctx, kind, kw):
+    """Same modes selected as by the oracle's restatement of Basis.basis_expansion, coefficients
+    to 1e-6 of the largest, convergence figures to 1e-9.  With every mode selected
+    (max_modes -1) the Gram matrix's condition number is ~1e15 and the coefficients of two
+    conjugate-gradient runs differ in rounding; there the fitted shapes are compared."""
+    from oracle import pymodal
+    every = kw.get("max_modes") == -1
+    g = modal.basis_grid(p_max=6 if every else 8, k_grid_size=30)
+    S = _shape(g, kind)
+    print("expansion", kind, kw, len(g["p1"]), "modes", len(g["w"]), "points", flush=True)
+    ra, rcov, rcorr, rmse, rmask = pymodal.basis_expansion(g, S, **kw)
+    da, dcov, dcorr, dmse, dmask = ctx.mode_basis_expansion(g, S, **kw)
+    np.testing.assert_array_equal(dmask, rmask)
+    np.testing.assert_allclose(dcov, rcov, rtol=1e-12)
+    if every:
+        Q = pymodal.mode_bispectra(g)
+        np.testing.assert_allclose(da @ Q, ra @ Q, rtol=0, atol=1e-6 * np.abs(S).max())
+        np.testing.assert_allclose(dmse, rmse, rtol=1e-3)
+        return
+    np.testing.assert_allclose(da, ra, rtol=0, atol=1e-6 * np.abs(ra).max())
+    np.testing.assert_allclose(dcorr, rcorr, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(dmse, rmse, rtol=1e-4, atol=1e-12)
+
+
+def test_basis_expansion_rejects_bad_input(ctx):
+    g = modal.basis_grid(p_max=4, k_grid_size=12)
+    S = np.ones((1, len(g["w"])))
+    bad = dict(g, i1=np.where(np.arange(len(g["i1"])) == 3, 12, g["i1"]))
+    with pytest.raises(aar.AarError):
+        ctx.mode_basis_expansion(bad, S)
+    with pytest.raises(aar.AarError):
+        ctx.mode_basis_expansion(g, S, max_modes=-2)

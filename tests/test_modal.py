@@ -47,3 +47,70 @@ def test_covariance_is_a_gram_matrix(grid):
                   for a, b, c in zip(grid["p1"], grid["p2"], grid["p3"])])
     np.testing.assert_allclose(cov, (Q * grid["w"]) @ Q.T, rtol=1e-10, atol=1e-14 * np.abs(cov).max())
     assert np.linalg.eigvalsh((cov + cov.T) / 2).min() > -1e-10 * np.abs(cov).max()
+
+
+# ---------------------------------------------------------------- basis expansion (oracle/pymodal.py)
+def _q_matrix(grid):
+    from oracle import pymodal
+    return pymodal.mode_bispectra(grid)
+
+
+def _good(grid):
+    p1, p2, p3 = grid["p1"], grid["p2"], grid["p3"]
+    return ~(((p1 <= 2) & (p2 == 0) & (p3 == 0)) | ((p1 == 1) & (p2 == 1) & (p3 == 0)))
+
+
+def _index(grid, t):
+    return int(np.nonzero((grid["p1"] == t[0]) & (grid["p2"] == t[1]) & (grid["p3"] == t[2]))[0][0])
+
+
+def test_expansion_of_a_shape_in_the_starting_span(grid):
+    """S = sum of four starting modes: no selection step, coefficients recovered, correlation 1."""
+    from oracle import pymodal
+    Q = _q_matrix(grid)
+    c = {(3, 0, 0): 1.0, (2, 1, 1): -0.5, (3, 2, 1): 0.25, (1, 1, 1): 2.0}
+    S = sum(v * Q[_index(grid, t)] for t, v in c.items())[None, :]
+    alpha, cov, corr, mse, masks = pymodal.basis_expansion(grid, S, RR_tol=1e-14)
+    start = (grid["p1"] <= 3) & (grid["p2"] <= 3) & (grid["p3"] <= 3)
+    np.testing.assert_array_equal(masks[0], start)
+    expect = np.zeros(len(grid["p1"]))
+    for t, v in c.items():
+        expect[_index(grid, t)] = v
+    np.testing.assert_allclose(alpha[0], expect, atol=1e-7 * np.abs(expect).max())
+    assert abs(corr[0] - 1) < 1e-12 and mse[0] < 1e-12
+    np.testing.assert_allclose(cov[0, 0], (S[0] * grid["w"]) @ S[0], rtol=1e-14)
+
+
+def test_expansion_over_all_modes_is_the_weighted_least_squares_fit(grid):
+    """max_modes -1 selects every mode: alpha is the w-weighted least-squares fit of S by the
+    modes outside the dropped set (numpy lstsq, independent of the CG path)."""
+    from oracle import pymodal
+    Q = _q_matrix(grid)
+    w = grid["w"]
+    k = grid["k"]
+    k1, k2, k3 = k[grid["i1"]], k[grid["i2"]], k[grid["i3"]]
+    S = ((k1 * k1 / (k2 * k3) + k2 * k2 / (k1 * k3) + k3 * k3 / (k1 * k2)) / 3)[None, :]
+    alpha, _, corr, mse, masks = pymodal.basis_expansion(grid, S, max_modes=-1)
+    assert masks[0].all()
+    good = _good(grid)
+    A = (Q[good] * np.sqrt(w)).T
+    sol, *_ = np.linalg.lstsq(A, S[0] * np.sqrt(w), rcond=None)
+    rec, ref = alpha[0] @ Q, sol @ Q[good]
+    np.testing.assert_allclose(rec, ref, atol=1e-7 * np.abs(S).max())
+    assert np.all(alpha[0][~good] == 0)
+    resid = ((S[0] - ref) ** 2 * w).sum() / ((S[0] ** 2) * w).sum()
+    assert mse[0] == pytest.approx(resid, rel=1e-4)  # MSE from the cancelling form SS + RR - 2 SR
+
+
+def test_selection_loop_recovers_a_mode_outside_the_start(grid):
+    """S = one mode with p1 = 4: the loop keeps adding modes while the residual is above the
+    tolerance, so it ends with that mode selected and the shape reproduced (the coefficients of the
+    other, nearly collinear modes are zero to the conditioning of their Gram matrix)."""
+    from oracle import pymodal
+    Q = _q_matrix(grid)
+    b = _index(grid, (4, 2, 1))
+    alpha, _, corr, mse, masks = pymodal.basis_expansion(grid, Q[b][None, :], RR_tol=1e-12, cond_tol=1e14)
+    assert masks[0][b] and masks[0].sum() > 20
+    assert alpha[0][b] == pytest.approx(1.0, abs=1e-4)
+    assert np.abs(np.delete(alpha[0], b)).max() < 1e-4
+    assert mse[0] < 1e-12 and abs(corr[0] - 1) < 1e-12
