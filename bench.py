@@ -200,7 +200,44 @@ def decomposition(aar, ctx, p_max=16, nk=150, reps=3):
     out["cpu_sample"] = {"kind": "reference" if lib is not None else "port", "k_grid": 40,
                          "points": len(small["w"]), "cpu_s": round(cdt, 3), "gpu_s": round(gdt, 4),
                          "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+    out["expansion"] = expansion(ctx)
     return out
+
+
+def expansion(ctx):
+    """Basis.basis_expansion (Aarambam/basis/cmbbest.pyx:320-490) of the local shape at the settings
+    of the reference's decompose.py and MakeICs example (N_modes 15, k_grid_size 150, ModeTol 1e-5,
+    MaxModeCount 150), and with a tolerance that runs the selection loop to 150 modes; the oracle's
+    numpy restatement times a 10-mode, 60-point-grid sample."""
+    from oracle import pymodal
+    from tests import modal
+
+    def local(g):
+        k = g["k"]
+        k1, k2, k3 = k[g["i1"]], k[g["i2"]], k[g["i3"]]
+        return ((k1 * k1 / (k2 * k3) + k2 * k2 / (k1 * k3) + k3 * k3 / (k1 * k2)) / 3)[None, :]
+
+    g = modal.basis_grid(p_max=15, k_grid_size=150, n_s=0.9660499)
+    S = local(g)
+    res = {"modes_3d": len(g["p1"]), "tetrapyd_points": len(g["w"])}
+    for name, tol in [("ModeTol_1e-5", 1e-5), ("ModeTol_1e-12", 1e-12)]:
+        t0 = time.perf_counter()
+        r = ctx.mode_basis_expansion(g, S, rtol=1e-10, Niter=1e3, RR_tol=tol, max_modes=150)
+        res[name] = {"s": round(time.perf_counter() - t0, 3), "selected": int(r[4][0].sum()),
+                     "correlation": float(r[2][0])}
+    small = modal.basis_grid(p_max=10, k_grid_size=60, n_s=0.9660499)
+    Ss = local(small)
+    kw = dict(rtol=1e-10, Niter=1e3, RR_tol=1e-5, max_modes=150)
+    t0 = time.perf_counter()
+    rs = ctx.mode_basis_expansion(small, Ss, **kw)
+    gdt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ro = pymodal.basis_expansion(small, Ss, **kw)
+    cdt = time.perf_counter() - t0
+    res["cpu_sample"] = {"kind": "port", "modes_3d": len(small["p1"]), "points": len(small["w"]),
+                         "cpu_s": round(cdt, 3), "gpu_s": round(gdt, 4),
+                         "same_selection": bool((ro[4] == rs[4]).all())}
+    return res
 
 
 def copy_peak(ctx, N=512, nf=8, iters=10):
