@@ -50,7 +50,7 @@ class AarCosmo(C.Structure):
 
 EXPORTS = ["aar_status_string", "aar_read_params", "aar_read_params_res", "aar_load_alpha", "aar_load_x0", "aar_load_transfer",
            "aar_read_glass", "aar_cosmology", "aar_seedtable", "aar_ranlxd_nth", "aar_write_gadget",
-           "aar_write_gadget_files",
+           "aar_write_gadget_files", "aar_task_layout", "aar_write_gadget_tasks", "aar_write_phi_tasks",
            "aar_write_phi", "aar_write_inputspec", "aar_ctx_create", "aar_ctx_destroy", "aar_ctx_sync", "aar_job_bytes",
            "aar_job_prepare", "aar_job_run", "aar_job_sync", "aar_job_device_output", "aar_job_fetch",
            "aar_job_fetch_field", "aar_job_last_seconds", "aar_job_destroy", "aar_fft3d",
@@ -97,6 +97,9 @@ def lib():
             "aar_write_gadget_files": (C.c_int, [PP, CP, fp, fp, C.c_longlong, C.POINTER(C.c_uint8), C.c_int,
                                                  C.c_longlong, C.POINTER(C.c_int)]),
             "aar_write_phi": (C.c_int, [PP, dp, C.c_int]),
+            "aar_task_layout": (C.c_int, [PP, TP, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]),
+            "aar_write_gadget_tasks": (C.c_int, [PP, CP, TP, fp, fp, C.c_longlong, C.c_int]),
+            "aar_write_phi_tasks": (C.c_int, [PP, dp, C.c_int, C.c_int, C.c_int]),
             "aar_write_inputspec": (C.c_int, [PP, TP, CP]),
             "aar_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
             "aar_ctx_destroy": (C.c_int, [vp]),

@@ -582,14 +582,9 @@ static int fwrite_all(const void* p, size_t sz, size_t n, FILE* fd) {
   return fwrite(p, sz, n, fd) == n ? AAR_OK : 777;
 }
 
-// One Gadget-1 file of save_local_data (save.c:162-520, NO64BITID): n particles
-// starting at global index first, of ntot in numfiles files.
-static int write_gadget_file(const std::string& fn, const aar_params* p, const aar_cosmo* c, const float* pos,
-                             const float* vel, long long first, long long n, long long ntot, int numfiles,
-                             const uint8_t* glass_header) {
-  FILE* fd = fopen(fn.c_str(), "wb");
-  if (!fd) return 10;
-  uint8_t hdr[256];
+// Gadget-1 header of save_local_data (save.c:195-266) for n of ntot particles in numfiles files.
+static void gadget_header(uint8_t* hdr, const aar_params* p, const aar_cosmo* c, const uint8_t* glass_header,
+                          long long n, long long ntot, int numfiles) {
   if (glass_header) memcpy(hdr, glass_header, 256);  // fields not set below keep the glass's bytes
   else memset(hdr, 0, 256);
   unsigned npart6[6] = {0, 0, 0, 0, 0, 0}, ntot6[6] = {0, 0, 0, 0, 0, 0};
@@ -616,6 +611,17 @@ static int write_gadget_file(const std::string& fn, const aar_params* p, const a
   memcpy(hdr + 160, &zero, 4);  // flag_stellarage
   memcpy(hdr + 164, &zero, 4);  // flag_metals
   memcpy(hdr + 168, &zero, 4);  // hashtabsize
+}
+
+// One Gadget-1 file of save_local_data (save.c:162-520, NO64BITID): n particles
+// starting at global index first, of ntot in numfiles files.
+static int write_gadget_file(const std::string& fn, const aar_params* p, const aar_cosmo* c, const float* pos,
+                             const float* vel, long long first, long long n, long long ntot, int numfiles,
+                             const uint8_t* glass_header) {
+  FILE* fd = fopen(fn.c_str(), "wb");
+  if (!fd) return 10;
+  uint8_t hdr[256];
+  gadget_header(hdr, p, c, glass_header, n, ntot, numfiles);
   int rc = AAR_OK;
   int dummy = 256;
   rc |= fwrite_all(&dummy, 4, 1, fd);
@@ -683,28 +689,37 @@ extern "C" int aar_write_gadget_files(const aar_params* p, const aar_cosmo* c, c
   return AAR_OK;
 }
 
-// Real-space potential dump (save.c:14-105), one task.
-extern "C" int aar_write_phi(const aar_params* p, const double* real, int non_gaussian) {
-  if (!p || !real) return AAR_ERR_BAD_ARG;
+// x-planes of task r of ntask in FFTW 2's MPI slab decomposition
+// (rfftwnd_mpi_local_sizes, main.c:1573-1588): blocks of ceil(N / ntask)
+// planes, the last non-empty task takes the remainder, later tasks none.
+static void fftw_slab(int N, int ntask, int r, int* start, int* nx) {
+  const int block = (N + ntask - 1) / ntask, used = (N + block - 1) / block;
+  if (r >= used) {
+    *start = 0;
+    *nx = 0;
+  } else {
+    *start = r * block;
+    *nx = r == used - 1 ? N - *start : block;
+  }
+}
+
+// Potential records of planes [x0, x0 + nx) of a real N^3 field (save.c:88-99).
+static int write_phi_file(const char* name, const aar_params* p, const double* real, int x0, int nx, int total) {
   const int N = p->Nmesh, W = 2 * (N / 2 + 1);
-  char name[1200];
-  snprintf(name, sizeof(name), "%s/%s_potential_%d", p->OutputDir, non_gaussian ? "Nongauss" : "Gauss",
-           p->Seed);
   FILE* fd = fopen(name, "wb");
   if (!fd) return 10;
-  int total = N * N * W + N * W;  // TotalSizePlusAdditional of one task
   int rc = fwrite_all(&total, 4, 1, fd);
   rc |= fwrite_all(&N, 4, 1, fd);
   rc |= fwrite_all(&p->Box, 8, 1, fd);
   // the reference issues two fwrite calls per element; here one x-plane of
   // {int coord, double value} records is packed and written at a time
   std::vector<unsigned char> rec((size_t)N * N * 12);
-  for (int i = 0; i < N && !rc; ++i) {
+  for (int i = 0; i < nx && !rc; ++i) {
     unsigned char* o = rec.data();
     for (int j = 0; j < N; ++j) {
-      const double* src = real + ((size_t)i * N + j) * N;
+      const double* src = real + ((size_t)(x0 + i) * N + j) * N;
       for (int k = 0; k < N; ++k, o += 12) {
-        const int coord = (i * N + j) * W + k;
+        const int coord = (i * N + j) * W + k;  // slab-local, as each task stores it
         memcpy(o, &coord, 4);
         memcpy(o + 4, &src[k], 8);
       }
@@ -713,6 +728,161 @@ extern "C" int aar_write_phi(const aar_params* p, const double* real, int non_ga
   }
   if (fclose(fd)) rc = 777;
   return rc ? 777 : AAR_OK;
+}
+
+// Real-space potential dump (save.c:14-105), one task.
+extern "C" int aar_write_phi(const aar_params* p, const double* real, int non_gaussian) {
+  if (!p || !real) return AAR_ERR_BAD_ARG;
+  const int N = p->Nmesh, W = 2 * (N / 2 + 1);
+  char name[1200];
+  snprintf(name, sizeof(name), "%s/%s_potential_%d", p->OutputDir, non_gaussian ? "Nongauss" : "Gauss",
+           p->Seed);
+  return write_phi_file(name, p, real, 0, N, N * N * W + N * W);  // TotalSizePlusAdditional of one task
+}
+
+// The dump as ntask tasks write it (save.c:14-105): one file per task with its
+// slab, named ..._StartX_%04d_NumX_%d when more than one task holds particles;
+// every header carries task 0's TotalSizePlusAdditional.
+extern "C" int aar_write_phi_tasks(const aar_params* p, const double* real, int non_gaussian, int ntask,
+                                   int ntask_with_n) {
+  if (!p || !real || ntask < 1) return AAR_ERR_BAD_ARG;
+  if (ntask_with_n <= 1) return aar_write_phi(p, real, non_gaussian);
+  const int N = p->Nmesh, W = 2 * (N / 2 + 1);
+  int s0, n0;
+  fftw_slab(N, ntask, 0, &s0, &n0);
+  const int total = n0 * N * W + N * W;
+  for (int r = 0; r < ntask; ++r) {
+    int x0, nx;
+    fftw_slab(N, ntask, r, &x0, &nx);
+    char name[1200];
+    snprintf(name, sizeof(name), "%s/%s_potential_%d_StartX_%04d_NumX_%d", p->OutputDir,
+             non_gaussian ? "Nongauss" : "Gauss", p->Seed, x0, nx);
+    const int rc = write_phi_file(name, p, real, x0, nx, total);
+    if (rc) return rc;
+  }
+  return AAR_OK;
+}
+
+// Task of every glass particle in every x-tile: the slab of its unperturbed x
+// (read_glass.c:129-146, x stored as float), task_of[ti * nglass + n].
+static void glass_tasks(const aar_params* p, const aar_tables* t, int ntask, std::vector<uint16_t>& task_of) {
+  const int N = p->Nmesh, T = p->GlassTileFac;
+  std::vector<int> slab_task(N);
+  for (int r = 0; r < ntask; ++r) {
+    int x0, nx;
+    fftw_slab(N, ntask, r, &x0, &nx);
+    for (int i = 0; i < nx; ++i) slab_task[x0 + i] = r;
+  }
+  task_of.assign((size_t)T * t->nglass, 0);
+  for (int ti = 0; ti < T; ++ti)
+    for (int n = 0; n < t->nglass; ++n) {
+      const float x = (float)(t->glass[3 * (size_t)n] / t->glass_box * (p->Box / T) + ti * (p->Box / T));
+      int slab = (int)(x / p->Box * N);
+      if (slab >= N) slab = N - 1;
+      task_of[(size_t)ti * t->nglass + n] = (uint16_t)slab_task[slab];
+    }
+}
+
+extern "C" int aar_task_layout(const aar_params* p, const aar_tables* t, int ntask, long long* npart_task,
+                               int* ntask_with_n) {
+  if (!p || !t || !t->glass || t->nglass < 1 || ntask < 1 || ntask > 65535 || p->GlassTileFac < 1 || p->Nmesh < 1)
+    return AAR_ERR_BAD_ARG;
+  std::vector<uint16_t> task_of;
+  glass_tasks(p, t, ntask, task_of);
+  std::vector<long long> cnt(ntask, 0);
+  const long long T2 = (long long)p->GlassTileFac * p->GlassTileFac;
+  for (size_t q = 0; q < task_of.size(); ++q) cnt[task_of[q]] += T2;  // every (tj, tk) repeats the x-tile
+  int with = 0;
+  for (int r = 0; r < ntask; ++r) {
+    if (npart_task) npart_task[r] = cnt[r];
+    with += cnt[r] > 0;
+  }
+  if (ntask_with_n) *ntask_with_n = with;
+  return AAR_OK;
+}
+
+// The snapshot as ntask tasks write it (write_particle_data, save.c:108-520):
+// task r's particles -- those whose unperturbed x lies in its slabs -- in
+// file order, with their run-wide IDs, in OutputDir/FileBase.r (FileBase when
+// one task holds them all); tasks without particles write no file.
+extern "C" int aar_write_gadget_tasks(const aar_params* p, const aar_cosmo* c, const aar_tables* t,
+                                      const float* pos, const float* vel, long long npart, int ntask) {
+  if (!p || !c || !t || !pos || !vel || !t->glass || ntask < 1 || ntask > 65535) return AAR_ERR_BAD_ARG;
+  if (p->NumFilesWrittenInParallel > ntask) return AAR_ERR_WRITE_PARALLEL;  // save.c:112-117
+  const int T = p->GlassTileFac;
+  const long long ng = t->nglass;
+  if ((long long)T * T * T * ng != npart) return AAR_ERR_BAD_ARG;
+  std::vector<uint16_t> task_of;
+  glass_tasks(p, t, ntask, task_of);
+  std::vector<long long> cnt(ntask, 0);
+  for (size_t q = 0; q < task_of.size(); ++q) cnt[task_of[q]] += (long long)T * T;
+  int with = 0;
+  for (int r = 0; r < ntask; ++r) with += cnt[r] > 0;
+  if (with <= 1) {
+    for (int r = 0; r < ntask; ++r)
+      if (cnt[r] == npart)
+        return write_gadget_file(std::string(p->OutputDir) + "/" + p->FileBase, p, c, pos, vel, 0, npart, npart,
+                                 1, t->glass_header);
+    return AAR_OK;
+  }
+  // x-tiles holding each task's particles
+  std::vector<int> tlo(T, 65535), thi(T, -1);
+  for (int ti = 0; ti < T; ++ti)
+    for (long long n = 0; n < ng; ++n) {
+      tlo[ti] = std::min<int>(tlo[ti], task_of[(size_t)ti * ng + n]);
+      thi[ti] = std::max<int>(thi[ti], task_of[(size_t)ti * ng + n]);
+    }
+  std::vector<float> buf(3 << 20);
+  std::vector<int> ids(1 << 20);
+  for (int r = 0; r < ntask; ++r) {
+    if (!cnt[r]) continue;
+    const std::string fn = std::string(p->OutputDir) + "/" + p->FileBase + "." + std::to_string(r);
+    FILE* fd = fopen(fn.c_str(), "wb");
+    if (!fd) return 10;
+    uint8_t hdr[256];
+    gadget_header(hdr, p, c, t->glass_header, cnt[r], npart, with);
+    int rc = AAR_OK, dummy = 256;
+    rc |= fwrite_all(&dummy, 4, 1, fd);
+    rc |= fwrite_all(hdr, 256, 1, fd);
+    rc |= fwrite_all(&dummy, 4, 1, fd);
+    // pass 0: positions, 1: velocities, 2: IDs, each gathered in particle order
+    for (int pass = 0; pass < 3 && !rc; ++pass) {
+      dummy = (int)(uint32_t)((pass < 2 ? 12 : 4) * (size_t)cnt[r]);
+      rc |= fwrite_all(&dummy, 4, 1, fd);
+      size_t fill = 0;
+      auto flush = [&]() {
+        if (pass < 2) rc |= fwrite_all(buf.data(), sizeof(float), 3 * fill, fd);
+        else rc |= fwrite_all(ids.data(), sizeof(int), fill, fd);
+        fill = 0;
+      };
+      for (int ti = 0; ti < T; ++ti) {
+        if (r < tlo[ti] || r > thi[ti]) continue;
+        const uint16_t* to = &task_of[(size_t)ti * ng];
+        for (int tj = 0; tj < T; ++tj)
+          for (int tk = 0; tk < T; ++tk) {
+            const long long base = (((long long)ti * T + tj) * T + tk) * ng;
+            for (long long n = 0; n < ng; ++n) {
+              if (to[n] != r) continue;
+              const long long q = base + n;
+              if (pass == 2) {
+                ids[fill] = (int)(q + 1);
+              } else {
+                const float* src = (pass == 0 ? pos : vel) + 3 * q;
+                buf[3 * fill] = src[0];
+                buf[3 * fill + 1] = src[1];
+                buf[3 * fill + 2] = src[2];
+              }
+              if (++fill == ids.size()) flush();
+            }
+          }
+      }
+      flush();
+      rc |= fwrite_all(&dummy, 4, 1, fd);
+    }
+    if (fclose(fd)) rc = 777;
+    if (rc) return 777;
+  }
+  return AAR_OK;
 }
 
 // inputspec_<FileBase>.txt with the Peacock & Dodds nonlinear spectrum (main.c:1620-1694).

@@ -7,7 +7,9 @@
 // optional potential dumps and inputspec file.  The default is the FP64
 // pairwise schedule, the closest to the reference; --fp32 selects the FP32
 // batched schedule.  The snapshot is one file unless a record would exceed
-// 2^31 - 1 bytes (or --max-record BYTES): then FileBase.<i> files.
+// 2^31 - 1 bytes (or --max-record BYTES): then FileBase.<i> files.  Under
+// mpirun / srun with N tasks, task 0 runs the job and writes the outputs in
+// the layout of N reference tasks; the other tasks exit.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -64,7 +66,31 @@ static void banner(const aar_params& p, int nglass) {  // main.c:104-139 (ResBas
   printf("\n%s\n\n", pstr);
 }
 
+// Tasks of an MPI launch (mpirun / srun set these before the program starts);
+// 1 and rank 0 when run directly.
+static void mpi_world(int* size, int* rank) {
+  const char* sz[] = {"OMPI_COMM_WORLD_SIZE", "PMI_SIZE", "MV2_COMM_WORLD_SIZE", "SLURM_NTASKS"};
+  const char* rk[] = {"OMPI_COMM_WORLD_RANK", "PMI_RANK", "MV2_COMM_WORLD_RANK", "SLURM_PROCID"};
+  *size = 1;
+  *rank = 0;
+  for (int q = 0; q < 4; ++q) {
+    const char* a = getenv(sz[q]);
+    const char* b = getenv(rk[q]);
+    if (a && b && atoi(a) > 0) {
+      *size = atoi(a);
+      *rank = atoi(b);
+      return;
+    }
+  }
+}
+
 int main(int argc, char** argv) {
+  // Launched as `mpirun -np N` (as the reference's examples do): task 0 runs the
+  // whole job on one GPU and writes the outputs in the layout N tasks write;
+  // the other tasks have nothing to do.
+  int ntask, rank;
+  mpi_world(&ntask, &rank);
+  if (rank != 0) return 0;
   if (argc < 2) {  // main.c:65-72
     fprintf(stdout, "\nParameters are missing.\n");
     fprintf(stdout, "Call with <ParameterFile>\n\n");
@@ -91,7 +117,7 @@ int main(int argc, char** argv) {
     }
   }
   const double t0 = now();
-  printf("RUNNING LPTPNG WITH %d TASKS", 1);
+  printf("RUNNING LPTPNG WITH %d TASKS", ntask);
   aar_params p;
   char msg[8192];
   const int prc = kRes ? aar_read_params_res(argv[1], &p, msg, sizeof(msg))
@@ -100,7 +126,7 @@ int main(int argc, char** argv) {
     fputs(msg, stdout);
     return 0;
   }
-  if (p.NumFilesWrittenInParallel > 1) die(24131, "NumFilesWrittenInParallel");  // save.c:112-117
+  if (p.NumFilesWrittenInParallel > ntask) die(24131, "NumFilesWrittenInParallel");  // save.c:112-117
   std::vector<double> alpha((size_t)p.N_modes * p.N_modes * p.N_modes), x0(2 * (size_t)p.N_modes);
   int rc;
   if ((rc = aar_load_alpha(p.FileWithAlphaCoeff, p.N_modes, alpha.data()))) die(rc, p.FileWithAlphaCoeff);
@@ -148,11 +174,15 @@ int main(int argc, char** argv) {
   if ((rc = aar_job_run(job, nullptr)) || (rc = aar_job_sync(job))) die(rc, "aar_job_run");
   printf(" Done [%.3f s on device, %.3f s prepare]\n", aar_job_last_seconds(job), tr - tp);
   const long long npart = (long long)p.GlassTileFac * p.GlassTileFac * p.GlassTileFac * nglass;
+  int with_n = 1;
+  if (ntask > 1 && (rc = aar_task_layout(&p, &t, ntask, nullptr, &with_n))) die(rc, "task layout");
   if (p.SavePotentialField == 1) {  // main.c:1096, 1135
     std::vector<double> f((size_t)p.Nmesh * p.Nmesh * p.Nmesh);
-    if ((rc = aar_job_fetch_field(job, AAR_FIELD_PHIG_REAL, f.data())) || (rc = aar_write_phi(&p, f.data(), 0)))
+    if ((rc = aar_job_fetch_field(job, AAR_FIELD_PHIG_REAL, f.data())) ||
+        (rc = aar_write_phi_tasks(&p, f.data(), 0, ntask, with_n)))
       die(rc, "write_phi");
-    if ((rc = aar_job_fetch_field(job, AAR_FIELD_PHING_REAL, f.data())) || (rc = aar_write_phi(&p, f.data(), 1)))
+    if ((rc = aar_job_fetch_field(job, AAR_FIELD_PHING_REAL, f.data())) ||
+        (rc = aar_write_phi_tasks(&p, f.data(), 1, ntask, with_n)))
       die(rc, "write_phi");
   }
   std::vector<float> pos(3 * (size_t)npart), vel(3 * (size_t)npart);
@@ -164,7 +194,10 @@ int main(int argc, char** argv) {
   // 4-byte record markers (12 npart >= 2^31, e.g. 1024^3) or --max-record: then FileBase.<i>
   // files as several tasks write them.
   int nfiles = 1;
-  if (12LL * npart <= max_record) {
+  if (ntask > 1) {
+    if ((rc = aar_write_gadget_tasks(&p, &c, &t, pos.data(), vel.data(), npart, ntask))) die(rc, "write_particle_data");
+    nfiles = with_n;
+  } else if (12LL * npart <= max_record) {
     if ((rc = aar_write_gadget(&p, &c, pos.data(), vel.data(), npart, ghdr))) die(rc, "write_particle_data");
   } else if ((rc = aar_write_gadget_files(&p, &c, pos.data(), vel.data(), npart, ghdr, 0, max_record,
                                           &nfiles))) {

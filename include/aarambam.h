@@ -24,6 +24,9 @@
  *   aar_write_gadget_files  Aarambam/LPT/Basis/save.c:108-520 with several writer
  *                        tasks (FileBase.<i>, num_files), save.c:184-187, 252
  *   aar_write_phi        Aarambam/LPT/Basis/save.c:14-105 (write_phi)
+ *   aar_task_layout, aar_write_gadget_tasks, aar_write_phi_tasks
+ *                        the same writers as run on several MPI tasks (main.c:1556-1599,
+ *                        read_glass.c:129-146, save.c:14-520)
  *   aar_write_inputspec  Aarambam/LPT/Basis/main.c:1631-1694 (print_spec)
  *
  * Status codes: 0 on success; values below 100000 are the reference's
@@ -151,6 +154,20 @@ int aar_write_gadget_files(const aar_params *p, const aar_cosmo *c, const float 
                            long long npart, const uint8_t *glass_header, int nfiles, long long max_record,
                            int *nfiles_out);
 int aar_write_phi(const aar_params *p, const double *real_nnn, int non_gaussian);
+/* The outputs as the reference writes them when run on ntask MPI tasks
+ * (x-slabs of FFTW 2's MPI decomposition: blocks of ceil(Nmesh / ntask) planes,
+ * main.c:1556-1599; particles to the task of their unperturbed x,
+ * read_glass.c:129-146).  aar_task_layout: particles per task and the number of
+ * tasks holding any (NTaskWithN).  aar_write_gadget_tasks: OutputDir/FileBase.<task>
+ * for every task with particles, num_files = NTaskWithN, run-wide IDs
+ * (save.c:108-520); FatalError 24131 when NumFilesWrittenInParallel > ntask.
+ * aar_write_phi_tasks: one dump per task, ..._StartX_%04d_NumX_%d when
+ * NTaskWithN > 1 (save.c:14-105). */
+int aar_task_layout(const aar_params *p, const aar_tables *t, int ntask, long long *npart_task /* ntask, or NULL */,
+                    int *ntask_with_n);
+int aar_write_gadget_tasks(const aar_params *p, const aar_cosmo *c, const aar_tables *t, const float *pos,
+                           const float *vel, long long npart, int ntask);
+int aar_write_phi_tasks(const aar_params *p, const double *real_nnn, int non_gaussian, int ntask, int ntask_with_n);
 int aar_write_inputspec(const aar_params *p, const aar_tables *t, const aar_cosmo *c);
 
 /* ---- device side ---- */

@@ -103,13 +103,19 @@ def _read_gadget(paths):
     return np.concatenate(pos), np.concatenate(vel)
 
 
-@pytest.mark.parametrize("exe,split", [("2LPTBasis", False), ("2LPTResBasis", False), ("2LPTBasis", True)])
-def test_executable_snapshot_vs_oracle(tmp_path, exe, split):
-    """split: --max-record 120000 puts the 32^3 particles in FileBase.0-3."""
+@pytest.mark.parametrize("exe,layout", [("2LPTBasis", "one"), ("2LPTResBasis", "one"), ("2LPTBasis", "split"),
+                                        ("2LPTResBasis", "tasks")])
+def test_executable_snapshot_vs_oracle(tmp_path, exe, layout):
+    """split: --max-record 120000 puts the 32^3 particles in FileBase.0-3; tasks: launched as task 0
+    of 3 MPI tasks, the snapshot in FileBase.0-2 and the potentials in three slab dumps, which
+    collate to the oracle's fields."""
     res = exe == "2LPTResBasis"
+    split, tasks = layout == "split", layout == "tasks"
     N = 32
     tk, tt = cases.load_transfer()
     kw = dict(Nmesh=N, Nsample=N, Nmax=N, N_modes=4, WhichTransfer=2, Seed=7)
+    if tasks:
+        kw.update(NumFilesWrittenInParallel=3, SavePotentialField=1)
     cfg = cases.res_config(2, A=0.9, w=8.0, phi=0.5, **kw) if res else cases.base_config(**kw)
     rows = cases.equilateral_rows()
     np.savetxt(tmp_path / "alpha.dat", rows, fmt=["%d", "%d", "%d", "%.17g"])
@@ -123,15 +129,27 @@ def test_executable_snapshot_vs_oracle(tmp_path, exe, split):
                            AiPath=str(tmp_path / "ai.dat"), FileBase="snap", OutputDir=str(tmp_path))
     print("running", exe, flush=True)
     args = [os.path.join(BIN, exe), str(pf)] + (["--max-record", "120000"] if split else [])
-    out = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    env = dict(os.environ, OMPI_COMM_WORLD_SIZE="3", OMPI_COMM_WORLD_RANK="0") if tasks else None
+    out = subprocess.run(args, capture_output=True, text=True, timeout=300, env=env)
     assert out.returncode == 0, out.stdout + out.stderr
     title = "Aarambam-2LPT-ResBasis" if res else "Aarambam-2LPT-Basis"
     assert title in out.stdout and ("A_Pk = 0.9000" in out.stdout) == res
-    files = [tmp_path / f"snap.{i}" for i in range(4)] if split else [tmp_path / "snap"]
-    assert (tmp_path / "snap").exists() != split
-    pos, vel = _read_gadget(files)
+    files = [tmp_path / f"snap.{i}" for i in range(4 if split else 3)] if split or tasks else [tmp_path / "snap"]
+    assert (tmp_path / "snap").exists() != (split or tasks)
     ref = orc.run(dict(cfg, transfer_k=tk, transfer_t=tt), orc.alpha_table(cfg["N_modes"], rows), x0, glass,
-                  100.0, want_fields=False)
+                  100.0, want_fields=tasks)
+    if tasks:
+        parts = [read_gadget_file(f) for f in files]
+        order = np.argsort(np.concatenate([q[3] for q in parts]))
+        pos = np.concatenate([q[1] for q in parts])[order]
+        vel = np.concatenate([q[2] for q in parts])[order]
+        assert all(q[0]["nfiles"] == 3 for q in parts)
+        pot = aar.collate_potential(str(tmp_path))
+        for key, name in [("Gauss_potential", "phig_real"), ("Nongauss_potential", "phing_real")]:
+            d = pot[key] - ref[name]
+            assert np.linalg.norm(d) <= 1e-10 * np.linalg.norm(ref[name]), key
+    else:
+        pos, vel = _read_gadget(files)
     box = cfg["Box"]
     dpos = np.abs(pos.astype(np.float64) - ref["pos"])
     dpos = np.minimum(dpos, box - dpos)

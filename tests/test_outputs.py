@@ -3,7 +3,9 @@ out (Aarambam/LPT/Basis/save.c:184-187, 252), the potential dump and its
 collation (Aarambam/utils.py:307-347, with the dump's own Nmesh and StartX in
 place of the file-name arithmetic)."""
 import ctypes as C
+import os
 import struct
+import subprocess
 
 import numpy as np
 import pytest
@@ -127,3 +129,109 @@ def test_collate_slab_dumps(tmp_path):
     _write_slab(tmp_path / "Gauss_potential_6_StartX_0000_NumX_12", N, 0, N, 100.0, g)
     with pytest.raises(ValueError, match="several seeds"):
         aar.collate_potential(str(tmp_path))
+
+
+# ---------------------------------------------------------------- the layout of several MPI tasks
+def fftw_slabs(N, ntask):
+    """FFTW 2's MPI slab decomposition (rfftwnd_mpi_local_sizes): blocks of ceil(N / ntask) planes,
+    the last used task takes the rest, later tasks none."""
+    block = -(-N // ntask)
+    used = -(-N // block)
+    out = []
+    for r in range(ntask):
+        if r >= used:
+            out.append((0, 0))
+        else:
+            out.append((r * block, N - r * block if r == used - 1 else block))
+    return out
+
+
+def _task_setup(tmp_path, N=16, T=2, n=8, box=100.0):
+    d = dict(cases.base_config(Nmesh=N, GlassTileFac=T, Box=box), OutputDir=str(tmp_path), FileBase="snap",
+             NumFilesWrittenInParallel=1, Seed=7)
+    p = api.params_from_dict(d)
+    rng = np.random.default_rng(5)
+    glass = (rng.random((n ** 3, 3)) * 100.0).astype(np.float32)  # glass BoxSize 100
+    t = api.tables(np.zeros(8), np.zeros(4), glass, 100.0)
+    c = api.AarCosmo()
+    assert aar.lib().aar_cosmology(C.byref(p), C.byref(t.c), C.byref(c)) == 0
+    npart = T ** 3 * n ** 3
+    pos = (rng.random((npart, 3)) * box).astype(np.float32)
+    vel = rng.standard_normal((npart, 3)).astype(np.float32)
+    # unperturbed x of every particle in file order (read_glass.c:129-146): tile-major, float
+    ti = np.repeat(np.arange(T ** 3) // (T * T), n ** 3)
+    gx = np.tile(glass[:, 0].astype(np.float64), T ** 3)
+    x = (gx / 100.0 * (box / T) + ti * (box / T)).astype(np.float32)
+    return p, c, t, pos, vel, x
+
+
+@pytest.mark.parametrize("ntask", [1, 3, 5, 16, 20])
+def test_task_layout_matches_slab_decomposition(tmp_path, ntask):
+    p, c, t, pos, vel, x = _task_setup(tmp_path)
+    N = p.Nmesh
+    slab_task = np.zeros(N, int)
+    for r, (x0, nx) in enumerate(fftw_slabs(N, ntask)):
+        slab_task[x0:x0 + nx] = r
+    slab = np.minimum((x.astype(np.float64) / p.Box * N).astype(int), N - 1)
+    expect = np.bincount(slab_task[slab], minlength=ntask)
+    got = np.zeros(ntask, np.int64)
+    with_n = C.c_int()
+    assert aar.lib().aar_task_layout(C.byref(p), C.byref(t.c), ntask,
+                                     got.ctypes.data_as(C.POINTER(C.c_longlong)), C.byref(with_n)) == 0
+    np.testing.assert_array_equal(got, expect)
+    assert with_n.value == (expect > 0).sum()
+
+
+def test_snapshot_in_the_layout_of_three_tasks(tmp_path):
+    p, c, t, pos, vel, x = _task_setup(tmp_path)
+    fp = C.POINTER(C.c_float)
+    assert aar.lib().aar_write_gadget_tasks(C.byref(p), C.byref(c), C.byref(t.c), pos.ctypes.data_as(fp),
+                                            vel.ctypes.data_as(fp), len(pos), 3) == 0
+    slabs = fftw_slabs(p.Nmesh, 3)
+    slab = np.minimum((x.astype(np.float64) / p.Box * p.Nmesh).astype(int), p.Nmesh - 1)
+    seen = []
+    for r in range(3):
+        head, ps, vs, ids = read_gadget_file(tmp_path / f"snap.{r}")
+        assert head["nfiles"] == 3 and head["ntot"][1] == len(pos) and head["npart"][1] == len(ps)
+        x0, nx = slabs[r]
+        q = ids - 1
+        assert np.all((slab[q] >= x0) & (slab[q] < x0 + nx)) and np.all(np.diff(q) > 0)
+        np.testing.assert_array_equal(ps, pos[q])
+        np.testing.assert_array_equal(vs, vel[q])
+        seen.append(q)
+    np.testing.assert_array_equal(np.sort(np.concatenate(seen)), np.arange(len(pos)))
+    assert not (tmp_path / "snap").exists()
+    # more writers than tasks: FatalError 24131 (save.c:112-117)
+    p.NumFilesWrittenInParallel = 4
+    assert aar.lib().aar_write_gadget_tasks(C.byref(p), C.byref(c), C.byref(t.c), pos.ctypes.data_as(fp),
+                                            vel.ctypes.data_as(fp), len(pos), 3) == 24131
+
+
+def test_potential_in_the_layout_of_three_tasks_collates(tmp_path):
+    """Slab dumps named ..._StartX_%04d_NumX_%d; both the reference's collation rule (Nmesh = sum
+    of NumX, slabs concatenated in sorted name order, Aarambam/utils.py:307-347) and
+    collate_potential rebuild the grid."""
+    p, *_ = _task_setup(tmp_path)
+    N = p.Nmesh
+    f = np.random.default_rng(8).standard_normal((N, N, N))
+    dp = C.POINTER(C.c_double)
+    assert aar.lib().aar_write_phi_tasks(C.byref(p), f.ctypes.data_as(dp), 0, 3, 3) == 0
+    names = sorted(q.name for q in tmp_path.iterdir())
+    assert names == [f"Gauss_potential_7_StartX_{x0:04d}_NumX_{nx}" for x0, nx in fftw_slabs(N, 3)]
+    dt = np.dtype([("ind", "<i4"), ("pot", "<f8")])
+    n_ref = sum(int(nm.split("_")[-1]) for nm in names)
+    arr = np.concatenate([np.memmap(tmp_path / nm, offset=16, dtype=dt, mode="r")["pot"] for nm in names])
+    assert n_ref == N
+    np.testing.assert_array_equal(arr.reshape(N, N, N), f)
+    W = 2 * (N // 2 + 1)
+    hdr = np.frombuffer((tmp_path / names[1]).read_bytes()[:8], "<i4")
+    assert hdr[0] == fftw_slabs(N, 3)[0][1] * N * W + N * W and hdr[1] == N  # task 0's TotalSizePlusAdditional
+    out = aar.collate_potential(str(tmp_path))
+    np.testing.assert_array_equal(out["Gauss_potential"], f)
+
+
+def test_executable_tasks_other_than_zero_exit(tmp_path):
+    exe = os.path.join(os.path.dirname(api.__file__), "bin", "2LPTBasis")
+    env = dict(os.environ, OMPI_COMM_WORLD_SIZE="4", OMPI_COMM_WORLD_RANK="2")
+    out = subprocess.run([exe, str(tmp_path / "missing")], capture_output=True, text=True, timeout=60, env=env)
+    assert out.returncode == 0 and out.stdout == "" and not list(tmp_path.iterdir())
