@@ -241,7 +241,19 @@ double orc_transfer(const orc_params *p, double k) {
   return t;
 }
 
-static double powerspec(const orc_ctx *cx, double k) { /* power.c:240-288 */
+/* WDM suppression of power.c:267-275 (Bode, Ostriker & Turok 2001, eq. A9, g_X = 1.5) */
+static double wdm_factor(const orc_params *p, double k, double power) {
+  if (p->WDM_On != 1) return power;
+  double alpha = 0.048 * pow((p->Omega - p->OmegaBaryon) / 0.4, 0.15) * pow(p->HubbleParam / 0.65, 1.3) *
+                 pow(1.0 / p->WDM_PartMass_in_kev, 1.15);
+  double Tf = pow(1 + pow(alpha * k * (3.085678e24 / p->UnitLength_in_cm), 2 * 1.2), -5.0 / 1.2);
+  return power * (Tf * Tf);
+}
+
+static double powerspec_base(const orc_ctx *cx, double k);
+static double powerspec(const orc_ctx *cx, double k) { return wdm_factor(cx->p, k, powerspec_base(cx, k)); }
+
+static double powerspec_base(const orc_ctx *cx, double k) { /* power.c:240-266 */
   const orc_params *p = cx->p;
   if (p->Variant == 1) { /* ResBasis/power.c:240-260 */
     double power = 0;

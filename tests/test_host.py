@@ -194,3 +194,15 @@ def test_write_phi_layout(tmp_path):
     i, j, k = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
     np.testing.assert_array_equal(rec["c"], ((i * N + j) * W + k).reshape(-1))
     np.testing.assert_array_equal(rec["v"], f)
+
+
+@pytest.mark.parametrize("which", [1, 2])
+def test_cosmology_with_wdm_bitwise_equal_to_oracle(which):
+    tk, tt = cases.load_transfer()
+    cfg = cases.base_config(WhichTransfer=which, transfer_k=tk, transfer_t=tt, WDM_On=1, WDM_PartMass_in_kev=1.5)
+    ref = orc.cosmology(cfg)
+    t = api.tables(np.zeros(27), np.zeros(6), cases.lattice_glass(4), 100.0, tk, tt)
+    ours = api.cosmology(api.params_from_dict(cfg), t)
+    for k, v in ref.items():
+        assert ours[k] == v, (k, ours[k], v)
+    assert ref["Norm"] != orc.cosmology(dict(cfg, WDM_On=0))["Norm"]

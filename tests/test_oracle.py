@@ -168,3 +168,24 @@ def test_ng_potential_vs_regrouped_numpy(table):
     assert err < 1e-12, err
     # the NG part is not negligible: Phi differs from phi_G
     assert np.linalg.norm(ours - phig) / np.linalg.norm(phig) > 1e-6
+
+
+def test_sigma2_with_wdm_suppression_vs_quad():
+    """WDM_On 1 multiplies P(k) by T_WDM(k)^2, T_WDM = (1 + (a k)^2.4)^(-5/1.2) with
+    a = 0.048 ((Om - Ob)/0.4)^0.15 (h/0.65)^1.3 (1/m_keV)^1.15 (power.c:267-275)."""
+    cfg = cases.base_config(WhichTransfer=0, WDM_On=1, WDM_PartMass_in_kev=0.5)
+    R = 8.0
+    ns = cfg["PrimordialIndex"]
+    a = 0.048 * ((cfg["Omega"] - cfg["OmegaBaryon"]) / 0.4) ** 0.15 * (cfg["HubbleParam"] / 0.65) ** 1.3 \
+        * (1 / cfg["WDM_PartMass_in_kev"]) ** 1.15
+
+    def integrand(k):
+        kr = R * k
+        w = 3 * (np.sin(kr) / kr ** 3 - np.cos(kr) / kr ** 2)
+        return 4 * np.pi * k * k * w * w * k ** ns * (1 + (a * k) ** 2.4) ** (-2 * 5 / 1.2)
+
+    ref = integrate.quad(integrand, 1e-12, 500.0 / R, limit=2000, epsrel=1e-11)[0]
+    ours = orc.lib().orc_sigma2_tophat(orc.make_params(cfg), 1.0, R)
+    assert ours == pytest.approx(ref, rel=1e-7)
+    plain = orc.lib().orc_sigma2_tophat(orc.make_params(dict(cfg, WDM_On=0)), 1.0, R)
+    assert ours < 0.9 * plain  # a 0.5 keV particle suppresses sigma_8 visibly
