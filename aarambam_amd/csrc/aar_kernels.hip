@@ -1033,22 +1033,22 @@ __device__ __forceinline__ double fmod_box(double x, double b) {
   return fmod(x, b);
 }
 
+// One particle: glass particle n of tile t (tile-major order of read_glass.c).
 template <typename T>
-__global__ void __launch_bounds__(256)
-    k_cic(const cx_t<T>* __restrict__ disp, const cx_t<T>* __restrict__ disp2, Geom g,
-          const float* __restrict__ glass, int nglass, double gbox, int tile, double box, double vp1,
-          double vp2, float* __restrict__ pos, float* __restrict__ vel, long long npart) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;  // glass particle
-  if (n >= nglass) return;
-  const int t = blockIdx.y;  // tile (tile-major order of read_glass.c)
+__device__ __forceinline__ void cic_particle(int n, int t, const cx_t<T>* __restrict__ disp,
+                                             const cx_t<T>* __restrict__ disp2, const Geom& g,
+                                             const float* __restrict__ glass, int nglass, double gbox, int tile,
+                                             double box, double vp1, double vp2, float* __restrict__ pos,
+                                             float* __restrict__ vel) {
   const long long p = (long long)t * nglass + n;
   const int N = g.N;
   const int ti = t / (tile * tile), tj = (t / tile) % tile, tk = t % tile;
   const double bt = box / tile;
+  const float3 gp = reinterpret_cast<const float3*>(glass)[n];  // one 12-byte load
   float P[3];
-  P[0] = (float)(glass[3 * n] / gbox * bt + ti * bt);
-  P[1] = (float)(glass[3 * n + 1] / gbox * bt + tj * bt);
-  P[2] = (float)(glass[3 * n + 2] / gbox * bt + tk * bt);
+  P[0] = (float)(gp.x / gbox * bt + ti * bt);
+  P[1] = (float)(gp.y / gbox * bt + tj * bt);
+  P[2] = (float)(gp.z / gbox * bt + tk * bt);
   double u = P[0] / box * N, v = P[1] / box * N, w = P[2] / box * N;
   int i = (int)u, j = (int)v, k = (int)w;
   if (i == N) i = N - 1;
@@ -1086,11 +1086,19 @@ __global__ void __launch_bounds__(256)
     V[a] = (float)(dis * vp1 - 3. / 7. * dis2 * vp2);
     P[a] = (float)fmod_box(fmod_box((double)P[a], box) + box, box);
   }
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    pos[3 * p + a] = P[a];
-    vel[3 * p + a] = V[a];
-  }
+  // one 12-byte store per array: a wavefront writes 768 contiguous bytes
+  reinterpret_cast<float3*>(pos)[p] = make_float3(P[0], P[1], P[2]);
+  reinterpret_cast<float3*>(vel)[p] = make_float3(V[0], V[1], V[2]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_cic(const cx_t<T>* __restrict__ disp, const cx_t<T>* __restrict__ disp2, Geom g,
+          const float* __restrict__ glass, int nglass, double gbox, int tile, double box, double vp1,
+          double vp2, float* __restrict__ pos, float* __restrict__ vel, long long npart) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;  // glass particle
+  if (n >= nglass) return;
+  cic_particle<T>(n, blockIdx.y, disp, disp2, g, glass, nglass, gbox, tile, box, vp1, vp2, pos, vel);
 }
 
 template <typename T>
