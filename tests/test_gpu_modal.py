@@ -64,6 +64,8 @@ def _shape(grid, kind):
         return ((k1 * k1 / (k2 * k3) + k2 * k2 / (k1 * k3) + k3 * k3 / (k1 * k2)) / 3)[None, :]
     if kind == "equilateral":
         return ((k1 + k2 - k3) * (k2 + k3 - k1) * (k3 + k1 - k2) / (k1 * k2 * k3))[None, :]
+    if kind == "two":  # the second shape's selection is driven by the first one's residual, as in the reference
+        return np.concatenate([_shape(grid, "local"), _shape(grid, "equilateral")])
     Q = pymodal.mode_bispectra(grid)
     n = int(np.nonzero((grid["p1"] == 5) & (grid["p2"] == 2) & (grid["p3"] == 1))[0][0])
     return Q[n][None, :]
@@ -74,6 +76,7 @@ def _shape(grid, kind):
     ("equilateral", dict(RR_tol=1e-8, max_modes=40, rtol=1e-10, Niter=1e3)),
     ("mode", dict(RR_tol=1e-12, cond_tol=1e14, max_modes=60)),
     ("local", dict(max_modes=-1)),
+    ("two", dict(RR_tol=1e-8, max_modes=30, rtol=1e-10, Niter=1e3, bad_modes="decompose")),
 ])
 def test_basis_expansion_matches_oracle(ctx, kind, kw):
     """Same modes selected as by the oracle's restatement of Basis.basis_expansion, coefficients
@@ -83,6 +86,10 @@ def test_basis_expansion_matches_oracle(ctx, kind, kw):
     from oracle import pymodal
     every = kw.get("max_modes") == -1
     g = modal.basis_grid(p_max=6 if every else 8, k_grid_size=30)
+    if kw.get("bad_modes") == "decompose":  # the mask of Aarambam/basis/decompose.py:125-129
+        p1, p2, p3 = g["p1"], g["p2"], g["p3"]
+        kw = dict(kw, bad_modes=(p1 < 4) & ~(((p1 == 1) & (p2 == 1) & (p3 == 1)) | ((p1 == 2) & (p2 == 1) & (p3 == 0))
+                                             | ((p1 == 3) & (p2 == 0) & (p3 == 0))))
     S = _shape(g, kind)
     print("expansion", kind, kw, len(g["p1"]), "modes", len(g["w"]), "points", flush=True)
     ra, rcov, rcorr, rmse, rmask = pymodal.basis_expansion(g, S, **kw)
