@@ -777,8 +777,10 @@ static void glass_tasks(const aar_params* p, const aar_tables* t, int ntask, std
   for (int ti = 0; ti < T; ++ti)
     for (int n = 0; n < t->nglass; ++n) {
       const float x = (float)(t->glass[3 * (size_t)n] / t->glass_box * (p->Box / T) + ti * (p->Box / T));
-      int slab = (int)(x / p->Box * N);
-      if (slab >= N) slab = N - 1;
+      // the reference truncates and clamps at N - 1; a position outside the box (or
+      // NaN) would index outside its slab table, here it goes to the nearest end
+      const double sd = (double)x / p->Box * N;
+      const int slab = !(sd >= 0) ? 0 : (sd >= N ? N - 1 : (int)sd);
       task_of[(size_t)ti * t->nglass + n] = (uint16_t)slab_task[slab];
     }
 }
