@@ -85,7 +85,10 @@ def survey_bytes(N, M, K, b):
     return S * (94 + 7 * M + 8 * K) + 24 * N ** 3
 
 
-def time_job(aar, ctx, cfg, alpha, x0, glass, gbox, prec, sched, steps, warmup, profile, graph=False):
+ERR_FIELDS = ["disp_x", "disp_y", "disp_z", "disp2_x", "disp2_y", "disp2_z", "phing_real"]
+
+
+def time_job(aar, ctx, cfg, alpha, x0, glass, gbox, prec, sched, steps, warmup, profile, graph=False, box=0):
     t = aar.tables(alpha, x0, glass, gbox)
     job = aar.Job(ctx, aar.params_from_dict(cfg), t, prec, sched)
     job.set_graph(graph)
@@ -104,6 +107,8 @@ def time_job(aar, ctx, cfg, alpha, x0, glass, gbox, prec, sched, steps, warmup, 
     wall = time.perf_counter() - t0
     stats = job.kernel_stats() if profile else {}
     info = dict(wall=wall, device=dev, stats=stats, M=None)
+    if box:  # a box^3 corner of the displacement and potential fields, after the timed region
+        info["box"] = {n: job.field_box(n, (0, 0, 0), (box, box, box)) for n in ERR_FIELDS}
     job.close()
     return info
 
@@ -303,8 +308,9 @@ def main():
     # x / y tile shapes for this mesh, picked by the autotuner before the timed region
     tsec, tvar = ctx.autotune(N, aar.AAR_FP32, nf=8)
     # timed region: eager launches, each bracketed by HIP events (per-kernel roofline)
+    eb = 0 if args.no_fp64 else min(128, N)
     r32 = time_job(aar, ctx, cfg, alpha, x0, glass, gbox, aar.AAR_FP32, aar.AAR_SCHED_BATCHED, args.steps,
-                   args.warmup, True)
+                   args.warmup, True, box=eb)
     rg = None
     if not args.no_graph:  # the same job replayed as one captured hipGraph
         rg = time_job(aar, ctx, cfg, alpha, x0, glass, gbox, aar.AAR_FP32, aar.AAR_SCHED_BATCHED, args.steps,
@@ -339,11 +345,17 @@ def main():
                            "value": round(N ** 3 * args.steps / rg["wall"] / 1e6, 3)}
     if not args.no_fp64:
         r64 = time_job(aar, ctx, cfg, alpha, x0, glass, gbox, aar.AAR_FP64, aar.AAR_SCHED_PLAIN,
-                       args.fp64_steps, 1, False)
+                       args.fp64_steps, 1, False, box=eb)
         v64 = N ** 3 * args.fp64_steps / r64["wall"] / 1e6
         out["fp64_plain"] = {"value": round(v64, 3), "unit": "Mcells/s",
                              "ms_per_step": round(r64["wall"] / args.fp64_steps * 1e3, 3),
                              "fp32_over_fp64": round(value / v64, 3)}
+        # accuracy of the timed FP32 path: relative L2 error against the FP64 pairwise path on a
+        # box^3 corner of every displacement field and of the non-Gaussian potential
+        err = {n: float(np.linalg.norm(r32["box"][n] - r64["box"][n]) / np.linalg.norm(r64["box"][n]))
+               for n in ERR_FIELDS}
+        out["fp32_rel_l2_vs_fp64"] = {"sub_box": eb, "max": float(f"{max(err.values()):.3g}"),
+                                      "fields": {n: float(f"{v:.3g}") for n, v in err.items()}}
     out["autotune"] = {"xy_tile_variant": {"x": tvar[0], "y": tvar[1]},
                        "us_per_launch_8_fields": {ax: [round(v * 1e6, 1) if v > 0 else None for v in tsec[i]]
                                                   for i, ax in enumerate(("x", "y"))},
