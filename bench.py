@@ -10,6 +10,7 @@ Nmax = 512, Eisenstein-Hu transfer, fNL = 100.  Headline: FP32 batched
 schedule; the FP64 pairwise schedule and the CPU oracle are reported beside it.
 """
 import argparse
+import itertools
 import json
 import os
 import sys
@@ -24,6 +25,7 @@ HBM_PEAK_GBS = 8000.0
 
 
 TEMPLATES = {
+    "scalari16": "ScalarI mass=1 decomposed on the device into 16 basis modes (ModeTol 1e-4, MaxModeCount 150)",
     "scalari": "ScalarI mass=1 15-mode table (Aarambam/defaults/AlphaTable.dat)",
     "le": "local+equilateral closed-form table, 12 modes",
     "local": "local closed-form table, 8 modes",
@@ -32,13 +34,45 @@ TEMPLATES = {
 }
 
 
-def workload(N=512, box=2000.0, template="scalari", seed=1234, fnl=100.0):
+_SCALARI16 = {}
+
+
+def scalari16_alpha(ctx, ns, box, nmax):
+    """AlphaTable of the ScalarI (mass 1) template for 16 basis modes, made the way
+    Aarambam/basis/decompose.py:110-166 makes it (Basis on a 150-point k grid from 2 pi / L to
+    2 pi Nmax / L, the shape symmetrised over the orderings, basis_expansion with rtol 1e-10,
+    1000 CG iterations, its mode mask, ModeTol 1e-4, MaxModeCount 150), with the expansion run on
+    the device (aar_mode_basis_expansion).  Cached per (n_s, k range)."""
+    from tests import modal, templates
+    key = (ns, round(2 * np.pi / box, 15), round(2 * np.pi / box * nmax, 12))
+    if key not in _SCALARI16:
+        g = modal.basis_grid(p_max=16, k_grid_size=150, k_min=2 * np.pi / box, k_max=2 * np.pi / box * nmax, n_s=ns)
+        S = templates.shape(g, lambda a, b, c: templates.scalar_i(a, b, c, ns, 1.0))
+        alpha, _, corr, _, _ = ctx.mode_basis_expansion(g, S, rtol=1e-10, Niter=1e3,
+                                                        bad_modes=templates.decompose_bad_modes(g), RR_tol=1e-4,
+                                                        max_modes=150)
+        table = np.zeros((16, 16, 16))
+        for n1, n2, n3, a in zip(g["p1"], g["p2"], g["p3"], alpha[0]):  # main.c:231-241, all six orders
+            for q in itertools.permutations((int(n1), int(n2), int(n3))):
+                table[q] = a
+        _SCALARI16[key] = (table, float(corr[0]), int((alpha[0] != 0).sum()))
+    return _SCALARI16[key]
+
+
+def x0_table(nm, ns, box, nmax):
     from tests import cases
-    nm = {"scalari": 15, "le": 12, "local": 8, "equil": 12, "dense12": 12}[template]
+    return cases.x0_closed_form(nm, ns, box, nmax)
+
+
+def workload(N=512, box=2000.0, template="scalari16", seed=1234, fnl=100.0, ctx=None):
+    from tests import cases
+    nm = {"scalari16": 16, "scalari": 15, "le": 12, "local": 8, "equil": 12, "dense12": 12}[template]
     cfg = cases.base_config(Nmesh=N, Nsample=N, Nmax=N, N_modes=nm, Box=box, GlassTileFac=max(1, N // 64),
                             Fnl=fnl, Seed=seed)
     x0 = cases.x0_closed_form(nm, cfg["PrimordialIndex"], box, N)
-    if template == "scalari":
+    if template == "scalari16":
+        alpha = scalari16_alpha(ctx, cfg["PrimordialIndex"], box, N)[0]
+    elif template == "scalari":
         alpha = load_alpha(os.path.join(cases.GOLDEN, "AlphaTable.dat"), 15)
     else:
         rows = {"local": cases.local_rows, "equil": cases.equilateral_rows, "le": cases.le_rows,
@@ -268,9 +302,16 @@ def profiled_traffic(kernel, N, template):
     return None
 
 
-def cpu_baseline(N=128):
+def cpu_baseline(N=128, template="scalari16", ctx=None, alpha=None):
+    """The oracle on one N^3 job of the timed workload's table (alpha, when given, is that table)."""
     from oracle import pyoracle as orc
-    cfg, alpha, x0, glass, gbox = workload(N, 2000.0 * N / 512)
+    cfg, a, x0, glass, gbox = workload(N, 2000.0 * N / 512, template if alpha is None else "le", ctx=ctx)
+    if alpha is not None:
+        nm = alpha.shape[0]
+        cfg["N_modes"] = nm
+        x0 = x0_table(nm, cfg["PrimordialIndex"], cfg["Box"], N)
+    else:
+        alpha = a
     t0 = time.perf_counter()
     orc.run(cfg, alpha, x0, glass, gbox, want_fields=False)
     dt = time.perf_counter() - t0
@@ -290,7 +331,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)
-    ap.add_argument("--template", default="scalari", choices=sorted(TEMPLATES))
+    ap.add_argument("--template", default="scalari16", choices=sorted(TEMPLATES))
     ap.add_argument("--box", type=float, default=2000.0)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -303,8 +344,8 @@ def main():
         sys.exit(2)
     import aarambam_amd as aar
     N = args.nmesh
-    cfg, alpha, x0, glass, gbox = workload(N, args.box, args.template)
     ctx = aar.Context(0)
+    cfg, alpha, x0, glass, gbox = workload(N, args.box, args.template, ctx=ctx)
     # x / y tile shapes for this mesh, picked by the autotuner before the timed region
     tsec, tvar = ctx.autotune(N, aar.AAR_FP32, nf=8)
     # timed region: eager launches, each bracketed by HIP events (per-kernel roofline)
@@ -322,13 +363,18 @@ def main():
     d = stats[dom]
     achieved = d["bytes"] / d["seconds"] / 1e9
     M, K = modal_sizes(alpha)
+    decomp_note = ""
+    if args.template == "scalari16":
+        _, corr16, nz16 = scalari16_alpha(ctx, cfg["PrimordialIndex"], args.box, N)
+        decomp_note = f": {nz16} nonzero coefficients, correlation {corr16:.6f}"
     out = {
         "metric": "Mcells/s for NG-2LPT IC at Nmesh=512, FP32 vs FP64; p50/p99 job latency",
         "value": round(value, 3), "unit": "Mcells/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Gaussian field, lattice glass)",
         "config": {"workload": f"NG-2LPT IC, Nmesh=Npart={N}^3, Lbox={args.box:g} Mpc/h, "
-                               f"{TEMPLATES[args.template]} (M={M}, K={K}), EH P(k), fNL=100, seed 1234",
+                               f"{TEMPLATES[args.template]}{decomp_note} (M={M}, K={K}), EH P(k), fNL=100, "
+                               f"seed 1234",
                    "precision": "fp32", "schedule": "batched"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -367,7 +413,7 @@ def main():
         out["decomposition"] = decomposition(aar, ctx)
     ctx.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_n)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.template, ctx, alpha)
     print(json.dumps(out))
 
 

@@ -64,6 +64,9 @@ def _shape(grid, kind):
         return ((k1 * k1 / (k2 * k3) + k2 * k2 / (k1 * k3) + k3 * k3 / (k1 * k2)) / 3)[None, :]
     if kind == "equilateral":
         return ((k1 + k2 - k3) * (k2 + k3 - k1) * (k3 + k1 - k2) / (k1 * k2 * k3))[None, :]
+    if kind == "scalari":  # the collider template of the benchmark's table (mass 1)
+        from tests import templates
+        return templates.shape(grid, lambda a, b, c: templates.scalar_i(a, b, c, 0.9660499, 1.0))
     if kind == "two":  # the second shape's selection is driven by the first one's residual, as in the reference
         return np.concatenate([_shape(grid, "local"), _shape(grid, "equilateral")])
     Q = pymodal.mode_bispectra(grid)
@@ -77,6 +80,7 @@ def _shape(grid, kind):
     ("mode", dict(RR_tol=1e-12, cond_tol=1e14, max_modes=60)),
     ("local", dict(max_modes=-1)),
     ("two", dict(RR_tol=1e-8, max_modes=30, rtol=1e-10, Niter=1e3, bad_modes="decompose")),
+    ("scalari", dict(RR_tol=1e-4, max_modes=60, rtol=1e-10, Niter=1e3, bad_modes="decompose")),
 ])
 def test_basis_expansion_matches_oracle(ctx, kind, kw):
     """Same modes selected as by the oracle's restatement of Basis.basis_expansion, coefficients

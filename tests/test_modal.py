@@ -114,3 +114,17 @@ def test_selection_loop_recovers_a_mode_outside_the_start(grid):
     assert alpha[0][b] == pytest.approx(1.0, abs=1e-4)
     assert np.abs(np.delete(alpha[0], b)).max() < 1e-4
     assert mse[0] < 1e-12 and abs(corr[0] - 1) < 1e-12
+
+
+def test_scalar_i_template_is_normalised_at_the_equilateral_point():
+    """The Scalar-I shape is normalised so that S(k, k, k) = k^(2(n_s - 1)) (each k enters as
+    k^((4 - n_s)/3) and every term is homogeneous of degree 0), and it is symmetric."""
+    from tests import templates
+    ns = 0.9649
+    k = np.geomspace(1e-3, 1.0, 7)
+    g = {"k": k, "i1": np.arange(7), "i2": np.arange(7), "i3": np.arange(7)}
+    S = templates.shape(g, lambda a, b, c: templates.scalar_i(a, b, c, ns, 1.0))[0]
+    np.testing.assert_allclose(S, k ** (2 * (ns - 1)), rtol=1e-12)
+    g2 = {"k": k, "i1": np.array([6, 5, 4]), "i2": np.array([4, 6, 5]), "i3": np.array([5, 4, 6])}
+    S2 = templates.shape(g2, lambda a, b, c: templates.scalar_i(a, b, c, ns, 1.0))[0]
+    np.testing.assert_allclose(S2, S2[0], rtol=1e-13)
