@@ -38,24 +38,22 @@ _SCALARI16 = {}
 
 
 def scalari16_alpha(ctx, ns, box, nmax):
-    """AlphaTable of the ScalarI (mass 1) template for 16 basis modes, made the way
-    Aarambam/basis/decompose.py:110-166 makes it (Basis on a 150-point k grid from 2 pi / L to
-    2 pi Nmax / L, the shape symmetrised over the orderings, basis_expansion with rtol 1e-10,
-    1000 CG iterations, its mode mask, ModeTol 1e-4, MaxModeCount 150), with the expansion run on
-    the device (aar_mode_basis_expansion).  Cached per (n_s, k range)."""
-    from tests import modal, templates
-    key = (ns, round(2 * np.pi / box, 15), round(2 * np.pi / box * nmax, 12))
+    """AlphaTable of the ScalarI (mass 1) template for 16 basis modes, made by
+    aarambam_amd.decompose the way Aarambam/basis/decompose.py:84-174 makes it (150-point k grid
+    from 2 pi / L to 2 pi Nmax / L, the shape symmetrised over the orderings, basis_expansion with
+    rtol 1e-10, 1000 CG iterations, its mode mask, ModeTol 1e-4, MaxModeCount 150), the expansion
+    on the device.  Cached per (n_s, box, Nmax)."""
+    from aarambam_amd import decompose
+    from tests import templates
+    key = (ns, box, nmax)
     if key not in _SCALARI16:
-        g = modal.basis_grid(p_max=16, k_grid_size=150, k_min=2 * np.pi / box, k_max=2 * np.pi / box * nmax, n_s=ns)
-        S = templates.shape(g, lambda a, b, c: templates.scalar_i(a, b, c, ns, 1.0))
-        alpha, _, corr, _, _ = ctx.mode_basis_expansion(g, S, rtol=1e-10, Niter=1e3,
-                                                        bad_modes=templates.decompose_bad_modes(g), RR_tol=1e-4,
-                                                        max_modes=150)
+        r = decompose.decompose(ctx, lambda a, b, c: templates.scalar_i(a, b, c, ns, 1.0), 16, ns, box, nmax,
+                                1e-4, 150)
         table = np.zeros((16, 16, 16))
-        for n1, n2, n3, a in zip(g["p1"], g["p2"], g["p3"], alpha[0]):  # main.c:231-241, all six orders
+        for n1, n2, n3, a in r["AlphaTable"]:  # main.c:231-241 fills all six orders
             for q in itertools.permutations((int(n1), int(n2), int(n3))):
                 table[q] = a
-        _SCALARI16[key] = (table, float(corr[0]), int((alpha[0] != 0).sum()))
+        _SCALARI16[key] = (table, r["correlation"], int((r["AlphaTable"][:, 3] != 0).sum()))
     return _SCALARI16[key]
 
 

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as orc
-from tests import modal
+from tests import cases, modal
 
 pytestmark = pytest.mark.gpu
 
@@ -118,3 +118,42 @@ def test_basis_expansion_rejects_bad_input(ctx):
         ctx.mode_basis_expansion(bad, S)
     with pytest.raises(aar.AarError):
         ctx.mode_basis_expansion(g, S, max_modes=-2)
+
+
+def test_decomposition_tables_drive_an_ic_job(ctx, tmp_path):
+    """Template -> aarambam_amd.decompose (expansion on the device) -> AlphaTable.dat / AiTable.dat in
+    the reference's format -> aar_load_alpha / aar_load_x0 -> an FP64 IC job that matches the oracle
+    run on the same tables to 1e-10."""
+    import ctypes as C
+    from aarambam_amd import decompose
+    from tests import templates
+    from tests.test_gpu_parity import _compare_job
+    ns, L, N = 0.9649, 1000.0, 32
+    r = decompose.decompose(ctx, lambda a, b, c: templates.scalar_i(a, b, c, ns, 1.0), 8, ns, L, N, 1e-4, 30,
+                            k_grid_size=24)
+    ref = pymodal_expansion_check(r, ns)
+    assert ref
+    decompose.write_tables(r, str(tmp_path))
+    alpha = np.empty(8 ** 3)
+    x0 = np.empty(16)
+    dp = C.POINTER(C.c_double)
+    assert aar.lib().aar_load_alpha(str(tmp_path / "AlphaTable.dat").encode(), 8, alpha.ctypes.data_as(dp)) == 0
+    assert aar.lib().aar_load_x0(str(tmp_path / "AiTable.dat").encode(), 8, x0.ctypes.data_as(dp)) == 0
+    rows = r["AlphaTable"][r["AlphaTable"][:, 3] != 0]
+    cfg = cases.base_config(Nmesh=N, Nsample=N, Nmax=N, N_modes=8, Box=L, PrimordialIndex=ns)
+    np.testing.assert_array_equal(alpha.reshape(8, 8, 8), orc.alpha_table(8, rows))
+    _compare_job(ctx, cfg, rows, x0.reshape(8, 2), aar.AAR_FP64, aar.AAR_SCHED_BATCHED, 1e-10)
+
+
+def pymodal_expansion_check(r, ns):
+    """The device expansion inside decompose() selects what the oracle's restatement selects."""
+    from aarambam_amd import decompose
+    from oracle import pymodal
+    from tests import templates
+    g = r["grid"]
+    S = decompose.shape_values(g, lambda a, b, c: templates.scalar_i(a, b, c, ns, 1.0))[None, :]
+    ra = pymodal.basis_expansion(g, S, rtol=1e-10, Niter=1e3, bad_modes=decompose.dropped_modes(g), RR_tol=1e-4,
+                                 max_modes=30)
+    np.testing.assert_array_equal(ra[4][0], r["selected"])
+    np.testing.assert_allclose(r["AlphaTable"][:, 3], ra[0][0], rtol=0, atol=1e-6 * np.abs(ra[0][0]).max())
+    return True

@@ -128,3 +128,21 @@ def test_scalar_i_template_is_normalised_at_the_equilateral_point():
     g2 = {"k": k, "i1": np.array([6, 5, 4]), "i2": np.array([4, 6, 5]), "i3": np.array([5, 4, 6])}
     S2 = templates.shape(g2, lambda a, b, c: templates.scalar_i(a, b, c, ns, 1.0))[0]
     np.testing.assert_allclose(S2, S2[0], rtol=1e-13)
+
+
+def test_product_basis_grid_matches_the_test_restatement():
+    """aarambam_amd.decompose.basis_grid (product) and tests/modal.basis_grid (test input) restate
+    cmbbest.Basis independently; the Legendre modes differ only in evaluation (power series of
+    scipy.special.legendre in the product, as the reference, recurrence in the tests)."""
+    from aarambam_amd import decompose
+    kmin, kmax, ns = 2 * np.pi / 1000, 2 * np.pi / 1000 * 256, 0.9660499
+    a = decompose.basis_grid(12, ns, kmin, kmax, 40)
+    b = modal.basis_grid(p_max=12, k_grid_size=40, k_min=kmin, k_max=kmax, n_s=ns)
+    for key in ["i1", "i2", "i3", "p1", "p2", "p3"]:
+        np.testing.assert_array_equal(a[key], b[key])
+    np.testing.assert_allclose(a["w"], b["w"], rtol=1e-13)
+    np.testing.assert_allclose(a["evals"], b["evals"], rtol=0, atol=1e-11 * np.abs(b["evals"]).max())
+    # AiTable: x0 of the reference's shipped table (defaults/AiTable.dat: N_modes 15, Nmax 256, L 512)
+    from tests import cases
+    ai = decompose.ai_table(15, ns, 2 * np.pi / 512, 2 * np.pi / 512 * 256)
+    np.testing.assert_allclose(ai[:, 1], cases.x0_file(15)[:, 0], rtol=1e-14)
