@@ -80,6 +80,28 @@ def test_dense12_fp64_batched_vs_oracle_64(ctx):
         assert rel_l2(dev[name], rd) <= 1e-10, name
 
 
+def wide_rows(nm=24, n_triples=50, seed=24):
+    """A table over 24 modes (more than the 16 mode fields the fused z pass holds in
+    registers), so the batched schedule takes its unfused C2R / products / R2C passes."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    triples = [(a, b, c) for a in range(nm) for b in range(a + 1) for c in range(b + 1)]
+    pick = rng.choice(len(triples), size=n_triples, replace=False)
+    return np.array([[*triples[i], rng.standard_normal()] for i in sorted(pick)])
+
+
+@pytest.mark.parametrize("prec,tol", [(aar.AAR_FP64, 1e-10), (aar.AAR_FP32, 1e-5)])
+def test_wide_table_beyond_the_fused_z_pass_vs_oracle_64(ctx, prec, tol):
+    rows = wide_rows()
+    cfg = c2_config(5, N=64, nm=24)
+    alpha, x0, glass, gbox = cases.job_inputs(cfg, rows)
+    assert len({int(r[1]) for r in rows} | {int(r[2]) for r in rows}) > 16
+    dev = run(ctx, cfg, rows, prec, aar.AAR_SCHED_BATCHED)
+    ref = orc.run(cfg, alpha, x0, glass, gbox)
+    assert rel_l2(dev["phi_spec"], ref["phi_spec"]) <= tol
+    for name, rd in zip(DISP, orc.lpt_fields(cfg, ref["phi_spec"])):
+        assert rel_l2(dev[name], rd) <= tol, name
+
+
 def test_dense12_256_fp32_vs_fp64(ctx):
     cfg = c2_config(8)
     r64 = run(ctx, cfg, cases.dense12_rows(), aar.AAR_FP64, aar.AAR_SCHED_PLAIN)
