@@ -149,12 +149,31 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
     a = 0;
     c0 = (t - nmain) * C;  // first y (x pass) or first x (y pass) of a Nyquist tile
   }
-  auto addr = [&](int pos, int col) -> long long {
-    if (main_tile) return (AXIS == 0 ? row_of(N, pos, a) : row_of(N, a, pos)) * H + c0 + col;
-    return g.nq + (AXIS == 0 ? row_of(N, pos, c0 + col) : row_of(N, c0 + col, pos));
+  // The offset of element (pos, col) is (pos / B) s1 + (pos % B) s2 + cterm(col), B = kXB, with
+  // tile-uniform strides s1, s2 and a column term that the thread computes once (each thread of
+  // the split-exchange tile keeps one column, col = threadIdx.x % C).  Offsets fit 32 bits: a
+  // field holds N^2 (N/2 + 1) < 2^31 complex values for N <= 1024.
+  static_assert(SPLIT, "the column term assumes one column per thread");
+  unsigned s1, s2;
+  auto cterm = [&](int col) -> unsigned {
+    if (main_tile) {
+      if (AXIS == 0) return (unsigned)(kXB * a * H + c0 + col);  // row_of(N, pos, a) * H + c0 + col
+      return (unsigned)(row_of(N, a, 0) * H + c0 + col);        // row_of(N, a, pos) = row_of(N, a, 0) + kXB pos
+    }
+    if (AXIS == 0) return (unsigned)(g.nq + (long long)kXB * (c0 + col));  // nq + row_of(N, pos, c0 + col)
+    return (unsigned)(g.nq + row_of(N, c0 + col, 0));                      // nq + row_of(N, c0 + col, pos)
   };
-  auto load = [&](int pos, int col) { return f[addr(pos, col)]; };
-  auto store = [&](int pos, int col, C2 v) { f[addr(pos, col)] = v; };
+  if (main_tile) {
+    s1 = AXIS == 0 ? (unsigned)(kXB * N * H) : (unsigned)(kXB * kXB * H);
+    s2 = AXIS == 0 ? (unsigned)H : (unsigned)(kXB * H);
+  } else {
+    s1 = AXIS == 0 ? (unsigned)(kXB * N) : (unsigned)(kXB * kXB);
+    s2 = AXIS == 0 ? 1u : (unsigned)kXB;
+  }
+  const unsigned ct = cterm(threadIdx.x % C);
+  auto addr = [&](int pos) -> unsigned { return (unsigned)(pos >> kXBLog) * s1 + (unsigned)(pos & (kXB - 1)) * s2 + ct; };
+  auto load = [&](int pos, int) { return f[addr(pos)]; };
+  auto store = [&](int pos, int, C2 v) { f[addr(pos)] = v; };
   if constexpr (SPLIT) {
     __syncthreads();  // twiddles
     fft_tile_split<T, L, C, NT, DIR>(tile_r, tw, load, store);
