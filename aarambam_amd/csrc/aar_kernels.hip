@@ -1087,8 +1087,19 @@ __device__ __forceinline__ void cic_particle(int n, int t, const cx_t<T>* __rest
   const W f3 = (1 - uu) * (vv) * (1 - ww), f4 = (1 - uu) * (vv) * (ww);
   const W f5 = (uu) * (1 - vv) * (1 - ww), f6 = (uu) * (1 - vv) * (ww);
   const W f7 = (uu) * (vv) * (1 - ww), f8 = (uu) * (vv) * (ww);
-  const long long c1 = row_of(N, i, j) * N, c2 = row_of(N, i, jj) * N;
-  const long long c3 = row_of(N, ii, j) * N, c4 = row_of(N, ii, jj) * N;
+  // the eight corner offsets, once for all six fields (a real field holds N^3 <= 2^30 values)
+  const unsigned c1 = (unsigned)(row_of(N, i, j) * N), c2 = (unsigned)(row_of(N, i, jj) * N);
+  const unsigned c3 = (unsigned)(row_of(N, ii, j) * N), c4 = (unsigned)(row_of(N, ii, jj) * N);
+  unsigned o[8] = {c1 + k, c1 + kk, c2 + k, c2 + kk, c3 + k, c3 + kk, c4 + k, c4 + kk};
+  // FP32: as byte offsets, which fit 32 bits (4 N^3 <= 2^32), so every load is the field's
+  // uniform base plus a 32-bit offset with no per-load address arithmetic
+  if constexpr (sizeof(T) == 4)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) o[q] <<= 2;
+  auto at = [&](const T* D, unsigned off) -> T {
+    if constexpr (sizeof(T) == 4) return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(D) + off);
+    else return D[off];
+  };
   // 1 / N^3 is a power of two: multiplying by it equals dividing by (float) N^3
   const float rn3f = 1.0f / (float)((long long)N * N * N);
   float V[3];
@@ -1096,10 +1107,10 @@ __device__ __forceinline__ void cic_particle(int n, int t, const cx_t<T>* __rest
   for (int a = 0; a < 3; ++a) {
     const T* D = reinterpret_cast<const T*>(disp + (long long)a * g.fs);
     const T* D2 = reinterpret_cast<const T*>(disp2 + (long long)a * g.fs);
-    double dis = (double)(D[c1 + k] * f1 + D[c1 + kk] * f2 + D[c2 + k] * f3 + D[c2 + kk] * f4 +
-                          D[c3 + k] * f5 + D[c3 + kk] * f6 + D[c4 + k] * f7 + D[c4 + kk] * f8);
-    double dis2 = (double)(D2[c1 + k] * f1 + D2[c1 + kk] * f2 + D2[c2 + k] * f3 + D2[c2 + kk] * f4 +
-                           D2[c3 + k] * f5 + D2[c3 + kk] * f6 + D2[c4 + k] * f7 + D2[c4 + kk] * f8);
+    double dis = (double)(at(D, o[0]) * f1 + at(D, o[1]) * f2 + at(D, o[2]) * f3 + at(D, o[3]) * f4 +
+                          at(D, o[4]) * f5 + at(D, o[5]) * f6 + at(D, o[6]) * f7 + at(D, o[7]) * f8);
+    double dis2 = (double)(at(D2, o[0]) * f1 + at(D2, o[1]) * f2 + at(D2, o[2]) * f3 + at(D2, o[3]) * f4 +
+                           at(D2, o[4]) * f5 + at(D2, o[5]) * f6 + at(D2, o[6]) * f7 + at(D2, o[7]) * f8);
     dis2 *= rn3f;
     P[a] = (float)((double)P[a] + (dis - 3. / 7. * dis2));
     V[a] = (float)(dis * vp1 - 3. / 7. * dis2 * vp2);
