@@ -62,6 +62,8 @@ struct aar_job {
   unsigned char* outc = nullptr;
   int *modes = nullptr, *kmodes = nullptr, *groups = nullptr;
   NgTerm* terms = nullptr;
+  int* fgroups = nullptr;  // the terms factored for the fused z pass
+  NgRec4* frecs = nullptr;
   NgWeight* weights = nullptr;
   double2* acc = nullptr;
   float* glass = nullptr;
@@ -105,8 +107,8 @@ struct aar_job {
   hipStream_t up = nullptr;
   std::vector<std::vector<char>> staged;
   // 2LPT source as a product sum of the six gradient fields, for the fused z pass
-  NgTerm* src_terms = nullptr;
   int* src_groups = nullptr;
+  NgRec4* src_recs = nullptr;
   bool fused_lpt = false;
   // slot map: phi_G, Phi, the work region; ZA and 2LPT displacement fields
   int s_phig = 0, s_phi = 1, s_reg = 2, s_disp = 8, s_disp2 = 2;
@@ -122,6 +124,11 @@ int ilog2i(int n) {
   return l;
 }
 
+struct NgFactored {
+  std::vector<int> groups;  // K + 1 offsets into recs, in records
+  std::vector<NgRec4> recs;
+};
+
 // Build the regrouped term list and the pairwise schedule from the alpha table
 // (main.c:564-1063): main triples i <= j <= K with alpha != 0, plus the two
 // Beta terms -alpha012 t012 g_2 [f0 f1] and +alpha012 t012 g_1 [f0 f2].
@@ -132,7 +139,49 @@ struct Schedule {
   std::vector<int> groups;                // K+1 offsets
   std::vector<PairWork> pairs;            // pairwise schedule
   std::vector<NgWeight> weights;
+  NgFactored fz;                          // terms factored for the fused z pass
 };
+
+// Greedy vertex cover of each group's pairs: repeatedly take the mode slot in the most
+// remaining pairs (the lowest slot on ties) and give it all of them.
+NgFactored ng_factor(const std::vector<NgTerm>& terms, const std::vector<int>& groups) {
+  std::vector<NgRec> recs;
+  NgFactored out;
+  out.groups.push_back(0);
+  for (size_t kk = 0; kk + 1 < groups.size(); ++kk) {
+    std::vector<NgTerm> left(terms.begin() + groups[kk], terms.begin() + groups[kk + 1]);
+    while (!left.empty()) {
+      std::map<int, int> deg;
+      for (const NgTerm& t : left) {
+        ++deg[t.i];
+        if (t.j != t.i) ++deg[t.j];
+      }
+      int v = left[0].i, best = -1;
+      for (auto& [m, d] : deg)
+        if (d > best) v = m, best = d;
+      const size_t h = recs.size();
+      recs.push_back({v, 0, 0.0});
+      std::vector<NgTerm> rest;
+      for (const NgTerm& t : left) {
+        if (t.i == v || t.j == v) {
+          const float cf = (float)t.c;
+          int bits;
+          memcpy(&bits, &cf, 4);
+          recs.push_back({t.i == v ? t.j : t.i, bits, t.c});
+        } else {
+          rest.push_back(t);
+        }
+      }
+      recs[h].aux = (int)(recs.size() - h - 1);
+      while (recs.size() % 4) recs.push_back({0, 0, 0.0});
+      left.swap(rest);
+    }
+    out.groups.push_back((int)recs.size());
+  }
+  out.recs.resize(recs.size() / 4);
+  for (size_t q = 0; q < recs.size(); ++q) out.recs[q / 4].q[q % 4] = recs[q];
+  return out;
+}
 
 Schedule build_schedule(const aar_params& p, const double* alpha) {
   const int Nm = p.N_modes;
@@ -188,6 +237,7 @@ Schedule build_schedule(const aar_params& p, const double* alpha) {
     s.weights.push_back({t.K, t.c});
     s.pairs.push_back(pw);
   }
+  s.fz = ng_factor(s.terms, s.groups);
   return s;
 }
 
@@ -264,7 +314,7 @@ template <typename T> struct Exec {
               (rc = timed(j, st, AAR_K_XY, 2 * j->M * S, [&] { return P::xy_pass(reg, j->M, g, tw(j), 1, +1, st); })))
             return rc;
           if ((rc = timed(j, st, AAR_K_PRODUCTS, (j->M + j->K) * S,
-                          [&] { return P::z_ng(reg, j->M, j->K, j->terms, j->groups, g, tw(j), st); })))
+                          [&] { return P::z_ng(reg, j->M, j->K, j->fgroups, j->frecs, g, tw(j), st); })))
             return rc;
           if ((rc = timed(j, st, AAR_K_XY, 2 * j->K * S, [&] { return P::xy_pass(reg, j->K, g, tw(j), 1, -1, st); })) ||
               (rc = timed(j, st, AAR_K_XY, 2 * j->K * S, [&] { return P::xy_pass(reg, j->K, g, tw(j), 0, -1, st); })))
@@ -296,7 +346,7 @@ template <typename T> struct Exec {
         if ((rc = timed(j, st, AAR_K_Z, 3 * (S + R), [&] { return P::z_pass(disp, 3, g, tw(j), +1, st); })))
           return rc;
         if ((rc = timed(j, st, AAR_K_PRODUCTS, 7 * S,
-                        [&] { return P::z_ng(reg, 6, 1, j->src_terms, j->src_groups, g, tw(j), st); })))
+                        [&] { return P::z_ng(reg, 6, 1, j->src_groups, j->src_recs, g, tw(j), st); })))
           return rc;
         if ((rc = timed(j, st, AAR_K_XY, 2 * S, [&] { return P::xy_pass(reg, 1, g, tw(j), 1, -1, st); })) ||
             (rc = timed(j, st, AAR_K_XY, 2 * S, [&] { return P::xy_pass(reg, 1, g, tw(j), 0, -1, st); })))
@@ -619,6 +669,12 @@ static int validate(const aar_params* p, const aar_tables* t) {
   return AAR_OK;
 }
 
+// src = f00 (f11 + f22) + f11 f22 - f01^2 - f02^2 - f12^2 over the gradient slots
+// 00, 01, 02, 11, 12, 22 (main.c:1286-1290), as one product group
+static NgFactored source_factored() {
+  return ng_factor({{0, 3, 1.0}, {0, 5, 1.0}, {3, 5, 1.0}, {1, 1, -1.0}, {2, 2, -1.0}, {4, 4, -1.0}}, {0, 6});
+}
+
 static int slots_needed(int M, int K, int sched) {
   if (sched == AAR_SCHED_PLAIN) return 2 + 9;
   return 2 + std::max(std::max(M, K), 9);
@@ -643,7 +699,9 @@ static long long job_footprint(const aar_params* p, const aar_tables* t, int pre
   add(4 * (plain ? 2 : (long long)s.groups.size()));
   add((long long)sizeof(NgTerm) * (plain ? 1 : (long long)s.terms.size()));
   add((long long)sizeof(NgWeight) * (long long)s.weights.size());
-  add((long long)sizeof(NgTerm) * 6), add(4 * 2);  // 2LPT source terms and groups
+  add(4 * (long long)s.fz.groups.size()), add((long long)sizeof(NgRec4) * (long long)s.fz.recs.size());
+  const NgFactored src = source_factored();  // 2LPT source
+  add(4 * (long long)src.groups.size()), add((long long)sizeof(NgRec4) * (long long)src.recs.size());
   if (p->WhichTransfer == 2) {
     std::vector<double> logk, logT;
     if (aar_host_transfer_tables(p, t, logk, logT)) return -1;
@@ -748,14 +806,11 @@ int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, i
   if ((rc = job_upload(j, &j->kv, kv)) || (rc = job_upload(j, &j->sinc, sinc)) || (rc = job_upload(j, &j->x0, x0)) ||
       (rc = job_upload(j, &j->outc, outc)) || (rc = job_upload(j, &j->modes, modes_dev)) ||
       (rc = job_upload(j, &j->kmodes, s.kmodes)) || (rc = job_upload(j, &j->groups, groups)) ||
-      (rc = job_upload(j, &j->terms, terms)) || (rc = job_upload(j, &j->weights, s.weights)))
+      (rc = job_upload(j, &j->terms, terms)) || (rc = job_upload(j, &j->weights, s.weights)) ||
+      (rc = job_upload(j, &j->fgroups, s.fz.groups)) || (rc = job_upload(j, &j->frecs, s.fz.recs)))
     return fail(rc);
-  // src = f00 (f11 + f22) + f11 f22 - f01^2 - f02^2 - f12^2 over the gradient slots
-  // 00, 01, 02, 11, 12, 22 (main.c:1286-1290)
-  const std::vector<NgTerm> src_terms = {{0, 3, 1.0}, {0, 5, 1.0}, {3, 5, 1.0},
-                                         {1, 1, -1.0}, {2, 2, -1.0}, {4, 4, -1.0}};
-  const std::vector<int> src_groups = {0, 6};
-  if ((rc = job_upload(j, &j->src_terms, src_terms)) || (rc = job_upload(j, &j->src_groups, src_groups)))
+  const NgFactored src = source_factored();
+  if ((rc = job_upload(j, &j->src_groups, src.groups)) || (rc = job_upload(j, &j->src_recs, src.recs)))
     return fail(rc);
   j->fused_lpt = schedule == AAR_SCHED_BATCHED && N >= 64;
   j->s_disp = j->s_reg + 6;
@@ -1024,7 +1079,8 @@ extern "C" int aar_job_destroy(aar_job* j) {
   if (j->ran) hipEventSynchronize(j->ev1);
   if (!j->ws) {
     void* bufs[] = {j->pool, j->seeds, j->kv, j->sinc, j->x0, j->logk, j->logT, j->outc, j->modes,
-                    j->kmodes, j->groups, j->terms, j->weights, j->src_terms, j->src_groups, j->acc, j->glass,
+                    j->kmodes, j->groups, j->terms, j->weights, j->fgroups, j->frecs,
+                    j->src_groups, j->src_recs, j->acc, j->glass,
                     j->pos, j->vel};
     for (void* b : bufs)
       if (b) hipFree(b);

@@ -34,6 +34,21 @@ struct NgTerm {
   double c;
 };
 
+// The same sum factored for the fused z pass: each output group is a list of
+// factors h[K] += f[v] * (sum_t c_t f[w_t]), one factor per mode slot of a
+// vertex cover of the group's (i, j) pairs, so a pair costs one multiply-add
+// and one register read instead of two of each.  A factor is a run of
+// records, padded to a multiple of four (one 64-byte scalar load each): a
+// header {v, n} and n parts {w_t, FP32 bits of c_t, c_t}.
+struct NgRec {
+  int slot;  // v (header) or w_t (part)
+  int aux;   // n (header) or the FP32 coefficient's bits (part)
+  double c;  // part: the coefficient
+};
+struct alignas(64) NgRec4 {
+  NgRec q[4];
+};
+
 // Weight list entry: W(k) += c * g_n(k).
 struct NgWeight {
   int n;     // mode index of the weight function
@@ -51,7 +66,8 @@ template <typename T> struct Pipeline {
 
   // fused z pass of the NG sum: M mode fields (x, y already inverse transformed)
   // -> z C2R -> products -> z R2C -> K fields over the first K slots
-  static int z_ng(cx_t<T>* f, int M, int K, const NgTerm* terms, const int* group_start, const Geom& g,
+  // groups: K + 1 offsets into recs, in records (multiples of four)
+  static int z_ng(cx_t<T>* f, int M, int K, const int* groups, const NgRec4* recs, const Geom& g,
                   const cx_t<T>* tw, hipStream_t st);
   static int gauss(cx_t<T>* phi, const Geom& g, const KSpace& ks, const uint32_t* seeds, double anorm,
                    double ns, double fac, double beta, double phase_shift, int fixed_amp, hipStream_t st);

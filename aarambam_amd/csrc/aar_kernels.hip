@@ -305,7 +305,7 @@ template <typename T, int H, int MMAX> struct ZNGCfg {
 
 template <typename T, int H, int MMAX>
 __global__ void __launch_bounds__((ZNGCfg<T, H, MMAX>::NT))
-    k_z_ng(cx_t<T>* base, int M, int K, const NgTerm* __restrict__ terms, const int* __restrict__ gs, Geom g,
+    k_z_ng(cx_t<T>* base, int M, int K, const int* __restrict__ gs, const NgRec4* __restrict__ recs, Geom g,
            const cx_t<T>* __restrict__ twg) {
   using C2 = cx_t<T>;
   using Cfg = ZNGCfg<T, H, MMAX>;
@@ -376,34 +376,63 @@ __global__ void __launch_bounds__((ZNGCfg<T, H, MMAX>::NT))
     C2 hv[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) hv[e] = cmake<C2>(0, 0);
-    for (int t = gs[kk]; t < gs[kk + 1]; ++t) {
-      const NgTerm tm = terms[t];
-      const T cf = (T)tm.c;
-      const int ti = tm.i, tj = tm.j;
-      // all reads with index ti, then all with tj: runs of one register index
-      C2 a[E], b[E];
-      auto get = [&](int m, C2* v) {
-        v[0] = cmake<C2>(fx0[m], fy0[m]);
-        v[1] = cmake<C2>(fx1[m], fy1[m]);
-        if constexpr (E >= 4) {
-          v[2] = cmake<C2>(fx2[m], fy2[m]);
-          v[3] = cmake<C2>(fx3[m], fy3[m]);
+    // h = sum over factors of f[v] * (sum over parts of c f[w]); every read of a mode slot is
+    // a run of indexed register reads with one uniform index
+    auto get = [&](int m, C2* v) {
+      v[0] = cmake<C2>(fx0[m], fy0[m]);
+      v[1] = cmake<C2>(fx1[m], fy1[m]);
+      if constexpr (E >= 4) {
+        v[2] = cmake<C2>(fx2[m], fy2[m]);
+        v[3] = cmake<C2>(fx3[m], fy3[m]);
+      }
+      if constexpr (E == 8) {
+        v[4] = cmake<C2>(fx4[m], fy4[m]);
+        v[5] = cmake<C2>(fx5[m], fy5[m]);
+        v[6] = cmake<C2>(fx6[m], fy6[m]);
+        v[7] = cmake<C2>(fx7[m], fy7[m]);
+      }
+    };
+    // one 64-byte scalar load per block of four records, issued before the header is examined
+    typedef int blk_t __attribute__((ext_vector_type(16)));
+    auto coef = [](const blk_t& k, int q) -> T {
+      if constexpr (sizeof(T) == 4) {
+        return __int_as_float(k[4 * q + 1]);
+      } else {
+        return __hiloint2double(k[4 * q + 3], k[4 * q + 2]);
+      }
+    };
+    for (int r = gs[kk]; r < gs[kk + 1];) {
+      const blk_t* R = reinterpret_cast<const blk_t*>(recs + (r >> 2));
+      blk_t blk = R[0];
+      const int v = blk[0], n = blk[1];
+      C2 u[E], b[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) u[e] = cmake<C2>(0, 0);
+      for (int b0 = 0; b0 <= n; b0 += 4) {  // records b0 .. b0 + 3 of the factor; record 0 is the header
+        if (b0) blk = R[b0 >> 2];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int t = b0 + q;
+          if (t >= 1 && t <= n) {
+            get(blk[4 * q], b);
+            __builtin_amdgcn_sched_barrier(0);
+            const T cf = coef(blk, q);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+              u[e].x = fma(cf, b[e].x, u[e].x);
+              u[e].y = fma(cf, b[e].y, u[e].y);
+            }
+          }
         }
-        if constexpr (E == 8) {
-          v[4] = cmake<C2>(fx4[m], fy4[m]);
-          v[5] = cmake<C2>(fx5[m], fy5[m]);
-          v[6] = cmake<C2>(fx6[m], fy6[m]);
-          v[7] = cmake<C2>(fx7[m], fy7[m]);
-        }
-      };
-      get(ti, a);
-      get(tj, b);
-      __builtin_amdgcn_sched_barrier(0);  // keep each run of indexed reads together
+      }
+      get(v, b);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        hv[e].x = fma(cf, a[e].x * b[e].x, hv[e].x);
-        hv[e].y = fma(cf, a[e].y * b[e].y, hv[e].y);
+        hv[e].x = fma(b[e].x, u[e].x, hv[e].x);
+        hv[e].y = fma(b[e].y, u[e].y, hv[e].y);
       }
+      r += (n + 4) & ~3;
     }
     tile_sync<WL>();  // the tile is free
 #pragma unroll
@@ -421,34 +450,34 @@ __global__ void __launch_bounds__((ZNGCfg<T, H, MMAX>::NT))
 }
 
 template <typename T, int H, int MMAX>
-static int launch_z_ng_m(cx_t<T>* base, int M, int K, const NgTerm* terms, const int* gs, const Geom& g,
+static int launch_z_ng_m(cx_t<T>* base, int M, int K, const int* gs, const NgRec4* recs, const Geom& g,
                          const cx_t<T>* tw, hipStream_t st) {
   constexpr int C = ZNGCfg<T, H, MMAX>::C, NT = ZNGCfg<T, H, MMAX>::NT;
   dim3 grid((unsigned)((long long)g.N * g.N / C));
-  hipLaunchKernelGGL((k_z_ng<T, H, MMAX>), grid, dim3(NT), 0, st, base, M, K, terms, gs, g, tw);
+  hipLaunchKernelGGL((k_z_ng<T, H, MMAX>), grid, dim3(NT), 0, st, base, M, K, gs, recs, g, tw);
   return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
 }
 
 template <typename T, int H>
-static int launch_z_ng(cx_t<T>* base, int M, int K, const NgTerm* terms, const int* gs, const Geom& g,
+static int launch_z_ng(cx_t<T>* base, int M, int K, const int* gs, const NgRec4* recs, const Geom& g,
                        const cx_t<T>* tw, hipStream_t st) {
-  if (M <= 4) return launch_z_ng_m<T, H, 4>(base, M, K, terms, gs, g, tw, st);
-  if (M <= 6) return launch_z_ng_m<T, H, 6>(base, M, K, terms, gs, g, tw, st);
-  if (M <= 8) return launch_z_ng_m<T, H, 8>(base, M, K, terms, gs, g, tw, st);
-  if (M <= 16) return launch_z_ng_m<T, H, 16>(base, M, K, terms, gs, g, tw, st);
+  if (M <= 4) return launch_z_ng_m<T, H, 4>(base, M, K, gs, recs, g, tw, st);
+  if (M <= 6) return launch_z_ng_m<T, H, 6>(base, M, K, gs, recs, g, tw, st);
+  if (M <= 8) return launch_z_ng_m<T, H, 8>(base, M, K, gs, recs, g, tw, st);
+  if (M <= 16) return launch_z_ng_m<T, H, 16>(base, M, K, gs, recs, g, tw, st);
   return AAR_ERR_UNSUPPORTED;
 }
 
 template <typename T>
-int Pipeline<T>::z_ng(cx_t<T>* f, int M, int K, const NgTerm* terms, const int* gs, const Geom& g,
+int Pipeline<T>::z_ng(cx_t<T>* f, int M, int K, const int* gs, const NgRec4* recs, const Geom& g,
                       const cx_t<T>* tw, hipStream_t st) {
   if (M > 16) return AAR_ERR_UNSUPPORTED;
   switch (g.H) {
-    case 32: return launch_z_ng<T, 32>(f, M, K, terms, gs, g, tw, st);
-    case 64: return launch_z_ng<T, 64>(f, M, K, terms, gs, g, tw, st);
-    case 128: return launch_z_ng<T, 128>(f, M, K, terms, gs, g, tw, st);
-    case 256: return launch_z_ng<T, 256>(f, M, K, terms, gs, g, tw, st);
-    case 512: return launch_z_ng<T, 512>(f, M, K, terms, gs, g, tw, st);
+    case 32: return launch_z_ng<T, 32>(f, M, K, gs, recs, g, tw, st);
+    case 64: return launch_z_ng<T, 64>(f, M, K, gs, recs, g, tw, st);
+    case 128: return launch_z_ng<T, 128>(f, M, K, gs, recs, g, tw, st);
+    case 256: return launch_z_ng<T, 256>(f, M, K, gs, recs, g, tw, st);
+    case 512: return launch_z_ng<T, 512>(f, M, K, gs, recs, g, tw, st);
     default: return AAR_ERR_UNSUPPORTED;
   }
 }
