@@ -1173,9 +1173,8 @@ int Pipeline<T>::cic(const cx_t<T>* disp3, const cx_t<T>* disp2_3, const Geom& g
 
 // Plain streaming copy of n 16-byte words: the reference kernel for the
 // measured HBM copy peak and for scaling the FETCH_SIZE / WRITE_SIZE counters.
-// Each block moves 4 x 256 words; every thread issues its 4 loads before its stores.
-// One 16-byte element per lane: on MI355X this shape copies faster than
-// several elements per lane (6.2 against 5.5 TB/s over 4.3 GB).
+// One word per lane: on MI355X this shape copies faster than several words
+// per lane (6.2 against 5.5 TB/s over 4.3 GB).
 __global__ void __launch_bounds__(256) k_copy16(const float4* __restrict__ src, float4* __restrict__ dst,
                                                 long long n) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
