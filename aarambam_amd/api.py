@@ -277,11 +277,11 @@ class Context:
         return s.value
 
     def autotune(self, n, precision=AAR_FP32, nf=8):
-        """Time the x / y tile shapes at n^3 and keep the fastest; returns (seconds[2][3], chosen[2])."""
-        sec = np.zeros(6, np.float64)
+        """Time the x / y tile shapes at n^3 and keep the fastest; returns (seconds[2][4], chosen[2])."""
+        sec = np.zeros(8, np.float64)
         ch = (C.c_int * 2)()
         _check(lib().aar_autotune(self.h, n, precision, nf, _dptr(sec), ch), "aar_autotune")
-        return sec.reshape(2, 3), [ch[0], ch[1]]
+        return sec.reshape(2, 4), [ch[0], ch[1]]
 
     def _modal(self, grid):
         c = lambda a, t: np.ascontiguousarray(a, dtype=t)

@@ -325,9 +325,11 @@ __device__ __forceinline__ void fft_stage_split(T* tile, const cx_t<T>* tw, cx_t
   }
 }
 
-template <typename T, int L, int C, int NT, int DIR, class LoadF, class StoreF>
-__device__ __forceinline__ void fft_tile_split(T* tile, const cx_t<T>* tw, LoadF load, StoreF store) {
-  using C2 = cx_t<T>;
+// The first-stage inputs of fft_tile_split, loaded into registers through the
+// caller's functor; split_transform finishes the transform from them.  Apart,
+// they let a caller hold the next tile's inputs while it transforms this one.
+template <typename T, int L, int C, int NT, class LoadF>
+__device__ __forceinline__ void split_load(cx_t<T>* v, LoadF load) {
   constexpr int E = C * L / NT;
   constexpr int RMAX = E < 16 ? E : 16;
   constexpr int R = plan_radix(L, RMAX, 0);
@@ -336,13 +338,23 @@ __device__ __forceinline__ void fft_tile_split(T* tile, const cx_t<T>* tw, LoadF
   constexpr int ROWS = NT / C;
   const int col = threadIdx.x % C;
   const int i0 = threadIdx.x / C;
-  C2 v[E];
 #pragma unroll
   for (int q = 0; q < NB; ++q)
 #pragma unroll
     for (int t = 0; t < R; ++t) v[q * R + t] = call_load(load, i0 + q * ROWS + t * STR, col, q * R + t);
+}
+
+template <typename T, int L, int C, int NT, int DIR, class StoreF>
+__device__ __forceinline__ void split_transform(T* tile, const cx_t<T>* tw, cx_t<T>* v, StoreF store) {
   __syncthreads();  // the tile may still be read by the previous transform
   fft_stage_split<T, L, C, NT, DIR, 0, StoreF>(tile, tw, v, store);
+}
+
+template <typename T, int L, int C, int NT, int DIR, class LoadF, class StoreF>
+__device__ __forceinline__ void fft_tile_split(T* tile, const cx_t<T>* tw, LoadF load, StoreF store) {
+  cx_t<T> v[C * L / NT];
+  split_load<T, L, C, NT>(v, load);
+  split_transform<T, L, C, NT, DIR>(tile, tw, v, store);
 }
 
 // Element slots of the first stage of fft_tile_split: the (pos, col) that

@@ -135,7 +135,7 @@ int stat_bin_fields(const cx_t<TS>* a, const cx_t<TS>* b, double scale, int kind
 int stat_sums(const double2* fields, const Geom& g, const int* ofs, const StatTerm* terms, int nout, int nchunks,
               double* partial, hipStream_t st);
 // x / y pass tile-shape variants (0 .. kXYVariantCount-1) per precision, mesh size and axis
-constexpr int kXYVariantCount = 3;
+constexpr int kXYVariantCount = 4;
 int aar_xy_variant(int fp64, int n, int axis);
 void aar_set_xy_variant(int fp64, int n, int axis, int v);
 // n uniforms of one ranlxd1 stream on the device (test hook)

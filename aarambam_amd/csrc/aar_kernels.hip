@@ -41,12 +41,18 @@ __host__ __device__ constexpr int tile_threads(int CL) {
 //   V = 1: 64-byte segments (half the LDS and threads per tile, so more
 //          tiles per CU at L = 1024; the XCD-aware block order keeps both
 //          halves of each 128-byte line on one XCD's L2);
-//   V = 2: 256-byte segments (fewer, larger tiles; L <= 512).
+//   V = 2: 256-byte segments (fewer, larger tiles; L <= 512);
+//   V = 3: 128-byte segments, four tiles in turn per workgroup, the next
+//          tile's loads in flight during this tile's transform (FP32,
+//          L >= 512: at L = 1024 one 16-wave tile fills a CU's registers,
+//          so without it the loads and the transform never overlap).
 template <typename T, int L, int V = 0> struct XYCfg {
+  static constexpr int TPW = V == 3 ? 4 : 1;  // tiles per workgroup
   static constexpr int CW = V == 1 ? Shape<T>::CMAX / 2 : (V == 2 ? Shape<T>::CMAX * 2 : Shape<T>::CMAX);
   static constexpr int C = CW < L / 2 ? CW : L / 2;
   static constexpr bool VALID =
-      C * L / 16 <= 1024 && (long)L * (C + 1) * sizeof(T) + (long)L * sizeof(cx_t<T>) <= 152 * 1024;
+      C * L / 16 <= 1024 && (long)L * (C + 1) * sizeof(T) + (long)L * sizeof(cx_t<T>) <= 152 * 1024 &&
+      (V != 3 || (sizeof(T) == 4 && L >= 512));
   static constexpr int NT = tile_threads(C * L);
   static constexpr bool SPLIT = true;  // real-valued LDS exchange tile
   // minimum wavefronts per SIMD the register allocation must allow: 64-byte
@@ -132,53 +138,66 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
   constexpr int C = XYCfg<T, L, V>::C, NT = XYCfg<T, L, V>::NT;
   constexpr bool SPLIT = XYCfg<T, L, V>::SPLIT;
   __shared__ T tile_r[SPLIT ? L * (C + 1) : 1];
-  __shared__ C2 tile_c[SPLIT ? 1 : TileIdx<T, L, C, false>::SIZE];
   __shared__ C2 tw[L];
   load_twiddles(tw, twg + L, L, NT);
   C2* f = base + (long long)blockIdx.y * g.fs;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  constexpr int TPW = XYCfg<T, L, V>::TPW;
   const int N = g.N, H = g.H;
-  // element (pos, col) of this tile -> complex offset in the field
-  int a, c0;
-  bool main_tile = t < nmain;
-  if (main_tile) {
-    const int per = H / C;
-    a = t / per;         // fixed y (x pass) or fixed x (y pass)
-    c0 = (t % per) * C;  // first kz of the tile
-  } else {
-    a = 0;
-    c0 = (t - nmain) * C;  // first y (x pass) or first x (y pass) of a Nyquist tile
-  }
-  // The offset of element (pos, col) is (pos / B) s1 + (pos % B) s2 + cterm(col), B = kXB, with
-  // tile-uniform strides s1, s2 and a column term that the thread computes once (each thread of
-  // the split-exchange tile keeps one column, col = threadIdx.x % C).  Offsets fit 32 bits: a
-  // field holds N^2 (N/2 + 1) < 2^31 complex values for N <= 1024.
+  // The offset of element (pos, col) of tile t is (pos / B) s1 + (pos % B) s2 + ct, B = kXB,
+  // with tile-uniform strides s1, s2 and a column term ct that the thread computes once per
+  // tile (each thread of the split-exchange tile keeps one column, col = threadIdx.x % C).
+  // Offsets fit 32 bits: a field holds N^2 (N/2 + 1) < 2^31 complex values for N <= 1024.
   static_assert(SPLIT, "the column term assumes one column per thread");
-  unsigned s1, s2;
-  auto cterm = [&](int col) -> unsigned {
-    if (main_tile) {
-      if (AXIS == 0) return (unsigned)(kXB * a * H + c0 + col);  // row_of(N, pos, a) * H + c0 + col
-      return (unsigned)(row_of(N, a, 0) * H + c0 + col);        // row_of(N, a, pos) = row_of(N, a, 0) + kXB pos
-    }
-    if (AXIS == 0) return (unsigned)(g.nq + (long long)kXB * (c0 + col));  // nq + row_of(N, pos, c0 + col)
-    return (unsigned)(g.nq + row_of(N, c0 + col, 0));                      // nq + row_of(N, c0 + col, pos)
+  struct Addr {
+    unsigned s1, s2, ct;
   };
-  if (main_tile) {
-    s1 = AXIS == 0 ? (unsigned)(kXB * N * H) : (unsigned)(kXB * kXB * H);
-    s2 = AXIS == 0 ? (unsigned)H : (unsigned)(kXB * H);
-  } else {
-    s1 = AXIS == 0 ? (unsigned)(kXB * N) : (unsigned)(kXB * kXB);
-    s2 = AXIS == 0 ? 1u : (unsigned)kXB;
-  }
-  const unsigned ct = cterm(threadIdx.x % C);
-  auto addr = [&](int pos) -> unsigned { return (unsigned)(pos >> kXBLog) * s1 + (unsigned)(pos & (kXB - 1)) * s2 + ct; };
-  auto load = [&](int pos, int) { return f[addr(pos)]; };
-  auto store = [&](int pos, int, C2 v) { f[addr(pos)] = v; };
-  if constexpr (SPLIT) {
+  auto tile_addr = [&](int t) {
+    Addr A;
+    const int col = threadIdx.x % C;
+    if (t < nmain) {
+      const int per = H / C;
+      const int a = t / per;         // fixed y (x pass) or fixed x (y pass)
+      const int c0 = (t % per) * C;  // first kz of the tile
+      A.s1 = AXIS == 0 ? (unsigned)(kXB * N * H) : (unsigned)(kXB * kXB * H);
+      A.s2 = AXIS == 0 ? (unsigned)H : (unsigned)(kXB * H);
+      // row_of(N, pos, a) * H + c0 + col, or row_of(N, a, pos) = row_of(N, a, 0) + kXB pos
+      A.ct = AXIS == 0 ? (unsigned)(kXB * a * H + c0 + col) : (unsigned)(row_of(N, a, 0) * H + c0 + col);
+    } else {
+      const int c0 = (t - nmain) * C;  // first y (x pass) or first x (y pass) of a Nyquist tile
+      A.s1 = AXIS == 0 ? (unsigned)(kXB * N) : (unsigned)(kXB * kXB);
+      A.s2 = AXIS == 0 ? 1u : (unsigned)kXB;
+      // nq + row_of(N, pos, c0 + col), or nq + row_of(N, c0 + col, pos)
+      A.ct = AXIS == 0 ? (unsigned)(g.nq + (long long)kXB * (c0 + col)) : (unsigned)(g.nq + row_of(N, c0 + col, 0));
+    }
+    return A;
+  };
+  auto off = [](const Addr& A, int pos) -> unsigned {
+    return (unsigned)(pos >> kXBLog) * A.s1 + (unsigned)(pos & (kXB - 1)) * A.s2 + A.ct;
+  };
+  if constexpr (TPW == 1) {
+    const Addr A = tile_addr(xcd_tile(blockIdx.x, gridDim.x));
     __syncthreads();  // twiddles
-    fft_tile_split<T, L, C, NT, DIR>(tile_r, tw, load, store);
+    fft_tile_split<T, L, C, NT, DIR>(
+        tile_r, tw, [&](int pos, int) { return f[off(A, pos)]; }, [&](int pos, int, C2 v) { f[off(A, pos)] = v; });
   } else {
-    fft_tile<T, L, C, NT, DIR, 1, false>(tile_c, tw, load, store);
+    // tiles t0 .. t0 + TPW - 1 (the last workgroup may hold fewer); the condition of every
+    // branch and loop below is uniform over the workgroup, so every barrier is reached
+    constexpr int E = C * L / NT;
+    const int ntiles = nmain + N / C;
+    const int t0 = xcd_tile(blockIdx.x, gridDim.x) * TPW;
+    C2 v[E], nv[E];
+    Addr A = tile_addr(t0);
+    split_load<T, L, C, NT>(nv, [&](int pos, int) { return f[off(A, pos)]; });
+    for (int i = 0; i < TPW && t0 + i < ntiles; ++i) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = nv[e];
+      const Addr Ac = A;
+      if (i + 1 < TPW && t0 + i + 1 < ntiles) {
+        A = tile_addr(t0 + i + 1);
+        split_load<T, L, C, NT>(nv, [&](int pos, int) { return f[off(A, pos)]; });
+      }
+      split_transform<T, L, C, NT, DIR>(tile_r, tw, v, [&](int pos, int, C2 x) { f[off(Ac, pos)] = x; });
+    }
   }
 }
 
@@ -191,7 +210,7 @@ static int launch_xy_v(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hip
     constexpr int C = Cfg::C, NT = Cfg::NT;
     const int nmain = g.N * (g.H / C);
     const int nnyq = g.N / C;
-    dim3 grid(nmain + nnyq, nf);
+    dim3 grid((nmain + nnyq + Cfg::TPW - 1) / Cfg::TPW, nf);
     hipLaunchKernelGGL((k_xy_pass<T, L, AXIS, DIR, V>), grid, dim3(NT), 0, st, f, g, tw, nmain);
     return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
   }
@@ -204,6 +223,7 @@ static int launch_xy(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipSt
     switch (v) {
       case 1: return launch_xy_v<T, L, AXIS, DIR, 1>(f, nf, g, tw, st);
       case 2: return launch_xy_v<T, L, AXIS, DIR, 2>(f, nf, g, tw, st);
+      case 3: return launch_xy_v<T, L, AXIS, DIR, 3>(f, nf, g, tw, st);
       default: break;
     }
   }

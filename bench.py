@@ -403,7 +403,7 @@ def main():
     out["autotune"] = {"xy_tile_variant": {"x": tvar[0], "y": tvar[1]},
                        "us_per_launch_8_fields": {ax: [round(v * 1e6, 1) if v > 0 else None for v in tsec[i]]
                                                   for i, ax in enumerate(("x", "y"))},
-                       "variants": "0: 128-B tile columns, 1: 64-B, 2: 256-B (same radix plan)"}
+                       "variants": "0: 128-B tile columns, 1: 64-B, 2: 256-B, 3: 128-B, four tiles per workgroup with the next tile's loads in flight (same radix plan)"}
     out["copy_peak_gbs"] = round(copy_peak(ctx, min(N, 512)), 1)
     if not args.no_serve:
         out["serving"] = serving(aar, ctx, args.serve_jobs)

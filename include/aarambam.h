@@ -306,12 +306,12 @@ int aar_server_destroy(aar_server *s);
  * of an n^3 grid and keeps the fastest per axis for later jobs.  Every shape
  * uses the same radix plan, so the choice never changes results.  A job's
  * first aar_job_prepare at n >= 256 tunes by itself on its own buffers.
- * seconds: optional [2][3] mean seconds per launch (-1: shape not built for n);
+ * seconds: optional [2][4] mean seconds per launch (-1: shape not built for n);
  * variant: optional [2] chosen shape per axis. */
 int aar_autotune(aar_ctx *ctx, int n, int precision, int nf, double *seconds, int *variant);
 /* tile shape in use for (n, precision, axis 0 = x, 1 = y) */
 int aar_xy_tile_variant(int n, int precision, int axis);
-/* pin a tile shape (0, 1, 2); a shape not built for n falls back to 0 at launch */
+/* pin a tile shape (0, 1, 2, 3); a shape not built for n falls back to 0 at launch */
 int aar_set_xy_tile_variant(int n, int precision, int axis, int variant);
 
 /* ---- building blocks (tests, tools) ---- */

@@ -167,22 +167,27 @@ def test_graph_replays_match_eager(ctx):
         eager.sync()
 
 
-def test_xy_tile_shapes_give_identical_transforms(ctx):
-    """Every x / y tile shape shares the radix plan: bit-identical FFTs."""
+@pytest.mark.parametrize("N", [256, 512])
+def test_xy_tile_shapes_give_identical_transforms(ctx, N):
+    """Every x / y tile shape shares the radix plan: bit-identical FFTs, both directions.  At 256^3
+    shape 3 (four tiles per workgroup) is not built and runs as shape 0; at 512^3 every shape
+    runs."""
     lib = aar.lib()
-    N = 256
     rng = np.random.default_rng(5)
     x = rng.standard_normal((N, N, N))
     saved = [lib.aar_xy_tile_variant(N, aar.AAR_FP32, a) for a in (0, 1)]
     outs = []
-    for v in range(3):
+    for v in range(4):
         for a in (0, 1):
             assert lib.aar_set_xy_tile_variant(N, aar.AAR_FP32, a, v) == 0
-        outs.append(ctx.fft3d(x, -1, aar.AAR_FP32))
+        y = ctx.fft3d(x, -1, aar.AAR_FP32)
+        outs.append((y, ctx.fft3d(y, +1, aar.AAR_FP32)))
     for a in (0, 1):
         lib.aar_set_xy_tile_variant(N, aar.AAR_FP32, a, saved[a])
-    for o in outs[1:]:
-        np.testing.assert_array_equal(o, outs[0])
+    for y, z in outs[1:]:
+        np.testing.assert_array_equal(y, outs[0][0])
+        np.testing.assert_array_equal(z, outs[0][1])
     sec, chosen = ctx.autotune(N, aar.AAR_FP32, nf=2)
-    assert all(0 <= c < 3 for c in chosen)
+    assert all(0 <= c < 4 for c in chosen)
     assert (sec[:, 0] > 0).all()
+    assert (sec[:, 3] > 0).all() == (N >= 512)
