@@ -57,7 +57,8 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_read_params_res", "aar_l
            "aar_gaussian_field", "aar_ranlxd_device", "aar_bench_pass", "aar_job_set_profiling", "aar_job_kernel_stats", "aar_job_set_graph",
            "aar_server_create", "aar_server_submit", "aar_server_wait", "aar_server_fetch",
            "aar_server_device_output", "aar_server_release", "aar_server_stats", "aar_server_destroy",
-           "aar_autotune", "aar_xy_tile_variant", "aar_set_xy_tile_variant", "aar_job_fetch_field_box",
+           "aar_autotune", "aar_xy_tile_variant", "aar_set_xy_tile_variant", "aar_xy_group_cols",
+           "aar_set_xy_group_cols", "aar_job_fetch_field_box",
            "aar_job_ng_statistics", "aar_mode_bispectra_covariance", "aar_mode_QS",
            "aar_mode_basis_expansion"]
 KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
@@ -143,6 +144,8 @@ def lib():
             "aar_autotune": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, C.POINTER(C.c_int)]),
             "aar_xy_tile_variant": (C.c_int, [C.c_int, C.c_int, C.c_int]),
             "aar_set_xy_tile_variant": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
+            "aar_xy_group_cols": (C.c_int, [C.c_int, C.c_int]),
+            "aar_set_xy_group_cols": (C.c_int, [C.c_int, C.c_int, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -87,6 +87,7 @@ struct aar_job {
     int kind;
     double bytes;
     hipEvent_t a, b;
+    int dispatches;  // kernel launches between the two events
   };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> evpool;
@@ -250,15 +251,15 @@ hipEvent_t next_event(aar_job* j) {
   return j->evpool[j->evnext++];
 }
 
-// Run one launcher; with profiling on, bracket it with events on `st`.
-template <class Fn> int timed(aar_job* j, hipStream_t st, int kind, double bytes, Fn&& fn) {
+// Run one launcher of `dispatches` kernel launches; with profiling on, bracket it with events on `st`.
+template <class Fn> int timed(aar_job* j, hipStream_t st, int kind, double bytes, Fn&& fn, int dispatches = 1) {
   if (!j->profiling) return fn();
   hipEvent_t a = next_event(j), b = next_event(j);
   if (!a || !b) return AAR_ERR_HIP;
   hipEventRecord(a, st);
   int rc = fn();
   hipEventRecord(b, st);
-  j->recs.push_back({kind, bytes, a, b});
+  j->recs.push_back({kind, bytes, a, b, dispatches});
   return rc;
 }
 
@@ -279,21 +280,22 @@ template <typename T> struct Exec {
     const double S = (double)g.fs * sizeof(C2);              // one spectrum / padded field
     const double R = (double)g.N * g.N * g.N * sizeof(T);     // one real field
     const double phase_shift = p.PhaseFlip == 1 ? AAR_PI : 0.0;
+    // x / y pass pairs, in the tuned column groups; one timed unit of `dispatches` launches
+    const int xyg = aar_xy_group(sizeof(T) == 8, g.N);
+    auto xy = [&](C2* f, int nf, int dir) {
+      return timed(j, st, AAR_K_XY, 4 * nf * S, [&] { return P::xy_pair(f, nf, g, tw(j), dir, st, xyg); },
+                   aar_xy_pair_dispatches(nf, g.H, xyg));
+    };
     auto c2r = [&](C2* f, int nf) {
       int r;
-      if ((r = timed(j, st, AAR_K_XY, 2 * nf * S, [&] { return P::xy_pass(f, nf, g, tw(j), 0, +1, st); })))
-        return r;
-      if ((r = timed(j, st, AAR_K_XY, 2 * nf * S, [&] { return P::xy_pass(f, nf, g, tw(j), 1, +1, st); })))
-        return r;
+      if ((r = xy(f, nf, +1))) return r;
       return timed(j, st, AAR_K_Z, nf * (S + R), [&] { return P::z_pass(f, nf, g, tw(j), +1, st); });
     };
     auto r2c = [&](C2* f, int nf) {
       int r;
       if ((r = timed(j, st, AAR_K_Z, nf * (S + R), [&] { return P::z_pass(f, nf, g, tw(j), -1, st); })))
         return r;
-      if ((r = timed(j, st, AAR_K_XY, 2 * nf * S, [&] { return P::xy_pass(f, nf, g, tw(j), 1, -1, st); })))
-        return r;
-      return timed(j, st, AAR_K_XY, 2 * nf * S, [&] { return P::xy_pass(f, nf, g, tw(j), 0, -1, st); });
+      return xy(f, nf, -1);
     };
     if ((rc = timed(j, st, AAR_K_GAUSS, S + 4.0 * g.N * g.N, [&] {
            return P::gauss(F(j, j->s_phig), g, j->ks, j->seeds, c.Anorm, p.PrimordialIndex, c.fac, c.Beta,
@@ -310,15 +312,11 @@ template <typename T> struct Exec {
         if (j->M <= 16 && g.N >= 64) {
           // x, y inverse passes of the M mode fields, then the fused z pass
           // (C2R, products, R2C), then y, x forward passes of the K products
-          if ((rc = timed(j, st, AAR_K_XY, 2 * j->M * S, [&] { return P::xy_pass(reg, j->M, g, tw(j), 0, +1, st); })) ||
-              (rc = timed(j, st, AAR_K_XY, 2 * j->M * S, [&] { return P::xy_pass(reg, j->M, g, tw(j), 1, +1, st); })))
-            return rc;
+          if ((rc = xy(reg, j->M, +1))) return rc;
           if ((rc = timed(j, st, AAR_K_PRODUCTS, (j->M + j->K) * S,
                           [&] { return P::z_ng(reg, j->M, j->K, j->fgroups, j->frecs, g, tw(j), st); })))
             return rc;
-          if ((rc = timed(j, st, AAR_K_XY, 2 * j->K * S, [&] { return P::xy_pass(reg, j->K, g, tw(j), 1, -1, st); })) ||
-              (rc = timed(j, st, AAR_K_XY, 2 * j->K * S, [&] { return P::xy_pass(reg, j->K, g, tw(j), 0, -1, st); })))
-            return rc;
+          if ((rc = xy(reg, j->K, -1))) return rc;
         } else {
           if ((rc = c2r(reg, j->M))) return rc;
           if ((rc = timed(j, st, AAR_K_PRODUCTS, (j->M + j->K) * R,
@@ -340,17 +338,13 @@ template <typename T> struct Exec {
         // the six gradient fields go through the fused z pass, whose one
         // product is the 2LPT source (never stored in real space), then y, x
         // forward passes of the source
-        if ((rc = timed(j, st, AAR_K_XY, 2 * 9 * S, [&] { return P::xy_pass(reg, 9, g, tw(j), 0, +1, st); })) ||
-            (rc = timed(j, st, AAR_K_XY, 2 * 9 * S, [&] { return P::xy_pass(reg, 9, g, tw(j), 1, +1, st); })))
-          return rc;
+        if ((rc = xy(reg, 9, +1))) return rc;
         if ((rc = timed(j, st, AAR_K_Z, 3 * (S + R), [&] { return P::z_pass(disp, 3, g, tw(j), +1, st); })))
           return rc;
         if ((rc = timed(j, st, AAR_K_PRODUCTS, 7 * S,
                         [&] { return P::z_ng(reg, 6, 1, j->src_groups, j->src_recs, g, tw(j), st); })))
           return rc;
-        if ((rc = timed(j, st, AAR_K_XY, 2 * S, [&] { return P::xy_pass(reg, 1, g, tw(j), 1, -1, st); })) ||
-            (rc = timed(j, st, AAR_K_XY, 2 * S, [&] { return P::xy_pass(reg, 1, g, tw(j), 0, -1, st); })))
-          return rc;
+        if ((rc = xy(reg, 1, -1))) return rc;
         if ((rc = timed(j, st, AAR_K_LPT, 4 * S, [&] { return P::disp2_spec(reg, disp2, g, j->ks, st); })))
           return rc;
       } else {
@@ -595,6 +589,29 @@ static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, double* sec, i
     if (!rc) aar_set_xy_variant(sizeof(T) == 8, N, axis, bv);
     if (chosen) chosen[axis] = bv;
   }
+  // then the column groups of the x / y pass pairs with those shapes: whole passes over the nf
+  // fields, or field by field in halves or quarters of the kz columns
+  double best = 1e30;
+  int bg = 0;
+  for (int gc : {0, g.H / 2, g.H / 4}) {
+    if (rc || (gc > 0 && gc < 16)) break;
+    int r = Pipeline<T>::xy_pair(d, nf, g, tw, +1, ctx->stream, gc);
+    double t = 1e30;
+    if (!r) {
+      hipEventRecord(a, ctx->stream);
+      for (int q = 0; q < iters && !r; ++q) r = Pipeline<T>::xy_pair(d, nf, g, tw, q % 2 ? -1 : +1, ctx->stream, gc);
+      hipEventRecord(b, ctx->stream);
+      if (hipEventSynchronize(b) != hipSuccess) r = AAR_ERR_HIP;
+      float ms = 0;
+      if (!r && hipEventElapsedTime(&ms, a, b) == hipSuccess) t = ms * 1e-3 / iters;
+    }
+    if (r) rc = r;
+    if (t < best) {
+      best = t;
+      bg = gc;
+    }
+  }
+  if (!rc) aar_set_xy_group(sizeof(T) == 8, N, bg);
   hipEventDestroy(a);
   hipEventDestroy(b);
   if (!scratch) hipFree(d);
@@ -603,6 +620,18 @@ static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, double* sec, i
     g_tuned[sizeof(T) == 8][ilog2i(N)] = true;
   }
   return rc;
+}
+
+extern "C" int aar_xy_group_cols(int n, int precision) {
+  if (!pow2(n) || (precision != AAR_FP32 && precision != AAR_FP64)) return AAR_ERR_BAD_ARG;
+  return aar_xy_group(precision == AAR_FP64, n);
+}
+
+extern "C" int aar_set_xy_group_cols(int n, int precision, int cols) {
+  if (!pow2(n) || (precision != AAR_FP32 && precision != AAR_FP64)) return AAR_ERR_BAD_ARG;
+  if (cols < 0 || cols > n / 2 || (cols && cols % 16)) return AAR_ERR_BAD_ARG;
+  aar_set_xy_group(precision == AAR_FP64, n, cols);
+  return AAR_OK;
 }
 
 extern "C" int aar_autotune(aar_ctx* ctx, int n, int precision, int nf, double* seconds, int* variant) {
@@ -916,7 +945,7 @@ extern "C" int aar_job_sync(aar_job* j) {
   for (const auto& r : j->recs) {
     float e = 0;
     AAR_HIP(hipEventElapsedTime(&e, r.a, r.b));
-    j->k_launch[r.kind] += 1;
+    j->k_launch[r.kind] += r.dispatches;
     j->k_sec[r.kind] += e * 1e-3;
     j->k_bytes[r.kind] += r.bytes;
   }

@@ -55,13 +55,26 @@ struct NgWeight {
   double c;  // coefficient
 };
 
+// Part of a field an x / y pass covers: kz columns [k0, k0 + cols) of the kz < N/2 block
+// (cols = 0: all of them) and, with nyq, the Nyquist plane.
+struct XYRange {
+  int k0 = 0, cols = 0;
+  bool nyq = true;
+};
+
 template <typename T> struct Pipeline {
   // FFT passes on nf consecutive fields starting at f (stride g.fs)
   static int c2r(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st);
   static int r2c(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st);
   // variant < 0: the tile shape from the variant table (aar_set_xy_variant)
   static int xy_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int axis, int dir,
-                     hipStream_t st, int variant = -1);
+                     hipStream_t st, int variant = -1, XYRange r = XYRange());
+  // x then y pass (dir > 0) or y then x (dir < 0).  group_cols > 0: field by field, in groups of
+  // group_cols kz columns (the last group may be narrower), so the first pass's output of a group
+  // can still be in the last-level cache when the second pass reads it; 0: each pass over all nf
+  // fields at once; < 0: the tuned choice for this precision and N (aar_set_xy_group).
+  static int xy_pair(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int dir, hipStream_t st,
+                     int group_cols);
   static int z_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int dir, hipStream_t st);
 
   // fused z pass of the NG sum: M mode fields (x, y already inverse transformed)
@@ -138,6 +151,10 @@ int stat_sums(const double2* fields, const Geom& g, const int* ofs, const StatTe
 constexpr int kXYVariantCount = 4;
 int aar_xy_variant(int fp64, int n, int axis);
 void aar_set_xy_variant(int fp64, int n, int axis, int v);
+int aar_xy_group(int fp64, int n);
+void aar_set_xy_group(int fp64, int n, int cols);
+// kernel launches of one xy_pair call
+int aar_xy_pair_dispatches(int nf, int H, int group_cols);
 // n uniforms of one ranlxd1 stream on the device (test hook)
 int aar_ranlxd_draws_dev(uint32_t seed, int n, double* out, hipStream_t st);
 

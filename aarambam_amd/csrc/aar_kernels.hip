@@ -73,6 +73,22 @@ void aar_set_xy_variant(int fp64, int n, int axis, int v) {
   if (l < 11 && axis >= 0 && axis < 2 && v >= 0 && v < kXYVariantCount) g_xy_variant[fp64 ? 1 : 0][l][axis] = v;
 }
 
+// Column groups of the x / y pass pairs: [precision][log2 N], 0 = whole passes over all fields
+static int g_xy_group[2][11] = {};
+int aar_xy_group(int fp64, int n) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l < 11 ? g_xy_group[fp64 ? 1 : 0][l] : 0;
+}
+void aar_set_xy_group(int fp64, int n, int cols) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  if (l < 11 && cols >= 0) g_xy_group[fp64 ? 1 : 0][l] = cols;
+}
+int aar_xy_pair_dispatches(int nf, int H, int group_cols) {
+  return group_cols > 0 ? 2 * nf * ((H + group_cols - 1) / group_cols) : 2;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; give
 // each XCD a contiguous range of tiles so neighbouring tiles share its L2.
 __device__ __forceinline__ int xcd_tile(int b, int nblk) {
@@ -133,7 +149,7 @@ __device__ __forceinline__ void r2c_post(const cx_t<T>* tile, const cx_t<T>* tw,
 // kz < N/2 block, the rest the Nyquist plane.
 template <typename T, int L, int AXIS, int DIR, int V>
 __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_waves_per_eu(XYCfg<T, L, V>::WPE)))
-    k_xy_pass(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg, int nmain) {
+    k_xy_pass(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg, int nmain, int gper, int k0) {
   using C2 = cx_t<T>;
   constexpr int C = XYCfg<T, L, V>::C, NT = XYCfg<T, L, V>::NT;
   constexpr bool SPLIT = XYCfg<T, L, V>::SPLIT;
@@ -155,9 +171,8 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
     Addr A;
     const int col = threadIdx.x % C;
     if (t < nmain) {
-      const int per = H / C;
-      const int a = t / per;         // fixed y (x pass) or fixed x (y pass)
-      const int c0 = (t % per) * C;  // first kz of the tile
+      const int a = t / gper;             // fixed y (x pass) or fixed x (y pass)
+      const int c0 = k0 + (t % gper) * C;  // first kz of the tile
       A.s1 = AXIS == 0 ? (unsigned)(kXB * N * H) : (unsigned)(kXB * kXB * H);
       A.s2 = AXIS == 0 ? (unsigned)H : (unsigned)(kXB * H);
       // row_of(N, pos, a) * H + c0 + col, or row_of(N, a, pos) = row_of(N, a, 0) + kXB pos
@@ -202,32 +217,36 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
 }
 
 template <typename T, int L, int AXIS, int DIR, int V>
-static int launch_xy_v(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st) {
+static int launch_xy_v(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, const XYRange& r) {
   using Cfg = XYCfg<T, L, V>;
   if constexpr (!Cfg::VALID) {
     return AAR_ERR_UNSUPPORTED;
   } else {
     constexpr int C = Cfg::C, NT = Cfg::NT;
-    const int nmain = g.N * (g.H / C);
-    const int nnyq = g.N / C;
+    const int cols = r.cols > 0 ? r.cols : g.H;
+    if (cols % C || r.k0 % C || r.k0 < 0 || r.k0 + cols > g.H) return AAR_ERR_UNSUPPORTED;
+    const int gper = cols / C;
+    const int nmain = g.N * gper;
+    const int nnyq = r.nyq ? g.N / C : 0;
+    if (nmain + nnyq == 0) return AAR_OK;
     dim3 grid((nmain + nnyq + Cfg::TPW - 1) / Cfg::TPW, nf);
-    hipLaunchKernelGGL((k_xy_pass<T, L, AXIS, DIR, V>), grid, dim3(NT), 0, st, f, g, tw, nmain);
+    hipLaunchKernelGGL((k_xy_pass<T, L, AXIS, DIR, V>), grid, dim3(NT), 0, st, f, g, tw, nmain, gper, r.k0);
     return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
   }
 }
 
 // Tile variants other than 0 are built for the sizes where the choice matters.
 template <typename T, int L, int AXIS, int DIR>
-static int launch_xy(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, int v) {
+static int launch_xy(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, int v, const XYRange& r) {
   if constexpr (L >= 256) {
     switch (v) {
-      case 1: return launch_xy_v<T, L, AXIS, DIR, 1>(f, nf, g, tw, st);
-      case 2: return launch_xy_v<T, L, AXIS, DIR, 2>(f, nf, g, tw, st);
-      case 3: return launch_xy_v<T, L, AXIS, DIR, 3>(f, nf, g, tw, st);
+      case 1: return launch_xy_v<T, L, AXIS, DIR, 1>(f, nf, g, tw, st, r);
+      case 2: return launch_xy_v<T, L, AXIS, DIR, 2>(f, nf, g, tw, st, r);
+      case 3: return launch_xy_v<T, L, AXIS, DIR, 3>(f, nf, g, tw, st, r);
       default: break;
     }
   }
-  return launch_xy_v<T, L, AXIS, DIR, 0>(f, nf, g, tw, st);
+  return launch_xy_v<T, L, AXIS, DIR, 0>(f, nf, g, tw, st, r);
 }
 
 // ------------------------------------------------------------------ z passes
@@ -504,18 +523,18 @@ int Pipeline<T>::z_ng(cx_t<T>* f, int M, int K, const int* gs, const NgRec4* rec
 
 template <typename T>
 int Pipeline<T>::xy_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int axis, int dir,
-                         hipStream_t st, int variant) {
+                         hipStream_t st, int variant, XYRange r) {
   const int v = variant >= 0 ? variant : aar_xy_variant(sizeof(T) == 8, g.N, axis);
   if (variant < 0 && v != 0) {  // a pinned shape that is not built for this size runs as shape 0
-    const int rc = xy_pass(f, nf, g, tw, axis, dir, st, v);
-    return rc == AAR_ERR_UNSUPPORTED ? xy_pass(f, nf, g, tw, axis, dir, st, 0) : rc;
+    const int rc = xy_pass(f, nf, g, tw, axis, dir, st, v, r);
+    return rc == AAR_ERR_UNSUPPORTED ? xy_pass(f, nf, g, tw, axis, dir, st, 0, r) : rc;
   }
-#define XY_CASE(LL)                                                                           \
-  case LL:                                                                                    \
-    if (axis == 0)                                                                            \
-      return dir > 0 ? launch_xy<T, LL, 0, 1>(f, nf, g, tw, st, v)                            \
-                     : launch_xy<T, LL, 0, -1>(f, nf, g, tw, st, v);                          \
-    return dir > 0 ? launch_xy<T, LL, 1, 1>(f, nf, g, tw, st, v) : launch_xy<T, LL, 1, -1>(f, nf, g, tw, st, v);
+#define XY_CASE(LL)                                                                                 \
+  case LL:                                                                                          \
+    if (axis == 0)                                                                                  \
+      return dir > 0 ? launch_xy<T, LL, 0, 1>(f, nf, g, tw, st, v, r)                               \
+                     : launch_xy<T, LL, 0, -1>(f, nf, g, tw, st, v, r);                             \
+    return dir > 0 ? launch_xy<T, LL, 1, 1>(f, nf, g, tw, st, v, r) : launch_xy<T, LL, 1, -1>(f, nf, g, tw, st, v, r);
   switch (g.N) {
     XY_CASE(16)
     XY_CASE(32)
@@ -528,6 +547,26 @@ int Pipeline<T>::xy_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, i
       return AAR_ERR_UNSUPPORTED;
   }
 #undef XY_CASE
+}
+
+template <typename T>
+int Pipeline<T>::xy_pair(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int dir, hipStream_t st,
+                         int group_cols) {
+  const int a0 = dir > 0 ? 0 : 1, a1 = 1 - a0;
+  int rc;
+  if (group_cols < 0) group_cols = aar_xy_group(sizeof(T) == 8, g.N);
+  if (group_cols <= 0) {
+    if ((rc = xy_pass(f, nf, g, tw, a0, dir, st))) return rc;
+    return xy_pass(f, nf, g, tw, a1, dir, st);
+  }
+  for (int q = 0; q < nf; ++q)
+    for (int k0 = 0; k0 < g.H; k0 += group_cols) {
+      const XYRange r{k0, k0 + group_cols < g.H ? group_cols : g.H - k0, k0 == 0};
+      cx_t<T>* fq = f + (long long)q * g.fs;
+      if ((rc = xy_pass(fq, 1, g, tw, a0, dir, st, -1, r)) || (rc = xy_pass(fq, 1, g, tw, a1, dir, st, -1, r)))
+        return rc;
+    }
+  return AAR_OK;
 }
 
 template <typename T>
@@ -548,8 +587,7 @@ int Pipeline<T>::z_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, in
 template <typename T>
 int Pipeline<T>::c2r(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st) {
   int rc;
-  if ((rc = xy_pass(f, nf, g, tw, 0, +1, st))) return rc;
-  if ((rc = xy_pass(f, nf, g, tw, 1, +1, st))) return rc;
+  if ((rc = xy_pair(f, nf, g, tw, +1, st, -1))) return rc;
   return z_pass(f, nf, g, tw, +1, st);
 }
 
@@ -557,8 +595,7 @@ template <typename T>
 int Pipeline<T>::r2c(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st) {
   int rc;
   if ((rc = z_pass(f, nf, g, tw, -1, st))) return rc;
-  if ((rc = xy_pass(f, nf, g, tw, 1, -1, st))) return rc;
-  return xy_pass(f, nf, g, tw, 0, -1, st);
+  return xy_pair(f, nf, g, tw, -1, st, -1);
 }
 
 // ------------------------------------------------------------------ Gaussian potential

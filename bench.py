@@ -344,8 +344,10 @@ def main():
     N = args.nmesh
     ctx = aar.Context(0)
     cfg, alpha, x0, glass, gbox = workload(N, args.box, args.template, ctx=ctx)
-    # x / y tile shapes for this mesh, picked by the autotuner before the timed region
+    # x / y tile shapes and x / y pair column groups for this mesh, picked by the autotuner
+    # before the timed region
     tsec, tvar = ctx.autotune(N, aar.AAR_FP32, nf=8)
+    xy_group = aar.lib().aar_xy_group_cols(N, aar.AAR_FP32)
     # timed region: eager launches, each bracketed by HIP events (per-kernel roofline)
     eb = 0 if args.no_fp64 else min(128, N)
     r32 = time_job(aar, ctx, cfg, alpha, x0, glass, gbox, aar.AAR_FP32, aar.AAR_SCHED_BATCHED, args.steps,
@@ -403,7 +405,8 @@ def main():
     out["autotune"] = {"xy_tile_variant": {"x": tvar[0], "y": tvar[1]},
                        "us_per_launch_8_fields": {ax: [round(v * 1e6, 1) if v > 0 else None for v in tsec[i]]
                                                   for i, ax in enumerate(("x", "y"))},
-                       "variants": "0: 128-B tile columns, 1: 64-B, 2: 256-B, 3: 128-B, four tiles per workgroup with the next tile's loads in flight (same radix plan)"}
+                       "variants": "0: 128-B tile columns, 1: 64-B, 2: 256-B, 3: 128-B, four tiles per workgroup with the next tile's loads in flight (same radix plan)",
+                       "xy_pair_group_cols": xy_group}
     out["copy_peak_gbs"] = round(copy_peak(ctx, min(N, 512)), 1)
     if not args.no_serve:
         out["serving"] = serving(aar, ctx, args.serve_jobs)

@@ -306,6 +306,8 @@ int aar_server_destroy(aar_server *s);
  * of an n^3 grid and keeps the fastest per axis for later jobs.  Every shape
  * uses the same radix plan, so the choice never changes results.  A job's
  * first aar_job_prepare at n >= 256 tunes by itself on its own buffers.
+ * Then it times the column groups of the x / y pass pairs (whole passes, halves, quarters)
+ * and keeps the fastest (aar_xy_group_cols).
  * seconds: optional [2][4] mean seconds per launch (-1: shape not built for n);
  * variant: optional [2] chosen shape per axis. */
 int aar_autotune(aar_ctx *ctx, int n, int precision, int nf, double *seconds, int *variant);
@@ -313,6 +315,13 @@ int aar_autotune(aar_ctx *ctx, int n, int precision, int nf, double *seconds, in
 int aar_xy_tile_variant(int n, int precision, int axis);
 /* pin a tile shape (0, 1, 2, 3); a shape not built for n falls back to 0 at launch */
 int aar_set_xy_tile_variant(int n, int precision, int axis, int variant);
+/* Column groups of the x / y pass pairs, also picked by aar_autotune: 0 runs each pass over all
+ * fields at once; cols > 0 (a multiple of 16, at most n / 2) runs them field by field over kz
+ * columns [k0, k0 + cols), so a group's intermediate can stay in the last-level cache between
+ * the two passes.  Results are bit-identical for every choice.  The getter returns the choice
+ * in use for (n, precision) or a negative status. */
+int aar_xy_group_cols(int n, int precision);
+int aar_set_xy_group_cols(int n, int precision, int cols);
 
 /* ---- building blocks (tests, tools) ---- */
 /* 3-D transforms with FFTW 2 rfftwnd conventions on a host array in the

@@ -167,6 +167,52 @@ def test_graph_replays_match_eager(ctx):
         eager.sync()
 
 
+AAR_ERR_BAD_ARG = 100001  # include/aarambam.h
+
+
+def test_xy_pair_column_groups_give_identical_results(ctx):
+    """The x / y pass pairs run over all fields at once or field by field in kz column groups
+    (96 columns at 512^3 leaves a narrower last group): bit-identical 3-D FFTs, and a batched
+    FP64 job at 64^3 in two 16-column groups matches the same job with whole passes bitwise
+    (and the oracle).  The setter rejects widths that are not multiples of 16 or exceed N/2."""
+    lib = aar.lib()
+    N = 512
+    saved = lib.aar_xy_group_cols(N, aar.AAR_FP32)
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal((N, N, N))
+    outs = []
+    for gc in (0, 128, 96):
+        assert lib.aar_set_xy_group_cols(N, aar.AAR_FP32, gc) == 0
+        assert lib.aar_xy_group_cols(N, aar.AAR_FP32) == gc
+        y = ctx.fft3d(x, -1, aar.AAR_FP32)
+        outs.append((y, ctx.fft3d(y, +1, aar.AAR_FP32)))
+    assert lib.aar_set_xy_group_cols(N, aar.AAR_FP32, saved) == 0
+    for y, z in outs[1:]:
+        np.testing.assert_array_equal(y, outs[0][0])
+        np.testing.assert_array_equal(z, outs[0][1])
+    assert lib.aar_set_xy_group_cols(N, aar.AAR_FP32, 8) == AAR_ERR_BAD_ARG
+    assert lib.aar_set_xy_group_cols(N, aar.AAR_FP32, 512) == AAR_ERR_BAD_ARG
+    assert lib.aar_set_xy_group_cols(100, aar.AAR_FP32, 16) == AAR_ERR_BAD_ARG
+
+    cfg = cases.base_config(Nmesh=64, Nsample=64, Nmax=64, N_modes=12)
+    alpha, x0, glass, gbox = cases.job_inputs(cfg, cases.equilateral_rows())
+    t = tables_for(cfg, alpha, x0, glass, gbox)
+    saved = lib.aar_xy_group_cols(64, aar.AAR_FP64)
+    res = []
+    for gc in (0, 16):
+        assert lib.aar_set_xy_group_cols(64, aar.AAR_FP64, gc) == 0
+        job = aar.Job(ctx, aar.params_from_dict(cfg), t, aar.AAR_FP64, aar.AAR_SCHED_BATCHED)
+        job.run()
+        job.sync()
+        res.append((job.fetch(), job.field("phing_real"), job.field("disp2_x")))
+        job.close()
+    assert lib.aar_set_xy_group_cols(64, aar.AAR_FP64, saved) == 0
+    for a, b in zip(res[0][0] + res[0][1:], res[1][0] + res[1][1:]):
+        np.testing.assert_array_equal(a, b)
+    ref = orc.run(cfg, alpha, x0, glass, gbox)
+    assert rel_l2(res[1][1], ref["phing_real"]) < 1e-10
+
+
 @pytest.mark.parametrize("N", [256, 512])
 def test_xy_tile_shapes_give_identical_transforms(ctx, N):
     """Every x / y tile shape shares the radix plan: bit-identical FFTs, both directions.  At 256^3
