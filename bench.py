@@ -335,6 +335,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=64)
     ap.add_argument("--no-decomp", action="store_true")
+    ap.add_argument("--xy-group", type=int, default=-1,
+                    help="pin the kz column groups of the x/y pass pairs (default: the autotuner's choice); "
+                         "counter runs serialize every launch, which skews the autotuner, so they pin it")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs one IC job per GPU on a single GPU; multi-GPU runs are not "
@@ -347,6 +350,8 @@ def main():
     # x / y tile shapes and x / y pair column groups for this mesh, picked by the autotuner
     # before the timed region
     tsec, tvar = ctx.autotune(N, aar.AAR_FP32, nf=8)
+    if args.xy_group >= 0 and aar.lib().aar_set_xy_group_cols(N, aar.AAR_FP32, args.xy_group) != 0:
+        sys.exit(f"--xy-group {args.xy_group}: not a valid column group for N = {N}")
     xy_group = aar.lib().aar_xy_group_cols(N, aar.AAR_FP32)
     # timed region: eager launches, each bracketed by HIP events (per-kernel roofline)
     eb = 0 if args.no_fp64 else min(128, N)
