@@ -3,6 +3,8 @@ aar_mode_QS) against the CPU oracle, which equals the reference's own
 tetrapyd.c bit for bit (tests/test_modal.py).  The device sums over the
 tetrapyd points in a different order (MFMA tiles, split over blocks), so the
 bar is 1e-12 relative to the largest entry."""
+import os
+
 import numpy as np
 import pytest
 
@@ -157,3 +159,43 @@ def pymodal_expansion_check(r, ns):
     np.testing.assert_array_equal(ra[4][0], r["selected"])
     np.testing.assert_allclose(r["AlphaTable"][:, 3], ra[0][0], rtol=0, atol=1e-6 * np.abs(ra[0][0]).max())
     return True
+
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.mark.parametrize("name", ["equil", "scalari"])
+def test_device_expansion_matches_reference_vectors(ctx, name):
+    """The device expansion against the reference's basis_expansion (vectors written by
+    tests/golden/make_expansion_vectors.py from the reference's cmbbest, 40-point k grid,
+    16 modes, decompose.py's settings): the same modes picked, coefficients to 1e-6 of the
+    largest, correlation to 1e-9."""
+    f = np.load(os.path.join(GOLD, "expansion_grid40.npz"))
+    g = {k: f[k] for k in ["k", "w", "i1", "i2", "i3", "evals", "p1", "p2", "p3"]}
+    corr, mse, tol, mm = f[f"corr_{name}"]
+    alpha, _, c, m, sel = ctx.mode_basis_expansion(g, f[f"S_{name}"][None, :], rtol=1e-10, Niter=1e3,
+                                                   bad_modes=f["bad_modes"], RR_tol=tol, max_modes=int(mm))
+    ref = f[f"alpha_{name}"]
+    np.testing.assert_array_equal(alpha[0] != 0, ref != 0)
+    np.testing.assert_allclose(alpha[0], ref, rtol=0, atol=1e-6 * np.abs(ref).max())
+    np.testing.assert_allclose(c[0], corr, rtol=0, atol=1e-9)
+
+
+def test_scalari16_decomposition_matches_reference_table(ctx):
+    """aarambam_amd.decompose (150-point k grid, expansion on the device) for the benchmark's
+    template -- ScalarI mass 1, 16 modes, n_s 0.9649, L 2000, Nmax 512, ModeTol 1e-4,
+    MaxModeCount 150 -- against the AlphaTable the reference's
+    BasicBasisDecompose.generate_alpha writes for it: the same 132 nonzero rows, coefficients to
+    1e-6 of the largest, the same correlation to 1e-9."""
+    from aarambam_amd import decompose
+    from tests import templates
+    path = os.path.join(GOLD, "scalari16_alphatable.txt")
+    ref = np.loadtxt(path)
+    with open(path) as fh:
+        corr = float(fh.readline().split("correlation ")[1].split(";")[0])
+    r = decompose.decompose(ctx, lambda a, b, c: templates.scalar_i(a, b, c, 0.9649, 1.0), 16, 0.9649, 2000.0, 512,
+                            1e-4, 150)
+    t = r["AlphaTable"][r["AlphaTable"][:, 3] != 0]
+    np.testing.assert_array_equal(t[:, :3], ref[:, :3])
+    np.testing.assert_allclose(t[:, 3], ref[:, 3], rtol=0, atol=1e-6 * np.abs(ref[:, 3]).max())
+    assert abs(r["correlation"] - corr) < 1e-9

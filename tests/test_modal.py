@@ -1,6 +1,8 @@
 """CPU checks of the decomposition contractions' oracle (oracle/oracle.c):
 against the reference's own tetrapyd.c built from its sources (oracle/_ref,
 when present) and against closed forms."""
+import os
+
 import numpy as np
 import pytest
 
@@ -146,3 +148,52 @@ def test_product_basis_grid_matches_the_test_restatement():
     from tests import cases
     ai = decompose.ai_table(15, ns, 2 * np.pi / 512, 2 * np.pi / 512 * 256)
     np.testing.assert_allclose(ai[:, 1], cases.x0_file(15)[:, 0], rtol=1e-14)
+
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def gold40():
+    """Vectors written by tests/golden/make_expansion_vectors.py from the reference's own cmbbest
+    (Basis grid, shapes and basis_expansion on a 40-point k grid, 16 modes)."""
+    return dict(np.load(os.path.join(GOLD, "expansion_grid40.npz")))
+
+
+def test_basis_grid_matches_reference_vectors(gold40):
+    """decompose.basis_grid against cmbbest.Basis itself: the k grid, tetrapyd points and weights,
+    mode functions and 3-D mode indices, bit for bit."""
+    from aarambam_amd import decompose
+    ns, kmin, kmax = gold40["params"]
+    g = decompose.basis_grid(16, ns, kmin, kmax, 40)
+    for key in ["k", "w", "i1", "i2", "i3", "evals", "p1", "p2", "p3"]:
+        np.testing.assert_array_equal(g[key], gold40[key], err_msg=key)
+
+
+def test_scalar_i_shape_matches_reference_vectors(gold40):
+    """The ScalarI (mass 1) shape on the tetrapyd points against the reference's model
+    (Aarambam/models/Colliders.py with decompose.py's symmetrisation)."""
+    from aarambam_amd import decompose
+    from tests import templates
+    ns, kmin, kmax = gold40["params"]
+    g = decompose.basis_grid(16, ns, kmin, kmax, 40)
+    S = templates.shape(g, lambda a, b, c: templates.scalar_i(a, b, c, ns, 1.0))[0]
+    np.testing.assert_allclose(S, gold40["S_scalari"], rtol=0, atol=1e-13 * np.abs(gold40["S_scalari"]).max())
+
+
+@pytest.mark.parametrize("name", ["equil", "scalari"])
+def test_oracle_expansion_matches_reference_vectors(gold40, name):
+    """oracle/pymodal.basis_expansion against the reference's basis_expansion with decompose.py's
+    settings: the same modes (the OMP loop's picks, 132 for ScalarI), the same coefficients to
+    1e-12 of the largest, the same correlation and MSE.  The ScalarI case walks ~115 selection
+    steps (about a minute)."""
+    from oracle import pymodal
+    g = {k: gold40[k] for k in ["k", "w", "i1", "i2", "i3", "evals", "p1", "p2", "p3"]}
+    corr, mse, tol, mm = gold40[f"corr_{name}"]
+    alpha, _, c, m, _ = pymodal.basis_expansion(g, gold40[f"S_{name}"][None, :], rtol=1e-10, Niter=1e3,
+                                                bad_modes=gold40["bad_modes"], RR_tol=tol, max_modes=int(mm))
+    ref = gold40[f"alpha_{name}"]
+    np.testing.assert_array_equal(alpha[0] != 0, ref != 0)
+    np.testing.assert_allclose(alpha[0], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    np.testing.assert_allclose(c[0], corr, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(m[0], mse, rtol=1e-9)
