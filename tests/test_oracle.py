@@ -189,3 +189,49 @@ def test_sigma2_with_wdm_suppression_vs_quad():
     assert ours == pytest.approx(ref, rel=1e-7)
     plain = orc.lib().orc_sigma2_tophat(orc.make_params(dict(cfg, WDM_On=0)), 1.0, R)
     assert ours < 0.9 * plain  # a 0.5 keV particle suppresses sigma_8 visibly
+
+
+def test_eisenstein_hu_transfer_limits_and_scalings():
+    """The no-wiggle Eisenstein & Hu (1998, ApJ 496, 605) transfer function of power.c:519-548,
+    pinned by what its published form fixes rather than by its formula:
+    * no baryons (OmegaBaryon 1e-14; 0 would select power.c's 0.04 h^2 default): T depends on k
+      only through q = k Theta_2.7^2 / (Omega h) (eq. 28 with Gamma = Omega h), so two
+      cosmologies of equal Omega h agree and others follow by rescaling k;
+    * q -> 0: (1 - T) / q^2 -> (14.2 + 731) / ln(2e); q -> oo: T 14.2 q^2 / ln(1.8 q) -> 1 (eq. 29);
+    * baryons rescale Gamma by alpha_Gamma (eq. 31) above the sound horizon s (eq. 26) and not
+      below it: T_b(k) = T_0(k / a_eff) with a_eff = alpha + (1 - alpha) / (1 + (0.43 k s)^4) (eq. 30).
+    """
+    theta2 = (2.728 / 2.7) ** 2
+
+    def T(k, **kw):
+        return orc.transfer(cases.base_config(WhichTransfer=1, **kw), k)
+
+    ks = np.logspace(-4, 2, 25)
+    cdm = dict(OmegaBaryon=1e-14)
+    a = [T(k, Omega=0.3, HubbleParam=0.7, **cdm) for k in ks]
+    b = [T(k, Omega=0.25, HubbleParam=0.84, **cdm) for k in ks]  # same Omega h
+    np.testing.assert_allclose(a, b, rtol=1e-12)
+    # Omega h = 0.18: k scaled by 0.18 / 0.21 gives the same q
+    c = [T(k * 0.18 / 0.21, Omega=0.3, HubbleParam=0.6, **cdm) for k in ks]
+    np.testing.assert_allclose(a, c, rtol=1e-12)
+
+    gam = 0.3 * 0.7
+    for q in [1e-5, 3e-6]:
+        t = T(q * gam / theta2, Omega=0.3, HubbleParam=0.7, **cdm)
+        assert (1 - t) / q ** 2 == pytest.approx(745.2 / np.log(2 * np.e), rel=2e-3)
+    for q in [1e5, 1e6]:
+        t = T(q * gam / theta2, Omega=0.3, HubbleParam=0.7, **cdm)
+        assert t * 14.2 * q * q / np.log(1.8 * q) == pytest.approx(1.0, rel=1e-4)
+
+    om, h, ob = 0.3, 0.7, 0.05
+    omh2, obh2 = om * h * h, ob * h * h
+    fb = obh2 / omh2
+    alpha = 1 - 0.328 * np.log(431 * omh2) * fb + 0.38 * np.log(22.3 * omh2) * fb * fb
+    s = 44.5 * np.log(9.83 / omh2) / np.sqrt(1 + 10 * obh2 ** 0.75) * h  # Mpc/h
+    for x in [1e-2, 1.0, 30.0]:  # 0.43 k s
+        k = x / (0.43 * s)
+        aeff = alpha + (1 - alpha) / (1 + x ** 4)
+        tb = T(k, Omega=om, HubbleParam=h, OmegaBaryon=ob)
+        t0 = T(k / aeff, Omega=om, HubbleParam=h, **cdm)
+        assert tb == pytest.approx(t0, rel=1e-9)
+    assert T(1e-7, Omega=om, HubbleParam=h, OmegaBaryon=ob) == pytest.approx(1.0, abs=1e-9)
