@@ -54,7 +54,6 @@ template <typename T, int L, int V = 0> struct XYCfg {
       C * L / 16 <= 1024 && (long)L * (C + 1) * sizeof(T) + (long)L * sizeof(cx_t<T>) <= 152 * 1024 &&
       (V != 3 || (sizeof(T) == 4 && L >= 512));
   static constexpr int NT = tile_threads(C * L);
-  static constexpr bool SPLIT = true;  // real-valued LDS exchange tile
   // minimum wavefronts per SIMD the register allocation must allow: 64-byte
   // tiles at L = 1024 fit three per CU by LDS when they stay within 80 VGPRs
   static constexpr int WPE = (sizeof(T) == 4 && L == 1024 && V == 1) ? 6 : 1;
@@ -152,8 +151,7 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
     k_xy_pass(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg, int nmain, int gper, int k0) {
   using C2 = cx_t<T>;
   constexpr int C = XYCfg<T, L, V>::C, NT = XYCfg<T, L, V>::NT;
-  constexpr bool SPLIT = XYCfg<T, L, V>::SPLIT;
-  __shared__ T tile_r[SPLIT ? L * (C + 1) : 1];
+  __shared__ T tile_r[L * (C + 1)];  // real-valued exchange tile (one component at a time)
   __shared__ C2 tw[L];
   load_twiddles(tw, twg + L, L, NT);
   C2* f = base + (long long)blockIdx.y * g.fs;
@@ -163,7 +161,6 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
   // with tile-uniform strides s1, s2 and a column term ct that the thread computes once per
   // tile (each thread of the split-exchange tile keeps one column, col = threadIdx.x % C).
   // Offsets fit 32 bits: a field holds N^2 (N/2 + 1) < 2^31 complex values for N <= 1024.
-  static_assert(SPLIT, "the column term assumes one column per thread");
   struct Addr {
     unsigned s1, s2, ct;
   };
