@@ -32,8 +32,10 @@ template <typename C> __host__ __device__ __forceinline__ C cmake(decltype(C::x)
 }
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmake<C>(a.x + b.x, a.y + b.y); }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmake<C>(a.x - b.x, a.y - b.y); }
+// Complex product with two fused multiply-adds (4 instructions instead of 6;
+// the build turns off automatic contraction, so this one is explicit).
 template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
-  return cmake<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return cmake<C>(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
 }
 template <typename C> __device__ __forceinline__ C cconj(C a) { return cmake<C>(a.x, -a.y); }
 template <typename C> __device__ __forceinline__ C cscale(C a, decltype(C::x) s) { return cmake<C>(a.x * s, a.y * s); }
