@@ -6,6 +6,8 @@
 // and radial arguments agree bit for bit with Aarambam/LPT/Basis/main.c.
 #include <math.h>
 
+#include <atomic>
+
 #include "aar_fft.h"
 #include "aar_kernels.h"
 #include "aar_kspace.h"
@@ -60,29 +62,30 @@ template <typename T, int L, int V = 0> struct XYCfg {
 };
 
 // Variant table: [precision 0 FP32 / 1 FP64][log2 N][axis]
-static int g_xy_variant[2][11][2] = {};
+// Tuning tables, read by every launch and written by the autotuner or a pin from any thread.
+static std::atomic<int> g_xy_variant[2][11][2] = {};
 int aar_xy_variant(int fp64, int n, int axis) {
   int l = 0;
   while ((1 << l) < n) ++l;
-  return (l < 11 && axis >= 0 && axis < 2) ? g_xy_variant[fp64 ? 1 : 0][l][axis] : 0;
+  return (l < 11 && axis >= 0 && axis < 2) ? g_xy_variant[fp64 ? 1 : 0][l][axis].load(std::memory_order_relaxed) : 0;
 }
 void aar_set_xy_variant(int fp64, int n, int axis, int v) {
   int l = 0;
   while ((1 << l) < n) ++l;
-  if (l < 11 && axis >= 0 && axis < 2 && v >= 0 && v < kXYVariantCount) g_xy_variant[fp64 ? 1 : 0][l][axis] = v;
+  if (l < 11 && axis >= 0 && axis < 2 && v >= 0 && v < kXYVariantCount) g_xy_variant[fp64 ? 1 : 0][l][axis].store(v, std::memory_order_relaxed);
 }
 
 // Column groups of the x / y pass pairs: [precision][log2 N], 0 = whole passes over all fields
-static int g_xy_group[2][11] = {};
+static std::atomic<int> g_xy_group[2][11] = {};
 int aar_xy_group(int fp64, int n) {
   int l = 0;
   while ((1 << l) < n) ++l;
-  return l < 11 ? g_xy_group[fp64 ? 1 : 0][l] : 0;
+  return l < 11 ? g_xy_group[fp64 ? 1 : 0][l].load(std::memory_order_relaxed) : 0;
 }
 void aar_set_xy_group(int fp64, int n, int cols) {
   int l = 0;
   while ((1 << l) < n) ++l;
-  if (l < 11 && cols >= 0) g_xy_group[fp64 ? 1 : 0][l] = cols;
+  if (l < 11 && cols >= 0) g_xy_group[fp64 ? 1 : 0][l].store(cols, std::memory_order_relaxed);
 }
 int aar_xy_pair_dispatches(int nf, int H, int group_cols) {
   return group_cols > 0 ? 2 * nf * ((H + group_cols - 1) / group_cols) : 2;
