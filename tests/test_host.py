@@ -206,3 +206,50 @@ def test_cosmology_with_wdm_bitwise_equal_to_oracle(which):
     for k, v in ref.items():
         assert ours[k] == v, (k, ours[k], v)
     assert ref["Norm"] != orc.cosmology(dict(cfg, WDM_On=0))["Norm"]
+
+
+def test_reciprocal_division_rounds_correctly(tmp_path):
+    """The CIC readout divides by the glass box and the box as q = a rb with two FMA corrections
+    (rb = RN(1/b), Markstein's iteration, aarambam_amd/csrc/aar_kernels.hip ddiv).  It must equal
+    IEEE a / b bit for bit: 2e7 operands per divisor (uniform in [0, 4b), float-valued, and
+    random exponents over 2^-30 .. 2^33) against the C library's division and fma."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc absent")
+    src = tmp_path / "ddiv.c"
+    src.write_text(r"""
+#include <math.h>
+#include <stdio.h>
+#include <stdint.h>
+#include <string.h>
+static uint64_t s = 88172645463325252ull;
+static uint64_t rnd(void) { s ^= s << 13; s ^= s >> 7; s ^= s << 17; return s; }
+static double ddiv(double a, double b, double rb) {
+  double q = a * rb, r = fma(-b, q, a);
+  q = fma(r, rb, q);
+  r = fma(-b, q, a);
+  return fma(r, rb, q);
+}
+int main(void) {
+  const double bs[] = {100.0, 1000.0, 2000.0, 250.0, 3.0, 7.0, 1e-3, 123.456, 2000.0 / 3.0, 0.1};
+  long long bad = 0;
+  for (int ib = 0; ib < 10; ++ib) {
+    const double b = bs[ib], rb = 1.0 / b;
+    for (long long i = 0; i < 20000000LL; ++i) {
+      uint64_t u = rnd();
+      double a;
+      if (i % 3 == 0) a = (double)(u >> 11) * 0x1p-53 * 4 * b;
+      else if (i % 3 == 1) a = (double)(float)((double)(u >> 40) * 0x1p-24 * 4 * b);
+      else { uint64_t bits = ((uint64_t)(1023 - 30 + (u >> 58)) << 52) | (rnd() >> 12); memcpy(&a, &bits, 8); }
+      if (ddiv(a, b, rb) != a / b) ++bad;
+    }
+  }
+  printf("%lld\n", bad);
+  return 0;
+}
+""")
+    exe = tmp_path / "ddiv"
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", str(exe), str(src), "-lm"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True, timeout=600)
+    assert int(out.stdout.strip()) == 0
