@@ -15,5 +15,5 @@ def pytest_configure(config):
 
 def pytest_collection_modifyitems(config, items):
     # GPU tests need the built HIP library and a visible device; they are
-    # selected with -m gpu and never silently skipped on a GPU box.
+    # selected with -m gpu and never silently skipped on a machine that has one.
     pass
