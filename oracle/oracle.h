@@ -1,8 +1,8 @@
 /*
  * CPU oracle for the NG-2LPT initial-condition stage.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP product
- * path is compared against.  Only tests/, __graft_entry__.smoke() and the
+ * TEST INFRASTRUCTURE ONLY.  This library is the CPU restatement that the HIP
+ * product path is compared against.  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load it; the product library never links,
  * loads or calls anything here.
  *
