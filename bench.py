@@ -340,8 +340,8 @@ def main():
                          "counter runs serialize every launch, which skews the autotuner, so they pin it")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs one IC job per GPU on a single GPU; multi-GPU runs are not "
-              "part of it (jobs shard across GPUs with no exchange).", file=sys.stderr)
+        print("bench.py: measures one GPU only; --gpus above 1 is not supported (one IC job runs on one "
+              "GPU, and independent jobs shard across GPUs with no exchange).", file=sys.stderr)
         sys.exit(2)
     import aarambam_amd as aar
     N = args.nmesh
