@@ -1103,17 +1103,17 @@ __global__ void __launch_bounds__(256)
   using C2 = cx_t<T>;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= g.fs) return;
-  int x, y, z;
-  decode_k(g, e, x, y, z);
-  // the path's precision throughout; one reciprocal replaces the divisions
-  using RT = T;
-  const RT kv[3] = {(RT)ks.kv[x], (RT)ks.kv[y], (RT)ks.kv[z]};
-  const RT kmag2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
-  const RT ff = (RT)1 / ((RT)ks.sinc[x] * (RT)ks.sinc[y] * (RT)ks.sinc[z]);
-  const RT f = kmag2 > (RT)0 ? ff * ff / kmag2 : (RT)0;  // CIC deconvolution / k^2
-  const C2 s = src[e];
+  with_k(g, e, [&](int x, int y, int z) {
+    // the path's precision throughout; one reciprocal replaces the divisions
+    using RT = T;
+    const RT kv[3] = {(RT)ks.kv[x], (RT)ks.kv[y], (RT)ks.kv[z]};
+    const RT kmag2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+    const RT ff = (RT)1 / ((RT)ks.sinc[x] * (RT)ks.sinc[y] * (RT)ks.sinc[z]);
+    const RT f = kmag2 > (RT)0 ? ff * ff / kmag2 : (RT)0;  // CIC deconvolution / k^2
+    const C2 s = src[e];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) out[(long long)a * g.fs + e] = cmake<C2>(s.y * kv[a] * f, -s.x * kv[a] * f);
+    for (int a = 0; a < 3; ++a) out[(long long)a * g.fs + e] = cmake<C2>(s.y * kv[a] * f, -s.x * kv[a] * f);
+  });
 }
 
 template <typename T>

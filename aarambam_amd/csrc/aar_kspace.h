@@ -17,6 +17,23 @@ __device__ __forceinline__ void decode_k(const Geom& g, long long e, int& x, int
   row_xy_log(g.nlog, r, x, y);
 }
 
+// Runs body(x, y, kz) for element e of a block of consecutive elements.  When the block lies in
+// one row of the kz < N/2 block (N/2 a multiple of the block size) the row, hence x and y, comes
+// from the block index: the body then sees wave-uniform x and y, and its table reads indexed by
+// them become scalar loads.  Otherwise each element is decoded on its own.
+template <class F> __device__ __forceinline__ void with_k(const Geom& g, long long e, F&& body) {
+  const long long e0 = (long long)blockIdx.x * blockDim.x;
+  if (g.H % (int)blockDim.x == 0 && e0 < g.nq) {
+    int x, y, z0;
+    decode_k(g, e0, x, y, z0);
+    body(x, y, (int)(e & (g.H - 1)));
+  } else {
+    int x, y, z;
+    decode_k(g, e, x, y, z);
+    body(x, y, z);
+  }
+}
+
 // main.c:452-462: SphereMode 1 keeps |k| L/2pi <= Nsample/2, else the cube.
 __device__ __forceinline__ bool outside_cut(const KSpace& ks, int x, int y, int z, double kmag1) {
   if (ks.sphere == 1) return kmag1 * ks.box / (2 * AAR_PI) > ks.nsh;
