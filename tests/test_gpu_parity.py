@@ -144,9 +144,23 @@ def test_job_fp32_error_vs_oracle(ctx, table):
     print("fp32 rel L2", table, res)
 
 
-def test_graph_replays_match_eager(ctx):
-    """The captured hipGraph gives the eager result on every replay, with eager work between."""
-    cfg = cases.base_config(Nmesh=32, Nsample=32, Nmax=32, N_modes=4)
+@pytest.mark.parametrize("N,group", [(32, None), (64, 16)])
+def test_graph_replays_match_eager(ctx, N, group):
+    """The captured hipGraph gives the eager result on every replay, with eager work between;
+    at 64^3 with the x / y pass pairs pinned to 16-column groups (two per field, field by field,
+    and the fused z pass)."""
+    lib = aar.lib()
+    saved = lib.aar_xy_group_cols(N, aar.AAR_FP32)
+    if group is not None:
+        assert lib.aar_set_xy_group_cols(N, aar.AAR_FP32, group) == 0
+    try:
+        _graph_vs_eager(ctx, N)
+    finally:
+        lib.aar_set_xy_group_cols(N, aar.AAR_FP32, saved)
+
+
+def _graph_vs_eager(ctx, N):
+    cfg = cases.base_config(Nmesh=N, Nsample=N, Nmax=N, N_modes=4)
     alpha, x0, glass, gbox = cases.job_inputs(cfg, cases.equilateral_rows())
     t = tables_for(cfg, alpha, x0, glass, gbox)
     eager = aar.Job(ctx, aar.params_from_dict(cfg), t, aar.AAR_FP32, aar.AAR_SCHED_BATCHED)
@@ -165,6 +179,8 @@ def test_graph_replays_match_eager(ctx):
         np.testing.assert_array_equal(g.field("phi_spec"), phi0)
         eager.run()  # unrelated eager work between replays
         eager.sync()
+    g.close()
+    eager.close()
 
 
 AAR_ERR_BAD_ARG = 100001  # include/aarambam.h
