@@ -646,14 +646,15 @@ __global__ void __launch_bounds__(kGaussRows)
       const double kmag = ks.sphere == 1 ? sqrt(kmag2) : 0.0;  // the cube cut uses per-axis flags
       if (!outside_cut(ks, i, j, k, kmag)) {
         if constexpr (sizeof(T) == 4) {
+          // the hardware exp2 / log2 / sin / cos (a few ulp; the phase is in [0, 3 pi))
           const float k2 = (float)kmag2;
-          float a = expf(0.5f * (float)ns * logf(k2));
+          float a = __expf(0.5f * (float)ns * __logf(k2));
           // the oscillation phase w k can be large: its factor is formed in FP64
           if (ks.res) a *= (float)res_modulate(ks, 1.0, sqrt(kmag2));
-          if (!fixed_amp) a *= -logf((float)u2);
+          if (!fixed_amp) a *= -__logf((float)u2);
           const float phig = amp32 * sqrtf(a) / k2;
           float sn, cs;
-          sincosf((float)(u1 * 2 * AAR_PI + phase_shift), &sn, &cs);
+          __sincosf((float)(u1 * 2 * AAR_PI + phase_shift), &sn, &cs);
           val = cmake<C2>(phig * cs, phig * sn);
         } else {
           double phase = u1 * 2 * AAR_PI;
