@@ -942,12 +942,21 @@ int Pipeline<T>::ng_combine_acc(const cx_t<T>* phig, const double2* acc, cx_t<T>
 // ------------------------------------------------------------------ 2LPT
 template <typename RT> __device__ RT transfer_dev(const TransferDev& tf, RT k) {
   if (tf.which == 1) {  // Eisenstein & Hu no-wiggle, power.c:519-548
+    // FP32 path: the hardware exp / log (a few ulp)
+    auto ex = [](RT v) -> RT {
+      if constexpr (sizeof(RT) == 4) return __expf(v);
+      else return exp(v);
+    };
+    auto lg = [](RT v) -> RT {
+      if constexpr (sizeof(RT) == 4) return __logf(v);
+      else return log(v);
+    };
     k *= (RT)tf.kconv;
     const RT a = (RT)tf.eh_a;
-    RT gamma = a + ((RT)1 - a) / ((RT)1 + exp((RT)4 * log((RT)0.43 * k * (RT)tf.eh_s)));
+    RT gamma = a + ((RT)1 - a) / ((RT)1 + ex((RT)4 * lg((RT)0.43 * k * (RT)tf.eh_s)));
     gamma *= (RT)tf.eh_omh;
     RT q = k * (RT)tf.eh_theta2 / gamma;
-    RT L0 = log((RT)tf.eh_twoe + (RT)1.8 * q);
+    RT L0 = lg((RT)tf.eh_twoe + (RT)1.8 * q);
     RT C0 = (RT)14.2 + (RT)731. / ((RT)1 + (RT)62.5 * q);
     return L0 / (L0 + C0 * q * q);
   }
@@ -1030,7 +1039,9 @@ __global__ void __launch_bounds__(256)
   if (x + y + z == 0) {
     re = im = 0;
   } else if (!outside_cut(ks, x, y, z, kmag1)) {
-    const RT kss = pow((RT)kmag1, (RT)ks.s);
+    RT kss;
+    if constexpr (sizeof(T) == 4) kss = __expf((float)ks.s * __logf((float)kmag1));  // FP32: hardware exp / log
+    else kss = pow(kmag1, ks.s);
     LegRecT<RT> leg;
     leg.init(-1 + 2 * (kss - (RT)ks.kmin_s) / (RT)(ks.kmax_s - ks.kmin_s));
     double nr = 0, ni = 0;
