@@ -590,10 +590,10 @@ static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, double* sec, i
     if (chosen) chosen[axis] = bv;
   }
   // then the column groups of the x / y pass pairs with those shapes: whole passes over the nf
-  // fields, or field by field in halves or quarters of the kz columns
+  // fields, or field by field over all kz columns, halves or quarters
   double best = 1e30;
   int bg = 0;
-  for (int gc : {0, g.H / 2, g.H / 4}) {
+  for (int gc : {0, g.H, g.H / 2, g.H / 4}) {
     if (rc || (gc > 0 && gc < 16)) break;
     int r = Pipeline<T>::xy_pair(d, nf, g, tw, +1, ctx->stream, gc);
     double t = 1e30;

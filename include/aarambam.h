@@ -306,8 +306,8 @@ int aar_server_destroy(aar_server *s);
  * of an n^3 grid and keeps the fastest per axis for later jobs.  Every shape
  * uses the same radix plan, so the choice never changes results.  A job's
  * first aar_job_prepare at n >= 256 tunes by itself on its own buffers.
- * Then it times the column groups of the x / y pass pairs (whole passes, halves, quarters)
- * and keeps the fastest (aar_xy_group_cols).
+ * Then it times the column groups of the x / y pass pairs (whole passes over all fields, or
+ * field by field over all columns, halves, quarters) and keeps the fastest (aar_xy_group_cols).
  * seconds: optional [2][4] mean seconds per launch (-1: shape not built for n);
  * variant: optional [2] chosen shape per axis. */
 int aar_autotune(aar_ctx *ctx, int n, int precision, int nf, double *seconds, int *variant);
