@@ -273,11 +273,31 @@ def lpt_fields(cfg, phi_spec):
     return [o.reshape(N, N, 2 * (N // 2 + 1))[:, :, :N].copy() for o in outs]
 
 
-def glass_tile(cfg, glass, glass_box):
+def glass_tile(cfg, glass, glass_box, with_ids=False):
     p = make_params(cfg)
     glass = np.ascontiguousarray(glass, dtype=np.float32).reshape(-1)
     ng = glass.size // 3
     npart = cfg["GlassTileFac"] ** 3 * ng
     pos = np.empty(3 * npart, np.float32)
-    lib().orc_glass_tile(C.byref(p), glass, ng, float(glass_box), pos, None)
-    return pos.reshape(-1, 3)
+    ids = np.zeros(npart, np.int32)
+    lib().orc_glass_tile(C.byref(p), glass, ng, float(glass_box), pos, ids.ctypes.data_as(C.c_void_p))
+    return (pos.reshape(-1, 3), ids) if with_ids else pos.reshape(-1, 3)
+
+
+def particles(cfg, pos, disp, disp2):
+    """CIC readout and update (main.c:1440-1516) of float32 positions (n, 3) from six real
+    (N, N, N) fields: disp x, y, z and disp2 x, y, z (disp2 not divided by N^3).
+    Returns (new positions, velocities)."""
+    N = cfg["Nmesh"]
+    p = make_params(cfg)
+    c = OrcCosmo(**cosmology(cfg))
+    pad = []
+    for f in list(disp) + list(disp2):
+        a = np.zeros((N, N, 2 * (N // 2 + 1)), np.float64)
+        a[:, :, :N] = f
+        pad.append(a)
+    ptrs = lambda fs: (C.c_void_p * 3)(*[f.ctypes.data for f in fs])
+    P = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1).copy()
+    V = np.zeros_like(P)
+    lib().orc_particles(C.byref(p), C.byref(c), P.size // 3, ptrs(pad[:3]), ptrs(pad[3:]), P, V)
+    return P.reshape(-1, 3), V.reshape(-1, 3)
