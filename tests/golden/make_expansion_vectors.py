@@ -5,7 +5,7 @@ The OMP loop of Basis.basis_expansion (Aarambam/basis/cmbbest.pyx:320-490) picks
 modes by the ten largest residual correlations and the smallest rise of a
 condition number: no closed form fixes which modes it adds, so these vectors
 are the only pin of that order.  The reference module is built from its sources
-by oracle/build_ref_cmbbest.py into oracle/_ref/ (`make -C oracle ref`); this
+by oracle/build_ref_cmbbest.py into oracle/_ref/ (`make -C oracle ref_cmbbest`); this
 script imports it with the reference's decompose.py and the ScalarI model
 (Aarambam/models/Colliders.py:4-72) and writes:
 
