@@ -212,3 +212,26 @@ def write_lattice_glass(path, n=64, box=100.0):
 
 # sha256 of Aarambam/defaults/dummy_glass_dmonly_64.dat
 REFERENCE_GLASS_SHA256 = "684dc7e1ce49983c3948cca40abc320726737090469df96c55574bdc7c6f4579"
+
+
+def scalari16_rows():
+    """AlphaTable rows of ScalarI (mass 1) decomposed into 16 basis modes (n_s 0.9649, L 2000,
+    Nmax 512, ModeTol 1e-4, MaxModeCount 150), written by the reference's own decomposition
+    (tests/golden/make_expansion_vectors.py)."""
+    return np.loadtxt(os.path.join(GOLDEN, "scalari16_alphatable.txt"), comments="#").reshape(-1, 4)
+
+
+def example_config(N=512, Nsample=None, Nmax=None, Box=1000.0, **kw):
+    """The reference's example run (examples/MakeICs.py:11-25, 52-60 with utils.py:9-58): the
+    shipped 15-mode ScalarI tables, tabulated CAMB transfer, OmegaBaryon 0, cube cut at
+    Nsample = Nmesh / 2 with Nmax = 256, a 64^3 glass tiled 4 times (256^3 particles on a
+    512^3 mesh), seed 42.  Smaller N scale Nsample, Nmax and the box with it (the glass is
+    then an (N/8)^3 lattice)."""
+    tk, tt = load_transfer()
+    f = N / 512
+    cfg = base_config(Nmesh=N, Nsample=Nsample or N // 2, Nmax=Nmax or int(256 * f), N_modes=15,
+                      Box=Box * f, GlassTileFac=4, Seed=42, Omega=0.3175, OmegaLambda=1 - 0.3175,
+                      OmegaBaryon=0.0, HubbleParam=0.6711, Sigma8=0.834, PrimordialIndex=0.9660499,
+                      Redshift=127.0, Fnl=100.0, WhichTransfer=2, transfer_k=tk, transfer_t=tt)
+    cfg.update(kw)
+    return cfg

@@ -188,7 +188,7 @@ AAR_ERR_BAD_ARG = 100001  # include/aarambam.h
 
 def test_xy_pair_column_groups_give_identical_results(ctx):
     """The x / y pass pairs run over all fields at once or field by field in kz column groups
-    (96 columns at 512^3 leaves a narrower last group): bit-identical 3-D FFTs, and a batched
+    (96 columns at 512^3 leaves a narrower last group; 256 is a whole field): bit-identical 3-D FFTs, and a batched
     FP64 job at 64^3 in two 16-column groups matches the same job with whole passes bitwise
     (and the oracle).  The setter rejects widths that are not multiples of 16 or exceed N/2."""
     lib = aar.lib()
@@ -197,7 +197,7 @@ def test_xy_pair_column_groups_give_identical_results(ctx):
     rng = np.random.default_rng(9)
     x = rng.standard_normal((N, N, N))
     outs = []
-    for gc in (0, 128, 96):
+    for gc in (0, 256, 128, 96):
         assert lib.aar_set_xy_group_cols(N, aar.AAR_FP32, gc) == 0
         assert lib.aar_xy_group_cols(N, aar.AAR_FP32) == gc
         y = ctx.fft3d(x, -1, aar.AAR_FP32)

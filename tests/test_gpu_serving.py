@@ -29,11 +29,17 @@ def ctx():
 def make_job(N, template, seed, fnl):
     if template == "scalari":
         cfg, rows, x0 = cases.shipped_table_config(N, Seed=seed, Fnl=fnl)
+    elif template == "scalari16":
+        cfg = cases.base_config(Nmesh=N, Nsample=N, Nmax=N, N_modes=16, Box=2000.0 * N / 512, Seed=seed, Fnl=fnl,
+                                GlassTileFac=max(1, N // 64))
+        rows, x0 = cases.scalari16_rows(), None
     else:
         nm = 8 if template == "local" else 12
-        cfg = cases.base_config(Nmesh=N, Nsample=N, Nmax=N, N_modes=nm, Seed=seed, Fnl=fnl)
+        cfg = cases.base_config(Nmesh=N, Nsample=N, Nmax=N, N_modes=nm, Seed=seed, Fnl=fnl,
+                                GlassTileFac=max(1, N // 64))
         rows, x0 = (cases.local_rows() if template == "local" else cases.equilateral_rows()), None
-    alpha, x0, glass, gbox = cases.job_inputs(cfg, rows, x0)
+    # a lattice glass of at most 64^3 particles, tiled (GlassTileFac) to N^3
+    alpha, x0, glass, gbox = cases.job_inputs(cfg, rows, x0, glass_n=N // cfg["GlassTileFac"])
     return aar.params_from_dict(cfg), aar.tables(alpha, x0, glass, gbox)
 
 
@@ -129,3 +135,36 @@ def test_kept_results_hold_their_memory_until_release(ctx):
     np.testing.assert_array_equal(srv.fetch(tb)[0], ref_b[0])
     srv.release(tb)
     srv.close()
+
+
+MIX_AT_SIZE = [  # (N, template, seed, fNL): the benchmark stream's sizes and templates, FP32 batched
+    (512, "scalari16", 101, -120.0),
+    (256, "equil", 102, 55.0),
+    (128, "local", 103, 190.0),
+    (512, "equil", 104, 80.0),
+    (256, "scalari16", 105, -30.0),
+    (128, "scalari16", 106, 12.5),
+    (256, "local", 107, -200.0),
+]
+
+
+def test_server_at_benchmark_sizes_equal_alone_and_in_the_mix(ctx):
+    """Jobs of 128^3, 256^3 and 512^3 with the local, equilateral and 16-mode ScalarI tables,
+    all submitted at once to four slots: at least three run together, and each result is
+    bit-identical to the same job run alone."""
+    specs = [s + (aar.AAR_FP32, aar.AAR_SCHED_BATCHED) for s in MIX_AT_SIZE]
+    jobs = [make_job(*s[:4]) for s in specs]
+    budget = sum(aar.lib().aar_job_bytes(p, t.c, s[4], s[5]) for (p, t), s in zip(jobs, specs)) + (1 << 26)
+    srv = aar.Server(ctx, budget, max_active=4)
+    tickets = [srv.submit(p, t, s[4], s[5], keep=True) for (p, t), s in zip(jobs, specs)]
+    for tk, s in zip(tickets, specs):
+        srv.wait(tk)
+        pos, vel = srv.fetch(tk)
+        srv.release(tk)
+        rp, rv = alone(ctx, s)
+        np.testing.assert_array_equal(pos, rp, err_msg=str(s))
+        np.testing.assert_array_equal(vel, rv, err_msg=str(s))
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(specs)
+    assert st["peak_active"] >= 3, st
