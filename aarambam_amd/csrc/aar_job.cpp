@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <mutex>
 #include <set>
@@ -695,6 +696,16 @@ static int validate(const aar_params* p, const aar_tables* t) {
   if (p->N_modes < 3 || !t->alpha || !t->x0) return AAR_ERR_BAD_ARG;
   if (!t->glass || t->nglass <= 0 || p->GlassTileFac <= 0) return AAR_ERR_BAD_ARG;
   if (p->WDM_Vtherm_On == 1 && p->WDM_On == 1) return AAR_ERR_UNSUPPORTED;
+  // The CIC gather indexes the fields with the cell of each particle's unperturbed position
+  // (main.c:1443-1461 clamps only i == N, j == N, k == N and i < 0): a glass coordinate outside
+  // [0, BoxSize] or not finite, or a box that is not positive, would index outside the grid.
+  if (!(p->Box > 0) || !std::isfinite(p->Box) || !(t->glass_box > 0) || !std::isfinite(t->glass_box))
+    return AAR_ERR_BAD_ARG;
+  const float gb = (float)t->glass_box;
+  for (size_t q = 0; q < 3 * (size_t)t->nglass; ++q) {
+    const float v = t->glass[q];
+    if (!(v >= 0.0f && v <= gb)) return AAR_ERR_BAD_ARG;  // NaN fails both
+  }
   return AAR_OK;
 }
 

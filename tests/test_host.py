@@ -253,3 +253,25 @@ int main(void) {
     subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", str(exe), str(src), "-lm"], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True, timeout=600)
     assert int(out.stdout.strip()) == 0
+
+
+def test_job_refuses_glass_outside_its_box():
+    """The CIC gather indexes the fields by each particle's unperturbed cell, so a glass
+    coordinate outside [0, BoxSize], a non-finite one or a box that is not positive is refused
+    before anything reaches the device (aar_job_bytes validates the job as aar_job_prepare does)."""
+    cfg = cases.base_config(Nmesh=32, Nsample=32, Nmax=32, N_modes=4)
+    alpha, x0, glass, gbox = cases.job_inputs(cfg)
+    p = aar.params_from_dict(cfg)
+    nbytes = lambda g, b: aar.lib().aar_job_bytes(p, api.tables(alpha, x0, g, b).c, aar.AAR_FP32,
+                                                  aar.AAR_SCHED_BATCHED)
+    assert nbytes(glass, gbox) > 0
+    edge = glass.copy()
+    edge[5] = gbox  # a coordinate on the upper face is the reference's clamped i == N case
+    assert nbytes(edge, gbox) > 0
+    for bad in (-1e-3, gbox * 1.01, np.nan, np.inf):
+        g = glass.copy()
+        g[7, 1] = bad
+        assert nbytes(g, gbox) == -1, bad
+    assert nbytes(glass, 0.0) == -1
+    assert nbytes(glass, -100.0) == -1
+    assert nbytes(glass, np.nan) == -1
