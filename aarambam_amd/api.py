@@ -60,7 +60,7 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_read_params_res", "aar_l
            "aar_autotune", "aar_xy_tile_variant", "aar_set_xy_tile_variant", "aar_xy_group_cols",
            "aar_set_xy_group_cols", "aar_job_fetch_field_box",
            "aar_job_ng_statistics", "aar_mode_bispectra_covariance", "aar_mode_QS",
-           "aar_mode_basis_expansion"]
+           "aar_mode_basis_expansion", "aar_job_reseed", "aar_ctx_cached_plans"]
 KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
 
 
@@ -108,6 +108,8 @@ def lib():
             "aar_job_bytes": (C.c_longlong, [PP, TP, C.c_int, C.c_int]),
             "aar_job_prepare": (C.c_int, [vp, PP, TP, C.c_int, C.c_int, C.POINTER(vp)]),
             "aar_job_run": (C.c_int, [vp, vp]),
+            "aar_job_reseed": (C.c_int, [vp, C.c_int, C.c_double]),
+            "aar_ctx_cached_plans": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
             "aar_job_sync": (C.c_int, [vp]),
             "aar_job_device_output": (C.c_int, [vp, C.POINTER(fp), C.POINTER(fp), C.POINTER(C.c_longlong)]),
             "aar_job_fetch": (C.c_int, [vp, fp, fp]),
@@ -279,6 +281,12 @@ class Context:
         _check(lib().aar_bench_pass(self.h, n, precision, kind, nf, iters, C.byref(s)), "aar_bench_pass")
         return s.value
 
+    def cached_plans(self):
+        """(count, device bytes) of the job tables the context keeps resident for later jobs."""
+        n, b = C.c_int(0), C.c_longlong(0)
+        _check(lib().aar_ctx_cached_plans(self.h, C.byref(n), C.byref(b)), "aar_ctx_cached_plans")
+        return n.value, b.value
+
     def autotune(self, n, precision=AAR_FP32, nf=8):
         """Time the x / y tile shapes at n^3 and keep the fastest; returns (seconds[2][4], chosen[2])."""
         sec = np.zeros(8, np.float64)
@@ -354,6 +362,14 @@ class Job:
 
     def run(self, stream=None):
         _check(lib().aar_job_run(self.h, stream), "aar_job_run")
+        return self
+
+    def reseed(self, seed, fnl=None):
+        """Make the next run a new IC of this job's tables with `seed` (and `fnl`): the seed table
+        is built now on the host, while earlier runs may still execute on the device."""
+        fnl = self.params.Fnl if fnl is None else float(fnl)
+        _check(lib().aar_job_reseed(self.h, int(seed), fnl), "aar_job_reseed")
+        self.params.Seed, self.params.Fnl = int(seed), fnl
         return self
 
     def sync(self):

@@ -5,10 +5,14 @@
 
 #include "../../include/aarambam.h"
 
-// aar_job_prepare with the job's device buffers carved from `ws` (ws_size bytes,
-// at least aar_job_bytes) and its table uploads enqueued on `up`.
+// aar_job_prepare with the job's own buffers carved from `ws` (ws_size bytes, at least
+// aar_job_ws_bytes) and its uploads enqueued on `up`.  plan_in_ws = 0: the tables come from the
+// context's plan cache (device-resident across jobs); 1: they are laid out at the start of `ws`
+// too, so nothing is allocated.
 int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, int precision, int schedule,
-                       void* ws, size_t ws_size, hipStream_t up, aar_job** out);
+                       void* ws, size_t ws_size, hipStream_t up, int plan_in_ws, aar_job** out);
+// Workspace bytes aar_job_prepare_ws carves for a job (-1 if the job is invalid).
+long long aar_job_ws_bytes(const aar_params* p, const aar_tables* t, int precision, int schedule, int plan_in_ws);
 // HIP device ordinal and stream of a context.
 int aar_ctx_device(const aar_ctx* ctx);
 hipStream_t aar_ctx_stream(const aar_ctx* ctx);

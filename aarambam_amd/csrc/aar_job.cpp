@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <set>
 #include <string>
@@ -25,6 +26,47 @@
 int aar_host_transfer_tables(const aar_params* p, const aar_tables* t, std::vector<double>& logk,
                              std::vector<double>& logT);
 
+namespace {
+
+struct PairWork {  // one (N_i, N_j) product of the pairwise schedule
+  int i, j;
+  int w_off, w_n;  // slice of the weight list
+};
+
+template <typename T> struct Exec;
+
+// The device-resident tables of a job: everything its kernels read except the seed table,
+// fNL and the job's own buffers (field slots, particles).  One plan serves every job of the
+// same parameters (Seed and Fnl aside), tables, precision and schedule: the context keeps
+// plans in a cache, so a stream of jobs that differ in seed or fNL uploads its tables once.
+// The device block is one buffer holding every table at 256-byte aligned offsets.
+struct Plan {
+  std::string key;
+  std::vector<double> alpha, x0in, tk, tt;  // the caller's tables, matched exactly on lookup
+  std::vector<float> glass_in;
+  aar_cosmo c{};
+  int M = 0, K = 0;
+  std::vector<int> modes_h, kmodes_h, groups_h;
+  std::vector<NgTerm> terms_h;
+  std::vector<PairWork> pairs;
+  std::vector<char> image;  // host image of the block
+  size_t bytes = 0;
+  void* block = nullptr;
+  bool owned = false;  // hipMalloc'd by the cache (freed with the plan)
+  size_t o_kv = 0, o_sinc = 0, o_x0 = 0, o_outc = 0, o_modes = 0, o_kmodes = 0, o_groups = 0, o_terms = 0,
+         o_weights = 0, o_fgroups = 0, o_frecs = 0, o_src_groups = 0, o_src_recs = 0, o_logk = 0, o_logT = 0,
+         o_glass = 0;
+  int nglass = 0;
+  double glass_box = 0;
+  int nt = 0;  // transfer table rows (WhichTransfer 2)
+  unsigned long long last_use = 0;
+  ~Plan() {
+    if (owned && block) hipFree(block);
+  }
+};
+
+}  // namespace
+
 struct aar_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -35,18 +77,11 @@ struct aar_ctx {
   // a job and repeat across jobs that differ only in mesh, table or seed
   std::mutex mu;
   std::map<std::string, aar_cosmo> cosmo_cache;
+  // device-resident plans (tables) by key, least recently used evicted first
+  std::mutex plan_mu;
+  std::vector<std::shared_ptr<Plan>> plans;
+  unsigned long long plan_clock = 0;
 };
-
-namespace {
-
-struct PairWork {  // one (N_i, N_j) product of the pairwise schedule
-  int i, j;
-  int w_off, w_n;  // slice of the weight list
-};
-
-template <typename T> struct Exec;
-
-}  // namespace
 
 struct aar_job {
   aar_ctx* ctx = nullptr;
@@ -108,6 +143,15 @@ struct aar_job {
   size_t ws_size = 0, ws_used = 0;
   hipStream_t up = nullptr;
   std::vector<std::vector<char>> staged;
+  // the tables this job reads (shared through the context's cache, or its own in a workspace)
+  std::shared_ptr<Plan> plan;
+  // seed tables of later runs (aar_job_reseed): built on the host into one of two pinned
+  // buffers and copied to `seeds` at the start of the next run; an event after each copy
+  // tells when its buffer may be rewritten
+  uint32_t* stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  bool stage_used[2] = {false, false};
+  int stage_next = 0, stage_pending = -1;
   // 2LPT source as a product sum of the six gradient fields, for the fused z pass
   int* src_groups = nullptr;
   NgRec4* src_recs = nullptr;
@@ -539,6 +583,7 @@ extern "C" int aar_ctx_sync(aar_ctx* c) {
 extern "C" int aar_ctx_destroy(aar_ctx* c) {
   if (!c) return AAR_OK;
   hipSetDevice(c->device);
+  c->plans.clear();  // jobs hold their plans until they are destroyed
   hipFree(c->tw32);
   hipFree(c->tw64);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -720,58 +765,284 @@ static int slots_needed(int M, int K, int sched) {
   return 2 + std::max(std::max(M, K), 9);
 }
 
-// Bytes of every device buffer aar_job_prepare allocates, in its order, each
-// rounded up to 256 bytes: the exact footprint of a job carved from a workspace.
-static long long job_footprint(const aar_params* p, const aar_tables* t, int precision, int schedule,
-                               const Schedule& s) {
-  const Geom g = make_geom(p->Nmesh);
-  const long long N = p->Nmesh, Nm = p->N_modes, cx = precision == AAR_FP32 ? 8 : 16;
-  const long long npart = (long long)p->GlassTileFac * p->GlassTileFac * p->GlassTileFac * t->nglass;
-  const bool plain = schedule == AAR_SCHED_PLAIN;
-  long long b = 0;
-  auto add = [&](long long n) { b = ((b + 255) & ~255LL) + (n ? n : 1); };
-  add(slots_needed((int)s.modes.size(), (int)s.kmodes.size(), schedule) * g.fs * cx);  // field pool
-  if (plain) add(g.fs * 16);                                                           // double accumulator
-  add(4 * N * N);                                                                      // seed table
-  add(8 * N), add(8 * N), add(8 * Nm), add(N);                                         // kv, sinc, x0, cut flags
-  add(4 * (plain ? Nm : (long long)s.modes.size()));
-  add(4 * (long long)s.kmodes.size());
-  add(4 * (plain ? 2 : (long long)s.groups.size()));
-  add((long long)sizeof(NgTerm) * (plain ? 1 : (long long)s.terms.size()));
-  add((long long)sizeof(NgWeight) * (long long)s.weights.size());
-  add(4 * (long long)s.fz.groups.size()), add((long long)sizeof(NgRec4) * (long long)s.fz.recs.size());
-  const NgFactored src = source_factored();  // 2LPT source
-  add(4 * (long long)src.groups.size()), add((long long)sizeof(NgRec4) * (long long)src.recs.size());
-  if (p->WhichTransfer == 2) {
-    std::vector<double> logk, logT;
-    if (aar_host_transfer_tables(p, t, logk, logT)) return -1;
-    add(8 * (long long)logk.size()), add(8 * (long long)logT.size());
+// The host half of a plan: schedule, k-space and transfer tables, glass, and the image of the
+// device block.  Needs no device (aar_job_bytes sizes a job with it).
+static int plan_layout(const aar_params* p, const aar_tables* t, int schedule, Plan& pl) {
+  const int N = p->Nmesh, Nm = p->N_modes;
+  Schedule s = build_schedule(*p, t->alpha);
+  pl.modes_h = s.modes;
+  pl.kmodes_h = s.kmodes;
+  pl.terms_h = s.terms;
+  pl.groups_h = s.groups;
+  pl.M = (int)s.modes.size();
+  pl.K = (int)s.kmodes.size();
+  pl.pairs = s.pairs;
+  if (pl.M > 32) return AAR_ERR_UNSUPPORTED;
+  std::vector<int> modes_dev = s.modes, groups = s.groups;
+  std::vector<NgTerm> terms = s.terms;
+  if (schedule == AAR_SCHED_PLAIN) {
+    // pairwise: fields are addressed by mode number, one product term f0*f1
+    modes_dev.resize(Nm);
+    for (int q = 0; q < Nm; ++q) modes_dev[q] = q;
+    terms = {NgTerm{0, 1, 1.0}};
+    groups = {0, 1};
   }
-  add(12LL * t->nglass);
-  add(12 * npart), add(12 * npart);  // pos, vel
-  return b;
+  // k-space tables (main.c:412-452, 1323-1343)
+  std::vector<double> kv(N), sinc(N), x0(Nm);
+  std::vector<unsigned char> outc(N);
+  for (int i = 0; i < N; ++i) {
+    kv[i] = (i < N / 2) ? i * 2 * AAR_PI / p->Box : -(N - i) * 2 * AAR_PI / p->Box;
+    outc[i] = fabs(kv[i]) * p->Box / (2 * AAR_PI) > p->Nsample / 2;
+    double f = 1;
+    if (kv[i] != 0) {
+      f = (kv[i] * p->Box / 2) / N;
+      f = sin(f) / f;
+    }
+    sinc[i] = f;
+  }
+  for (int q = 0; q < Nm; ++q) x0[q] = t->x0[2 * q];
+  std::vector<double> logk, logT;
+  int rc;
+  if (p->WhichTransfer == 2 && (rc = aar_host_transfer_tables(p, t, logk, logT))) return rc;
+  pl.nt = (int)logk.size();
+  const NgFactored src = source_factored();
+  // pack every table at a 256-byte aligned offset of one block
+  struct Part {
+    const void* src;
+    size_t n;
+    size_t* at;
+  };
+  auto bytes_of = [](const auto& v) { return sizeof(v[0]) * v.size(); };
+  const Part parts[] = {
+      {kv.data(), bytes_of(kv), &pl.o_kv},
+      {sinc.data(), bytes_of(sinc), &pl.o_sinc},
+      {x0.data(), bytes_of(x0), &pl.o_x0},
+      {outc.data(), bytes_of(outc), &pl.o_outc},
+      {modes_dev.data(), bytes_of(modes_dev), &pl.o_modes},
+      {s.kmodes.data(), bytes_of(s.kmodes), &pl.o_kmodes},
+      {groups.data(), bytes_of(groups), &pl.o_groups},
+      {terms.data(), bytes_of(terms), &pl.o_terms},
+      {s.weights.data(), bytes_of(s.weights), &pl.o_weights},
+      {s.fz.groups.data(), bytes_of(s.fz.groups), &pl.o_fgroups},
+      {s.fz.recs.data(), bytes_of(s.fz.recs), &pl.o_frecs},
+      {src.groups.data(), bytes_of(src.groups), &pl.o_src_groups},
+      {src.recs.data(), bytes_of(src.recs), &pl.o_src_recs},
+      {logk.data(), bytes_of(logk), &pl.o_logk},
+      {logT.data(), bytes_of(logT), &pl.o_logT},
+      {t->glass, sizeof(float) * 3 * (size_t)t->nglass, &pl.o_glass},
+  };
+  size_t off = 0;
+  for (const Part& q : parts) {
+    off = (off + 255) & ~(size_t)255;
+    *q.at = off;
+    off += q.n ? q.n : 1;
+  }
+  pl.bytes = (off + 255) & ~(size_t)255;
+  pl.image.assign(pl.bytes, 0);
+  for (const Part& q : parts)
+    if (q.n) memcpy(pl.image.data() + *q.at, q.src, q.n);
+  pl.nglass = t->nglass;
+  pl.glass_box = t->glass_box;
+  return AAR_OK;
+}
+
+// Bytes of a job's own device buffers (field slots, pairwise accumulator, seed table,
+// positions, velocities), each rounded up to 256 bytes.
+static size_t job_own_bytes(const aar_params* p, const aar_tables* t, int precision, int schedule, int M, int K) {
+  const Geom g = make_geom(p->Nmesh);
+  const size_t N = p->Nmesh, cx = precision == AAR_FP32 ? 8 : 16;
+  const size_t npart = (size_t)p->GlassTileFac * p->GlassTileFac * p->GlassTileFac * t->nglass;
+  size_t b = 0;
+  auto add = [&](size_t n) { b = ((b + 255) & ~(size_t)255) + (n ? n : 1); };
+  add(slots_needed(M, K, schedule) * (size_t)g.fs * cx);
+  if (schedule == AAR_SCHED_PLAIN) add((size_t)g.fs * 16);
+  add(4 * N * N);
+  add(12 * npart), add(12 * npart);
+  return (b + 255) & ~(size_t)255;
+}
+
+static std::string plan_key(const aar_params* p, const aar_tables* t, int precision, int schedule) {
+  std::string k;
+  auto put = [&](const void* v, size_t n) { k.append((const char*)v, n); };
+  const int i[] = {p->Nmesh, p->Nsample, p->Nmax, p->N_modes, p->SphereMode, p->FixedAmplitude, p->PhaseFlip,
+                   p->WhichSpectrum, p->WhichTransfer, p->GlassTileFac, p->WDM_On, p->WDM_Vtherm_On, p->Variant,
+                   precision, schedule, t->ntransfer, t->nglass};
+  const double d[] = {p->Box, p->Omega, p->OmegaLambda, p->OmegaBaryon, p->OmegaDM_2ndSpecies, p->HubbleParam,
+                      p->Sigma8, p->PrimordialIndex, p->Redshift, p->ShapeGamma, p->UnitLength_in_cm,
+                      p->UnitMass_in_g, p->UnitVelocity_in_cm_per_s, p->InputSpectrum_UnitLength_in_cm,
+                      p->WDM_PartMass_in_kev, p->A_res, p->w_res, p->phi_res, t->glass_box};
+  put(i, sizeof i);
+  put(d, sizeof d);
+  return k;
+}
+
+static bool plan_matches(const Plan& pl, const std::string& key, const aar_params* p, const aar_tables* t) {
+  const size_t nm = (size_t)p->N_modes;
+  auto same = [](const auto& v, const auto* x, size_t n) {
+    return v.size() == n && (n == 0 || memcmp(v.data(), x, sizeof(v[0]) * n) == 0);
+  };
+  const bool tab = p->WhichTransfer == 2 && t->ntransfer > 0 && t->transfer_k && t->transfer_t;
+  return pl.key == key && same(pl.alpha, t->alpha, nm * nm * nm) && same(pl.x0in, t->x0, 2 * nm) &&
+         (tab ? same(pl.tk, t->transfer_k, (size_t)t->ntransfer) && same(pl.tt, t->transfer_t, (size_t)t->ntransfer)
+              : pl.tk.empty()) &&
+         same(pl.glass_in, t->glass, 3 * (size_t)t->nglass);
+}
+
+// Device views of a plan's block and the kernels' k-space and transfer parameters.
+static void plan_views(const aar_params* p, Plan& pl, aar_job* j) {
+  char* b = (char*)pl.block;
+  j->kv = (double*)(b + pl.o_kv);
+  j->sinc = (double*)(b + pl.o_sinc);
+  j->x0 = (double*)(b + pl.o_x0);
+  j->outc = (unsigned char*)(b + pl.o_outc);
+  j->modes = (int*)(b + pl.o_modes);
+  j->kmodes = (int*)(b + pl.o_kmodes);
+  j->groups = (int*)(b + pl.o_groups);
+  j->terms = (NgTerm*)(b + pl.o_terms);
+  j->weights = (NgWeight*)(b + pl.o_weights);
+  j->fgroups = (int*)(b + pl.o_fgroups);
+  j->frecs = (NgRec4*)(b + pl.o_frecs);
+  j->src_groups = (int*)(b + pl.o_src_groups);
+  j->src_recs = (NgRec4*)(b + pl.o_src_recs);
+  j->logk = pl.nt ? (double*)(b + pl.o_logk) : nullptr;
+  j->logT = pl.nt ? (double*)(b + pl.o_logT) : nullptr;
+  j->glass = (float*)(b + pl.o_glass);
+  j->nglass = pl.nglass;
+  j->glass_box = pl.glass_box;
+  j->c = pl.c;
+  j->M = pl.M;
+  j->K = pl.K;
+  j->modes_h = pl.modes_h;
+  j->kmodes_h = pl.kmodes_h;
+  j->terms_h = pl.terms_h;
+  j->groups_h = pl.groups_h;
+  j->pairs = pl.pairs;
+  KSpace& ks = j->ks;
+  ks.kv = j->kv;
+  ks.outc = j->outc;
+  ks.sinc = j->sinc;
+  ks.sphere = p->SphereMode;
+  ks.box = p->Box;
+  ks.nsh = (double)(p->Nsample / 2);
+  ks.s = (4 - p->PrimordialIndex) / 3;
+  const double kmin = 2 * AAR_PI / p->Box, kmax = 2 * AAR_PI / p->Box * p->Nmax;
+  ks.kmin_s = pow(kmin, (4 - p->PrimordialIndex) / 3);
+  ks.kmax_s = pow(kmax, (4 - p->PrimordialIndex) / 3);
+  ks.x0 = j->x0;
+  ks.res = p->Variant != 1 ? 0 : (p->WhichSpectrum == 2 ? 2 : 3);
+  ks.res_A = p->A_res;
+  ks.res_w = p->w_res;
+  ks.res_phi = p->phi_res;
+  TransferDev& tf = j->tf;
+  tf = TransferDev{};
+  tf.which = p->WhichTransfer;
+  if (p->WhichTransfer == 1) {  // k-independent part of tk_eh (power.c:526-540)
+    const double hubble = p->HubbleParam, omegam = p->Omega;
+    double ombh2 = p->OmegaBaryon * p->HubbleParam * p->HubbleParam;
+    if (p->OmegaBaryon == 0) ombh2 = 0.04 * p->HubbleParam * p->HubbleParam;
+    const double theta = 2.728 / 2.7, ommh2 = omegam * hubble * hubble;
+    tf.kconv = 3.085678e24 / p->UnitLength_in_cm;
+    tf.eh_s = 44.5 * log(9.83 / ommh2) / sqrt(1. + 10. * exp(0.75 * log(ombh2))) * hubble;
+    tf.eh_a = 1. - 0.328 * log(431. * ommh2) * ombh2 / ommh2 +
+              0.380 * log(22.3 * ommh2) * (ombh2 / ommh2) * (ombh2 / ommh2);
+    tf.eh_theta2 = theta * theta;
+    tf.eh_omh = omegam * hubble;
+    tf.eh_twoe = 2. * exp(1.);
+  } else if (p->WhichTransfer == 2) {
+    tf.kconv = p->InputSpectrum_UnitLength_in_cm / p->UnitLength_in_cm;
+    tf.nt = pl.nt;
+    tf.logk = j->logk;
+    tf.logT = j->logT;
+  }
+}
+
+// The context's plan for (p, t, precision, schedule): found in its cache, or laid out,
+// allocated as one block and uploaded.  At most kPlanCache plans stay cached; the least
+// recently used plan that no job holds is dropped first.
+static constexpr size_t kPlanCache = 32;
+
+static int ctx_plan(aar_ctx* ctx, const aar_params* p, const aar_tables* t, int precision, int schedule,
+                    std::shared_ptr<Plan>* out) {
+  const std::string key = plan_key(p, t, precision, schedule);
+  {
+    std::lock_guard<std::mutex> g(ctx->plan_mu);
+    for (auto& pl : ctx->plans)
+      if (plan_matches(*pl, key, p, t)) {
+        pl->last_use = ++ctx->plan_clock;
+        *out = pl;
+        return AAR_OK;
+      }
+  }
+  auto pl = std::make_shared<Plan>();
+  int rc = ctx_cosmology(ctx, p, t, &pl->c);
+  if (!rc) rc = plan_layout(p, t, schedule, *pl);
+  if (rc) return rc;
+  pl->key = key;
+  const size_t nm = (size_t)p->N_modes;
+  pl->alpha.assign(t->alpha, t->alpha + nm * nm * nm);
+  pl->x0in.assign(t->x0, t->x0 + 2 * nm);
+  if (p->WhichTransfer == 2 && t->ntransfer > 0 && t->transfer_k && t->transfer_t) {
+    pl->tk.assign(t->transfer_k, t->transfer_k + t->ntransfer);
+    pl->tt.assign(t->transfer_t, t->transfer_t + t->ntransfer);
+  }
+  pl->glass_in.assign(t->glass, t->glass + 3 * (size_t)t->nglass);
+  if ((rc = device_alloc(&pl->block, pl->bytes))) return rc;
+  pl->owned = true;
+  if (hipMemcpy(pl->block, pl->image.data(), pl->bytes, hipMemcpyHostToDevice) != hipSuccess) return AAR_ERR_HIP;
+  std::vector<char>().swap(pl->image);  // the device holds it now
+  std::lock_guard<std::mutex> g(ctx->plan_mu);
+  for (auto& other : ctx->plans)  // built meanwhile by another thread: use that one
+    if (plan_matches(*other, key, p, t)) {
+      other->last_use = ++ctx->plan_clock;
+      *out = other;
+      return AAR_OK;
+    }
+  pl->last_use = ++ctx->plan_clock;
+  ctx->plans.push_back(pl);
+  while (ctx->plans.size() > kPlanCache) {
+    size_t victim = ctx->plans.size();
+    for (size_t q = 0; q < ctx->plans.size(); ++q)
+      if (ctx->plans[q].use_count() == 1 &&
+          (victim == ctx->plans.size() || ctx->plans[q]->last_use < ctx->plans[victim]->last_use))
+        victim = q;
+    if (victim == ctx->plans.size()) break;  // every cached plan is held by a job
+    ctx->plans.erase(ctx->plans.begin() + victim);
+  }
+  *out = pl;
+  return AAR_OK;
 }
 
 extern "C" long long aar_job_bytes(const aar_params* p, const aar_tables* t, int precision, int schedule) {
   if (validate(p, t)) return -1;
-  return job_footprint(p, t, precision, schedule, build_schedule(*p, t->alpha));
+  Plan pl;
+  if (plan_layout(p, t, schedule, pl)) return -1;
+  return (long long)(pl.bytes + job_own_bytes(p, t, precision, schedule, pl.M, pl.K));
+}
+
+long long aar_job_ws_bytes(const aar_params* p, const aar_tables* t, int precision, int schedule, int plan_in_ws) {
+  if (plan_in_ws) return aar_job_bytes(p, t, precision, schedule);
+  if (validate(p, t)) return -1;
+  const Schedule s = build_schedule(*p, t->alpha);
+  return (long long)job_own_bytes(p, t, precision, schedule, (int)s.modes.size(), (int)s.kmodes.size());
 }
 
 extern "C" int aar_job_prepare(aar_ctx* ctx, const aar_params* p, const aar_tables* t, int precision,
                                int schedule, aar_job** out) {
-  return aar_job_prepare_ws(ctx, p, t, precision, schedule, nullptr, 0, nullptr, out);
+  return aar_job_prepare_ws(ctx, p, t, precision, schedule, nullptr, 0, nullptr, 0, out);
 }
 
-// Prepare a job whose device buffers are carved from `ws` (ws_size bytes,
-// at least aar_job_bytes) and whose table uploads are enqueued on `up`;
-// ws = nullptr gives the job its own hipMalloc'd buffers.
+// Prepare a job.  ws = nullptr: the job's own buffers are hipMalloc'd; otherwise they are carved
+// from `ws` (ws_size bytes, at least aar_job_ws_bytes) and the uploads are enqueued on `up`.
+// plan_in_ws = 0: the tables come from the context's plan cache; 1: the job lays out its own
+// tables at the start of `ws` (no allocation anywhere).
 int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, int precision, int schedule,
-                       void* ws, size_t ws_size, hipStream_t up, aar_job** out) {
+                       void* ws, size_t ws_size, hipStream_t up, int plan_in_ws, aar_job** out) {
   if (!ctx || !out) return AAR_ERR_BAD_ARG;
   int rc = validate(p, t);
   if (rc) return rc;
   if (precision != AAR_FP32 && precision != AAR_FP64) return AAR_ERR_BAD_ARG;
   if (schedule != AAR_SCHED_PLAIN && schedule != AAR_SCHED_BATCHED) return AAR_ERR_BAD_ARG;
+  if (plan_in_ws && !ws) return AAR_ERR_BAD_ARG;
   AAR_HIP(hipSetDevice(ctx->device));
   aar_job* j = new aar_job();
   j->ctx = ctx;
@@ -787,44 +1058,23 @@ int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, i
     aar_job_destroy(j);
     return code;
   };
-  if ((rc = ctx_cosmology(ctx, p, t, &j->c))) return fail(rc);
   const int N = p->Nmesh;
-  // schedule
-  Schedule s = build_schedule(*p, t->alpha);
-  j->modes_h = s.modes;
-  j->kmodes_h = s.kmodes;
-  j->terms_h = s.terms;
-  j->groups_h = s.groups;
-  j->M = (int)s.modes.size();
-  j->K = (int)s.kmodes.size();
-  j->pairs = s.pairs;
-  if (j->M > 32) return fail(AAR_ERR_UNSUPPORTED);
-  std::vector<int> modes_dev = s.modes, groups = s.groups;
-  std::vector<NgTerm> terms = s.terms;
-  if (schedule == AAR_SCHED_PLAIN) {
-    // pairwise: fields are addressed by mode number, one product term f0*f1
-    modes_dev.resize(N > 0 ? p->N_modes : 0);
-    for (int q = 0; q < p->N_modes; ++q) modes_dev[q] = q;
-    terms = {NgTerm{0, 1, 1.0}};
-    groups = {0, 1};
-  }
-  // k-space tables (main.c:412-452, 1323-1343)
-  std::vector<double> kv(N), sinc(N), x0(p->N_modes);
-  std::vector<unsigned char> outc(N);
-  for (int i = 0; i < N; ++i) {
-    kv[i] = (i < N / 2) ? i * 2 * AAR_PI / p->Box : -(N - i) * 2 * AAR_PI / p->Box;
-    outc[i] = fabs(kv[i]) * p->Box / (2 * AAR_PI) > p->Nsample / 2;
-    double f = 1;
-    if (kv[i] != 0) {
-      f = (kv[i] * p->Box / 2) / N;
-      f = sin(f) / f;
+  if (plan_in_ws) {
+    auto pl = std::make_shared<Plan>();
+    if ((rc = ctx_cosmology(ctx, p, t, &pl->c)) || (rc = plan_layout(p, t, schedule, *pl))) return fail(rc);
+    if ((rc = job_alloc(j, &pl->block, pl->bytes))) return fail(rc);
+    if (up) {  // the image stays alive with the plan until the job is destroyed
+      if (hipMemcpyAsync(pl->block, pl->image.data(), pl->bytes, hipMemcpyHostToDevice, up) != hipSuccess)
+        return fail(AAR_ERR_HIP);
+    } else if (hipMemcpy(pl->block, pl->image.data(), pl->bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      return fail(AAR_ERR_HIP);
     }
-    sinc[i] = f;
+    j->plan = pl;
+  } else if ((rc = ctx_plan(ctx, p, t, precision, schedule, &j->plan))) {
+    return fail(rc);
   }
-  for (int q = 0; q < p->N_modes; ++q) x0[q] = t->x0[2 * q];
-  std::vector<double> logk, logT;
-  if (p->WhichTransfer == 2 && (rc = aar_host_transfer_tables(p, t, logk, logT))) return fail(rc);
-  // device buffers
+  plan_views(p, *j->plan, j);
+  // the job's own buffers
   j->nslots = slots_needed(j->M, j->K, schedule);
   if ((rc = job_alloc(j, &j->pool, (size_t)j->nslots * j->g.fs * j->cxsize))) return fail(rc);
   if (schedule == AAR_SCHED_PLAIN && (rc = job_alloc(j, (void**)&j->acc, (size_t)j->g.fs * 16)))
@@ -843,68 +1093,58 @@ int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, i
   std::vector<uint32_t> seeds((size_t)N * N);
   aar_seedtable(N, p->Seed, seeds.data());
   if ((rc = job_upload(j, &j->seeds, seeds))) return fail(rc);
-  if ((rc = job_upload(j, &j->kv, kv)) || (rc = job_upload(j, &j->sinc, sinc)) || (rc = job_upload(j, &j->x0, x0)) ||
-      (rc = job_upload(j, &j->outc, outc)) || (rc = job_upload(j, &j->modes, modes_dev)) ||
-      (rc = job_upload(j, &j->kmodes, s.kmodes)) || (rc = job_upload(j, &j->groups, groups)) ||
-      (rc = job_upload(j, &j->terms, terms)) || (rc = job_upload(j, &j->weights, s.weights)) ||
-      (rc = job_upload(j, &j->fgroups, s.fz.groups)) || (rc = job_upload(j, &j->frecs, s.fz.recs)))
-    return fail(rc);
-  const NgFactored src = source_factored();
-  if ((rc = job_upload(j, &j->src_groups, src.groups)) || (rc = job_upload(j, &j->src_recs, src.recs)))
-    return fail(rc);
   j->fused_lpt = schedule == AAR_SCHED_BATCHED && N >= 64;
   j->s_disp = j->s_reg + 6;
   j->s_disp2 = j->s_reg + (j->fused_lpt ? 1 : 0);
-  if (p->WhichTransfer == 2 && ((rc = job_upload(j, &j->logk, logk)) || (rc = job_upload(j, &j->logT, logT))))
-    return fail(rc);
-  j->nglass = t->nglass;
-  j->glass_box = t->glass_box;
-  std::vector<float> gl(t->glass, t->glass + 3 * (size_t)t->nglass);
-  if ((rc = job_upload(j, &j->glass, gl))) return fail(rc);
   j->npart = (long long)p->GlassTileFac * p->GlassTileFac * p->GlassTileFac * t->nglass;
   if ((rc = job_alloc(j, (void**)&j->pos, sizeof(float) * 3 * j->npart)) ||
       (rc = job_alloc(j, (void**)&j->vel, sizeof(float) * 3 * j->npart)))
     return fail(rc);
-  // kernel-side views
-  KSpace& ks = j->ks;
-  ks.kv = j->kv;
-  ks.outc = j->outc;
-  ks.sinc = j->sinc;
-  ks.sphere = p->SphereMode;
-  ks.box = p->Box;
-  ks.nsh = (double)(p->Nsample / 2);
-  ks.s = (4 - p->PrimordialIndex) / 3;
-  const double kmin = 2 * AAR_PI / p->Box, kmax = 2 * AAR_PI / p->Box * p->Nmax;
-  ks.kmin_s = pow(kmin, (4 - p->PrimordialIndex) / 3);
-  ks.kmax_s = pow(kmax, (4 - p->PrimordialIndex) / 3);
-  ks.x0 = j->x0;
-  ks.res = p->Variant != 1 ? 0 : (p->WhichSpectrum == 2 ? 2 : 3);
-  ks.res_A = p->A_res;
-  ks.res_w = p->w_res;
-  ks.res_phi = p->phi_res;
-  TransferDev& tf = j->tf;
-  tf.which = p->WhichTransfer;
-  if (p->WhichTransfer == 1) {  // k-independent part of tk_eh (power.c:526-540)
-    const double hubble = p->HubbleParam, omegam = p->Omega;
-    double ombh2 = p->OmegaBaryon * p->HubbleParam * p->HubbleParam;
-    if (p->OmegaBaryon == 0) ombh2 = 0.04 * p->HubbleParam * p->HubbleParam;
-    const double theta = 2.728 / 2.7, ommh2 = omegam * hubble * hubble;
-    tf.kconv = 3.085678e24 / p->UnitLength_in_cm;
-    tf.eh_s = 44.5 * log(9.83 / ommh2) / sqrt(1. + 10. * exp(0.75 * log(ombh2))) * hubble;
-    tf.eh_a = 1. - 0.328 * log(431. * ommh2) * ombh2 / ommh2 +
-              0.380 * log(22.3 * ommh2) * (ombh2 / ommh2) * (ombh2 / ommh2);
-    tf.eh_theta2 = theta * theta;
-    tf.eh_omh = omegam * hubble;
-    tf.eh_twoe = 2. * exp(1.);
-  } else if (p->WhichTransfer == 2) {
-    tf.kconv = p->InputSpectrum_UnitLength_in_cm / p->UnitLength_in_cm;
-    tf.nt = (int)logk.size();
-    tf.logk = j->logk;
-    tf.logT = j->logT;
-  }
   AAR_HIP(hipEventCreate(&j->ev0));
   AAR_HIP(hipEventCreate(&j->ev1));
   *out = j;
+  return AAR_OK;
+}
+
+static void drop_graph(aar_job* j);
+
+extern "C" int aar_job_reseed(aar_job* j, int seed, double fnl) {
+  if (!j || j->ws) return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(j->ctx->device));
+  const size_t n = (size_t)j->g.N * j->g.N;
+  if (!j->stage[0]) {
+    for (int b = 0; b < 2; ++b) {
+      if (hipHostMalloc((void**)&j->stage[b], n * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+        j->stage[b] = nullptr;
+        return AAR_ERR_ALLOC;
+      }
+      AAR_HIP(hipEventCreateWithFlags(&j->stage_ev[b], hipEventDisableTiming));
+    }
+  }
+  const int b = j->stage_next;
+  if (j->stage_pending == b) return AAR_ERR_BAD_ARG;  // cannot happen: two buffers alternate
+  if (j->stage_used[b]) AAR_HIP(hipEventSynchronize(j->stage_ev[b]));  // its last copy has finished
+  aar_seedtable(j->g.N, seed, j->stage[b]);
+  j->stage_pending = b;
+  j->stage_next = b ^ 1;
+  j->p.Seed = seed;
+  if (fnl != j->p.Fnl) {
+    j->p.Fnl = fnl;
+    if (j->graph_exec) {  // fNL is a kernel argument of the captured graph: capture again
+      if (j->ran) AAR_HIP(hipEventSynchronize(j->ev1));
+      drop_graph(j);
+    }
+  }
+  return AAR_OK;
+}
+
+extern "C" int aar_ctx_cached_plans(aar_ctx* ctx, int* count, long long* bytes) {
+  if (!ctx) return AAR_ERR_BAD_ARG;
+  std::lock_guard<std::mutex> g(ctx->plan_mu);
+  long long b = 0;
+  for (auto& pl : ctx->plans) b += (long long)pl->bytes;
+  if (count) *count = (int)ctx->plans.size();
+  if (bytes) *bytes = b;
   return AAR_OK;
 }
 
@@ -912,7 +1152,13 @@ extern "C" int aar_job_run(aar_job* j, void* stream) {
   if (!j) return AAR_ERR_BAD_ARG;
   AAR_HIP(hipSetDevice(j->ctx->device));
   hipStream_t st = stream ? (hipStream_t)stream : j->ctx->stream;
-  j->recs.clear();
+  if (j->stage_pending >= 0) {  // the seed table of aar_job_reseed, ahead of the job's kernels
+    const int b = j->stage_pending;
+    AAR_HIP(hipMemcpyAsync(j->seeds, j->stage[b], sizeof(uint32_t) * j->g.N * j->g.N, hipMemcpyHostToDevice, st));
+    AAR_HIP(hipEventRecord(j->stage_ev[b], st));
+    j->stage_used[b] = true;
+    j->stage_pending = -1;
+  }
   AAR_HIP(hipEventRecord(j->ev0, st));
   if (j->use_graph) {
     if (!j->graph_exec) {  // capture every launch of the job once, then replay
@@ -935,7 +1181,7 @@ extern "C" int aar_job_run(aar_job* j, void* stream) {
     }
     AAR_HIP(hipGraphLaunch(j->graph_exec, st));
   } else {
-    j->evnext = 0;
+    // per-kind events accumulate over runs until aar_job_sync reads them
     int rc = j->prec == AAR_FP32 ? Exec<float>::run(j, st) : Exec<double>::run(j, st);
     if (rc) return rc;
   }
@@ -961,6 +1207,7 @@ extern "C" int aar_job_sync(aar_job* j) {
     j->k_bytes[r.kind] += r.bytes;
   }
   j->recs.clear();
+  j->evnext = 0;
   return AAR_OK;
 }
 
@@ -982,6 +1229,8 @@ extern "C" int aar_job_set_graph(aar_job* j, int on) {
 extern "C" int aar_job_set_profiling(aar_job* j, int on) {
   if (!j) return AAR_ERR_BAD_ARG;
   j->profiling = on != 0;
+  j->recs.clear();  // launches not yet read by aar_job_sync are not counted
+  j->evnext = 0;
   for (int k = 0; k < AAR_NKINDS; ++k) {
     j->k_launch[k] = 0;
     j->k_sec[k] = 0;
@@ -1117,16 +1366,21 @@ extern "C" int aar_job_destroy(aar_job* j) {
   if (!j) return AAR_OK;
   hipSetDevice(j->ctx->device);
   if (j->ran) hipEventSynchronize(j->ev1);
-  if (!j->ws) {
-    void* bufs[] = {j->pool, j->seeds, j->kv, j->sinc, j->x0, j->logk, j->logT, j->outc, j->modes,
-                    j->kmodes, j->groups, j->terms, j->weights, j->fgroups, j->frecs,
-                    j->src_groups, j->src_recs, j->acc, j->glass,
-                    j->pos, j->vel};
+  if (!j->ws) {  // the job's own buffers; the tables belong to the plan
+    void* bufs[] = {j->pool, j->seeds, j->acc, j->pos, j->vel};
     for (void* b : bufs)
       if (b) hipFree(b);
   } else if (j->up && !j->ran) {
     hipStreamSynchronize(j->up);  // uploads from the staged host copies may still be in flight
   }
+  for (int b = 0; b < 2; ++b) {
+    if (j->stage_ev[b]) {
+      hipEventSynchronize(j->stage_ev[b]);
+      hipEventDestroy(j->stage_ev[b]);
+    }
+    if (j->stage[b]) hipHostFree(j->stage[b]);
+  }
+  j->plan.reset();
   if (j->graph_exec) hipGraphExecDestroy(j->graph_exec);
   if (j->graph) hipGraphDestroy(j->graph);
   for (hipEvent_t e : j->evpool) hipEventDestroy(e);

@@ -1,19 +1,27 @@
 // Serving loop: continuous batching of IC jobs on one device.
 //
 // The reference runs one initial-condition job per process (main.c:53-102).
-// A server instead keeps one context (twiddle tables, streams) and one HBM
-// arena resident and accepts a stream of jobs of mixed Nmesh, table and seed:
-//   * admission is FIFO by footprint: the job at the head of the queue starts
-//     as soon as a stream slot is free and the arena has a contiguous piece of
-//     aar_job_bytes() for it (first fit, neighbours merged on free);
-//   * an admitted job is prepared inside its piece of the arena (no hipMalloc,
-//     so no device-wide synchronisation) and enqueued on its slot's stream, so
-//     the kernels of up to max_active jobs overlap on the device;
+// A server instead keeps one context (twiddle tables, the plan cache of
+// device-resident tables, streams) and one HBM arena resident and accepts a
+// stream of jobs of mixed Nmesh, table and seed:
+//   * admission is FIFO by footprint: the job at the head of the queue is
+//     admitted as soon as a stream slot is free and the arena has a contiguous
+//     piece for the job's own buffers (field slots, seed table, particles;
+//     first fit, neighbours merged on free);
+//   * the tables of a job (k-space tables, modal schedule, transfer table,
+//     glass) come from the context's plan cache, so jobs that share them with
+//     an earlier job upload nothing but their seed table;
+//   * admitted jobs are prepared by a small pool of worker threads (the host
+//     seed table, the uploads) and enqueued on their slot's stream, so the
+//     admission thread never waits on host work and the kernels of up to
+//     max_active jobs overlap on the device;
 //   * completion is signalled by a host function enqueued after the job's last
 //     kernel, which timestamps it and wakes the scheduler to admit more work.
-// Latency of a job = completion time - submission time.
+// Latency of a job = completion time - submission time; queueing = admission
+// time - submission time.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -49,6 +57,7 @@ struct Req {
   int status = AAR_OK;
   enum State { PENDING, RUNNING, DONE } state = PENDING;
   bool finished = false;  // set by the completion host function
+  bool prepared = false;  // the worker has handed over the job (or its failure)
   double t_submit = 0, t_start = 0, t_done = 0;
   aar_server* srv = nullptr;
 };
@@ -67,10 +76,12 @@ struct aar_server {
   std::mutex mu;
   std::condition_variable cv;
   std::deque<Req*> pending;
-  std::vector<Req*> running;
+  std::deque<Req*> admitted;  // admitted, waiting for a preparing worker
+  std::vector<Req*> running;  // admitted and not yet retired
   std::unordered_map<long long, Req*> reqs;
   std::thread th;
-  bool stop = false;
+  std::vector<std::thread> workers;
+  bool stop = false, stop_workers = false;
   long long next_id = 1, submitted = 0, completed = 0, peak_bytes = 0, in_use = 0;
   int peak_active = 0;
 
@@ -103,6 +114,7 @@ struct aar_server {
   }
 
   void loop();
+  void worker();
 };
 
 namespace {
@@ -129,7 +141,7 @@ void aar_server::loop() {
     // retire finished jobs
     for (size_t q = 0; q < running.size();) {
       Req* r = running[q];
-      if (!r->finished) {
+      if (!r->finished || !r->prepared) {
         ++q;
         continue;
       }
@@ -164,28 +176,46 @@ void aar_server::loop() {
       r->slot = slot;
       r->off = off;
       r->state = Req::RUNNING;
+      r->t_start = now_s();
       running.push_back(r);
       in_use += (long long)r->bytes;
       if (in_use > peak_bytes) peak_bytes = in_use;
       if ((int)running.size() > peak_active) peak_active = (int)running.size();
-      lk.unlock();
-      hipStream_t st = streams[slot];
-      int rc = aar_job_prepare_ws(ctx, &r->p, &r->t, r->prec, r->sched, arena + off, r->bytes, st, &r->job);
-      if (!rc) rc = aar_job_run(r->job, st);
-      if (!rc && hipLaunchHostFunc(st, on_done, r) != hipSuccess) rc = AAR_ERR_HIP;
-      if (rc) hipStreamSynchronize(st);  // nothing of a failed job may still use its piece
-      lk.lock();
-      r->t_start = now_s();
-      if (rc) {
-        r->status = rc;
-        r->finished = true;
-        r->t_done = r->t_start;
-      }
+      admitted.push_back(r);
+      cv.notify_all();
     }
     if (stop && running.empty()) break;
     bool work = false;
-    for (Req* r : running) work |= r->finished;
+    for (Req* r : running) work |= r->finished && r->prepared;
     if (!work) cv.wait(lk);
+  }
+}
+
+// Prepares admitted jobs inside their arena pieces and enqueues them on their slots' streams.
+void aar_server::worker() {
+  hipSetDevice(device);
+  std::unique_lock<std::mutex> lk(mu);
+  for (;;) {
+    cv.wait(lk, [&] { return stop_workers || !admitted.empty(); });
+    if (admitted.empty()) break;  // stop_workers, and nothing left to prepare
+    Req* r = admitted.front();
+    admitted.pop_front();
+    lk.unlock();
+    hipStream_t st = streams[r->slot];
+    aar_job* jb = nullptr;
+    int rc = aar_job_prepare_ws(ctx, &r->p, &r->t, r->prec, r->sched, arena + r->off, r->bytes, st, 0, &jb);
+    if (!rc) rc = aar_job_run(jb, st);
+    if (!rc && hipLaunchHostFunc(st, on_done, r) != hipSuccess) rc = AAR_ERR_HIP;
+    if (rc) hipStreamSynchronize(st);  // nothing of a failed job may still use its piece
+    lk.lock();
+    r->job = jb;
+    r->prepared = true;
+    if (rc) {
+      r->status = rc;
+      r->finished = true;
+      r->t_done = now_s();
+    }
+    cv.notify_all();
   }
 }
 
@@ -212,6 +242,7 @@ extern "C" int aar_server_create(aar_ctx* ctx, long long hbm_budget, int max_act
       return AAR_ERR_HIP;
     }
   s->th = std::thread([s] { s->loop(); });
+  for (int q = 0; q < std::min(max_active, 4); ++q) s->workers.emplace_back([s] { s->worker(); });
   *out = s;
   return AAR_OK;
 }
@@ -221,7 +252,7 @@ extern "C" int aar_server_submit(aar_server* s, const aar_params* p, const aar_t
   if (!s || !p || !t || !ticket || !t->alpha || !t->x0 || !t->glass || t->nglass <= 0) return AAR_ERR_BAD_ARG;
   if (precision != AAR_FP32 && precision != AAR_FP64) return AAR_ERR_BAD_ARG;
   if (schedule != AAR_SCHED_PLAIN && schedule != AAR_SCHED_BATCHED) return AAR_ERR_BAD_ARG;
-  const long long b = aar_job_bytes(p, t, precision, schedule);
+  const long long b = aar_job_ws_bytes(p, t, precision, schedule, 0);  // tables live in the plan cache
   if (b < 0) return AAR_ERR_BAD_ARG;
   if ((size_t)b > s->budget) return AAR_ERR_HBM_BUDGET;
   Req* r = new Req();
@@ -345,6 +376,13 @@ extern "C" int aar_server_destroy(aar_server* s) {
   }
   s->cv.notify_all();
   if (s->th.joinable()) s->th.join();  // the loop ends once every running job has finished
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop_workers = true;
+  }
+  s->cv.notify_all();
+  for (auto& w : s->workers)
+    if (w.joinable()) w.join();
   hipSetDevice(s->device);
   for (auto& kv : s->reqs) {
     if (kv.second->job) aar_job_destroy(kv.second->job);

@@ -20,6 +20,7 @@
  *                        power.c:391-448 (initialize_powerspectrum), main.c:305-313, 372
  *   aar_seedtable        Aarambam/LPT/Basis/main.c:315-349
  *   aar_job_prepare/run  Aarambam/LPT/Basis/main.c:149-1532 (displacement_fields)
+ *   aar_job_reseed       Aarambam/LPT/Basis/main.c:317-349 (seed table of a new Seed)
  *   aar_write_gadget     Aarambam/LPT/Basis/save.c:108-520 (write_particle_data, save_local_data)
  *   aar_write_gadget_files  Aarambam/LPT/Basis/save.c:108-520 with several writer
  *                        tasks (FileBase.<i>, num_files), save.c:184-187, 252
@@ -175,12 +176,27 @@ int aar_ctx_create(int device, aar_ctx **out);
 int aar_ctx_destroy(aar_ctx *ctx);
 /* wait for all work on the context's device (hipDeviceSynchronize) */
 int aar_ctx_sync(aar_ctx *ctx);
-/* HBM bytes a job of this shape allocates (admission control). */
+/* HBM bytes a job of this shape allocates (admission control): its own buffers (field
+ * slots, seed table, particles) plus its tables (k-space tables, modal schedule, transfer
+ * table, glass), which the context keeps resident and shares with later jobs of the same
+ * parameters (Seed and Fnl aside), tables, precision and schedule. */
 long long aar_job_bytes(const aar_params *p, const aar_tables *t, int precision, int schedule);
+/* Number and device bytes of the job tables the context holds for later jobs (at most 32;
+ * the least recently used one that no job holds is dropped first).  No counterpart in the
+ * reference, which reads its tables once per process (main.c:215-289, power.c:66-153,
+ * read_glass.c:10-88). */
+int aar_ctx_cached_plans(aar_ctx *ctx, int *count, long long *bytes);
 int aar_job_prepare(aar_ctx *ctx, const aar_params *p, const aar_tables *t, int precision,
                     int schedule, aar_job **out);
 /* enqueue the whole job on `stream` (a hipStream_t, NULL = the context stream) */
 int aar_job_run(aar_job *job, void *stream);
+/* Make the next aar_job_run produce the IC of another Seed and Fnl with the same tables and
+ * buffers, as a fresh aar_job_prepare would (main.c:317-349 builds the seed table from Seed;
+ * Fnl enters at main.c:1104-1126).  The seed table is built on the host at this call, so it
+ * overlaps with earlier runs still executing on the device; it is copied to the device at the
+ * start of the next run, ahead of that run's kernels.  Results of the previous run must be
+ * fetched before that next run is enqueued. */
+int aar_job_reseed(aar_job *job, int seed, double fnl);
 /* wait for the job's last run (returns at once if it never ran) */
 int aar_job_sync(aar_job *job);
 /* device pointers of the result arrays (npart*3 floats each), valid until destroy */
@@ -283,7 +299,8 @@ int aar_mode_basis_expansion(aar_ctx *ctx, double *alpha, double *shape_cov, dou
  * job run alone. */
 typedef struct aar_server aar_server;
 int aar_server_create(aar_ctx *ctx, long long hbm_budget, int max_active, aar_server **out);
-/* Queue a job (tables are copied).  keep = 0: its arena piece is returned as
+/* Queue a job (tables are copied; the job's own buffers come from the arena, its tables from
+ * the context's cache of device-resident tables, outside the arena).  keep = 0: its arena piece is returned as
  * soon as it completes and results are dropped; keep = 1: results stay on the
  * device until aar_server_release.  AAR_ERR_HBM_BUDGET if the job can never fit. */
 int aar_server_submit(aar_server *s, const aar_params *p, const aar_tables *t, int precision, int schedule,
