@@ -60,7 +60,8 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_read_params_res", "aar_l
            "aar_autotune", "aar_xy_tile_variant", "aar_set_xy_tile_variant", "aar_xy_group_cols",
            "aar_set_xy_group_cols", "aar_job_fetch_field_box",
            "aar_job_ng_statistics", "aar_mode_bispectra_covariance", "aar_mode_QS",
-           "aar_mode_basis_expansion", "aar_job_reseed", "aar_ctx_cached_plans"]
+           "aar_mode_basis_expansion", "aar_job_reseed", "aar_ctx_cached_plans",
+           "aar_job_run_noalloc"]
 KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
 
 
@@ -109,6 +110,7 @@ def lib():
             "aar_job_prepare": (C.c_int, [vp, PP, TP, C.c_int, C.c_int, C.POINTER(vp)]),
             "aar_job_run": (C.c_int, [vp, vp]),
             "aar_job_reseed": (C.c_int, [vp, C.c_int, C.c_double]),
+            "aar_job_run_noalloc": (C.c_int, [vp, PP, TP, C.c_int, C.c_int, vp, vp, vp]),
             "aar_ctx_cached_plans": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
             "aar_job_sync": (C.c_int, [vp]),
             "aar_job_device_output": (C.c_int, [vp, C.POINTER(fp), C.POINTER(fp), C.POINTER(C.c_longlong)]),

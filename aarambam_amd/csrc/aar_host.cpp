@@ -39,6 +39,7 @@ extern "C" const char* aar_status_string(int s) {
     case AAR_ERR_NMAX: return "Nmax < Nsample (main.c:200 assert)";
     case AAR_ERR_UNSUPPORTED: return "configuration not supported";
     case AAR_ERR_NO_DEVICE: return "no HIP device";
+    case AAR_ERR_SCRATCH: return "job needs more device memory than the fixed 64 MiB scratch of aar_job_run_noalloc";
     case 100010: return "parameter file error";
     default: return "unknown status";
   }

@@ -145,6 +145,7 @@ struct aar_job {
   std::vector<std::vector<char>> staged;
   // the tables this job reads (shared through the context's cache, or its own in a workspace)
   std::shared_ptr<Plan> plan;
+  bool ext_out = false;  // pos / vel are the caller's device arrays
   // seed tables of later runs (aar_job_reseed): built on the host into one of two pinned
   // buffers and copied to `seeds` at the start of the next run; an event after each copy
   // tells when its buffer may be rewritten
@@ -546,12 +547,41 @@ template <typename V> int job_upload(aar_job* j, V** dst, const std::vector<V>& 
 
 }  // namespace
 
+// ------------------------------------------------------------------ scratch
+// Device memory of the allocation-free entry point (aar_job_run_noalloc): every buffer of a job
+// besides its inputs and outputs (tables, seed table, field slots) is carved from this array.
+// Each GPU the library runs on holds its own copy until the process ends.  Host code never
+// passes the array's name to a kernel or a copy: it uses the address hipGetSymbolAddress
+// returned for the device at set-up (g_scratch_ptr).
+static constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// With `device` current: look up this device's copy of the scratch array once.
+static int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return AAR_ERR_BAD_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return AAR_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    fprintf(stderr, "hipGetSymbolAddress(scratch) failed on device %d: %s\n", device, hipGetErrorString(e));
+    return AAR_ERR_HIP;
+  }
+  g_scratch_ptr[device] = ptr;
+  return AAR_OK;
+}
+
 // ------------------------------------------------------------------ context
 extern "C" int aar_ctx_create(int device, aar_ctx** out) {
   if (!out) return AAR_ERR_BAD_ARG;
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return AAR_ERR_NO_DEVICE;
   AAR_HIP(hipSetDevice(device));
+  const int src = scratch_setup(device);
+  if (src) return src;
   aar_ctx* c = new aar_ctx();
   c->device = device;
   AAR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -848,7 +878,8 @@ static int plan_layout(const aar_params* p, const aar_tables* t, int schedule, P
 
 // Bytes of a job's own device buffers (field slots, pairwise accumulator, seed table,
 // positions, velocities), each rounded up to 256 bytes.
-static size_t job_own_bytes(const aar_params* p, const aar_tables* t, int precision, int schedule, int M, int K) {
+static size_t job_own_bytes(const aar_params* p, const aar_tables* t, int precision, int schedule, int M, int K,
+                            bool particles = true) {
   const Geom g = make_geom(p->Nmesh);
   const size_t N = p->Nmesh, cx = precision == AAR_FP32 ? 8 : 16;
   const size_t npart = (size_t)p->GlassTileFac * p->GlassTileFac * p->GlassTileFac * t->nglass;
@@ -857,7 +888,7 @@ static size_t job_own_bytes(const aar_params* p, const aar_tables* t, int precis
   add(slots_needed(M, K, schedule) * (size_t)g.fs * cx);
   if (schedule == AAR_SCHED_PLAIN) add((size_t)g.fs * 16);
   add(4 * N * N);
-  add(12 * npart), add(12 * npart);
+  if (particles) add(12 * npart), add(12 * npart);
   return (b + 255) & ~(size_t)255;
 }
 
@@ -1028,7 +1059,7 @@ long long aar_job_ws_bytes(const aar_params* p, const aar_tables* t, int precisi
 
 extern "C" int aar_job_prepare(aar_ctx* ctx, const aar_params* p, const aar_tables* t, int precision,
                                int schedule, aar_job** out) {
-  return aar_job_prepare_ws(ctx, p, t, precision, schedule, nullptr, 0, nullptr, 0, out);
+  return aar_job_prepare_ws(ctx, p, t, precision, schedule, nullptr, 0, nullptr, 0, out, nullptr, nullptr);
 }
 
 // Prepare a job.  ws = nullptr: the job's own buffers are hipMalloc'd; otherwise they are carved
@@ -1036,7 +1067,8 @@ extern "C" int aar_job_prepare(aar_ctx* ctx, const aar_params* p, const aar_tabl
 // plan_in_ws = 0: the tables come from the context's plan cache; 1: the job lays out its own
 // tables at the start of `ws` (no allocation anywhere).
 int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, int precision, int schedule,
-                       void* ws, size_t ws_size, hipStream_t up, int plan_in_ws, aar_job** out) {
+                       void* ws, size_t ws_size, hipStream_t up, int plan_in_ws, aar_job** out, float* out_pos,
+                       float* out_vel) {
   if (!ctx || !out) return AAR_ERR_BAD_ARG;
   int rc = validate(p, t);
   if (rc) return rc;
@@ -1097,11 +1129,18 @@ int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, i
   j->s_disp = j->s_reg + 6;
   j->s_disp2 = j->s_reg + (j->fused_lpt ? 1 : 0);
   j->npart = (long long)p->GlassTileFac * p->GlassTileFac * p->GlassTileFac * t->nglass;
-  if ((rc = job_alloc(j, (void**)&j->pos, sizeof(float) * 3 * j->npart)) ||
-      (rc = job_alloc(j, (void**)&j->vel, sizeof(float) * 3 * j->npart)))
+  if (out_pos && out_vel) {  // the caller's device arrays
+    j->pos = out_pos;
+    j->vel = out_vel;
+    j->ext_out = true;
+  } else if ((rc = job_alloc(j, (void**)&j->pos, sizeof(float) * 3 * j->npart)) ||
+             (rc = job_alloc(j, (void**)&j->vel, sizeof(float) * 3 * j->npart))) {
     return fail(rc);
-  AAR_HIP(hipEventCreate(&j->ev0));
-  AAR_HIP(hipEventCreate(&j->ev1));
+  }
+  if (!j->ext_out) {
+    AAR_HIP(hipEventCreate(&j->ev0));
+    AAR_HIP(hipEventCreate(&j->ev1));
+  }
   *out = j;
   return AAR_OK;
 }
@@ -1136,6 +1175,37 @@ extern "C" int aar_job_reseed(aar_job* j, int seed, double fnl) {
     }
   }
   return AAR_OK;
+}
+
+extern "C" int aar_job_run_noalloc(aar_ctx* ctx, const aar_params* p, const aar_tables* t, int precision,
+                                   int schedule, float* pos, float* vel, void* stream) {
+  if (!ctx || !pos || !vel) return AAR_ERR_BAD_ARG;
+  int rc = validate(p, t);
+  if (rc) return rc;
+  if (precision != AAR_FP32 && precision != AAR_FP64) return AAR_ERR_BAD_ARG;
+  if (schedule != AAR_SCHED_PLAIN && schedule != AAR_SCHED_BATCHED) return AAR_ERR_BAD_ARG;
+  {
+    Plan pl;
+    if ((rc = plan_layout(p, t, schedule, pl))) return rc;
+    if (pl.bytes + job_own_bytes(p, t, precision, schedule, pl.M, pl.K, false) > kScratchBytes)
+      return AAR_ERR_SCRATCH;
+  }
+  void* ws = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    ws = g_scratch_ptr[ctx->device];
+  }
+  if (!ws) return AAR_ERR_HIP;  // set up by aar_ctx_create
+  AAR_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  aar_job* j = nullptr;
+  if ((rc = aar_job_prepare_ws(ctx, p, t, precision, schedule, ws, kScratchBytes, st, 1, &j, pos, vel)))
+    return rc;
+  rc = precision == AAR_FP32 ? Exec<float>::run(j, st) : Exec<double>::run(j, st);
+  // the uploads read host copies the job holds: wait for the work before dropping them
+  if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = AAR_ERR_HIP;
+  aar_job_destroy(j);
+  return rc;
 }
 
 extern "C" int aar_ctx_cached_plans(aar_ctx* ctx, int* count, long long* bytes) {
@@ -1367,7 +1437,7 @@ extern "C" int aar_job_destroy(aar_job* j) {
   hipSetDevice(j->ctx->device);
   if (j->ran) hipEventSynchronize(j->ev1);
   if (!j->ws) {  // the job's own buffers; the tables belong to the plan
-    void* bufs[] = {j->pool, j->seeds, j->acc, j->pos, j->vel};
+    void* bufs[] = {j->pool, j->seeds, j->acc, j->ext_out ? nullptr : j->pos, j->ext_out ? nullptr : j->vel};
     for (void* b : bufs)
       if (b) hipFree(b);
   } else if (j->up && !j->ran) {

@@ -203,7 +203,8 @@ void aar_server::worker() {
     lk.unlock();
     hipStream_t st = streams[r->slot];
     aar_job* jb = nullptr;
-    int rc = aar_job_prepare_ws(ctx, &r->p, &r->t, r->prec, r->sched, arena + r->off, r->bytes, st, 0, &jb);
+    int rc = aar_job_prepare_ws(ctx, &r->p, &r->t, r->prec, r->sched, arena + r->off, r->bytes, st, 0, &jb, nullptr,
+                                nullptr);
     if (!rc) rc = aar_job_run(jb, st);
     if (!rc && hipLaunchHostFunc(st, on_done, r) != hipSuccess) rc = AAR_ERR_HIP;
     if (rc) hipStreamSynchronize(st);  // nothing of a failed job may still use its piece

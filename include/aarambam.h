@@ -58,7 +58,8 @@ enum {
   AAR_ERR_HBM_BUDGET = 100003,     /* job does not fit the admitted HBM budget */
   AAR_ERR_NMAX = 100004,           /* Nmax < Nsample (main.c:200 assert) */
   AAR_ERR_UNSUPPORTED = 100005,    /* a configuration the reference does not run either */
-  AAR_ERR_NO_DEVICE = 100006
+  AAR_ERR_NO_DEVICE = 100006,
+  AAR_ERR_SCRATCH = 100007         /* job exceeds the fixed scratch of aar_job_run_noalloc */
 };
 
 /* The 38 tags of the reference parameter file (read_param.c:27-184), and the
@@ -197,6 +198,19 @@ int aar_job_run(aar_job *job, void *stream);
  * start of the next run, ahead of that run's kernels.  Results of the previous run must be
  * fetched before that next run is enqueued. */
 int aar_job_reseed(aar_job *job, int seed, double fnl);
+/* One whole IC job that allocates nothing: no hipMalloc, hipFree or other allocating device
+ * routine.  No counterpart in the reference; it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid it).  It
+ * computes what aar_job_prepare + aar_job_run compute for the same inputs, bit for bit, with
+ * every device buffer besides the inputs and outputs (tables, seed table, field slots) taken
+ * from a 64 MiB scratch array the library holds on each GPU until the process ends (enough for
+ * Nmesh <= 64 in either precision and schedule); larger jobs get AAR_ERR_SCRATCH.  pos and vel
+ * are the caller's device arrays of 3 npart floats (npart = GlassTileFac^3 nglass).  The work
+ * is enqueued on `stream` (NULL: the context stream) and the call returns when it has finished.
+ * Every call on one GPU shares that GPU's scratch, whose contents persist from the last call:
+ * calls that overlap on one GPU (other threads, other contexts of the device) race. */
+int aar_job_run_noalloc(aar_ctx *ctx, const aar_params *p, const aar_tables *t, int precision, int schedule,
+                        float *pos, float *vel, void *stream);
 /* wait for the job's last run (returns at once if it never ran) */
 int aar_job_sync(aar_job *job);
 /* device pointers of the result arrays (npart*3 floats each), valid until destroy */
