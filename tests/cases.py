@@ -8,21 +8,16 @@ import os
 
 import numpy as np
 
+from aarambam_amd import workloads as _w
+
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
-UNITS = dict(UnitLength_in_cm=3.085678e24, UnitMass_in_g=1.989e43, UnitVelocity_in_cm_per_s=1e5,
-             InputSpectrum_UnitLength_in_cm=3.085678e24)
+UNITS = _w.UNITS
 
 
 def base_config(**kw):
     """Plain FP64 CPU-runnable case: Nmesh=64, Lbox=1000, EH P(k), local template."""
-    cfg = dict(Nmesh=64, Nsample=64, Nmax=64, N_modes=8, Seed=1234, SphereMode=0,
-               FixedAmplitude=0, PhaseFlip=0, WhichSpectrum=0, WhichTransfer=1, GlassTileFac=1,
-               WDM_On=0, Box=1000.0, Omega=0.3, OmegaLambda=0.7, OmegaBaryon=0.05,
-               HubbleParam=0.7, Sigma8=0.8, PrimordialIndex=0.9649, Redshift=50.0, Fnl=100.0,
-               ShapeGamma=0.0, WDM_PartMass_in_kev=10.0, **UNITS)
-    cfg.update(kw)
-    return cfg
+    return _w.config(**kw)
 
 
 def load_transfer():
@@ -46,53 +41,16 @@ def x0_file(Nm, path=None):
     return x0
 
 
-def local_rows():
-    """Local template as one separable triple (SingleField.py:19-27)."""
-    return np.array([[3, 0, 0, 1.0]])
-
-
-def equilateral_rows():
-    """Equilateral template triples (SingleField.py:45-78)."""
-    return np.array([[3, 0, 0, -3.0], [1, 1, 1, -2.0], [2, 1, 0, 6.0]])
-
-
-def le_rows():
-    """Local + equilateral: the sum of the two templates, so coefficients of the
-    same triple add (the (3,0,0) entry becomes 1 - 3 = -2)."""
-    merged = {}
-    for n1, n2, n3, a in list(local_rows()) + list(equilateral_rows()):
-        key = tuple(sorted((int(n1), int(n2), int(n3)), reverse=True))
-        merged[key] = merged.get(key, 0.0) + float(a)
-    return np.array([[*k, v] for k, v in sorted(merged.items())])
-
-
-def dense12_rows(n_triples=60, seed=12):
-    """Synthetic dense 12-mode table: 60 distinct triples p1 >= p2 >= p3 < 12 with
-    alpha ~ N(0, 1) from PCG64(seed): 39 distinct (p2, p3) product pairs."""
-    rng = np.random.Generator(np.random.PCG64(seed))
-    triples = [(a, b, c) for a in range(12) for b in range(a + 1) for c in range(b + 1)]
-    pick = rng.choice(len(triples), size=n_triples, replace=False)
-    return np.array([[*triples[i], rng.standard_normal()] for i in sorted(pick)])
+local_rows = _w.local_rows  # SingleField.py:19-27
+equilateral_rows = _w.equilateral_rows  # SingleField.py:45-78
+le_rows = _w.le_rows
+dense12_rows = _w.dense12_rows
+lattice_glass = _w.lattice_glass
 
 
 def x0_closed_form(Nm, ns, Box, Nmax):
     """AiTable x0[p] = P_p(-1 - 2 kmin^s/(kmax^s - kmin^s)), decompose.py:176-216."""
-    from scipy import special
-    s = (4 - ns) / 3
-    kmin = 2 * np.pi / Box
-    kmax = kmin * Nmax
-    kbar = -1 - 2 * kmin ** s / (kmax ** s - kmin ** s)
-    x0 = np.zeros((Nm, 2))
-    for p in range(Nm):
-        x0[p, 0] = special.eval_legendre(p, kbar)
-    return x0
-
-
-def lattice_glass(n=64, box=100.0):
-    """A regular n^3 lattice in the layout of Aarambam/defaults/dummy_glass_dmonly_64.dat."""
-    g = np.arange(n, dtype=np.float64) * (box / n)
-    x, y, z = np.meshgrid(g, g, g, indexing="ij")
-    return np.stack([x, y, z], -1).reshape(-1, 3).astype(np.float32)
+    return _w.x0_table(Nm, ns, Box, Nmax)
 
 
 def job_inputs(cfg, rows=None, x0=None, glass_n=None):
