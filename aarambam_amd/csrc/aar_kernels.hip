@@ -768,10 +768,8 @@ __global__ void __launch_bounds__(256)
   }
   const C2 p = phi[e];
   using RT = T;  // radial functions in the path's precision
-  RT kss;
   const RT k2 = (RT)kmag2;
-  if constexpr (sizeof(T) == 4) kss = exp2f(0.5f * (float)ks.s * log2f(k2));  // k^s = 2^(s/2 log2 k^2)
-  else kss = pow(kmag1, ks.s);
+  const RT kss = kpow_s<RT>(kmag2, kmag1, ks.s);
   const RT xb = -1 + 2 * (kss - (RT)ks.kmin_s) / (RT)(ks.kmax_s - ks.kmin_s);
   const RT k3s = kss * kss * kss;  // k^{4-ns}
   LegRecT<RT> leg;
@@ -886,7 +884,7 @@ __global__ void __launch_bounds__(256)
     const double kmag1 = sqrt(kmag2);
     if (!outside_cut(ks, x, y, z, kmag1)) {
       using RT = T;
-      const RT kss = pow((RT)kmag1, (RT)ks.s);
+      const RT kss = kpow_s<RT>(kmag2, kmag1, ks.s);
       LegRecT<RT> leg;
       leg.init(-1 + 2 * (kss - (RT)ks.kmin_s) / (RT)(ks.kmax_s - ks.kmin_s));
       double nr = 0, ni = 0;
@@ -1039,9 +1037,7 @@ __global__ void __launch_bounds__(256)
   if (x + y + z == 0) {
     re = im = 0;
   } else if (!outside_cut(ks, x, y, z, kmag1)) {
-    RT kss;
-    if constexpr (sizeof(T) == 4) kss = __expf((float)ks.s * __logf((float)kmag1));  // FP32: hardware exp / log
-    else kss = pow(kmag1, ks.s);
+    const RT kss = kpow_s<RT>(kmag2, kmag1, ks.s);
     LegRecT<RT> leg;
     leg.init(-1 + 2 * (kss - (RT)ks.kmin_s) / (RT)(ks.kmax_s - ks.kmin_s));
     double nr = 0, ni = 0;

@@ -56,6 +56,14 @@ __device__ __forceinline__ double res_modulate(const KSpace& ks, double v, doubl
 }
 
 // Upward Legendre recurrence, GSL gsl_sf_legendre_Pl (legendre_poly.c).
+// k^s of a mode, s = (4 - n_s)/3, from |k|^2 (FP32) or |k| (FP64).  The mode fields and the
+// combine weights both take it from here, so on the FP32 path their Legendre arguments agree:
+// 2^(s/2 log2 |k|^2) with the hardware exp2 / log2; the FP64 path keeps pow(|k|, s).
+template <typename RT> __device__ __forceinline__ RT kpow_s(double kmag2, double kmag1, double s) {
+  if constexpr (sizeof(RT) == 4) return exp2f(0.5f * (float)s * log2f((float)kmag2));
+  else return pow(kmag1, s);
+}
+
 // RT is the precision of the radial functions: FP64 on the FP64 path, FP32 on
 // the FP32 path (|x| <= 1 and orders <= 31 keep the recurrence stable).
 template <typename RT> struct LegRecT {
