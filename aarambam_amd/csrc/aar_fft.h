@@ -143,15 +143,30 @@ template <int R, int DIR, typename C> __device__ __forceinline__ void dftR(C* v)
 //  column-major (RM = false): element (pos, col) at pos*CP + col, thread t
 //    serves column t % C, so a wavefront touches 16 adjacent columns at a few
 //    positions -- coalesced when columns are adjacent in memory (x, y passes);
-//  row-major (RM = true): element at col*LP + pos + pos/16, thread t serves
+//  row-major (RM = true): element at col*LP + pos + pad(pos), thread t serves
 //    butterfly t % (L/R) of column t / (L/R), so a wavefront runs along
 //    positions -- coalesced when each column is a contiguous row (z passes).
-template <typename T, int L, int C, bool RM> struct TileIdx {
+// The LDS of gfx950 serves a 64-bit access in passes of 32 lanes over 32
+// eight-byte banks.  pad() is a sum of shifted position bits: the butterfly
+// reads (i + t L/R) and Stockham writes ((i - k) R + k + u P) split into a
+// per-lane part and a constant whose bits do not overlap, so pad() of the
+// sum is pad(lane part) + pad(constant) and the constant folds into the
+// instruction's offset.  For FP32 rows with E = 4 or 8 elements per thread,
+// 2 (pos >> 5) + (pos >> 6) or 2 (pos >> 5) + (pos >> 7) keeps most of the
+// writes' lanes on distinct banks, where pos >> 4 left every write and most
+// reads of these plans two-way conflicted; other tiles keep pos >> 4.
+template <typename T, int L, int C, bool RM, int E = 0> struct TileIdx {
+  static constexpr int PADK = (RM && sizeof(T) == 4 && (E == 4 || E == 8)) ? E : 0;
+  __host__ __device__ static constexpr int pad(int pos) {
+    if constexpr (PADK == 4) return 2 * (pos >> 5) + (pos >> 6);
+    else if constexpr (PADK == 8) return 2 * (pos >> 5) + (pos >> 7);
+    else return pos >> 4;
+  }
   static constexpr int CP = tile_cp<T>(L, C);
-  static constexpr int LP = L + L / 16;
+  static constexpr int LP = L + pad(L - 1) + 1 + (PADK && L == 64 ? 2 : 0) + (PADK && L == 32 ? 4 : 0);
   static constexpr int SIZE = RM ? C * LP : L * CP;
   __device__ static __forceinline__ int at(int pos, int col) {
-    if constexpr (RM) return col * LP + pos + (pos >> 4);
+    if constexpr (RM) return col * LP + pos + pad(pos);
     else return pos * CP + col;
   }
 };
@@ -183,8 +198,8 @@ template <typename T, int L, int C, int NT, int DIR, int TWS, int S, bool SRC_LD
           class StoreF>
 __device__ __forceinline__ void fft_stage(cx_t<T>* tile, const cx_t<T>* tw, LoadF& load, StoreF& store) {
   using C2 = cx_t<T>;
-  using TI = TileIdx<T, L, C, RM>;
   constexpr int E = C * L / NT;
+  using TI = TileIdx<T, L, C, RM, E>;
   constexpr int RMAX = E < 16 ? E : 16;
   constexpr int NS = plan_nstages(L, RMAX);
   constexpr int R = plan_radix(L, RMAX, S);

@@ -116,7 +116,7 @@ template <typename T, int H, int C, int NT, bool WL>
 __device__ __forceinline__ void r2c_post(const cx_t<T>* tile, const cx_t<T>* tw, cx_t<T>* f, long long r0,
                                          const Geom& g) {
   using C2 = cx_t<T>;
-  using TI = TileIdx<T, H, C, true>;
+  using TI = TileIdx<T, H, C, true, C * H / NT>;
   C2* rows = f + r0 * H;
   constexpr int U = H / 2;
   constexpr int GT = WL ? 64 : NT, GC = WL ? C * 64 / NT : C;
@@ -261,7 +261,7 @@ __global__ void __launch_bounds__((ZCfg<T, H>::NT))
   using C2 = cx_t<T>;
   constexpr int C = ZCfg<T, H>::C, NT = ZCfg<T, H>::NT;
   constexpr int N = 2 * H;
-  using TI = TileIdx<T, H, C, true>;
+  using TI = TileIdx<T, H, C, true, C * H / NT>;
   __shared__ C2 tile[TI::SIZE];
   __shared__ C2 tw[N];
   __shared__ C2 xh[C];
@@ -298,7 +298,7 @@ __global__ void __launch_bounds__((ZCfg<T, H>::NT))
   using C2 = cx_t<T>;
   constexpr int C = ZCfg<T, H>::C, NT = ZCfg<T, H>::NT;
   constexpr int N = 2 * H;
-  using TI = TileIdx<T, H, C, true>;
+  using TI = TileIdx<T, H, C, true, C * H / NT>;
   __shared__ C2 tile[TI::SIZE];
   __shared__ C2 tw[N];
   load_twiddles(tw, twg + N, N, NT);
@@ -351,7 +351,7 @@ __global__ void __launch_bounds__((ZNGCfg<T, H, MMAX>::NT))
   constexpr int C = Cfg::C, NT = Cfg::NT, E = Cfg::E;
   constexpr bool WL = Cfg::WL;
   constexpr int N = 2 * H;
-  using TI = TileIdx<T, H, C, true>;
+  using TI = TileIdx<T, H, C, true, C * H / NT>;
   __shared__ C2 tile[TI::SIZE];
   __shared__ C2 tw[N];
   load_twiddles(tw, twg + N, N, NT);
@@ -1242,13 +1242,19 @@ __device__ __forceinline__ void cic_particle(int n, int t, const cx_t<T>* __rest
   reinterpret_cast<float3*>(vel)[p] = make_float3(V[0], V[1], V[2]);
 }
 
+// Block b of the flattened grid covers 256 consecutive glass particles of one tile: with the
+// lattice glass, four z-rows at one x, so blocks b and b + 1 share a y-row of corners and b and
+// b + 64 / 256 an x-plane.  The grid is remapped so that each XCD takes a contiguous range of
+// blocks (xcd_tile): neighbouring blocks then read their shared rows through one L2.
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_cic(const cx_t<T>* __restrict__ disp, const cx_t<T>* __restrict__ disp2, Geom g,
           const float* __restrict__ glass, int nglass, CicConst cc, float* __restrict__ pos, float* __restrict__ vel) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;  // glass particle
+  const int bpt = gridDim.x;  // blocks per glass tile
+  const int lb = xcd_tile(blockIdx.y * bpt + blockIdx.x, bpt * gridDim.y);
+  const int n = (lb % bpt) * blockDim.x + threadIdx.x;  // glass particle
   if (n >= nglass) return;
-  cic_particle<T>(n, blockIdx.y, disp, disp2, g, glass, nglass, cc, pos, vel);
+  cic_particle<T>(n, lb / bpt, disp, disp2, g, glass, nglass, cc, pos, vel);
 }
 
 template <typename T>
