@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "../../include/aarambam.h"
@@ -417,7 +418,7 @@ struct Cosmo {
   // Romberg integration of Numerical Recipes (nrsrc/qromb.c, trapzd.c,
   // polint.c): trapezoid refinements, K=5 point polynomial extrapolation to
   // h=0, stop when |error estimate| <= 1e-8 |value|, at most 40 steps.
-  template <class Fn> double romberg(Fn f, double a, double b) const {
+  template <class Fn> static double romberg(Fn f, double a, double b) {
     double s[42], h[43];
     double st = 0;
     h[1] = 1.0;
@@ -578,6 +579,63 @@ extern "C" unsigned long aar_ranlxd_nth(unsigned long seed, int luxury, int n) {
   return k;
 }
 
+// ------------------------------------------------------------------ WDM thermal velocities
+// With WDM_On 1 and WDM_Vtherm_On 1 the reference's snapshot writer adds a random thermal
+// velocity to every particle's velocity as it writes it (save.c:350-356; the particle array
+// itself keeps the ZA + 2LPT velocities), drawn by power.c:624-707 from drand48, which the
+// reference never seeds: every writing task starts from drand48's default state.
+namespace {
+// drand48 (POSIX): X <- (0x5DEECE66D X + 0xB) mod 2^48 from X = 0x1234ABCD330E; value X 2^-48.
+struct Drand48 {
+  uint64_t x = 0x1234ABCD330EULL;
+  double next() {
+    x = (0x5DEECE66DULL * x + 0xBULL) & ((1ULL << 48) - 1);
+    return ldexp((double)x, -48);
+  }
+};
+
+// fermi_dirac_init (power.c:634-663): the cumulative distribution of x^2 / (e^x + 1) on 2000
+// speeds in [0, 20] by Romberg integration, and WDM_V0 in internal units.
+struct FermiDirac {
+  static constexpr int kLen = 2000;
+  double vel[kLen], cum[kLen], cummax[kLen];
+  double v0 = 0;
+  explicit FermiDirac(const aar_params* p) {
+    for (int i = 0; i < kLen; i++) {
+      vel[i] = 20.0 * i / (kLen - 1.0);
+      cum[i] = Cosmo::romberg([](double x) { return x * x / (exp(x) + 1); }, 0, vel[i]);
+    }
+    for (int i = 0; i < kLen; i++) cum[i] /= cum[kLen - 1];
+    // the running maximum: power.c's linear search stops at the first i with p <= cum[i + 1],
+    // which is also the first i with p <= max(cum[1..i + 1]), so a binary search on it agrees
+    cummax[0] = cum[0];
+    for (int i = 1; i < kLen; i++) cummax[i] = std::max(cummax[i - 1], cum[i]);
+    v0 = 0.012 * (1 + p->Redshift) * pow((p->Omega - p->OmegaBaryon) / 0.3, 1.0 / 3) *
+         pow(p->HubbleParam / 0.65, 2.0 / 3) * pow(1.0 / p->WDM_PartMass_in_kev, 4.0 / 3);
+    printf("\nWarm dark matter rms velocity dispersion at starting redshift = %g km/sec\n\n", 3.59714 * v0);
+    v0 *= 1.0e5 / p->UnitVelocity_in_cm_per_s;
+    v0 *= sqrt(1 + p->Redshift);  // peculiar to Gadget's cosmological velocity
+  }
+  double speed(Drand48& r) const {  // get_fermi_dirac_vel, power.c:667-683
+    const double pr = r.next();
+    int i = (int)(std::lower_bound(cummax + 1, cummax + kLen - 1, pr) - (cummax + 1));
+    if (i > kLen - 2) i = kLen - 2;
+    const double u = (pr - cum[i]) / (cum[i + 1] - cum[i]);
+    return vel[i] * (1 - u) + vel[i + 1] * u;
+  }
+  void kick(float* v3, Drand48& r) const {  // add_WDM_thermal_speeds, power.c:687-707
+    const double v = v0 * speed(r);
+    const double phi = 2 * M_PI * r.next();
+    const double theta = acos(2 * r.next() - 1);
+    v3[0] += v * sin(theta) * cos(phi);
+    v3[1] += v * sin(theta) * sin(phi);
+    v3[2] += v * cos(theta);
+  }
+};
+
+bool wdm_thermal(const aar_params* p) { return p->WDM_On == 1 && p->WDM_Vtherm_On == 1; }
+}  // namespace
+
 // ------------------------------------------------------------------ writers
 static int fwrite_all(const void* p, size_t sz, size_t n, FILE* fd) {
   return fwrite(p, sz, n, fd) == n ? AAR_OK : 777;
@@ -616,9 +674,10 @@ static void gadget_header(uint8_t* hdr, const aar_params* p, const aar_cosmo* c,
 
 // One Gadget-1 file of save_local_data (save.c:162-520, NO64BITID): n particles
 // starting at global index first, of ntot in numfiles files.
+// fdist (WDM thermal speeds, or NULL): the file's writer draws them from drand48's first state on.
 static int write_gadget_file(const std::string& fn, const aar_params* p, const aar_cosmo* c, const float* pos,
                              const float* vel, long long first, long long n, long long ntot, int numfiles,
-                             const uint8_t* glass_header) {
+                             const uint8_t* glass_header, const FermiDirac* fdist) {
   FILE* fd = fopen(fn.c_str(), "wb");
   if (!fd) return 10;
   uint8_t hdr[256];
@@ -634,7 +693,18 @@ static int write_gadget_file(const std::string& fn, const aar_params* p, const a
   rc |= fwrite_all(pos + 3 * first, sizeof(float), 3 * (size_t)n, fd);
   rc |= fwrite_all(&dummy, 4, 1, fd);
   rc |= fwrite_all(&dummy, 4, 1, fd);
-  rc |= fwrite_all(vel + 3 * first, sizeof(float), 3 * (size_t)n, fd);
+  if (fdist) {
+    Drand48 rng;
+    std::vector<float> vb(3 << 20);
+    for (long long s = 0; s < n; s += (long long)vb.size() / 3) {
+      const long long m = std::min<long long>((long long)vb.size() / 3, n - s);
+      memcpy(vb.data(), vel + 3 * (first + s), sizeof(float) * 3 * (size_t)m);
+      for (long long q = 0; q < m; ++q) fdist->kick(&vb[3 * q], rng);
+      rc |= fwrite_all(vb.data(), sizeof(float), 3 * (size_t)m, fd);
+    }
+  } else {
+    rc |= fwrite_all(vel + 3 * first, sizeof(float), 3 * (size_t)n, fd);
+  }
   rc |= fwrite_all(&dummy, 4, 1, fd);
   dummy = (int)(uint32_t)(sizeof(int) * (size_t)n);
   rc |= fwrite_all(&dummy, 4, 1, fd);
@@ -656,8 +726,9 @@ extern "C" int aar_write_gadget(const aar_params* p, const aar_cosmo* c, const f
   if (!p || !c || !pos || !vel || npart < 0) return AAR_ERR_BAD_ARG;
   if (p->NumFilesWrittenInParallel > 1) return AAR_ERR_WRITE_PARALLEL;  // save.c:112-117, NTask = 1
   if (npart == 0) return AAR_OK;
+  std::unique_ptr<FermiDirac> fdist(wdm_thermal(p) ? new FermiDirac(p) : nullptr);
   return write_gadget_file(std::string(p->OutputDir) + "/" + p->FileBase, p, c, pos, vel, 0, npart, npart, 1,
-                           glass_header);
+                           glass_header, fdist.get());
 }
 
 // The snapshot split over files the way several tasks write it (save.c:184-187, 252:
@@ -681,10 +752,11 @@ extern "C" int aar_write_gadget_files(const aar_params* p, const aar_cosmo* c, c
   if ((npart + nfiles - 1) / nfiles > per_max || nfiles > npart) return AAR_ERR_BAD_ARG;
   if (nfiles_out) *nfiles_out = nfiles;
   const std::string base = std::string(p->OutputDir) + "/" + p->FileBase;
-  for (int f = 0; f < nfiles; ++f) {
+  std::unique_ptr<FermiDirac> fdist(wdm_thermal(p) ? new FermiDirac(p) : nullptr);
+  for (int f = 0; f < nfiles; ++f) {  // each file as its own writing task
     const long long a = npart * f / nfiles, b = npart * (f + 1) / nfiles;
     const std::string fn = nfiles > 1 ? base + "." + std::to_string(f) : base;
-    const int rc = write_gadget_file(fn, p, c, pos, vel, a, b - a, npart, nfiles, glass_header);
+    const int rc = write_gadget_file(fn, p, c, pos, vel, a, b - a, npart, nfiles, glass_header, fdist.get());
     if (rc) return rc;
   }
   return AAR_OK;
@@ -821,11 +893,12 @@ extern "C" int aar_write_gadget_tasks(const aar_params* p, const aar_cosmo* c, c
   for (size_t q = 0; q < task_of.size(); ++q) cnt[task_of[q]] += (long long)T * T;
   int with = 0;
   for (int r = 0; r < ntask; ++r) with += cnt[r] > 0;
+  std::unique_ptr<FermiDirac> fdist(wdm_thermal(p) ? new FermiDirac(p) : nullptr);  // task 0 prints
   if (with <= 1) {
     for (int r = 0; r < ntask; ++r)
       if (cnt[r] == npart)
         return write_gadget_file(std::string(p->OutputDir) + "/" + p->FileBase, p, c, pos, vel, 0, npart, npart,
-                                 1, t->glass_header);
+                                 1, t->glass_header, fdist.get());
     return AAR_OK;
   }
   // x-tiles holding each task's particles
@@ -848,7 +921,9 @@ extern "C" int aar_write_gadget_tasks(const aar_params* p, const aar_cosmo* c, c
     rc |= fwrite_all(&dummy, 4, 1, fd);
     rc |= fwrite_all(hdr, 256, 1, fd);
     rc |= fwrite_all(&dummy, 4, 1, fd);
-    // pass 0: positions, 1: velocities, 2: IDs, each gathered in particle order
+    // pass 0: positions, 1: velocities (with the task's own thermal speeds), 2: IDs, each gathered
+    // in particle order
+    Drand48 rng;  // every task's drand48 starts from the default state
     for (int pass = 0; pass < 3 && !rc; ++pass) {
       dummy = (int)(uint32_t)((pass < 2 ? 12 : 4) * (size_t)cnt[r]);
       rc |= fwrite_all(&dummy, 4, 1, fd);
@@ -874,6 +949,7 @@ extern "C" int aar_write_gadget_tasks(const aar_params* p, const aar_cosmo* c, c
                 buf[3 * fill] = src[0];
                 buf[3 * fill + 1] = src[1];
                 buf[3 * fill + 2] = src[2];
+                if (pass == 1 && fdist) fdist->kick(&buf[3 * fill], rng);
               }
               if (++fill == ids.size()) flush();
             }

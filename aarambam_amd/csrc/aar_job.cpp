@@ -770,7 +770,6 @@ static int validate(const aar_params* p, const aar_tables* t) {
   if (p->Nmax < p->Nsample) return AAR_ERR_NMAX;
   if (p->N_modes < 3 || !t->alpha || !t->x0) return AAR_ERR_BAD_ARG;
   if (!t->glass || t->nglass <= 0 || p->GlassTileFac <= 0) return AAR_ERR_BAD_ARG;
-  if (p->WDM_Vtherm_On == 1 && p->WDM_On == 1) return AAR_ERR_UNSUPPORTED;
   // The CIC gather indexes the fields with the cell of each particle's unperturbed position
   // (main.c:1443-1461 clamps only i == N, j == N, k == N and i < 0): a glass coordinate outside
   // [0, BoxSize] or not finite, or a box that is not positive, would index outside the grid.

@@ -1024,6 +1024,55 @@ void orc_particles(const orc_params *p, const orc_cosmo *c, long long npart,
 }
 
 /* ------------------------------------------------------------------------ */
+/* WDM thermal velocities of the snapshot writer, save.c:350-356 with         */
+/* power.c:624-707: drand48 (never seeded, so from POSIX's default state),    */
+/* the Fermi-Dirac table by Romberg, the reference's linear table search.     */
+/* ------------------------------------------------------------------------ */
+static uint64_t orc_d48 = 0x1234ABCD330EULL;
+void orc_drand48_reset(void) { orc_d48 = 0x1234ABCD330EULL; }
+double orc_drand48(void) {
+  orc_d48 = (0x5DEECE66DULL * orc_d48 + 0xBULL) & ((1ULL << 48) - 1);
+  return ldexp((double)orc_d48, -48);
+}
+
+#define ORC_FD_LEN 2000
+static double fermi_dirac_kernel(const orc_ctx *cx, double x) { /* power.c:629-632 */
+  (void)cx;
+  return x * x / (exp(x) + 1);
+}
+
+void orc_wdm_thermal(const orc_params *p, long long npart, float *vel) {
+  static double fvel[ORC_FD_LEN], fcum[ORC_FD_LEN];
+  for (int i = 0; i < ORC_FD_LEN; i++) { /* power.c:638-646 */
+    fvel[i] = 20.0 * i / (ORC_FD_LEN - 1.0);
+    fcum[i] = nr_qromb(fermi_dirac_kernel, NULL, 0, fvel[i]);
+  }
+  for (int i = 0; i < ORC_FD_LEN; i++) fcum[i] /= fcum[ORC_FD_LEN - 1];
+  double v0 = 0.012 * (1 + p->Redshift) * pow((p->Omega - p->OmegaBaryon) / 0.3, 1.0 / 3) *
+              pow(p->HubbleParam / 0.65, 2.0 / 3) * pow(1.0 / p->WDM_PartMass_in_kev, 4.0 / 3);
+  v0 *= 1.0e5 / p->UnitVelocity_in_cm_per_s;
+  v0 *= sqrt(1 + p->Redshift);
+  orc_drand48_reset();
+  for (long long n = 0; n < npart; n++) {
+    double pr = orc_drand48(); /* get_fermi_dirac_vel, power.c:667-683 */
+    int i = 0;
+    while (i < ORC_FD_LEN - 2)
+      if (pr > fcum[i + 1])
+        i++;
+      else
+        break;
+    double u = (pr - fcum[i]) / (fcum[i + 1] - fcum[i]);
+    double v = v0 * (fvel[i] * (1 - u) + fvel[i + 1] * u);
+    double phi = 2 * M_PI * orc_drand48(); /* add_WDM_thermal_speeds, power.c:687-707 */
+    double theta = acos(2 * orc_drand48() - 1);
+    double vx = v * sin(theta) * cos(phi), vy = v * sin(theta) * sin(phi), vz = v * cos(theta);
+    vel[3 * n] += vx;
+    vel[3 * n + 1] += vy;
+    vel[3 * n + 2] += vz;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 int orc_run(const orc_params *p, const double *alpha, const double *x0, const float *glass,
             int nglass, double glass_box, float *pos, float *vel, int *ids, double *phig_spec,
             double *phi_spec, double *phig_real, double *phing_real) {

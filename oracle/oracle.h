@@ -111,6 +111,13 @@ void orc_glass_tile(const orc_params *p, const float *glass, int nglass, double 
 void orc_particles(const orc_params *p, const orc_cosmo *c, long long npart,
                    double *const disp[3], double *const disp2[3], float *pos, float *vel);
 
+/* WDM thermal velocities the snapshot writer adds (save.c:350-356, power.c:624-707): kicks
+ * vel (3 npart floats, particle order) in place, drawing from drand48 reset to its default
+ * state; orc_drand48 is that generator (POSIX's 48-bit LCG). */
+void orc_wdm_thermal(const orc_params *p, long long npart, float *vel);
+void orc_drand48_reset(void);
+double orc_drand48(void);
+
 /* Whole displacement_fields() for one rank; optional outputs may be NULL. */
 int orc_run(const orc_params *p, const double *alpha, const double *x0, const float *glass,
             int nglass, double glass_box, float *pos, float *vel, int *ids,

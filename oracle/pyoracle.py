@@ -75,6 +75,10 @@ def lib():
         L.orc_glass_tile.argtypes = [P, fp, C.c_int, C.c_double, fp, C.c_void_p]
         L.orc_particles.argtypes = [P, C.POINTER(OrcCosmo), C.c_longlong, C.c_void_p, C.c_void_p,
                                     fp, fp]
+        L.orc_wdm_thermal.argtypes = [P, C.c_longlong, fp]
+        L.orc_drand48.restype = C.c_double
+        L.orc_drand48.argtypes = []
+        L.orc_drand48_reset.argtypes = []
         L.orc_run.argtypes = [P, dp, dp, fp, C.c_int, C.c_double, fp, fp, C.c_void_p,
                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         modal = [dp, ip, ip, ip, C.c_int, ip, ip, ip, C.c_int, dp, C.c_int, C.c_int]
@@ -301,3 +305,19 @@ def particles(cfg, pos, disp, disp2):
     V = np.zeros_like(P)
     lib().orc_particles(C.byref(p), C.byref(c), P.size // 3, ptrs(pad[:3]), ptrs(pad[3:]), P, V)
     return P.reshape(-1, 3), V.reshape(-1, 3)
+
+
+def wdm_thermal(cfg, vel):
+    """Velocities as the snapshot writer writes them with WDM_On 1 and WDM_Vtherm_On 1
+    (save.c:350-356, power.c:624-707), drand48 from its default state."""
+    p = make_params(cfg)
+    v = np.ascontiguousarray(vel, dtype=np.float32).reshape(-1).copy()
+    lib().orc_wdm_thermal(C.byref(p), v.size // 3, v)
+    return v.reshape(-1, 3)
+
+
+def drand48(n):
+    """n values of the oracle's drand48 from its default state."""
+    L = lib()
+    L.orc_drand48_reset()
+    return np.array([L.orc_drand48() for _ in range(n)])

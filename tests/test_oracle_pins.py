@@ -10,7 +10,9 @@ What each check fixes, independently of the oracle's own formulas:
   CIC deconvolution have closed forms;
 * CIC readout (main.c:1440-1516): nodes read back exactly, linear fields interpolate exactly,
   the clamps at i, j, k == N and the periodic wrap;
-* glass tiling (read_glass.c:176-214): positions and IDs of a 2 x 2 x 2 tiling by hand.
+* glass tiling (read_glass.c:176-214): positions and IDs of a 2 x 2 x 2 tiling by hand;
+* WDM thermal speeds (power.c:624-707): drand48 against the C library's, the speeds' moments
+  against the Fermi-Dirac closed forms, isotropic directions.
 """
 import numpy as np
 import pytest
@@ -284,3 +286,41 @@ def test_glass_tiling_order_and_ids_by_hand():
             (14, 15, 16)]
     np.testing.assert_array_equal(pos, np.array(want, np.float32))
     np.testing.assert_array_equal(ids, np.arange(1, 17))
+
+
+def test_drand48_matches_libc():
+    """The oracle's drand48 (POSIX's 48-bit generator, which the reference's thermal speeds draw
+    from without seeding) against the C library's own drand48 reset to the default state."""
+    import ctypes
+    import ctypes.util
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    libc.drand48.restype = ctypes.c_double
+    libc.seed48.argtypes = [ctypes.POINTER(ctypes.c_ushort)]
+    libc.seed48((ctypes.c_ushort * 3)(0x330E, 0xABCD, 0x1234))  # X0 = 0x1234ABCD330E, POSIX's default
+    want = np.array([libc.drand48() for _ in range(5000)])
+    np.testing.assert_array_equal(orc.drand48(5000), want)
+
+
+def test_wdm_thermal_speeds_follow_fermi_dirac():
+    """Kicks added to zero velocities: speeds |v| / V0 distributed as x^2 / (e^x + 1) on [0, 20],
+    whose moments are (1 - 2^-n) n! zeta(n + 1) / (1.5 zeta(3)): mean 3.1514, rms 3.5971 (the
+    constant power.c:653 prints); directions isotropic; V0 = 0.012 (1 + z) ((Om - Ob)/0.3)^(1/3)
+    (h/0.65)^(2/3) m^(-4/3) km/s, times sqrt(1 + z) in Gadget units (power.c:649-661)."""
+    from scipy.special import zeta
+    cfg = cases.base_config(WDM_On=1, WDM_PartMass_in_kev=0.7, Redshift=40.0)
+    n = 200000
+    v = orc.wdm_thermal(cfg, np.zeros((n, 3), np.float32)).astype(np.float64)
+    z = cfg["Redshift"]
+    v0 = (0.012 * (1 + z) * ((cfg["Omega"] - cfg["OmegaBaryon"]) / 0.3) ** (1 / 3) * (cfg["HubbleParam"] / 0.65) ** (2 / 3)
+          * (1 / cfg["WDM_PartMass_in_kev"]) ** (4 / 3) * 1e5 / cfg["UnitVelocity_in_cm_per_s"] * np.sqrt(1 + z))
+    s = np.linalg.norm(v, axis=1) / v0
+    import math
+    mom = lambda k: (1 - 2.0 ** -k) * math.factorial(k) * zeta(k + 1)
+    mean, rms = mom(3) / mom(2), np.sqrt(mom(4) / mom(2))
+    assert abs(rms - 3.59714) < 1e-4
+    assert s.mean() == pytest.approx(mean, abs=0.02)
+    assert np.sqrt((s * s).mean()) == pytest.approx(rms, abs=0.02)
+    assert s.max() <= 20.0 + 1e-4
+    u = v / np.linalg.norm(v, axis=1)[:, None]
+    assert np.abs(u.mean(axis=0)).max() < 0.01
+    np.testing.assert_allclose((u * u).mean(axis=0), 1 / 3, atol=0.01)

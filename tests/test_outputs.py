@@ -235,3 +235,42 @@ def test_executable_tasks_other_than_zero_exit(tmp_path):
     env = dict(os.environ, OMPI_COMM_WORLD_SIZE="4", OMPI_COMM_WORLD_RANK="2")
     out = subprocess.run([exe, str(tmp_path / "missing")], capture_output=True, text=True, timeout=60, env=env)
     assert out.returncode == 0 and out.stdout == "" and not list(tmp_path.iterdir())
+
+
+def test_wdm_thermal_speeds_in_the_snapshot(tmp_path):
+    """WDM_On 1 with WDM_Vtherm_On 1: the writer adds thermal speeds to the velocities it writes,
+    from drand48's default state in each writing task (save.c:350-356, power.c:624-707), as the
+    oracle's restatement of the reference's loop adds them; the caller's array is unchanged."""
+    from oracle import pyoracle as orc
+    n = 3000
+    p, c, pos, vel = _setup(tmp_path, n)
+    p.WDM_On, p.WDM_Vtherm_On, p.WDM_PartMass_in_kev = 1, 1, 0.5
+    cfg = dict(cases.base_config(Nmesh=16), WDM_On=1, WDM_PartMass_in_kev=0.5)
+    keep = vel.copy()
+    assert aar.lib().aar_write_gadget(C.byref(p), C.byref(c), pos.ctypes.data_as(FP), vel.ctypes.data_as(FP), n,
+                                      None) == 0
+    np.testing.assert_array_equal(vel, keep)
+    _, ps, vs, _ = read_gadget_file(tmp_path / "snap")
+    np.testing.assert_array_equal(ps, pos)
+    want = orc.wdm_thermal(cfg, vel)
+    np.testing.assert_array_equal(vs, want)
+    assert np.abs(vs - vel).max() > 0
+    # split over three files: every file's writer starts its own drand48 sequence
+    rc, got = _split(p, c, pos, vel, 3, 0)
+    assert rc == 0 and got == 3
+    for f in range(3):
+        a, b = n * f // 3, n * (f + 1) // 3
+        _, _, vs, _ = read_gadget_file(tmp_path / f"snap.{f}")
+        np.testing.assert_array_equal(vs, orc.wdm_thermal(cfg, vel[a:b]))
+
+
+def test_wdm_thermal_speeds_in_the_layout_of_three_tasks(tmp_path):
+    from oracle import pyoracle as orc
+    p, c, t, pos, vel, x = _task_setup(tmp_path)
+    p.WDM_On, p.WDM_Vtherm_On = 1, 1
+    cfg = dict(cases.base_config(Nmesh=p.Nmesh), WDM_On=1)
+    assert aar.lib().aar_write_gadget_tasks(C.byref(p), C.byref(c), C.byref(t.c), pos.ctypes.data_as(FP),
+                                            vel.ctypes.data_as(FP), len(pos), 3) == 0
+    for r in range(3):
+        _, _, vs, ids = read_gadget_file(tmp_path / f"snap.{r}")
+        np.testing.assert_array_equal(vs, orc.wdm_thermal(cfg, vel[ids - 1]))
