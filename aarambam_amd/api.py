@@ -61,7 +61,7 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_read_params_res", "aar_l
            "aar_set_xy_group_cols", "aar_job_fetch_field_box",
            "aar_job_ng_statistics", "aar_mode_bispectra_covariance", "aar_mode_QS",
            "aar_mode_basis_expansion", "aar_job_reseed", "aar_ctx_cached_plans",
-           "aar_job_run_noalloc"]
+           "aar_job_run_noalloc", "aar_autotune_plans", "aar_xy_radix_plan", "aar_set_xy_radix_plan"]
 KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
 
 
@@ -147,6 +147,9 @@ def lib():
             "aar_server_destroy": (C.c_int, [vp]),
             "aar_autotune": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, C.POINTER(C.c_int)]),
             "aar_xy_tile_variant": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+            "aar_autotune_plans": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, C.POINTER(C.c_int)]),
+            "aar_xy_radix_plan": (C.c_int, [C.c_int, C.c_int]),
+            "aar_set_xy_radix_plan": (C.c_int, [C.c_int, C.c_int, C.c_int]),
             "aar_set_xy_tile_variant": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
             "aar_xy_group_cols": (C.c_int, [C.c_int, C.c_int]),
             "aar_set_xy_group_cols": (C.c_int, [C.c_int, C.c_int, C.c_int]),
@@ -282,6 +285,13 @@ class Context:
         s = C.c_double(0)
         _check(lib().aar_bench_pass(self.h, n, precision, kind, nf, iters, C.byref(s)), "aar_bench_pass")
         return s.value
+
+    def autotune_plans(self, n, precision=AAR_FP32, nf=8):
+        """aar_autotune with the radix plans' timings: (seconds per plan [3], chosen plan)."""
+        sec = np.full(3, -1.0)
+        pl = C.c_int(0)
+        _check(lib().aar_autotune_plans(self.h, n, precision, nf, _dptr(sec), C.byref(pl)), "aar_autotune_plans")
+        return sec, pl.value
 
     def cached_plans(self):
         """(count, device bytes) of the job tables the context keeps resident for later jobs."""

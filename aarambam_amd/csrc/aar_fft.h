@@ -29,16 +29,22 @@ template <class F, class V> __device__ __forceinline__ void call_store(F& f, int
 
 __host__ __device__ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
 
-// Balanced radix plan: ceil(log2 L / log2 Rmax) stages, bits spread evenly,
-// larger radices first.
+// Radix plans of ceil(log2 L / log2 Rmax) stages (the fewest LDS exchanges):
+//   PL = 0: the bits spread evenly, larger radices first (512 = 8 8 8, 1024 = 16 8 8);
+//   PL = 1: radix Rmax for every stage but the last, which takes the rest (16 16 2, 16 16 4);
+//   PL = 2: the rest first, then radix Rmax (2 16 16, 4 16 16).
+// The plans differ in rounding only; the autotuner picks one per precision and mesh size for
+// the x / y passes.
 __host__ __device__ constexpr int plan_nstages(int L, int Rmax) {
   return (ilog2c(L) + ilog2c(Rmax) - 1) / ilog2c(Rmax);
 }
-__host__ __device__ constexpr int plan_radix(int L, int Rmax, int s) {
-  return 1 << (ilog2c(L) / plan_nstages(L, Rmax) + (s < ilog2c(L) % plan_nstages(L, Rmax) ? 1 : 0));
+__host__ __device__ constexpr int plan_radix(int L, int Rmax, int s, int PL = 0) {
+  return PL == 1 ? (s < plan_nstages(L, Rmax) - 1 ? Rmax : 1 << (ilog2c(L) - ilog2c(Rmax) * (plan_nstages(L, Rmax) - 1)))
+       : PL == 2 ? (s > 0 ? Rmax : 1 << (ilog2c(L) - ilog2c(Rmax) * (plan_nstages(L, Rmax) - 1)))
+                 : 1 << (ilog2c(L) / plan_nstages(L, Rmax) + (s < ilog2c(L) % plan_nstages(L, Rmax) ? 1 : 0));
 }
-__host__ __device__ constexpr int plan_prefix(int L, int Rmax, int s) {
-  return s == 0 ? 1 : plan_prefix(L, Rmax, s - 1) * plan_radix(L, Rmax, s - 1);
+__host__ __device__ constexpr int plan_prefix(int L, int Rmax, int s, int PL = 0) {
+  return s == 0 ? 1 : plan_prefix(L, Rmax, s - 1, PL) * plan_radix(L, Rmax, s - 1, PL);
 }
 
 // LDS row pitch of a tile: one padding column when the tile plus its twiddle
@@ -273,15 +279,15 @@ __device__ __forceinline__ void fft_tile(cx_t<T>* tile, const cx_t<T>* tw, LoadF
 // half the LDS of the complex tile, so twice as many columns fit per CU.
 // Column-major layout only; the first stage loads and the last stage stores
 // through the caller's functors.
-template <typename T, int L, int C, int NT, int DIR, int S, class StoreF>
+template <typename T, int L, int C, int NT, int DIR, int S, int PL, class StoreF>
 __device__ __forceinline__ void fft_stage_split(T* tile, const cx_t<T>* tw, cx_t<T>* v, StoreF& store) {
   using C2 = cx_t<T>;
   constexpr int CP = C + 1;
   constexpr int E = C * L / NT;
   constexpr int RMAX = E < 16 ? E : 16;
   constexpr int NS = plan_nstages(L, RMAX);
-  constexpr int R = plan_radix(L, RMAX, S);
-  constexpr int P = plan_prefix(L, RMAX, S);
+  constexpr int R = plan_radix(L, RMAX, S, PL);
+  constexpr int P = plan_prefix(L, RMAX, S, PL);
   constexpr int NB = E / R;
   constexpr int ROWS = NT / C;
   const int col = threadIdx.x % C;
@@ -309,7 +315,7 @@ __device__ __forceinline__ void fft_stage_split(T* tile, const cx_t<T>* tw, cx_t
       for (int u = 0; u < R; ++u) store((i - k) * R + k + u * P, col, v[q * R + u]);
     }
   } else {
-    constexpr int R2 = plan_radix(L, RMAX, S + 1);
+    constexpr int R2 = plan_radix(L, RMAX, S + 1, PL);
     constexpr int NB2 = E / R2;
     constexpr int STR2 = L / R2;
     C2 w[E];
@@ -336,18 +342,18 @@ __device__ __forceinline__ void fft_stage_split(T* tile, const cx_t<T>* tw, cx_t
       }
       __syncthreads();
     }
-    fft_stage_split<T, L, C, NT, DIR, S + 1, StoreF>(tile, tw, w, store);
+    fft_stage_split<T, L, C, NT, DIR, S + 1, PL, StoreF>(tile, tw, w, store);
   }
 }
 
 // The first-stage inputs of fft_tile_split, loaded into registers through the
 // caller's functor; split_transform finishes the transform from them.  Apart,
 // they let a caller hold the next tile's inputs while it transforms this one.
-template <typename T, int L, int C, int NT, class LoadF>
+template <typename T, int L, int C, int NT, int PL = 0, class LoadF>
 __device__ __forceinline__ void split_load(cx_t<T>* v, LoadF load) {
   constexpr int E = C * L / NT;
   constexpr int RMAX = E < 16 ? E : 16;
-  constexpr int R = plan_radix(L, RMAX, 0);
+  constexpr int R = plan_radix(L, RMAX, 0, PL);
   constexpr int NB = E / R;
   constexpr int STR = L / R;
   constexpr int ROWS = NT / C;
@@ -359,26 +365,26 @@ __device__ __forceinline__ void split_load(cx_t<T>* v, LoadF load) {
     for (int t = 0; t < R; ++t) v[q * R + t] = call_load(load, i0 + q * ROWS + t * STR, col, q * R + t);
 }
 
-template <typename T, int L, int C, int NT, int DIR, class StoreF>
+template <typename T, int L, int C, int NT, int DIR, int PL = 0, class StoreF>
 __device__ __forceinline__ void split_transform(T* tile, const cx_t<T>* tw, cx_t<T>* v, StoreF store) {
   __syncthreads();  // the tile may still be read by the previous transform
-  fft_stage_split<T, L, C, NT, DIR, 0, StoreF>(tile, tw, v, store);
+  fft_stage_split<T, L, C, NT, DIR, 0, PL, StoreF>(tile, tw, v, store);
 }
 
-template <typename T, int L, int C, int NT, int DIR, class LoadF, class StoreF>
+template <typename T, int L, int C, int NT, int DIR, int PL = 0, class LoadF, class StoreF>
 __device__ __forceinline__ void fft_tile_split(T* tile, const cx_t<T>* tw, LoadF load, StoreF store) {
   cx_t<T> v[C * L / NT];
-  split_load<T, L, C, NT>(v, load);
-  split_transform<T, L, C, NT, DIR>(tile, tw, v, store);
+  split_load<T, L, C, NT, PL>(v, load);
+  split_transform<T, L, C, NT, DIR, PL>(tile, tw, v, store);
 }
 
 // Element slots of the first stage of fft_tile_split: the (pos, col) that
 // slot e of this thread is loaded from.
-template <typename T, int L, int C, int NT>
+template <typename T, int L, int C, int NT, int PL = 0>
 __device__ __forceinline__ void split_slot(int e, int& pos, int& col) {
   constexpr int E = C * L / NT;
   constexpr int RMAX = E < 16 ? E : 16;
-  constexpr int R = plan_radix(L, RMAX, 0);
+  constexpr int R = plan_radix(L, RMAX, 0, PL);
   constexpr int STR = L / R;
   constexpr int ROWS = NT / C;
   col = threadIdx.x % C;

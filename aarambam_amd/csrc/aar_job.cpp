@@ -630,7 +630,8 @@ static std::mutex g_tune_mu;
 static bool g_tuned[2][11] = {};
 
 template <typename T>
-static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, double* sec, int* chosen) {
+static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, double* sec, int* chosen,
+                      double* plan_sec = nullptr) {
   using C2 = cx_t<T>;
   const Geom g = make_geom(N);
   C2* d = (C2*)scratch;
@@ -665,28 +666,39 @@ static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, double* sec, i
     if (!rc) aar_set_xy_variant(sizeof(T) == 8, N, axis, bv);
     if (chosen) chosen[axis] = bv;
   }
-  // then the column groups of the x / y pass pairs with those shapes: whole passes over the nf
-  // fields, or field by field over all kz columns, halves or quarters
+  // then, jointly, the radix plan of the x / y passes (n = 512, 1024: three stages in different
+  // radix orders with the same tile shapes; plans differ in rounding only) and the column groups
+  // of the x / y pass pairs (whole passes over the nf fields, or field by field over all kz
+  // columns, halves or quarters), each pair timed both ways
+  const int nplan = (N == 512 || N == 1024) ? kXYPlanCount : 1;
   double best = 1e30;
-  int bg = 0;
-  for (int gc : {0, g.H, g.H / 2, g.H / 4}) {
-    if (rc || (gc > 0 && gc < 16)) break;
-    int r = Pipeline<T>::xy_pair(d, nf, g, tw, +1, ctx->stream, gc);
-    double t = 1e30;
-    if (!r) {
-      hipEventRecord(a, ctx->stream);
-      for (int q = 0; q < iters && !r; ++q) r = Pipeline<T>::xy_pair(d, nf, g, tw, q % 2 ? -1 : +1, ctx->stream, gc);
-      hipEventRecord(b, ctx->stream);
-      if (hipEventSynchronize(b) != hipSuccess) r = AAR_ERR_HIP;
-      float ms = 0;
-      if (!r && hipEventElapsedTime(&ms, a, b) == hipSuccess) t = ms * 1e-3 / iters;
+  int bg = 0, bp = 0;
+  for (int pl = 0; pl < nplan && !rc; ++pl) {
+    aar_set_xy_plan(sizeof(T) == 8, N, pl);
+    double bestp = 1e30;
+    for (int gc : {0, g.H, g.H / 2, g.H / 4}) {
+      if (rc || (gc > 0 && gc < 16)) break;
+      int r = Pipeline<T>::xy_pair(d, nf, g, tw, +1, ctx->stream, gc);
+      double t = 1e30;
+      if (!r) {
+        hipEventRecord(a, ctx->stream);
+        for (int q = 0; q < iters && !r; ++q) r = Pipeline<T>::xy_pair(d, nf, g, tw, q % 2 ? -1 : +1, ctx->stream, gc);
+        hipEventRecord(b, ctx->stream);
+        if (hipEventSynchronize(b) != hipSuccess) r = AAR_ERR_HIP;
+        float ms = 0;
+        if (!r && hipEventElapsedTime(&ms, a, b) == hipSuccess) t = ms * 1e-3 / iters;
+      }
+      if (r) rc = r;
+      bestp = std::min(bestp, t);
+      if (t < best) {
+        best = t;
+        bg = gc;
+        bp = pl;
+      }
     }
-    if (r) rc = r;
-    if (t < best) {
-      best = t;
-      bg = gc;
-    }
+    if (plan_sec) plan_sec[pl] = bestp < 1e30 ? bestp : -1.0;
   }
+  aar_set_xy_plan(sizeof(T) == 8, N, rc ? 0 : bp);
   if (!rc) aar_set_xy_group(sizeof(T) == 8, N, bg);
   hipEventDestroy(a);
   hipEventDestroy(b);
@@ -715,6 +727,29 @@ extern "C" int aar_autotune(aar_ctx* ctx, int n, int precision, int nf, double* 
   AAR_HIP(hipSetDevice(ctx->device));
   return precision == AAR_FP32 ? autotune_t<float>(ctx, n, nf, nullptr, seconds, variant)
                                : autotune_t<double>(ctx, n, nf, nullptr, seconds, variant);
+}
+
+extern "C" int aar_autotune_plans(aar_ctx* ctx, int n, int precision, int nf, double* plan_seconds, int* plan) {
+  if (!ctx || !pow2(n) || nf < 1 || (precision != AAR_FP32 && precision != AAR_FP64)) return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(ctx->device));
+  if (plan_seconds)
+    for (int q = 0; q < kXYPlanCount; ++q) plan_seconds[q] = -1.0;
+  const int rc = precision == AAR_FP32 ? autotune_t<float>(ctx, n, nf, nullptr, nullptr, nullptr, plan_seconds)
+                                       : autotune_t<double>(ctx, n, nf, nullptr, nullptr, nullptr, plan_seconds);
+  if (plan) *plan = aar_xy_plan(precision == AAR_FP64, n);
+  return rc;
+}
+
+extern "C" int aar_xy_radix_plan(int n, int precision) {
+  if (!pow2(n) || (precision != AAR_FP32 && precision != AAR_FP64)) return AAR_ERR_BAD_ARG;
+  return aar_xy_plan(precision == AAR_FP64, n);
+}
+
+extern "C" int aar_set_xy_radix_plan(int n, int precision, int plan) {
+  if (!pow2(n) || (precision != AAR_FP32 && precision != AAR_FP64)) return AAR_ERR_BAD_ARG;
+  if (plan < 0 || plan >= kXYPlanCount || (plan && n != 512 && n != 1024)) return AAR_ERR_BAD_ARG;
+  aar_set_xy_plan(precision == AAR_FP64, n, plan);
+  return AAR_OK;
 }
 
 extern "C" int aar_set_xy_tile_variant(int n, int precision, int axis, int variant) {

@@ -152,6 +152,10 @@ constexpr int kXYVariantCount = 4;
 int aar_xy_variant(int fp64, int n, int axis);
 void aar_set_xy_variant(int fp64, int n, int axis, int v);
 int aar_xy_group(int fp64, int n);
+// radix plans of the x / y passes (0 .. kXYPlanCount-1; others than 0 only for n = 512, 1024)
+constexpr int kXYPlanCount = 3;
+int aar_xy_plan(int fp64, int n);
+void aar_set_xy_plan(int fp64, int n, int plan);
 void aar_set_xy_group(int fp64, int n, int cols);
 // kernel launches of one xy_pair call
 int aar_xy_pair_dispatches(int nf, int H, int group_cols);

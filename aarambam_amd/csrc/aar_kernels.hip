@@ -75,6 +75,19 @@ void aar_set_xy_variant(int fp64, int n, int axis, int v) {
   if (l < 11 && axis >= 0 && axis < 2 && v >= 0 && v < kXYVariantCount) g_xy_variant[fp64 ? 1 : 0][l][axis].store(v, std::memory_order_relaxed);
 }
 
+// Radix plans of the x / y passes: [precision][log2 N] (aar_fft.h plan_radix)
+static std::atomic<int> g_xy_plan[2][11] = {};
+int aar_xy_plan(int fp64, int n) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l < 11 ? g_xy_plan[fp64 ? 1 : 0][l].load(std::memory_order_relaxed) : 0;
+}
+void aar_set_xy_plan(int fp64, int n, int plan) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  if (l < 11 && plan >= 0 && plan < kXYPlanCount) g_xy_plan[fp64 ? 1 : 0][l].store(plan, std::memory_order_relaxed);
+}
+
 // Column groups of the x / y pass pairs: [precision][log2 N], 0 = whole passes over all fields
 static std::atomic<int> g_xy_group[2][11] = {};
 int aar_xy_group(int fp64, int n) {
@@ -149,7 +162,7 @@ __device__ __forceinline__ void r2c_post(const cx_t<T>* tile, const cx_t<T>* tw,
 // ------------------------------------------------------------------ x / y passes
 // One tile = C adjacent columns of one field.  Tiles [0, nmain) cover the
 // kz < N/2 block, the rest the Nyquist plane.
-template <typename T, int L, int AXIS, int DIR, int V>
+template <typename T, int L, int AXIS, int DIR, int V, int PL>
 __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_waves_per_eu(XYCfg<T, L, V>::WPE)))
     k_xy_pass(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg, int nmain, int gper, int k0) {
   using C2 = cx_t<T>;
@@ -192,7 +205,7 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
   if constexpr (TPW == 1) {
     const Addr A = tile_addr(xcd_tile(blockIdx.x, gridDim.x));
     __syncthreads();  // twiddles
-    fft_tile_split<T, L, C, NT, DIR>(
+    fft_tile_split<T, L, C, NT, DIR, PL>(
         tile_r, tw, [&](int pos, int) { return f[off(A, pos)]; }, [&](int pos, int, C2 v) { f[off(A, pos)] = v; });
   } else {
     // tiles t0 .. t0 + TPW - 1 (the last workgroup may hold fewer); the condition of every
@@ -202,21 +215,21 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
     const int t0 = xcd_tile(blockIdx.x, gridDim.x) * TPW;
     C2 v[E], nv[E];
     Addr A = tile_addr(t0);
-    split_load<T, L, C, NT>(nv, [&](int pos, int) { return f[off(A, pos)]; });
+    split_load<T, L, C, NT, PL>(nv, [&](int pos, int) { return f[off(A, pos)]; });
     for (int i = 0; i < TPW && t0 + i < ntiles; ++i) {
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = nv[e];
       const Addr Ac = A;
       if (i + 1 < TPW && t0 + i + 1 < ntiles) {
         A = tile_addr(t0 + i + 1);
-        split_load<T, L, C, NT>(nv, [&](int pos, int) { return f[off(A, pos)]; });
+        split_load<T, L, C, NT, PL>(nv, [&](int pos, int) { return f[off(A, pos)]; });
       }
-      split_transform<T, L, C, NT, DIR>(tile_r, tw, v, [&](int pos, int, C2 x) { f[off(Ac, pos)] = x; });
+      split_transform<T, L, C, NT, DIR, PL>(tile_r, tw, v, [&](int pos, int, C2 x) { f[off(Ac, pos)] = x; });
     }
   }
 }
 
-template <typename T, int L, int AXIS, int DIR, int V>
+template <typename T, int L, int AXIS, int DIR, int V, int PL>
 static int launch_xy_v(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, const XYRange& r) {
   using Cfg = XYCfg<T, L, V>;
   if constexpr (!Cfg::VALID) {
@@ -230,23 +243,37 @@ static int launch_xy_v(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hip
     const int nnyq = r.nyq ? g.N / C : 0;
     if (nmain + nnyq == 0) return AAR_OK;
     dim3 grid((nmain + nnyq + Cfg::TPW - 1) / Cfg::TPW, nf);
-    hipLaunchKernelGGL((k_xy_pass<T, L, AXIS, DIR, V>), grid, dim3(NT), 0, st, f, g, tw, nmain, gper, r.k0);
+    hipLaunchKernelGGL((k_xy_pass<T, L, AXIS, DIR, V, PL>), grid, dim3(NT), 0, st, f, g, tw, nmain, gper, r.k0);
     return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
   }
 }
 
-// Tile variants other than 0 are built for the sizes where the choice matters.
-template <typename T, int L, int AXIS, int DIR>
-static int launch_xy(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, int v, const XYRange& r) {
+// Tile variants other than 0 are built for the sizes where the choice matters; radix plans other
+// than 0 for L = 512 and 1024 (three stages of radix 16, 8, 4 or 2 in different orders).
+template <typename T, int L, int AXIS, int DIR, int PL>
+static int launch_xy_p(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, int v,
+                       const XYRange& r) {
   if constexpr (L >= 256) {
     switch (v) {
-      case 1: return launch_xy_v<T, L, AXIS, DIR, 1>(f, nf, g, tw, st, r);
-      case 2: return launch_xy_v<T, L, AXIS, DIR, 2>(f, nf, g, tw, st, r);
-      case 3: return launch_xy_v<T, L, AXIS, DIR, 3>(f, nf, g, tw, st, r);
+      case 1: return launch_xy_v<T, L, AXIS, DIR, 1, PL>(f, nf, g, tw, st, r);
+      case 2: return launch_xy_v<T, L, AXIS, DIR, 2, PL>(f, nf, g, tw, st, r);
+      case 3: return launch_xy_v<T, L, AXIS, DIR, 3, PL>(f, nf, g, tw, st, r);
       default: break;
     }
   }
-  return launch_xy_v<T, L, AXIS, DIR, 0>(f, nf, g, tw, st, r);
+  return launch_xy_v<T, L, AXIS, DIR, 0, PL>(f, nf, g, tw, st, r);
+}
+
+template <typename T, int L, int AXIS, int DIR>
+static int launch_xy(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, int v, const XYRange& r) {
+  if constexpr (L >= 512) {
+    switch (aar_xy_plan(sizeof(T) == 8, L)) {
+      case 1: return launch_xy_p<T, L, AXIS, DIR, 1>(f, nf, g, tw, st, v, r);
+      case 2: return launch_xy_p<T, L, AXIS, DIR, 2>(f, nf, g, tw, st, v, r);
+      default: break;
+    }
+  }
+  return launch_xy_p<T, L, AXIS, DIR, 0>(f, nf, g, tw, st, v, r);
 }
 
 // ------------------------------------------------------------------ z passes

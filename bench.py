@@ -174,7 +174,7 @@ def north_star_1024(aar, ctx, steps=3, fp64_steps=2):
     N = 1024
     cfg, alpha, x0, glass, gbox = workload(N, 2000.0, "le")
     tabs = aar.tables(alpha, x0, glass, gbox)
-    ctx.autotune(N, aar.AAR_FP32, nf=8)
+    psec, plan = ctx.autotune_plans(N, aar.AAR_FP32, nf=8)
     r32 = time_stream(aar, ctx, cfg, tabs, aar.AAR_FP32, aar.AAR_SCHED_BATCHED, steps, 1, True, box=256)
     r64 = time_stream(aar, ctx, cfg, tabs, aar.AAR_FP64, aar.AAR_SCHED_PLAIN, fp64_steps, 0, False, box=256)
     M, K = modal_sizes(alpha)
@@ -188,6 +188,9 @@ def north_star_1024(aar, ctx, steps=3, fp64_steps=2):
             "fp64_plain": {"ms_per_job": round(ms64, 2), "mcells_s": round(N ** 3 / ms64 / 1e3, 1),
                            "steps": fp64_steps},
             "fp32_over_fp64": round(ms64 / ms32, 3),
+            "autotune": {"radix_plan": plan, "us_per_xy_pair_8_fields": [round(v * 1e6, 1) for v in psec],
+                         "xy_tile_variant": [aar.lib().aar_xy_tile_variant(N, aar.AAR_FP32, a) for a in (0, 1)],
+                         "xy_pair_group_cols": aar.lib().aar_xy_group_cols(N, aar.AAR_FP32)},
             "job_roofline": {"bytes_per_job": B, "frac": round(B / (ms32 * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)},
             "dominant_kernel": {"kernel": dom, "achieved_gbs": round(achieved, 1),
                                 "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -417,6 +420,7 @@ def main():
     # x / y tile shapes and x / y pair column groups for this mesh, picked by the autotuner
     # before the timed region
     tsec, tvar = ctx.autotune(N, aar.AAR_FP32, nf=8)
+    psec, pchosen = ctx.autotune_plans(N, aar.AAR_FP32, nf=8)  # the same search, reporting the plans
     if args.xy_group >= 0 and aar.lib().aar_set_xy_group_cols(N, aar.AAR_FP32, args.xy_group) != 0:
         sys.exit(f"--xy-group {args.xy_group}: not a valid column group for N = {N}")
     xy_group = aar.lib().aar_xy_group_cols(N, aar.AAR_FP32)
@@ -479,7 +483,11 @@ def main():
                        "us_per_launch_8_fields": {ax: [round(v * 1e6, 1) if v > 0 else None for v in tsec[i]]
                                                   for i, ax in enumerate(("x", "y"))},
                        "variants": "0: 128-B tile columns, 1: 64-B, 2: 256-B, 3: 128-B, four tiles per workgroup with the next tile's loads in flight (same radix plan)",
-                       "xy_pair_group_cols": xy_group}
+                       "xy_pair_group_cols": xy_group,
+                       "radix_plan": {"chosen": pchosen,
+                                      "us_per_xy_pair_8_fields": [round(v * 1e6, 1) if v > 0 else None for v in psec],
+                                      "plans": "0: radices spread evenly (512 = 8 8 8, 1024 = 16 8 8), "
+                                               "1: 16 first (16 16 2, 16 16 4), 2: 16 last (2 16 16, 4 16 16)"}}
     out["copy_peak_gbs"] = round(copy_peak(ctx, min(N, 512)), 1)
     if not args.no_1024:
         out["north_star_1024"] = north_star_1024(aar, ctx)

@@ -337,11 +337,23 @@ int aar_server_destroy(aar_server *s);
  * of an n^3 grid and keeps the fastest per axis for later jobs.  Every shape
  * uses the same radix plan, so the choice never changes results.  A job's
  * first aar_job_prepare at n >= 256 tunes by itself on its own buffers.
- * Then it times the column groups of the x / y pass pairs (whole passes over all fields, or
+ * Then it times the radix plans of the x / y passes at n = 512 and 1024 (aar_autotune_plans) and
+ * the column groups of the x / y pass pairs (whole passes over all fields, or
  * field by field over all columns, halves, quarters) and keeps the fastest (aar_xy_group_cols).
  * seconds: optional [2][4] mean seconds per launch (-1: shape not built for n);
  * variant: optional [2] chosen shape per axis. */
 int aar_autotune(aar_ctx *ctx, int n, int precision, int nf, double *seconds, int *variant);
+/* aar_autotune, also reporting the radix plans of the x / y passes it timed at n = 512 and 1024:
+ * plan_seconds (optional, 3 entries, -1 where not timed) mean seconds of one x + y pass over nf
+ * fields per plan; plan (optional) the plan kept.  Plans: 0 radices spread evenly (512 = 8 8 8,
+ * 1024 = 16 8 8), 1 radix 16 first (16 16 2, 16 16 4), 2 radix 16 last (2 16 16, 4 16 16).  Unlike
+ * the tile shapes, plans change the rounding of the transforms (by a few units in the last place).
+ * Counterpart of FFTW's planner choosing among algorithms (rfftw3d_mpi_create_plan, main.c:1566). */
+int aar_autotune_plans(aar_ctx *ctx, int n, int precision, int nf, double *plan_seconds, int *plan);
+/* radix plan in use for (n, precision), or a negative status; the setter pins one (0 only for
+ * meshes other than 512 and 1024) */
+int aar_xy_radix_plan(int n, int precision);
+int aar_set_xy_radix_plan(int n, int precision, int plan);
 /* tile shape in use for (n, precision, axis 0 = x, 1 = y) */
 int aar_xy_tile_variant(int n, int precision, int axis);
 /* pin a tile shape (0, 1, 2, 3); a shape not built for n falls back to 0 at launch */

@@ -148,3 +148,48 @@ def test_fft_512_matches_numpy(ctx, prec, tol):
     spec = ref * np.exp(2j * np.pi * rng.random(ref.shape))  # non-Hermitian on the kz = 0, N/2 planes
     ref2 = np.fft.irfftn(spec, s=(N, N, N), axes=(0, 1, 2)) * N ** 3
     assert rel_l2(ctx.fft3d(spec, +1, prec), ref2) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [(aar.AAR_FP64, 1e-12), (aar.AAR_FP32, 2e-6)])
+def test_fft_512_radix_plans_match_numpy(ctx, prec, tol):
+    """Each radix plan of the x / y passes (8 8 8, 16 16 2, 2 16 16 at 512) against numpy."""
+    N = 512
+    lib = aar.lib()
+    saved = lib.aar_xy_radix_plan(N, prec)
+    rng = np.random.default_rng(77)
+    real = rng.standard_normal((N, N, N))
+    ref = np.fft.rfftn(real)
+    outs = []
+    try:
+        for plan in range(3):
+            assert lib.aar_set_xy_radix_plan(N, prec, plan) == 0 and lib.aar_xy_radix_plan(N, prec) == plan
+            out = ctx.fft3d(real, -1, prec)
+            assert rel_l2(out, ref) < tol, plan
+            outs.append(out)
+    finally:
+        lib.aar_set_xy_radix_plan(N, prec, saved)
+    assert any(np.any(o != outs[0]) for o in outs[1:])  # the plans round differently
+    assert lib.aar_set_xy_radix_plan(256, prec, 1) == 100001  # plans other than 0 exist at 512, 1024
+
+
+def test_1024_job_radix_plans_agree(ctx):
+    """The 1024^3 local + equilateral FP32 job with each radix plan: displacement and potential
+    sub-boxes agree with plan 0's to the FP32 rounding of the transforms."""
+    N = 1024
+    lib = aar.lib()
+    saved = lib.aar_xy_radix_plan(N, aar.AAR_FP32)
+    cfg = cases.base_config(Nmesh=N, Nsample=N, Nmax=N, N_modes=12, Box=2000.0, GlassTileFac=16, Seed=99)
+    alpha, x0, glass, gbox = cases.job_inputs(cfg, cases.le_rows(), glass_n=64)
+    t = aar.tables(alpha, x0, glass, gbox)
+    boxes = []
+    try:
+        for plan in range(3):
+            assert lib.aar_set_xy_radix_plan(N, aar.AAR_FP32, plan) == 0
+            boxes.append(run_job(ctx, cfg, t, aar.AAR_FP32, aar.AAR_SCHED_BATCHED, box=64))
+    finally:
+        lib.aar_set_xy_radix_plan(N, aar.AAR_FP32, saved)
+    for b in boxes[1:]:
+        for n in DISP + ["phing_real"]:
+            assert rel_l2(b[n], boxes[0][n]) < 2e-6, n
+    sec, plan = ctx.autotune_plans(N, aar.AAR_FP32, nf=4)
+    assert (sec > 0).all() and plan == int(np.argmin(sec))
