@@ -337,23 +337,18 @@ extern "C" int aar_mode_basis_expansion(aar_ctx* ctx, double* alpha, double* sha
       for (double v : rr) RR += v;
       if (fabs(RR / scov[0]) <= rr_tol) break;
       if (nsel == mmax - 1) break;
-      // condition numbers of the selection and of the selection plus each candidate
+      // condition number of the selection: the selection alone, its first index as the
+      // "candidate" repeating a row, so build it as an m x m matrix from the first m - 1
+      // indices plus the last
       std::vector<int> cand;
       for (int i = 0; i < n; ++i)
         if (!msk[i]) cand.push_back(i);
       const int nc = (int)cand.size();
-      if ((rc = bt.ensure(m + 1, nc))) return rc;
-      AAR_HIP(hipMemcpyAsync(cand_d, cand.data(), sizeof(int) * nc, hipMemcpyHostToDevice, st));
-      // the selection alone: its first index as the "candidate" repeats a row,
-      // so build it as an m x m matrix from the first m - 1 indices plus the last
+      if ((rc = bt.ensure(m + 1, std::min(nc, 10)))) return rc;
       if ((rc = modal_bordered_launch(qqt_d, n, sel_d, m - 1, sel_d + (m - 1), 1, bt.mats, st)) ||
           (rc = eigs(m, 1, ev)))
         return rc;
       const double cond_here = cond_of(ev.data(), m);
-      if ((rc = modal_bordered_launch(qqt_d, n, sel_d, m, cand_d, nc, bt.mats, st)) || (rc = eigs(m + 1, nc, ev)))
-        return rc;
-      std::vector<double> fcond(nc);
-      for (int c = 0; c < nc; ++c) fcond[c] = cond_of(ev.data() + (size_t)c * (m + 1), m + 1) / cond_here - 1;
       if (cond_here > cond_tol) break;
       // correlation of every mode with the residual: QS of R, normalised
       if ((rc = modal_qs_launch(R_d, 1, b.w, b.i1, b.i2, b.i3, npts, b.p1, b.p2, b.p3, nmodes, b.ev, pmax, nk,
@@ -364,15 +359,25 @@ extern "C" int aar_mode_basis_expansion(aar_ctx* ctx, double* alpha, double* sha
       AAR_HIP(hipStreamSynchronize(st));
       std::vector<double> qr(nc);
       for (int c = 0; c < nc; ++c) qr[c] = bad[cand[c]] ? 0.0 : QR[cand[c]] / norms[cand[c]] / sqrt(RR);
-      // the ten candidates most correlated with the residual (ascending |QR|),
-      // of them the one whose addition raises the condition number least
+      // the ten candidates most correlated with the residual (ascending |QR|), of them the one
+      // whose addition raises the condition number of the selection least.  The reference
+      // evaluates the condition number with every candidate and reads it for these ten only,
+      // so the bordered matrices of the ten are the only ones decomposed here.
       std::vector<int> order(nc);
       for (int c = 0; c < nc; ++c) order[c] = c;
       std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return fabs(qr[a]) < fabs(qr[c]); });
-      const int first = std::max(0, nc - 10);
-      int best = order[first];
-      for (int k = first + 1; k < nc; ++k)
-        if (fcond[order[k]] < fcond[best]) best = order[k];
+      const int first = std::max(0, nc - 10), ntop = nc - first;
+      std::vector<int> top(ntop);
+      for (int k = 0; k < ntop; ++k) top[k] = cand[order[first + k]];
+      AAR_HIP(hipMemcpyAsync(cand_d, top.data(), sizeof(int) * ntop, hipMemcpyHostToDevice, st));
+      if ((rc = modal_bordered_launch(qqt_d, n, sel_d, m, cand_d, ntop, bt.mats, st)) || (rc = eigs(m + 1, ntop, ev)))
+        return rc;
+      std::vector<double> fcond(ntop);
+      for (int k = 0; k < ntop; ++k) fcond[k] = cond_of(ev.data() + (size_t)k * (m + 1), m + 1) / cond_here - 1;
+      int bk = 0;
+      for (int k = 1; k < ntop; ++k)
+        if (fcond[k] < fcond[bk]) bk = k;
+      const int best = order[first + bk];
       msk[cand[best]] = 1;
       ++nsel;
     }

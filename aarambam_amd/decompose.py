@@ -25,7 +25,9 @@ def basis_grid(N_modes, n_s, k_min, k_max, k_grid_size=150):
     kw = np.where(k > k_median * 2, 1 / np.sqrt(k), 1 / k)
     kw[k < 2 * k_min] = 1 / np.sqrt(k[k < 2 * k_min])
     # tetrapyd points k1 >= k2 >= k3 with k2 + k3 >= k1 (cmbbest.pyx:170-214)
-    t = np.array([[a, b, c] for a in range(k_grid_size) for b in range(a + 1) for c in range(b + 1)]).T
+    ar = np.arange(k_grid_size)
+    A, B, Cc = (g.reshape(-1) for g in np.meshgrid(ar, ar, ar, indexing="ij"))  # a major, c minor
+    t = np.stack([A, B, Cc])[:, (A >= B) & (B >= Cc)]
     keep = k[t[1]] + k[t[2]] >= k[t[0]]
     i1, i2, i3 = t[:, keep]
     w = np.prod(kw[np.stack([i1, i2, i3])], axis=0)
