@@ -61,7 +61,8 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_read_params_res", "aar_l
            "aar_set_xy_group_cols", "aar_job_fetch_field_box",
            "aar_job_ng_statistics", "aar_mode_bispectra_covariance", "aar_mode_QS",
            "aar_mode_basis_expansion", "aar_job_reseed", "aar_ctx_cached_plans",
-           "aar_job_run_noalloc", "aar_autotune_plans", "aar_xy_radix_plan", "aar_set_xy_radix_plan"]
+           "aar_job_run_noalloc", "aar_autotune_plans", "aar_xy_radix_plan", "aar_set_xy_radix_plan",
+           "aar_sym_eig_extremes"]
 KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
 
 
@@ -131,6 +132,7 @@ def lib():
             "aar_fft3d": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp]),
             "aar_gaussian_field": (C.c_int, [vp, PP, TP, C.c_int, dp]),
             "aar_ranlxd_device": (C.c_int, [vp, C.c_uint32, C.c_int, dp]),
+            "aar_sym_eig_extremes": (C.c_int, [vp, C.c_int, C.c_int, dp, dp]),
             "aar_bench_pass": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp]),
             "aar_job_set_profiling": (C.c_int, [vp, C.c_int]),
             "aar_job_set_graph": (C.c_int, [vp, C.c_int]),
@@ -334,6 +336,17 @@ class Context:
         out = np.zeros(S.shape[0] * args[8])
         _check(lib().aar_mode_QS(self.h, _dptr(out), _dptr(S), S.shape[0], *args), "aar_mode_QS")
         return out.reshape(S.shape[0], args[8])
+
+    def sym_eig_extremes(self, mats):
+        """(min |lambda|, max |lambda|) of each symmetric matrix of mats (nmat x m x m), from the
+        condition-number kernel of the basis expansion."""
+        mats = np.ascontiguousarray(mats, dtype=np.float64)
+        if mats.ndim == 2:
+            mats = mats[None]
+        nmat, m = mats.shape[0], mats.shape[1]
+        ext = np.zeros((nmat, 2))
+        _check(lib().aar_sym_eig_extremes(self.h, m, nmat, _dptr(mats), _dptr(ext)), "aar_sym_eig_extremes")
+        return ext
 
     def mode_basis_expansion(self, grid, S, regularization=0.0, Niter=1e4, rtol=1e-12, bad_modes=None,
                              RR_tol=1e-2, cond_tol=1e9, max_modes=None):

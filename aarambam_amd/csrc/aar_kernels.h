@@ -134,6 +134,8 @@ int modal_residual_launch(const double* S, const double* w, const int* i1, const
 // ncand column-major (nsel+1)^2 matrices qq[idx][idx], idx = sel followed by cand[b]
 int modal_bordered_launch(const double* qq, int nmodes, const int* sel, int nsel, const int* cand, int ncand,
                           double* out, hipStream_t st);
+// min and max eigenvalue magnitude of nmat symmetric m x m matrices (overwritten): ext[2b], ext[2b + 1]
+int modal_extremes_launch(double* mats, int m, int nmat, double* ext, hipStream_t st);
 
 // ---- binned statistics (aar_stats.hip)
 struct StatTerm {  // one weighted product of real fields a, b, c (c < 0: none)

@@ -257,6 +257,182 @@ __global__ void k_mode_bordered(const double* __restrict__ qq, int nmodes, const
   out[(long long)b * m * m + e] = qq[(long long)ir * nmodes + ic];
 }
 
+// Sum (max when kMax) of one value per thread over the block; every thread gets
+// the result.  The tree has a fixed shape, so the result is deterministic.
+template <bool kMax>
+__device__ double block_all(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const double u = __shfl_xor(v, o);
+    v = kMax ? fmax(v, u) : v + u;
+  }
+  const int wave = threadIdx.x >> 6;
+  __syncthreads();  // red may still be read by the previous call
+  if ((threadIdx.x & 63) == 0) red[wave] = v;
+  __syncthreads();
+  v = red[0];
+  for (int q = 1; q < kThreads / 64; ++q) v = kMax ? fmax(v, red[q]) : v + red[q];
+  return v;
+}
+
+// Number of eigenvalues of the symmetric tridiagonal matrix (d, off-diagonal
+// squares e2) below x: the signs of the LDL^T pivots of T - x I (Sturm count by
+// Kahan's recurrence, a pivot smaller than pivmin taken as -pivmin as LAPACK's
+// dstebz does).
+__device__ int sturm_count(const double* d, const double* e2, int m, double x, double pivmin) {
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  int c = q < 0;
+  for (int i = 1; i < m; ++i) {
+    q = (d[i] - x) - e2[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    c += q < 0;
+  }
+  return c;
+}
+
+// The k-th smallest eigenvalue (1-based) of the tridiagonal by multisection:
+// the wavefront's 64 lanes count at 64 evenly spaced points of [lo, hi] and
+// keep the sub-interval where the count reaches k, until the interval is
+// within two ulps of its end points.  Invariant: count(lo) < k <= count(hi).
+__device__ double kth_eigenvalue(const double* d, const double* e2, int m, int k, double lo, double hi,
+                                 double pivmin) {
+  const int lane = threadIdx.x & 63;
+  for (int it = 0; it < 64; ++it) {
+    const double tol = 2 * __DBL_EPSILON__ * fmax(fabs(lo), fabs(hi)) + 2 * pivmin;
+    if (hi - lo <= tol) break;
+    const double step = (hi - lo) / 65;
+    const double x = lo + (lane + 1) * step;
+    const unsigned long long hit = __ballot(sturm_count(d, e2, m, x, pivmin) >= k);
+    if (hit == 0) {
+      lo = __shfl(x, 63);
+    } else {
+      const int j = __builtin_ctzll(hit);
+      const double xl = __shfl(x, j > 0 ? j - 1 : 0);
+      hi = __shfl(x, j);
+      if (j > 0) lo = xl;
+    }
+  }
+  return 0.5 * (lo + hi);
+}
+
+// Smallest and largest eigenvalue magnitude of each symmetric m x m matrix of
+// mats (column-major, both triangles, overwritten): ext[2b] = min |l|,
+// ext[2b + 1] = max |l|.  For a symmetric matrix these are its extreme singular
+// values, whose ratio is np.linalg.cond's 2-norm condition number.
+// One block per matrix: Householder reduction to tridiagonal form (Golub and
+// Van Loan, Matrix Computations, 4th ed., Algorithm 8.3.1, with house() of
+// Algorithm 5.1.1), then bisection for eigenvalues by index, one index per
+// wavefront: the smallest, the largest, and the two either side of zero.
+// kLds: the reduction works on a copy of the matrix in LDS (m^2 doubles), else
+// in place in global memory.
+template <bool kLds>
+__global__ void __launch_bounds__(kThreads) k_mode_extremes(double* __restrict__ mats, int m, double* __restrict__ ext) {
+  extern __shared__ double smem[];
+  double* Ag = mats + (size_t)blockIdx.x * m * m;
+  double* A = kLds ? smem : Ag;
+  double* v = smem + (kLds ? (size_t)m * m : 0);  // [m] Householder vector
+  double* wv = v + m;        // [m] p, then w
+  double* d = wv + m;        // [m] diagonal
+  double* e2 = d + m;        // [m] squared off-diagonal
+  double* part = e2 + m;     // [kThreads] matvec partial sums
+  double* red = part + kThreads;  // [kThreads / 64]
+  const int t = threadIdx.x;
+  if (kLds) {
+    for (int e = t; e < m * m; e += kThreads) A[e] = Ag[e];
+    __syncthreads();
+  }
+  for (int k = 0; k + 2 < m; ++k) {
+    const int n2 = m - k - 1, o = k + 1;  // trailing block: rows and columns o .. m - 1
+    const double* x = A + (size_t)k * m + o;
+    double s = 0;
+    for (int i = 1 + t; i < n2; i += kThreads) s += x[i] * x[i];
+    const double sigma = block_all<false>(s, red);
+    const double alpha = x[0];
+    if (t == 0) d[k] = A[(size_t)k * m + k];
+    if (sigma == 0) {  // column already reduced: H = I
+      if (t == 0) e2[k] = alpha * alpha;
+      continue;
+    }
+    const double mu = sqrt(alpha * alpha + sigma);
+    const double beta = alpha <= 0 ? alpha - mu : -sigma / (alpha + mu);
+    const double tau = 2 * beta * beta / (sigma + beta * beta);
+    if (t == 0) e2[k] = mu * mu;
+    for (int i = t; i < n2; i += kThreads) v[i] = i == 0 ? 1.0 : x[i] / beta;
+    __syncthreads();
+    // p = tau A22 v: rows over the threads, the columns split into nsl slices
+    const int nsl = n2 <= kThreads ? kThreads / n2 : 1;
+    if (nsl > 1) {
+      const int i = t % n2, sl = t / n2;
+      if (sl < nsl) {
+        double acc = 0;
+        for (int j = sl; j < n2; j += nsl) acc += A[(size_t)(o + j) * m + o + i] * v[j];
+        part[sl * n2 + i] = acc;
+      }
+      __syncthreads();
+      if (t < n2) {
+        double acc = 0;
+        for (int q = 0; q < nsl; ++q) acc += part[q * n2 + t];
+        wv[t] = tau * acc;
+      }
+    } else {
+      for (int i = t; i < n2; i += kThreads) {
+        double acc = 0;
+        for (int j = 0; j < n2; ++j) acc += A[(size_t)(o + j) * m + o + i] * v[j];
+        wv[i] = tau * acc;
+      }
+    }
+    __syncthreads();
+    double pv = 0;
+    for (int i = t; i < n2; i += kThreads) pv += wv[i] * v[i];
+    const double c = 0.5 * tau * block_all<false>(pv, red);
+    for (int i = t; i < n2; i += kThreads) wv[i] -= c * v[i];  // w = p - (tau p.v / 2) v
+    __syncthreads();
+    // A22 -= v w^T + w v^T
+    if (nsl > 1) {
+      const int i = t % n2, sl = t / n2;
+      if (sl < nsl)
+        for (int j = sl; j < n2; j += nsl) A[(size_t)(o + j) * m + o + i] -= v[i] * wv[j] + wv[i] * v[j];
+    } else {
+      for (int i = t; i < n2; i += kThreads)
+        for (int j = 0; j < n2; ++j) A[(size_t)(o + j) * m + o + i] -= v[i] * wv[j] + wv[i] * v[j];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    if (m >= 2) {
+      d[m - 2] = A[(size_t)(m - 2) * m + m - 2];
+      const double e = A[(size_t)(m - 2) * m + m - 1];
+      e2[m - 2] = e * e;
+    }
+    d[m - 1] = A[(size_t)(m - 1) * m + m - 1];
+  }
+  __syncthreads();
+  // Gershgorin interval and the pivot floor (LAPACK dstebz's pivmin and widening)
+  double lo = INFINITY, hi = -INFINITY, emax = 0;
+  for (int i = t; i < m; i += kThreads) {
+    const double r = (i > 0 ? sqrt(e2[i - 1]) : 0.0) + (i + 1 < m ? sqrt(e2[i]) : 0.0);
+    lo = fmin(lo, d[i] - r);
+    hi = fmax(hi, d[i] + r);
+    if (i + 1 < m) emax = fmax(emax, e2[i]);
+  }
+  lo = -block_all<true>(-lo, red);
+  hi = block_all<true>(hi, red);
+  const double pivmin = __DBL_MIN__ * fmax(1.0, block_all<true>(emax, red));
+  const double tnorm = fmax(fabs(lo), fabs(hi));
+  lo -= 2.1 * tnorm * __DBL_EPSILON__ * m + 4.2 * pivmin;
+  hi += 2.1 * tnorm * __DBL_EPSILON__ * m + 4.2 * pivmin;
+  const int nneg = sturm_count(d, e2, m, 0.0, pivmin);
+  const int wave = t >> 6;
+  const int idx[4] = {1, m, nneg > 1 ? nneg : 1, nneg < m ? nneg + 1 : m};
+  const double l = m == 1 ? d[0] : kth_eigenvalue(d, e2, m, idx[wave], lo, hi, pivmin);
+  if ((t & 63) == 0) part[wave] = fabs(l);  // part is idle since the reduction's last barrier
+  __syncthreads();
+  if (t == 0) {
+    ext[2 * blockIdx.x] = fmin(part[2], part[3]);
+    ext[2 * blockIdx.x + 1] = fmax(part[0], part[1]);
+  }
+}
+
 }  // namespace
 
 int modal_residual_launch(const double* S, const double* w, const int* i1, const int* i2, const int* i3, int npts,
@@ -306,5 +482,25 @@ int modal_qs_launch(const double* S, int nshapes, const double* w, const int* i1
   if (hipGetLastError() != hipSuccess) return AAR_ERR_HIP;
   hipLaunchKernelGGL(k_mode_qs_reduce, dim3((unsigned)((nshapes * nmodes + 255) / 256)), dim3(256), 0, st, partial,
                      nsplit, nshapes, nmodes, qs);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+int modal_extremes_launch(double* mats, int m, int nmat, double* ext, hipStream_t st) {
+  const size_t lds = sizeof(double) * (4 * (size_t)m + kThreads + kThreads / 64);
+  if (m < 1 || nmat < 1 || lds > 64 * 1024) return AAR_ERR_UNSUPPORTED;
+  // the matrix itself in LDS when it fits the device's per-block LDS (160 KiB on
+  // gfx950: m <= 140)
+  int dev = 0, lds_max = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    return AAR_ERR_HIP;
+  const size_t lds_in = lds + sizeof(double) * (size_t)m * m;
+  if (lds_in <= (size_t)lds_max &&
+      hipFuncSetAttribute((const void*)k_mode_extremes<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lds_in) == hipSuccess) {
+    hipLaunchKernelGGL(k_mode_extremes<true>, dim3(nmat), dim3(kThreads), lds_in, st, mats, m, ext);
+  } else {
+    hipLaunchKernelGGL(k_mode_extremes<false>, dim3(nmat), dim3(kThreads), lds, st, mats, m, ext);
+  }
   return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
 }

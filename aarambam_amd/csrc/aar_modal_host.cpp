@@ -8,7 +8,6 @@
 #include <algorithm>
 #include <vector>
 
-#include <rocsolver/rocsolver.h>
 
 #include "../../include/aarambam.h"
 #include "aar_common.h"
@@ -137,8 +136,23 @@ void cg_solve(const std::vector<double>& A, int n, const std::vector<double>& b,
     } else {
       p = r;
     }
+    // q = A p, eight rows at a time: eight independent sums, each still taken
+    // in column order, so q is the same as row by row
     double pq = 0;
-    for (int i = 0; i < n; ++i) {
+    int i = 0;
+    for (; i + 8 <= n; i += 8) {
+      double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      const double* row = &A[(size_t)i * n];
+      for (int j = 0; j < n; ++j) {
+        const double pj = p[j];
+        for (int r = 0; r < 8; ++r) s[r] += row[(size_t)r * n + j] * pj;
+      }
+      for (int r = 0; r < 8; ++r) {
+        q[i + r] = s[r];
+        pq += p[i + r] * s[r];
+      }
+    }
+    for (; i < n; ++i) {
       double s = 0;
       const double* row = &A[(size_t)i * n];
       for (int j = 0; j < n; ++j) s += row[j] * p[j];
@@ -166,15 +180,9 @@ struct Dev {  // device buffers of one expansion
   }
 };
 
-// np.linalg.cond (2-norm) of symmetric matrices from their eigenvalues: max |l| / min |l|
-double cond_of(const double* d, int m) {
-  double lo = fabs(d[0]), hi = fabs(d[0]);
-  for (int i = 1; i < m; ++i) {
-    lo = std::min(lo, fabs(d[i]));
-    hi = std::max(hi, fabs(d[i]));
-  }
-  return lo > 0 ? hi / lo : INFINITY;
-}
+// np.linalg.cond (2-norm) of a symmetric matrix from its extreme eigenvalue
+// magnitudes (min |l|, max |l|)
+double cond_of(const double* ext) { return ext[0] > 0 ? ext[1] / ext[0] : INFINITY; }
 
 }  // namespace
 
@@ -247,53 +255,31 @@ extern "C" int aar_mode_basis_expansion(aar_ctx* ctx, double* alpha, double* sha
     }
   AAR_HIP(hipMemcpyAsync(qqt_d, QQt.data(), sizeof(double) * QQt.size(), hipMemcpyHostToDevice, st));
   const int mmax = max_modes == 0 ? n : max_modes;
-  rocblas_handle h = nullptr;
-  if (rocblas_create_handle(&h) != rocblas_status_success) return AAR_ERR_HIP;
-  struct HandleGuard {
-    rocblas_handle h;
-    ~HandleGuard() { rocblas_destroy_handle(h); }
-  } hg{h};
-  if (rocblas_set_stream(h, st) != rocblas_status_success) return AAR_ERR_HIP;
-  // eigenvalues of ncand symmetric m x m matrices (column-major, in bt.mats); the
-  // batch buffers grow with the selection and are freed on every exit
+  // ncand symmetric m x m matrices (column-major, in bt.mats) and their extreme
+  // eigenvalue magnitudes (bt.ext); the buffers grow with the selection and are
+  // freed on every exit
   struct Batch {
-    double *mats = nullptr, *eig = nullptr, *off = nullptr;
-    rocblas_int* info = nullptr;
+    double *mats = nullptr, *ext = nullptr;
     size_t cap_m = 0, cap_v = 0;
     ~Batch() { release(); }
     void release() {
-      void* b[] = {mats, eig, off, info};
-      for (void* x : b)
-        if (x) hipFree(x);
-      mats = eig = off = nullptr;
-      info = nullptr;
+      if (mats) hipFree(mats);
+      if (ext) hipFree(ext);
+      mats = ext = nullptr;
     }
     int ensure(int m, int ncand) {
-      const size_t nm = (size_t)m * m * ncand, nv = (size_t)m * ncand;
-      if (nm <= cap_m && nv <= cap_v && (size_t)ncand <= cap_v) return AAR_OK;
+      const size_t nm = (size_t)m * m * ncand, nv = 2 * (size_t)ncand;
+      if (nm <= cap_m && nv <= cap_v) return AAR_OK;
       release();
       cap_m = std::max(nm, cap_m + cap_m / 2);
-      cap_v = std::max(nv, cap_v + cap_v / 2);
+      cap_v = std::max(nv, cap_v);
       if (aar_device_alloc((void**)&mats, sizeof(double) * cap_m) ||
-          aar_device_alloc((void**)&eig, sizeof(double) * cap_v) ||
-          aar_device_alloc((void**)&off, sizeof(double) * cap_v) ||
-          aar_device_alloc((void**)&info, sizeof(rocblas_int) * cap_v))
+          aar_device_alloc((void**)&ext, sizeof(double) * cap_v))
         return AAR_ERR_ALLOC;
       return AAR_OK;
     }
   } bt;
-  auto eigs = [&](int m, int ncand, std::vector<double>& out) -> int {
-    if (rocsolver_dsyev_strided_batched(h, rocblas_evect_none, rocblas_fill_lower, m, bt.mats, m,
-                                        (rocblas_stride)m * m, bt.eig, m, bt.off, m, bt.info,
-                                        ncand) != rocblas_status_success)
-      return AAR_ERR_HIP;
-    out.resize((size_t)m * ncand);
-    if (hipMemcpyAsync(out.data(), bt.eig, sizeof(double) * out.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return AAR_ERR_HIP;
-    return AAR_OK;
-  };
-  std::vector<double> x, ev;
+  std::vector<double> x;
   for (int model = 0; model < nshapes; ++model) {
     std::vector<double> qst(n);
     for (int i = 0; i < n; ++i) qst[i] = (bad[i] ? 0.0 : QS[(size_t)model * n + i]) / norms[i];
@@ -317,6 +303,8 @@ extern "C" int aar_mode_basis_expansion(aar_ctx* ctx, double* alpha, double* sha
       cg_solve(A, m, rhs, tol, maxit, x);
       return idx;
     };
+    double ext_here[2] = {0, 0};
+    bool have_here = false;
     while (true) {
       if (n <= nsel) break;
       std::vector<int> idx = solve(rtol, niter);
@@ -330,33 +318,34 @@ extern "C" int aar_mode_basis_expansion(aar_ctx* ctx, double* alpha, double* sha
       if ((rc = modal_residual_launch(S_d, b.w, b.i1, b.i2, b.i3, npts, sel_d, m, coef_d, b.p1, b.p2, b.p3, b.ev,
                                       pmax, nk, R_d, rr_d, st)))
         return rc;
-      std::vector<double> rr(nrb);
+      // queued behind the residual, read after one wait: its weighted norm RR, the
+      // condition number of the selection and the correlation of every mode with
+      // the residual (QS of R); the last two are not needed when RR already
+      // converged.  The selection's condition number is that of the bordered
+      // matrix the previous step picked (the same modes, so the same spectrum);
+      // on the first step, its first m - 1 indices bordered by the last.
+      std::vector<int> cand;
+      for (int i = 0; i < n; ++i)
+        if (!msk[i]) cand.push_back(i);
+      const int nc = (int)cand.size();
+      if ((rc = bt.ensure(m + 1, std::max(1, std::min(nc, 10))))) return rc;
+      if (!have_here && ((rc = modal_bordered_launch(qqt_d, n, sel_d, m - 1, sel_d + (m - 1), 1, bt.mats, st)) ||
+                         (rc = modal_extremes_launch(bt.mats, m, 1, bt.ext, st))))
+        return rc;
+      if ((rc = modal_qs_launch(R_d, 1, b.w, b.i1, b.i2, b.i3, npts, b.p1, b.p2, b.p3, nmodes, b.ev, pmax, nk,
+                                qs_split, part_d, qs_d, st)))
+        return rc;
+      std::vector<double> rr(nrb), QR(n);
       AAR_HIP(hipMemcpyAsync(rr.data(), rr_d, sizeof(double) * nrb, hipMemcpyDeviceToHost, st));
+      if (!have_here) AAR_HIP(hipMemcpyAsync(ext_here, bt.ext, sizeof(ext_here), hipMemcpyDeviceToHost, st));
+      AAR_HIP(hipMemcpyAsync(QR.data(), qs_d, sizeof(double) * n, hipMemcpyDeviceToHost, st));
       AAR_HIP(hipStreamSynchronize(st));
       double RR = 0;
       for (double v : rr) RR += v;
       if (fabs(RR / scov[0]) <= rr_tol) break;
       if (nsel == mmax - 1) break;
-      // condition number of the selection: the selection alone, its first index as the
-      // "candidate" repeating a row, so build it as an m x m matrix from the first m - 1
-      // indices plus the last
-      std::vector<int> cand;
-      for (int i = 0; i < n; ++i)
-        if (!msk[i]) cand.push_back(i);
-      const int nc = (int)cand.size();
-      if ((rc = bt.ensure(m + 1, std::min(nc, 10)))) return rc;
-      if ((rc = modal_bordered_launch(qqt_d, n, sel_d, m - 1, sel_d + (m - 1), 1, bt.mats, st)) ||
-          (rc = eigs(m, 1, ev)))
-        return rc;
-      const double cond_here = cond_of(ev.data(), m);
+      const double cond_here = cond_of(ext_here);
       if (cond_here > cond_tol) break;
-      // correlation of every mode with the residual: QS of R, normalised
-      if ((rc = modal_qs_launch(R_d, 1, b.w, b.i1, b.i2, b.i3, npts, b.p1, b.p2, b.p3, nmodes, b.ev, pmax, nk,
-                                qs_split, part_d, qs_d, st)))
-        return rc;
-      std::vector<double> QR(n);
-      AAR_HIP(hipMemcpyAsync(QR.data(), qs_d, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-      AAR_HIP(hipStreamSynchronize(st));
       std::vector<double> qr(nc);
       for (int c = 0; c < nc; ++c) qr[c] = bad[cand[c]] ? 0.0 : QR[cand[c]] / norms[cand[c]] / sqrt(RR);
       // the ten candidates most correlated with the residual (ascending |QR|), of them the one
@@ -370,16 +359,23 @@ extern "C" int aar_mode_basis_expansion(aar_ctx* ctx, double* alpha, double* sha
       std::vector<int> top(ntop);
       for (int k = 0; k < ntop; ++k) top[k] = cand[order[first + k]];
       AAR_HIP(hipMemcpyAsync(cand_d, top.data(), sizeof(int) * ntop, hipMemcpyHostToDevice, st));
-      if ((rc = modal_bordered_launch(qqt_d, n, sel_d, m, cand_d, ntop, bt.mats, st)) || (rc = eigs(m + 1, ntop, ev)))
+      if ((rc = modal_bordered_launch(qqt_d, n, sel_d, m, cand_d, ntop, bt.mats, st)) ||
+          (rc = modal_extremes_launch(bt.mats, m + 1, ntop, bt.ext, st)))
         return rc;
+      std::vector<double> ext(2 * (size_t)ntop);
+      AAR_HIP(hipMemcpyAsync(ext.data(), bt.ext, sizeof(double) * ext.size(), hipMemcpyDeviceToHost, st));
+      AAR_HIP(hipStreamSynchronize(st));
       std::vector<double> fcond(ntop);
-      for (int k = 0; k < ntop; ++k) fcond[k] = cond_of(ev.data() + (size_t)k * (m + 1), m + 1) / cond_here - 1;
+      for (int k = 0; k < ntop; ++k) fcond[k] = cond_of(ext.data() + 2 * k) / cond_here - 1;
       int bk = 0;
       for (int k = 1; k < ntop; ++k)
         if (fcond[k] < fcond[bk]) bk = k;
       const int best = order[first + bk];
       msk[cand[best]] = 1;
       ++nsel;
+      ext_here[0] = ext[2 * bk];
+      ext_here[1] = ext[2 * bk + 1];
+      have_here = true;
     }
     std::vector<int> idx = solve(1e-16, 1000000);
     for (int i = 0; i < n; ++i) alpha[(size_t)model * n + i] = 0.0;
@@ -403,5 +399,21 @@ extern "C" int aar_mode_basis_expansion(aar_ctx* ctx, double* alpha, double* sha
     if (mse) mse[a] = fabs(ss + rr - 2 * sr) / ss;
   }
   if (shape_cov) memcpy(shape_cov, scov.data(), sizeof(double) * scov.size());
+  return AAR_OK;
+}
+
+extern "C" int aar_sym_eig_extremes(aar_ctx* ctx, int m, int nmat, const double* mats, double* ext) {
+  if (!ctx || !mats || !ext || m < 1 || nmat < 1) return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(aar_ctx_device(ctx)));
+  hipStream_t st = aar_ctx_stream(ctx);
+  Dev d;
+  double *mats_d, *ext_d;
+  int rc;
+  const size_t nm = (size_t)m * m * nmat;
+  if ((rc = d.alloc(&mats_d, nm)) || (rc = d.alloc(&ext_d, 2 * (size_t)nmat))) return rc;
+  AAR_HIP(hipMemcpyAsync(mats_d, mats, sizeof(double) * nm, hipMemcpyHostToDevice, st));
+  if ((rc = modal_extremes_launch(mats_d, m, nmat, ext_d, st))) return rc;
+  AAR_HIP(hipMemcpyAsync(ext, ext_d, sizeof(double) * 2 * nmat, hipMemcpyDeviceToHost, st));
+  AAR_HIP(hipStreamSynchronize(st));
   return AAR_OK;
 }

@@ -291,9 +291,11 @@ int aar_mode_QS(aar_ctx *ctx, double *QS, const double *S, int n_shapes, const d
  * shape.  Outputs: alpha[s*n_modes + n], shape_cov (n_shapes^2, may be NULL),
  * corr and mse per shape (the reference's convergence_correlation and
  * convergence_MSE, may be NULL), selected[s*n_modes + n] (may be NULL).  QQ,
- * QS, the residuals and their projections run on the device, the Gram
- * matrices' eigenvalues through rocSOLVER (syev, batched over the
- * candidates); the conjugate gradients on the host. */
+ * QS, the residuals and their projections run on the device, and so do the
+ * condition numbers (np.linalg.cond in the reference): the extreme eigenvalue
+ * magnitudes of the selection's Gram matrix and of the ten bordered ones, one
+ * block each (Householder tridiagonalisation, then bisection on Sturm
+ * counts); the conjugate gradients run on the host. */
 int aar_mode_basis_expansion(aar_ctx *ctx, double *alpha, double *shape_cov, double *corr, double *mse,
                              uint8_t *selected, const double *S, int n_shapes, const double *tetra_weights,
                              const int *tetra_i1, const int *tetra_i2, const int *tetra_i3, int tetra_npts,
@@ -379,6 +381,10 @@ int aar_gaussian_field(aar_ctx *ctx, const aar_params *p, const aar_tables *t, i
 int aar_bench_pass(aar_ctx *ctx, int n, int precision, int kind, int nf, int iters, double *sec);
 /* Device ranlxd draws for tests: `n` uniforms of stream `seed`. */
 int aar_ranlxd_device(aar_ctx *ctx, uint32_t seed, int n, double *out);
+/* The condition-number kernel of aar_mode_basis_expansion on its own, for tests:
+ * ext[2b] = min |lambda| and ext[2b + 1] = max |lambda| of the symmetric m x m
+ * matrices mats[b*m*m ...] (host arrays, b < nmat).  m <= 1982. */
+int aar_sym_eig_extremes(aar_ctx *ctx, int m, int nmat, const double *mats, double *ext);
 
 #ifdef __cplusplus
 }

@@ -164,6 +164,45 @@ def pymodal_expansion_check(r, ns):
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
 
+def _sym_cases(rng):
+    """Symmetric test matrices for the condition-number kernel: normalised Gram matrices (the
+    expansion's kind), shifted to be indefinite, block-diagonal (columns already reduced),
+    diagonal, and sizes either side of one block's 256 threads."""
+    out = []
+    for m in (1, 2, 3, 17, 64, 130, 257, 300):
+        X = rng.standard_normal((m + 7, m)) * np.exp(rng.uniform(-4, 0, m))
+        G = X.T @ X
+        dn = np.sqrt(np.diag(G))
+        G = G / dn[:, None] / dn[None, :]
+        out.append(("gram", G))
+        out.append(("indefinite", G - 0.5 * np.eye(m)))
+        if m >= 3:
+            B = G.copy()
+            B[: m // 2, m // 2:] = 0
+            B[m // 2:, : m // 2] = 0
+            out.append(("block", B))
+        out.append(("diagonal", np.diag(rng.uniform(-2, 3, m))))
+    return out
+
+
+def test_condition_kernel_matches_numpy(ctx):
+    """The extreme eigenvalue magnitudes behind the expansion's condition numbers against
+    numpy's eigvalsh: within 1e-13 m of the largest magnitude (the Householder reduction's
+    backward error), and np.linalg.cond to 1e-9 where it is below 1e4."""
+    rng = np.random.default_rng(77)
+    for kind, A in _sym_cases(rng):
+        m = A.shape[0]
+        ext = ctx.sym_eig_extremes(np.stack([A, A.T.copy()]))
+        ev = np.abs(np.linalg.eigvalsh(A))
+        tol = 1e-13 * max(m, 4) * ev.max()
+        for e in ext:
+            assert abs(e[0] - ev.min()) <= tol and abs(e[1] - ev.max()) <= tol, (kind, m, e, ev.min(), ev.max())
+            c = np.linalg.cond(A)
+            if c < 1e4:
+                assert abs(e[1] / e[0] / c - 1) <= 1e-9, (kind, m, e[1] / e[0], c)
+
+
+
 @pytest.mark.parametrize("name", ["equil", "scalari"])
 def test_device_expansion_matches_reference_vectors(ctx, name):
     """The device expansion against the reference's basis_expansion (vectors written by
