@@ -408,6 +408,8 @@ def main():
     ap.add_argument("--xy-group", type=int, default=-1,
                     help="pin the kz column groups of the x/y pass pairs (default: the autotuner's choice); "
                          "counter runs serialize every launch, which skews the autotuner, so they pin it")
+    ap.add_argument("--xy-plan", type=int, default=-1,
+                    help="pin the x/y radix plan (default: the autotuner's choice); counter runs pin it too")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: measures one GPU only; --gpus above 1 is not supported (one IC job runs on one "
@@ -424,6 +426,8 @@ def main():
     psec, pchosen = ctx.autotune_plans(N, aar.AAR_FP32, nf=8)  # the same search, reporting the plans
     if args.xy_group >= 0 and aar.lib().aar_set_xy_group_cols(N, aar.AAR_FP32, args.xy_group) != 0:
         sys.exit(f"--xy-group {args.xy_group}: not a valid column group for N = {N}")
+    if args.xy_plan >= 0 and aar.lib().aar_set_xy_radix_plan(N, aar.AAR_FP32, args.xy_plan) != 0:
+        sys.exit(f"--xy-plan {args.xy_plan}: not a valid radix plan for N = {N}")
     xy_group = aar.lib().aar_xy_group_cols(N, aar.AAR_FP32)
     # timed region: a stream of jobs with fresh seeds, eager launches each bracketed by HIP events
     eb = 0 if args.no_fp64 else min(128, N)
