@@ -54,7 +54,7 @@ struct Plan {
   void* block = nullptr;
   bool owned = false;  // hipMalloc'd by the cache (freed with the plan)
   size_t o_kv = 0, o_sinc = 0, o_x0 = 0, o_outc = 0, o_modes = 0, o_kmodes = 0, o_groups = 0, o_terms = 0,
-         o_weights = 0, o_fgroups = 0, o_frecs = 0, o_src_groups = 0, o_src_recs = 0, o_logk = 0, o_logT = 0,
+         o_weights = 0, o_fgroups = 0, o_frecs = 0, o_logk = 0, o_logT = 0,
          o_glass = 0;
   int nglass = 0;
   double glass_box = 0;
@@ -153,9 +153,6 @@ struct aar_job {
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
   bool stage_used[2] = {false, false};
   int stage_next = 0, stage_pending = -1;
-  // 2LPT source as a product sum of the six gradient fields, for the fused z pass
-  int* src_groups = nullptr;
-  NgRec4* src_recs = nullptr;
   bool fused_lpt = false;
   // slot map: phi_G, Phi, the work region; ZA and 2LPT displacement fields
   int s_phig = 0, s_phi = 1, s_reg = 2, s_disp = 8, s_disp2 = 2;
@@ -388,7 +385,7 @@ template <typename T> struct Exec {
         if ((rc = timed(j, st, AAR_K_Z, 3 * (S + R), [&] { return P::z_pass(disp, 3, g, tw(j), +1, st); })))
           return rc;
         if ((rc = timed(j, st, AAR_K_PRODUCTS, 7 * S,
-                        [&] { return P::z_ng(reg, 6, 1, j->src_groups, j->src_recs, g, tw(j), st); })))
+                        [&] { return P::z_source(reg, g, tw(j), st); })))
           return rc;
         if ((rc = xy(reg, 1, -1))) return rc;
         if ((rc = timed(j, st, AAR_K_LPT, 4 * S, [&] { return P::disp2_spec(reg, disp2, g, j->ks, st); })))
@@ -818,12 +815,6 @@ static int validate(const aar_params* p, const aar_tables* t) {
   return AAR_OK;
 }
 
-// src = f00 (f11 + f22) + f11 f22 - f01^2 - f02^2 - f12^2 over the gradient slots
-// 00, 01, 02, 11, 12, 22 (main.c:1286-1290), as one product group
-static NgFactored source_factored() {
-  return ng_factor({{0, 3, 1.0}, {0, 5, 1.0}, {3, 5, 1.0}, {1, 1, -1.0}, {2, 2, -1.0}, {4, 4, -1.0}}, {0, 6});
-}
-
 static int slots_needed(int M, int K, int sched) {
   if (sched == AAR_SCHED_PLAIN) return 2 + 9;
   return 2 + std::max(std::max(M, K), 9);
@@ -869,7 +860,6 @@ static int plan_layout(const aar_params* p, const aar_tables* t, int schedule, P
   int rc;
   if (p->WhichTransfer == 2 && (rc = aar_host_transfer_tables(p, t, logk, logT))) return rc;
   pl.nt = (int)logk.size();
-  const NgFactored src = source_factored();
   // pack every table at a 256-byte aligned offset of one block
   struct Part {
     const void* src;
@@ -889,8 +879,6 @@ static int plan_layout(const aar_params* p, const aar_tables* t, int schedule, P
       {s.weights.data(), bytes_of(s.weights), &pl.o_weights},
       {s.fz.groups.data(), bytes_of(s.fz.groups), &pl.o_fgroups},
       {s.fz.recs.data(), bytes_of(s.fz.recs), &pl.o_frecs},
-      {src.groups.data(), bytes_of(src.groups), &pl.o_src_groups},
-      {src.recs.data(), bytes_of(src.recs), &pl.o_src_recs},
       {logk.data(), bytes_of(logk), &pl.o_logk},
       {logT.data(), bytes_of(logT), &pl.o_logT},
       {t->glass, sizeof(float) * 3 * (size_t)t->nglass, &pl.o_glass},
@@ -967,8 +955,6 @@ static void plan_views(const aar_params* p, Plan& pl, aar_job* j) {
   j->weights = (NgWeight*)(b + pl.o_weights);
   j->fgroups = (int*)(b + pl.o_fgroups);
   j->frecs = (NgRec4*)(b + pl.o_frecs);
-  j->src_groups = (int*)(b + pl.o_src_groups);
-  j->src_recs = (NgRec4*)(b + pl.o_src_recs);
   j->logk = pl.nt ? (double*)(b + pl.o_logk) : nullptr;
   j->logT = pl.nt ? (double*)(b + pl.o_logT) : nullptr;
   j->glass = (float*)(b + pl.o_glass);

@@ -82,6 +82,9 @@ template <typename T> struct Pipeline {
   // groups: K + 1 offsets into recs, in records (multiples of four)
   static int z_ng(cx_t<T>* f, int M, int K, const int* groups, const NgRec4* recs, const Geom& g,
                   const cx_t<T>* tw, hipStream_t st);
+  // fused z pass of the 2LPT source: the six gradient fields 00, 01, 02, 11, 12, 22 in
+  // slots 0-5 (x, y already inverse transformed) -> z C2R -> source -> z R2C into slot 0
+  static int z_source(cx_t<T>* f, const Geom& g, const cx_t<T>* tw, hipStream_t st);
   static int gauss(cx_t<T>* phi, const Geom& g, const KSpace& ks, const uint32_t* seeds, double anorm,
                    double ns, double fac, double beta, double phase_shift, int fixed_amp, hipStream_t st);
   // out[m] = q_{modes[m]}(k) * phi(k) for m < M (cut and origin -> 0)

@@ -359,25 +359,35 @@ static int launch_z(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int di
 // over the first K field slots.  Replaces z-C2R of M fields + the product
 // kernel + z-R2C of K fields ((M + K) spectra of HBM traffic instead of
 // 3(M + K)).
-template <typename T, int H, int MMAX> struct ZNGCfg {
+template <typename T, int H, int MMAX, int FIX = 0> struct ZNGCfg {
   // complex elements per thread: 8 keeps the 1024-point rows in one
-  // wavefront each when at most 6 mode fields sit in registers
-  static constexpr int E = sizeof(T) == 4 ? (H == 512 && MMAX <= 6 ? 8 : 4) : 2;
+  // wavefront each when at most 6 mode fields sit in registers.  The 2LPT
+  // source (FIX) holds at most three fields at once and keeps 4: its 118
+  // VGPRs give four waves per SIMD where 8 elements took 266 (one wave), and
+  // the kernel runs 13 % faster at 1024^3 for its workgroup barriers
+  static constexpr int E = sizeof(T) == 4 ? (H == 512 && MMAX <= 6 && !FIX ? 8 : 4) : 2;
   static constexpr int C = sizeof(T) == 4 && (H < 512 || E == 8) ? 4 : 2;  // rows per block
   static constexpr int NT = C * H / E;
   // every row's exchanges inside one wavefront: no workgroup barriers
   static constexpr bool WL = wave_local(H, E) && NT % 64 == 0;
 };
 
-template <typename T, int H, int MMAX>
-__global__ void __launch_bounds__((ZNGCfg<T, H, MMAX>::NT))
+// FIX = 0: the NG sum, M mode fields and K products from the factored records;
+// FIX = 1: the 2LPT source, src = f00 (f11 + f22) + f11 f22 - f01^2 - f02^2 - f12^2
+// (main.c:1286-1290) of the six gradient fields 00, 01, 02, 11, 12, 22 in slots 0-5,
+// one product written over slot 0.  With FIX the mode loop is unrolled, so every
+// register index is a constant (no indexed register moves) and the product is
+// the reference's expression in its order.
+template <typename T, int H, int MMAX, int FIX = 0>
+__global__ void __launch_bounds__((ZNGCfg<T, H, MMAX, FIX>::NT))
     k_z_ng(cx_t<T>* base, int M, int K, const int* __restrict__ gs, const NgRec4* __restrict__ recs, Geom g,
            const cx_t<T>* __restrict__ twg) {
   using C2 = cx_t<T>;
-  using Cfg = ZNGCfg<T, H, MMAX>;
+  using Cfg = ZNGCfg<T, H, MMAX, FIX>;
   constexpr int C = Cfg::C, NT = Cfg::NT, E = Cfg::E;
   constexpr bool WL = Cfg::WL;
   constexpr int N = 2 * H;
+  static_assert(!FIX || MMAX == 6, "the 2LPT source has six fields");
   using TI = TileIdx<T, H, C, true, C * H / NT>;
   __shared__ C2 tile[TI::SIZE];
   __shared__ C2 tw[N];
@@ -405,9 +415,10 @@ __global__ void __launch_bounds__((ZNGCfg<T, H, MMAX>::NT))
       b[e] = pos == 0 ? fm[g.nq + r0 + col] : row[H - pos];
     }
   };
-  fetch(0, ra, rb);
-  for (int m = 0; m < M; ++m) {
-    if (m + 1 < M) fetch(m + 1, na, nb);
+  auto c2r_mode = [&](int m, int next) {  // next: the slot to prefetch, or -1
+    // keeps the scheduler from hoisting later fields' loads (unrolled mode loop) above this one
+    __builtin_amdgcn_sched_barrier(0);
+    if (next >= 0) fetch(next, na, nb);
     auto load = [&](int k, int, int e) {
       const C2 A = ra[e], B = rb[e];
       if (k == 0) return cmake<C2>(A.x + B.x, A.x - B.x);  // B = X[H] here
@@ -437,27 +448,75 @@ __global__ void __launch_bounds__((ZNGCfg<T, H, MMAX>::NT))
       ra[e] = na[e];
       rb[e] = nb[e];
     }
+  };
+  // every read of a mode slot is a run of register reads with one uniform index
+  auto get = [&](int m, C2* v) {
+    v[0] = cmake<C2>(fx0[m], fy0[m]);
+    v[1] = cmake<C2>(fx1[m], fy1[m]);
+    if constexpr (E >= 4) {
+      v[2] = cmake<C2>(fx2[m], fy2[m]);
+      v[3] = cmake<C2>(fx3[m], fy3[m]);
+    }
+    if constexpr (E == 8) {
+      v[4] = cmake<C2>(fx4[m], fy4[m]);
+      v[5] = cmake<C2>(fx5[m], fy5[m]);
+      v[6] = cmake<C2>(fx6[m], fy6[m]);
+      v[7] = cmake<C2>(fx7[m], fy7[m]);
+    }
+  };
+  // z R2C of one product (values hv at the thread's last-stage slots) into slot kk
+  auto r2c_product = [&](int kk, const C2* hv) {
+    tile_sync<WL>();  // the tile is free
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      int pos, col;
+      last_slot<T, H, C, NT, true>(e, pos, col);
+      tile[TI::at(pos, col)] = hv[e];
+    }
+    auto load = [&](int pos, int c) { return tile[TI::at(pos, c)]; };
+    auto store = [&](int pos, int c, C2 v) { tile[TI::at(pos, c)] = v; };
+    fft_tile<T, H, C, NT, -1, 2, true, true, WL>(tile, tw, load, store);
+    tile_sync<WL>();
+    r2c_post<T, H, C, NT, WL>(tile, tw, base + (long long)kk * g.fs, r0, g);
+  };
+  if constexpr (FIX == 1) {
+    // fields in the order the expression consumes them (00, 11, 22, then 01, 02, 12), so at
+    // most three are held at once and each square is subtracted as soon as its field is ready
+    C2 a00[E], a11[E], a22[E], hv[E];
+    fetch(0, ra, rb);
+    c2r_mode(0, 3);
+    c2r_mode(3, 5);
+    c2r_mode(5, 1);
+    get(0, a00);
+    get(3, a11);
+    get(5, a22);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      hv[e].x = a00[e].x * (a11[e].x + a22[e].x) + a11[e].x * a22[e].x;
+      hv[e].y = a00[e].y * (a11[e].y + a22[e].y) + a11[e].y * a22[e].y;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {  // slots 1, 2, 4: f01, f02, f12
+      const int m = q == 2 ? 4 : q + 1;
+      c2r_mode(m, q == 0 ? 2 : q == 1 ? 4 : -1);
+      get(m, a00);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        hv[e].x = hv[e].x - a00[e].x * a00[e].x;
+        hv[e].y = hv[e].y - a00[e].y * a00[e].y;
+      }
+    }
+    r2c_product(0, hv);
+    return;
+  } else {
+    fetch(0, ra, rb);
+    for (int m = 0; m < M; ++m) c2r_mode(m, m + 1 < M ? m + 1 : -1);
   }
   for (int kk = 0; kk < K; ++kk) {
     C2 hv[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) hv[e] = cmake<C2>(0, 0);
-    // h = sum over factors of f[v] * (sum over parts of c f[w]); every read of a mode slot is
-    // a run of indexed register reads with one uniform index
-    auto get = [&](int m, C2* v) {
-      v[0] = cmake<C2>(fx0[m], fy0[m]);
-      v[1] = cmake<C2>(fx1[m], fy1[m]);
-      if constexpr (E >= 4) {
-        v[2] = cmake<C2>(fx2[m], fy2[m]);
-        v[3] = cmake<C2>(fx3[m], fy3[m]);
-      }
-      if constexpr (E == 8) {
-        v[4] = cmake<C2>(fx4[m], fy4[m]);
-        v[5] = cmake<C2>(fx5[m], fy5[m]);
-        v[6] = cmake<C2>(fx6[m], fy6[m]);
-        v[7] = cmake<C2>(fx7[m], fy7[m]);
-      }
-    };
+    // h = sum over factors of f[v] * (sum over parts of c f[w])
     // one 64-byte scalar load per block of four records, issued before the header is examined
     typedef int blk_t __attribute__((ext_vector_type(16)));
     auto coef = [](const blk_t& k, int q) -> T {
@@ -500,18 +559,7 @@ __global__ void __launch_bounds__((ZNGCfg<T, H, MMAX>::NT))
       }
       r += (n + 4) & ~3;
     }
-    tile_sync<WL>();  // the tile is free
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      int pos, col;
-      last_slot<T, H, C, NT, true>(e, pos, col);
-      tile[TI::at(pos, col)] = hv[e];
-    }
-    auto load = [&](int pos, int c) { return tile[TI::at(pos, c)]; };
-    auto store = [&](int pos, int c, C2 v) { tile[TI::at(pos, c)] = v; };
-    fft_tile<T, H, C, NT, -1, 2, true, true, WL>(tile, tw, load, store);
-    tile_sync<WL>();
-    r2c_post<T, H, C, NT, WL>(tile, tw, base + (long long)kk * g.fs, r0, g);
+    r2c_product(kk, hv);
   }
 }
 
@@ -532,6 +580,26 @@ static int launch_z_ng(cx_t<T>* base, int M, int K, const int* gs, const NgRec4*
   if (M <= 8) return launch_z_ng_m<T, H, 8>(base, M, K, gs, recs, g, tw, st);
   if (M <= 16) return launch_z_ng_m<T, H, 16>(base, M, K, gs, recs, g, tw, st);
   return AAR_ERR_UNSUPPORTED;
+}
+
+template <typename T, int H>
+static int launch_z_source(cx_t<T>* base, const Geom& g, const cx_t<T>* tw, hipStream_t st) {
+  constexpr int C = ZNGCfg<T, H, 6, 1>::C, NT = ZNGCfg<T, H, 6, 1>::NT;
+  dim3 grid((unsigned)((long long)g.N * g.N / C));
+  hipLaunchKernelGGL((k_z_ng<T, H, 6, 1>), grid, dim3(NT), 0, st, base, 6, 1, nullptr, nullptr, g, tw);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+template <typename T>
+int Pipeline<T>::z_source(cx_t<T>* f, const Geom& g, const cx_t<T>* tw, hipStream_t st) {
+  switch (g.H) {
+    case 32: return launch_z_source<T, 32>(f, g, tw, st);
+    case 64: return launch_z_source<T, 64>(f, g, tw, st);
+    case 128: return launch_z_source<T, 128>(f, g, tw, st);
+    case 256: return launch_z_source<T, 256>(f, g, tw, st);
+    case 512: return launch_z_source<T, 512>(f, g, tw, st);
+    default: return AAR_ERR_UNSUPPORTED;
+  }
 }
 
 template <typename T>
