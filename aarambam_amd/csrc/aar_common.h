@@ -53,7 +53,8 @@ template <int DIR, typename C> __device__ __forceinline__ C cmul_i(C a) {
 // as real[r][z] (N^3 reals) in the bytes of the main block.
 struct Geom {
   int N, H;
-  int nlog, hlog;  // log2 N, log2 H (N is a power of two)
+  int nlog, hlog;  // log2 N, log2 H when N is a power of two
+  int pow2;        // N is a power of two (else the mixed-radix FFT and divisions decode rows)
   long long nq;  // offset (complex elements) of the Nyquist plane = N*N*H
   long long fs;  // field stride (complex elements) = N*N*(H+1)
 };
@@ -78,10 +79,37 @@ __host__ __device__ __forceinline__ void row_xy_log(int nlog, unsigned r, int& x
   x = (int)((t >> nlog) << kXBLog) + (int)(r & (kXB - 1));
 }
 
+// Mesh sizes: powers of two from 16 to 1024, and, through the mixed-radix FFT,
+// the other multiples of 16 up to 1024 whose prime factors are 2, 3, 5 and 7
+// (48, 80, 96, 112, 144, 160, 192, ..., 768, ..., 1008).  N % 16 == 0 keeps
+// every x-block of 8 planes whole and N/2 a multiple of the Gaussian kernel's
+// 8-mode chunks.
+__host__ __device__ constexpr bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+__host__ __device__ constexpr bool smooth7(int n) {
+  for (int p = 2; p <= 7; ++p)
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+__host__ __device__ constexpr bool mesh_ok(int n) { return n >= 16 && n <= 1024 && n % 16 == 0 && smooth7(n); }
+
+// Twiddle tables: exp(-2 pi i m / L) at tw[L + m] for the powers of two L <= 2048
+// (4096 entries), then one table of N entries per mesh size N that is not a
+// power of two, in increasing N, at tw_offset(N).
+constexpr int kTwPow2 = 4096;
+__host__ __device__ constexpr int tw_offset(int n) {
+  if (is_pow2(n)) return n;
+  int off = kTwPow2;
+  for (int m = 16; m < n; m += 16)
+    if (mesh_ok(m) && !is_pow2(m)) off += m;
+  return off;
+}
+constexpr int kTwTotal = tw_offset(1024 + 16) > kTwPow2 ? tw_offset(1024 + 16) : kTwPow2;
+
 static inline Geom make_geom(int N) {
   Geom g;
   g.N = N;
   g.H = N / 2;
+  g.pow2 = is_pow2(N);
   g.nlog = 0;
   while ((1 << g.nlog) < N) ++g.nlog;
   g.hlog = g.nlog - 1;

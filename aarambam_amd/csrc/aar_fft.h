@@ -443,3 +443,135 @@ template <typename C2>
 __device__ __forceinline__ void load_twiddles(C2* dst, const C2* src, int n, int nt) {
   for (int m = threadIdx.x; m < n; m += nt) dst[m] = src[m];
 }
+
+// ---------------------------------------------------------------- mixed-radix transforms
+// Lengths that are not powers of two (mesh sizes 48, 96, 160, 384, 768, ...) run through a
+// Stockham transform with a run-time plan of radices 8, 4, 2, 3, 5 and 7.  C independent
+// vectors are interleaved in LDS, element (pos, c) at pos * (C + 1) + c; each stage reads one
+// buffer and writes the other (the caller's two buffers), so a thread holds one butterfly at a
+// time and the plan needs no per-thread element count divisible by every radix.  Stage s of
+// radix R with P = the product of the earlier radices computes, for every butterfly
+// i in [0, L/R) with k = i mod P:  v_t = x[i + t L/R] w^(t k L/(P R)),  y = DFT_R(v),
+// x'[(i - k) R + k + u P] = y_u  (the same recurrence as the power-of-two tiles above).
+struct GPlan {
+  int L, ns;
+  int r[12];
+};
+
+// radices for a length L whose prime factors are 2, 3, 5 and 7: eights first, then a four
+// or a two, then threes, fives and sevens (host)
+inline GPlan make_gplan(int L) {
+  GPlan p{};
+  p.L = L;
+  int n = L;
+  while (n % 8 == 0 && p.ns < 12) p.r[p.ns++] = 8, n /= 8;
+  if (n % 4 == 0) p.r[p.ns++] = 4, n /= 4;
+  if (n % 2 == 0) p.r[p.ns++] = 2, n /= 2;
+  for (int q : {3, 5, 7})
+    while (n % q == 0 && p.ns < 12) p.r[p.ns++] = q, n /= q;
+  if (n != 1) p.ns = 0;  // not 7-smooth: no plan
+  return p;
+}
+
+// cos and sin of 2 pi m / R for R = 3, 5, 7
+template <int R> __device__ __forceinline__ double cos_r(int m) {
+  if constexpr (R == 3) return -0.5;
+  else if constexpr (R == 5) {
+    return (m == 1 || m == 4) ? 0.309016994374947424102293417182819059 : -0.809016994374947424102293417182819059;
+  } else {
+    return (m == 1 || m == 6) ? 0.623489801858733530525004884004239810
+           : (m == 2 || m == 5) ? -0.222520933956314404288902564496794759
+                                : -0.900968867902419126236102319507445051;
+  }
+}
+template <int R> __device__ __forceinline__ double sin_r(int m) {
+  const double s = m < R - m ? 1.0 : -1.0;
+  const int a = m < R - m ? m : R - m;
+  if constexpr (R == 3) return s * 0.866025403784438646763723170752936183;
+  else if constexpr (R == 5) {
+    return s * (a == 1 ? 0.951056516295153572116439333379382143 : 0.587785252292473129168705954639072769);
+  } else {
+    return s * (a == 1 ? 0.781831482468029808708444526674057750
+                : a == 2 ? 0.974927912181823607018131682993931217
+                         : 0.433883739117558120475768332848358754);
+  }
+}
+
+// DFT of odd prime length R by its definition, y_u = sum_t v_t exp(DIR 2 pi i u t / R)
+template <int R, int DIR, typename C> __device__ __forceinline__ void dftp(C* v) {
+  using T = decltype(C::x);
+  C y[R];
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    C acc = v[0];
+#pragma unroll
+    for (int t = 1; t < R; ++t) {
+      const int m = (u * t) % R;
+      if (m == 0) {
+        acc = cadd(acc, v[t]);
+      } else {
+        const T c = (T)cos_r<R>(m), s = (T)(DIR * sin_r<R>(m));
+        acc.x = fma(v[t].x, c, fma(-v[t].y, s, acc.x));
+        acc.y = fma(v[t].x, s, fma(v[t].y, c, acc.y));
+      }
+    }
+    y[u] = acc;
+  }
+#pragma unroll
+  for (int u = 0; u < R; ++u) v[u] = y[u];
+}
+
+template <int R, int DIR, typename C> __device__ __forceinline__ void dft_any(C* v) {
+  if constexpr (R == 3 || R == 5 || R == 7) dftp<R, DIR>(v);
+  else dftR<R, DIR>(v);
+}
+
+// One stage of radix R over C interleaved vectors: src -> dst.  tw[m * tws] = exp(-2 pi i m / L).
+template <typename T, int R, int DIR, int C>
+__device__ __forceinline__ void gstage(const cx_t<T>* src, cx_t<T>* dst, const cx_t<T>* tw, int tws, int L, int P,
+                                       int nt) {
+  using C2 = cx_t<T>;
+  constexpr int CP = C + 1;
+  const int S = L / R, tm = L / (P * R);
+  for (int q = threadIdx.x; q < C * S; q += nt) {
+    const int c = q % C, i = q / C, k = i % P;
+    C2 v[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) v[t] = src[(i + t * S) * CP + c];
+    if (P > 1) {
+#pragma unroll
+      for (int t = 1; t < R; ++t) {
+        C2 w = tw[t * k * tm * tws];
+        if constexpr (DIR > 0) w.y = -w.y;
+        v[t] = cmul(v[t], w);
+      }
+    }
+    dft_any<R, DIR>(v);
+#pragma unroll
+    for (int u = 0; u < R; ++u) dst[((i - k) * R + k + u * P) * CP + c] = v[u];
+  }
+}
+
+// The whole transform, starting in a (already written and synchronised); returns the buffer
+// that holds the result (a or b), synchronised.
+template <typename T, int DIR, int C>
+__device__ cx_t<T>* gfft(cx_t<T>* a, cx_t<T>* b, const cx_t<T>* tw, int tws, const GPlan& pl, int nt) {
+  int P = 1;
+  for (int s = 0; s < pl.ns; ++s) {
+    const int R = pl.r[s];
+    switch (R) {
+      case 2: gstage<T, 2, DIR, C>(a, b, tw, tws, pl.L, P, nt); break;
+      case 3: gstage<T, 3, DIR, C>(a, b, tw, tws, pl.L, P, nt); break;
+      case 4: gstage<T, 4, DIR, C>(a, b, tw, tws, pl.L, P, nt); break;
+      case 5: gstage<T, 5, DIR, C>(a, b, tw, tws, pl.L, P, nt); break;
+      case 7: gstage<T, 7, DIR, C>(a, b, tw, tws, pl.L, P, nt); break;
+      default: gstage<T, 8, DIR, C>(a, b, tw, tws, pl.L, P, nt); break;
+    }
+    __syncthreads();
+    P *= R;
+    cx_t<T>* t = a;
+    a = b;
+    b = t;
+  }
+  return a;
+}

@@ -352,7 +352,7 @@ template <typename T> struct Exec {
         if ((rc = timed(j, st, AAR_K_MODES, (1 + j->M) * S,
                         [&] { return P::modes_spec(F(j, j->s_phig), reg, j->M, j->modes, g, j->ks, st); })))
           return rc;
-        if (j->M <= 16 && g.N >= 64) {
+        if (j->M <= 16 && g.N >= 64 && g.pow2) {
           // x, y inverse passes of the M mode fields, then the fused z pass
           // (C2R, products, R2C), then y, x forward passes of the K products
           if ((rc = xy(reg, j->M, +1))) return rc;
@@ -582,8 +582,8 @@ extern "C" int aar_ctx_create(int device, aar_ctx** out) {
   aar_ctx* c = new aar_ctx();
   c->device = device;
   AAR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  std::vector<float2> t32(4096);
-  std::vector<double2> t64(4096);
+  std::vector<float2> t32(kTwTotal);
+  std::vector<double2> t64(kTwTotal);
   make_twiddles<float>(t32.data());
   make_twiddles<double>(t64.data());
   int rc = upload(&c->tw32, t32);
@@ -798,7 +798,7 @@ static int ctx_cosmology(aar_ctx* ctx, const aar_params* p, const aar_tables* t,
 
 static int validate(const aar_params* p, const aar_tables* t) {
   if (!p || !t) return AAR_ERR_BAD_ARG;
-  if (!pow2(p->Nmesh)) return AAR_ERR_UNSUPPORTED;
+  if (!mesh_ok(p->Nmesh)) return AAR_ERR_UNSUPPORTED;
   if (p->Nmax < p->Nsample) return AAR_ERR_NMAX;
   if (p->N_modes < 3 || !t->alpha || !t->x0) return AAR_ERR_BAD_ARG;
   if (!t->glass || t->nglass <= 0 || p->GlassTileFac <= 0) return AAR_ERR_BAD_ARG;
@@ -1131,7 +1131,7 @@ int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, i
   if ((rc = job_alloc(j, &j->pool, (size_t)j->nslots * j->g.fs * j->cxsize))) return fail(rc);
   if (schedule == AAR_SCHED_PLAIN && (rc = job_alloc(j, (void**)&j->acc, (size_t)j->g.fs * 16)))
     return fail(rc);
-  if (!ws && N >= 256) {  // first job of this size and precision: tune the x / y tiles on its pool
+  if (!ws && N >= 256 && is_pow2(N)) {  // first job of this size and precision: tune the x / y tiles on its pool
     bool tuned;
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1145,7 +1145,7 @@ int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, i
   std::vector<uint32_t> seeds((size_t)N * N);
   aar_seedtable(N, p->Seed, seeds.data());
   if ((rc = job_upload(j, &j->seeds, seeds))) return fail(rc);
-  j->fused_lpt = schedule == AAR_SCHED_BATCHED && N >= 64;
+  j->fused_lpt = schedule == AAR_SCHED_BATCHED && N >= 64 && is_pow2(N);
   j->s_disp = j->s_reg + 6;
   j->s_disp2 = j->s_reg + (j->fused_lpt ? 1 : 0);
   j->npart = (long long)p->GlassTileFac * p->GlassTileFac * p->GlassTileFac * t->nglass;
@@ -1625,7 +1625,7 @@ template <typename T> static int fft3d_t(aar_ctx* ctx, int N, int dir, const dou
 }
 
 extern "C" int aar_fft3d(aar_ctx* ctx, int n, int precision, int dir, const double* in, double* out) {
-  if (!ctx || !in || !out || !pow2(n) || (dir != 1 && dir != -1)) return AAR_ERR_BAD_ARG;
+  if (!ctx || !in || !out || !mesh_ok(n) || (dir != 1 && dir != -1)) return AAR_ERR_BAD_ARG;
   AAR_HIP(hipSetDevice(ctx->device));
   return precision == AAR_FP32 ? fft3d_t<float>(ctx, n, dir, in, out) : fft3d_t<double>(ctx, n, dir, in, out);
 }

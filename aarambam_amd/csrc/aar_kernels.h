@@ -168,4 +168,4 @@ int aar_xy_pair_dispatches(int nf, int H, int group_cols);
 int aar_ranlxd_draws_dev(uint32_t seed, int n, double* out, hipStream_t st);
 
 // Twiddle tables: tw[L + m] = exp(-2 pi i m / L) for every power of two L <= 2048.
-template <typename T> void make_twiddles(cx_t<T>* host_out /* 4096 entries */);
+template <typename T> void make_twiddles(cx_t<T>* host_out /* kTwTotal entries */);

@@ -18,11 +18,15 @@ template <typename T> void make_twiddles(cx_t<T>* tw) {
   const long double tp = 6.283185307179586476925286766559005768L;
   tw[0] = cmake<cx_t<T>>(0, 0);
   tw[1] = cmake<cx_t<T>>(0, 0);
-  for (int L = 2; L <= 2048; L *= 2)
+  auto fill = [&](int L, int at) {
     for (int m = 0; m < L; ++m) {
       long double a = tp * (long double)m / (long double)L;
-      tw[L + m] = cmake<cx_t<T>>((T)cosl(a), (T)(-sinl(a)));
+      tw[at + m] = cmake<cx_t<T>>((T)cosl(a), (T)(-sinl(a)));
     }
+  };
+  for (int L = 2; L <= 2048; L *= 2) fill(L, L);
+  for (int n = 16; n <= 1024; n += 16)
+    if (mesh_ok(n) && !is_pow2(n)) fill(n, tw_offset(n));
 }
 template void make_twiddles<float>(float2*);
 template void make_twiddles<double>(double2*);
@@ -63,13 +67,16 @@ template <typename T, int L, int V = 0> struct XYCfg {
 
 // Variant table: [precision 0 FP32 / 1 FP64][log2 N][axis]
 // Tuning tables, read by every launch and written by the autotuner or a pin from any thread.
+// Only powers of two have entries; other mesh sizes run the mixed-radix passes (0 everywhere).
 static std::atomic<int> g_xy_variant[2][11][2] = {};
 int aar_xy_variant(int fp64, int n, int axis) {
+  if (!is_pow2(n)) return 0;
   int l = 0;
   while ((1 << l) < n) ++l;
   return (l < 11 && axis >= 0 && axis < 2) ? g_xy_variant[fp64 ? 1 : 0][l][axis].load(std::memory_order_relaxed) : 0;
 }
 void aar_set_xy_variant(int fp64, int n, int axis, int v) {
+  if (!is_pow2(n)) return;
   int l = 0;
   while ((1 << l) < n) ++l;
   if (l < 11 && axis >= 0 && axis < 2 && v >= 0 && v < kXYVariantCount) g_xy_variant[fp64 ? 1 : 0][l][axis].store(v, std::memory_order_relaxed);
@@ -78,11 +85,13 @@ void aar_set_xy_variant(int fp64, int n, int axis, int v) {
 // Radix plans of the x / y passes: [precision][log2 N] (aar_fft.h plan_radix)
 static std::atomic<int> g_xy_plan[2][11] = {};
 int aar_xy_plan(int fp64, int n) {
+  if (!is_pow2(n)) return 0;
   int l = 0;
   while ((1 << l) < n) ++l;
   return l < 11 ? g_xy_plan[fp64 ? 1 : 0][l].load(std::memory_order_relaxed) : 0;
 }
 void aar_set_xy_plan(int fp64, int n, int plan) {
+  if (!is_pow2(n)) return;
   int l = 0;
   while ((1 << l) < n) ++l;
   if (l < 11 && plan >= 0 && plan < kXYPlanCount) g_xy_plan[fp64 ? 1 : 0][l].store(plan, std::memory_order_relaxed);
@@ -91,11 +100,13 @@ void aar_set_xy_plan(int fp64, int n, int plan) {
 // Column groups of the x / y pass pairs: [precision][log2 N], 0 = whole passes over all fields
 static std::atomic<int> g_xy_group[2][11] = {};
 int aar_xy_group(int fp64, int n) {
+  if (!is_pow2(n)) return 0;
   int l = 0;
   while ((1 << l) < n) ++l;
   return l < 11 ? g_xy_group[fp64 ? 1 : 0][l].load(std::memory_order_relaxed) : 0;
 }
 void aar_set_xy_group(int fp64, int n, int cols) {
+  if (!is_pow2(n)) return;
   int l = 0;
   while ((1 << l) < n) ++l;
   if (l < 11 && cols >= 0) g_xy_group[fp64 ? 1 : 0][l].store(cols, std::memory_order_relaxed);
@@ -352,6 +363,175 @@ static int launch_z(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int di
   return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
 }
 
+// ------------------------------------------------------------------ mixed-radix passes
+// Mesh sizes that are not powers of two: the same passes over the same layout, with the
+// mixed-radix Stockham transform (gfft, aar_fft.h) on C interleaved vectors in LDS.  Dynamic
+// LDS: the N-point twiddle table, then two buffers of C (L + 1) ... vectors of L elements.
+template <typename T> static int gen_cols(int L, int N) {
+  for (int C : {8, 4, 2})
+    if ((2LL * L * (C + 1) + N) * (long long)sizeof(cx_t<T>) <= 150 * 1024) return C;
+  return 2;
+}
+template <typename T> static size_t gen_lds(int L, int N, int C) {
+  return (2 * (size_t)L * (C + 1) + N) * sizeof(cx_t<T>);
+}
+constexpr int kGenThreads = 256;
+
+// x or y pass: tile t of C adjacent kz columns (t < nmain) or C adjacent rows of the Nyquist
+// plane, field blockIdx.y; twg: the N-point table (tw_offset(N))
+template <typename T, int C, int AXIS, int DIR>
+__global__ void __launch_bounds__(kGenThreads)
+    k_xy_gen(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg, GPlan pl, int nmain) {
+  using C2 = cx_t<T>;
+  constexpr int CP = C + 1;
+  extern __shared__ char smem[];
+  const int N = g.N, H = g.H;
+  C2* tw = reinterpret_cast<C2*>(smem);
+  C2* a = tw + N;
+  C2* b = a + (size_t)N * CP;
+  for (int m = threadIdx.x; m < N; m += kGenThreads) tw[m] = twg[m];
+  C2* f = base + (long long)blockIdx.y * g.fs;
+  const int t = blockIdx.x;
+  // offset of element pos of column c of this tile
+  auto off = [&](int pos, int c) -> long long {
+    if (t < nmain) {
+      const int per = H / C, q = t / per, kz = (t % per) * C + c;  // q: fixed y (x pass) or x (y pass)
+      return (AXIS == 0 ? row_of(N, pos, q) : row_of(N, q, pos)) * H + kz;
+    }
+    const int rc = (t - nmain) * C + c;  // fixed y (x pass) or x (y pass) of a Nyquist row
+    return g.nq + (AXIS == 0 ? row_of(N, pos, rc) : row_of(N, rc, pos));
+  };
+  for (int q = threadIdx.x; q < C * N; q += kGenThreads) {
+    const int c = q % C, pos = q / C;
+    a[pos * CP + c] = f[off(pos, c)];
+  }
+  __syncthreads();
+  const C2* r = gfft<T, DIR, C>(a, b, tw, 1, pl, kGenThreads);
+  for (int q = threadIdx.x; q < C * N; q += kGenThreads) {
+    const int c = q % C, pos = q / C;
+    f[off(pos, c)] = r[pos * CP + c];
+  }
+}
+
+template <typename T, int C, int AXIS, int DIR>
+static int launch_xy_gen_c(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st) {
+  const GPlan pl = make_gplan(g.N);
+  if (!pl.ns || g.H % C || g.N % C) return AAR_ERR_UNSUPPORTED;
+  const size_t lds = gen_lds<T>(g.N, g.N, C);
+  static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_xy_gen<T, C, AXIS, DIR>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+  if (attr != hipSuccess) return AAR_ERR_HIP;
+  const int nmain = g.N * (g.H / C);
+  dim3 grid(nmain + g.N / C, nf);
+  hipLaunchKernelGGL((k_xy_gen<T, C, AXIS, DIR>), grid, dim3(kGenThreads), lds, st, f, g, tw + tw_offset(g.N), pl,
+                     nmain);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+template <typename T, int AXIS, int DIR>
+static int launch_xy_gen(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st) {
+  switch (gen_cols<T>(g.N, g.N)) {
+    case 8: return launch_xy_gen_c<T, 8, AXIS, DIR>(f, nf, g, tw, st);
+    case 4: return launch_xy_gen_c<T, 4, AXIS, DIR>(f, nf, g, tw, st);
+    default: return launch_xy_gen_c<T, 2, AXIS, DIR>(f, nf, g, tw, st);
+  }
+}
+
+// z pass over C rows (blockIdx.x) of field blockIdx.y through an H-point transform: the
+// half-complex <-> real conversions of k_z_c2r / k_z_r2c around gfft (H-point twiddles are
+// every second entry of the N-point table)
+template <typename T, int C, int DIR>
+__global__ void __launch_bounds__(kGenThreads)
+    k_z_gen(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg, GPlan pl) {
+  using C2 = cx_t<T>;
+  constexpr int CP = C + 1;
+  extern __shared__ char smem[];
+  const int N = g.N, H = g.H;
+  C2* tw = reinterpret_cast<C2*>(smem);
+  C2* a = tw + N;
+  C2* b = a + (size_t)H * CP;
+  for (int m = threadIdx.x; m < N; m += kGenThreads) tw[m] = twg[m];
+  __syncthreads();
+  C2* f = base + (long long)blockIdx.y * g.fs;
+  const long long r0 = (long long)blockIdx.x * C;
+  C2* rows = f + r0 * H;
+  if constexpr (DIR > 0) {
+    // Y[k] = (X_k + X*_{H-k}) + i (X_k - X*_{H-k}) e^{+2 pi i k/N}; X_H from the Nyquist plane
+    for (int q = threadIdx.x; q < C * H; q += kGenThreads) {
+      const int c = q / H, k = q % H;
+      const C2* row = rows + (long long)c * H;
+      C2 y;
+      if (k == 0) {
+        const T x0 = row[0].x, xh = f[g.nq + r0 + c].x;
+        y = cmake<C2>(x0 + xh, x0 - xh);
+      } else {
+        const C2 A = row[k], B = row[H - k];
+        C2 w = tw[k];
+        w.y = -w.y;
+        const C2 sm = cadd(A, cconj(B)), d = csub(A, cconj(B));
+        const C2 dw = cmul(d, w);
+        y = cmake<C2>(sm.x - dw.y, sm.y + dw.x);
+      }
+      a[k * CP + c] = y;
+    }
+    __syncthreads();
+    const C2* r = gfft<T, +1, C>(a, b, tw, 2, pl, kGenThreads);
+    for (int q = threadIdx.x; q < C * H; q += kGenThreads) {
+      const int c = q / H, m = q % H;
+      rows[(long long)c * H + m] = r[m * CP + c];
+    }
+  } else {
+    for (int q = threadIdx.x; q < C * H; q += kGenThreads) {
+      const int c = q / H, m = q % H;
+      a[m * CP + c] = rows[(long long)c * H + m];
+    }
+    __syncthreads();
+    const C2* r = gfft<T, -1, C>(a, b, tw, 2, pl, kGenThreads);
+    // X_k = E_k + e^{-2 pi i k/N} O_k from Z_k and Z_{H-k}; X_0 and the Nyquist entry from Z_0
+    for (int q = threadIdx.x; q < C * H; q += kGenThreads) {
+      const int c = q / H, k = q % H;
+      C2* out = rows + (long long)c * H;
+      if (k == 0) {
+        const C2 z = r[c];
+        out[0] = cmake<C2>(z.x + z.y, (T)0);
+        f[g.nq + r0 + c] = cmake<C2>(z.x - z.y, (T)0);
+      } else {
+        const C2 A = r[k * CP + c], B = r[(H - k) * CP + c];
+        const C2 Ev = cscale(cadd(A, cconj(B)), (T)0.5);
+        const C2 dd = csub(A, cconj(B));
+        const C2 O = cmake<C2>(dd.y * (T)0.5, -dd.x * (T)0.5);
+        out[k] = cadd(Ev, cmul(O, tw[k]));
+      }
+    }
+  }
+}
+
+template <typename T, int C, int DIR>
+static int launch_z_gen_c(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st) {
+  const GPlan pl = make_gplan(g.H);
+  if (!pl.ns || ((long long)g.N * g.N) % C) return AAR_ERR_UNSUPPORTED;
+  const size_t lds = gen_lds<T>(g.H, g.N, C);
+  static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_z_gen<T, C, DIR>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+  if (attr != hipSuccess) return AAR_ERR_HIP;
+  dim3 grid((unsigned)((long long)g.N * g.N / C), nf);
+  hipLaunchKernelGGL((k_z_gen<T, C, DIR>), grid, dim3(kGenThreads), lds, st, f, g, tw + tw_offset(g.N), pl);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+
+template <typename T>
+static int launch_z_gen(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int dir, hipStream_t st) {
+  const int C = gen_cols<T>(g.H, g.N);
+  if (dir > 0) {
+    if (C == 8) return launch_z_gen_c<T, 8, +1>(f, nf, g, tw, st);
+    if (C == 4) return launch_z_gen_c<T, 4, +1>(f, nf, g, tw, st);
+    return launch_z_gen_c<T, 2, +1>(f, nf, g, tw, st);
+  }
+  if (C == 8) return launch_z_gen_c<T, 8, -1>(f, nf, g, tw, st);
+  if (C == 4) return launch_z_gen_c<T, 4, -1>(f, nf, g, tw, st);
+  return launch_z_gen_c<T, 2, -1>(f, nf, g, tw, st);
+}
+
 // ------------------------------------------------------------------ fused z pass of the NG sum
 // For C grid rows at a time: half-complex -> real along z for each of the M
 // mode fields (kept in registers), the products h_K = sum c f_i f_j of the
@@ -592,6 +772,7 @@ static int launch_z_source(cx_t<T>* base, const Geom& g, const cx_t<T>* tw, hipS
 
 template <typename T>
 int Pipeline<T>::z_source(cx_t<T>* f, const Geom& g, const cx_t<T>* tw, hipStream_t st) {
+  if (!g.pow2) return AAR_ERR_UNSUPPORTED;
   switch (g.H) {
     case 32: return launch_z_source<T, 32>(f, g, tw, st);
     case 64: return launch_z_source<T, 64>(f, g, tw, st);
@@ -605,7 +786,7 @@ int Pipeline<T>::z_source(cx_t<T>* f, const Geom& g, const cx_t<T>* tw, hipStrea
 template <typename T>
 int Pipeline<T>::z_ng(cx_t<T>* f, int M, int K, const int* gs, const NgRec4* recs, const Geom& g,
                       const cx_t<T>* tw, hipStream_t st) {
-  if (M > 16) return AAR_ERR_UNSUPPORTED;
+  if (M > 16 || !g.pow2) return AAR_ERR_UNSUPPORTED;
   switch (g.H) {
     case 32: return launch_z_ng<T, 32>(f, M, K, gs, recs, g, tw, st);
     case 64: return launch_z_ng<T, 64>(f, M, K, gs, recs, g, tw, st);
@@ -619,6 +800,11 @@ int Pipeline<T>::z_ng(cx_t<T>* f, int M, int K, const int* gs, const NgRec4* rec
 template <typename T>
 int Pipeline<T>::xy_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int axis, int dir,
                          hipStream_t st, int variant, XYRange r) {
+  if (!g.pow2) {  // whole passes only: column groups belong to the power-of-two tiles
+    if (r.k0 != 0 || (r.cols != 0 && r.cols != g.H) || !r.nyq) return AAR_ERR_UNSUPPORTED;
+    if (axis == 0) return dir > 0 ? launch_xy_gen<T, 0, 1>(f, nf, g, tw, st) : launch_xy_gen<T, 0, -1>(f, nf, g, tw, st);
+    return dir > 0 ? launch_xy_gen<T, 1, 1>(f, nf, g, tw, st) : launch_xy_gen<T, 1, -1>(f, nf, g, tw, st);
+  }
   const int v = variant >= 0 ? variant : aar_xy_variant(sizeof(T) == 8, g.N, axis);
   if (variant < 0 && v != 0) {  // a pinned shape that is not built for this size runs as shape 0
     const int rc = xy_pass(f, nf, g, tw, axis, dir, st, v, r);
@@ -649,7 +835,7 @@ int Pipeline<T>::xy_pair(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, i
                          int group_cols) {
   const int a0 = dir > 0 ? 0 : 1, a1 = 1 - a0;
   int rc;
-  if (group_cols < 0) group_cols = aar_xy_group(sizeof(T) == 8, g.N);
+  if (group_cols < 0) group_cols = g.pow2 ? aar_xy_group(sizeof(T) == 8, g.N) : 0;
   if (group_cols <= 0) {
     if ((rc = xy_pass(f, nf, g, tw, a0, dir, st))) return rc;
     return xy_pass(f, nf, g, tw, a1, dir, st);
@@ -666,6 +852,7 @@ int Pipeline<T>::xy_pair(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, i
 
 template <typename T>
 int Pipeline<T>::z_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int dir, hipStream_t st) {
+  if (!g.pow2) return launch_z_gen<T>(f, nf, g, tw, dir, st);
   switch (g.H) {
     case 8: return launch_z<T, 8>(f, nf, g, tw, dir, st);
     case 16: return launch_z<T, 16>(f, nf, g, tw, dir, st);
@@ -1254,12 +1441,11 @@ __device__ __forceinline__ double ddiv(double a, double b, double rb) {
 }
 
 // Per-launch constants of the CIC readout, computed on the host with the same IEEE operations
-// the kernel would use: bt = box / tile, rn3f = 1 / (float) N^3, and the reciprocals of gbox
-// and box for ddiv.
+// the kernel would use: bt = box / tile, n3f = (float) N^3 and its reciprocal, and the
+// reciprocals of gbox and box for ddiv.
 struct CicConst {
-  double gbox, rgbox, box, rbox, bt, vp1, vp2;
-  float rn3f;
-  int tile;
+  double gbox, rgbox, box, rbox, bt, vp1, vp2, n3f, rn3;
+  int tile, n3pow2;
 };
 
 // One particle: glass particle n of tile t (tile-major order of read_glass.c).
@@ -1310,8 +1496,7 @@ __device__ __forceinline__ void cic_particle(int n, int t, const cx_t<T>* __rest
     if constexpr (sizeof(T) == 4) return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(D) + off);
     else return D[off];
   };
-  // 1 / N^3 is a power of two: multiplying by it equals dividing by (float) N^3
-  const float rn3f = cc.rn3f;
+
   // all 48 gathers and interpolations first, in one branch-free block so the loads can be
   // issued together; then the position updates with their wrap branches
   double dis[3], dis2[3];
@@ -1323,7 +1508,9 @@ __device__ __forceinline__ void cic_particle(int n, int t, const cx_t<T>* __rest
                       at(D, o[4]) * f5 + at(D, o[5]) * f6 + at(D, o[6]) * f7 + at(D, o[7]) * f8);
     dis2[a] = (double)(at(D2, o[0]) * f1 + at(D2, o[1]) * f2 + at(D2, o[2]) * f3 + at(D2, o[3]) * f4 +
                        at(D2, o[4]) * f5 + at(D2, o[5]) * f6 + at(D2, o[6]) * f7 + at(D2, o[7]) * f8);
-    dis2[a] *= rn3f;
+    // dis2 / (float) N^3 (main.c:1503): for a power-of-two N the reciprocal is exact and the
+    // product equals the quotient; otherwise the correctly rounded ddiv
+    dis2[a] = cc.n3pow2 ? dis2[a] * cc.rn3 : ddiv(dis2[a], cc.n3f, cc.rn3);
   }
   float V[3];
 #pragma unroll
@@ -1361,7 +1548,9 @@ int Pipeline<T>::cic(const cx_t<T>* disp3, const cx_t<T>* disp2_3, const Geom& g
   CicConst cc;
   cc.gbox = gbox, cc.rgbox = 1.0 / gbox, cc.box = box, cc.rbox = 1.0 / box, cc.bt = box / tile;
   cc.vp1 = vp1, cc.vp2 = vp2, cc.tile = tile;
-  cc.rn3f = 1.0f / (float)((long long)g.N * g.N * g.N);
+  cc.n3f = (double)(float)((long long)g.N * g.N * g.N);
+  cc.rn3 = 1.0 / cc.n3f;
+  cc.n3pow2 = g.pow2;
   hipLaunchKernelGGL((k_cic<T>), dim3((unsigned)((nglass + 255) / 256), (unsigned)ntile), dim3(256), 0, st,
                      disp3, disp2_3, g, glass, nglass, cc, pos, vel);
   return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;

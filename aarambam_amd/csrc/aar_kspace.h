@@ -5,16 +5,31 @@
 #include "aar_kernels.h"
 
 __device__ __forceinline__ void decode_k(const Geom& g, long long e, int& x, int& y, int& kz) {
-  // N*N rows fit 32 bits for N <= 1024 (2^20 rows)
+  // N*N rows fit 32 bits for N <= 1024 (2^20 rows), and so does a whole field
   unsigned r;
-  if (e < g.nq) {
-    kz = (int)(e & (g.H - 1));
-    r = (unsigned)(e >> g.hlog);
+  if (g.pow2) {
+    if (e < g.nq) {
+      kz = (int)(e & (g.H - 1));
+      r = (unsigned)(e >> g.hlog);
+    } else {
+      kz = g.H;
+      r = (unsigned)(e - g.nq);
+    }
+    row_xy_log(g.nlog, r, x, y);
   } else {
-    kz = g.H;
-    r = (unsigned)(e - g.nq);
+    const unsigned ue = (unsigned)e, uh = (unsigned)g.H, un = (unsigned)g.N;
+    if (e < g.nq) {
+      r = ue / uh;
+      kz = (int)(ue - r * uh);
+    } else {
+      kz = g.H;
+      r = (unsigned)(e - g.nq);
+    }
+    const unsigned t = r >> kXBLog;
+    const unsigned xb = t / un;
+    y = (int)(t - xb * un);
+    x = (int)(xb << kXBLog) + (int)(r & (kXB - 1));
   }
-  row_xy_log(g.nlog, r, x, y);
 }
 
 // Runs body(x, y, kz) for element e of a block of consecutive elements.  When the block lies in
@@ -26,7 +41,7 @@ template <class F> __device__ __forceinline__ void with_k(const Geom& g, long lo
   if (g.H % (int)blockDim.x == 0 && e0 < g.nq) {
     int x, y, z0;
     decode_k(g, e0, x, y, z0);
-    body(x, y, (int)(e & (g.H - 1)));
+    body(x, y, (int)(e - e0) + z0);
   } else {
     int x, y, z;
     decode_k(g, e, x, y, z);

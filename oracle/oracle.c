@@ -438,7 +438,40 @@ static cplx *twiddle_table(int n, int sign) {
   return w;
 }
 
+/* w has n/2 + 1 entries exp(sign*2*pi*i*m/n); the others by conjugate symmetry */
+static inline cplx root_of(const cplx *w, int n, long e) { return e <= n / 2 ? w[e] : conj(w[n - e]); }
+
+/* mixed-radix decimation in time for lengths that are not powers of two (FFTW takes any
+   size): y = DFT_n of x[0], x[s], ..., x[(n-1)s], split by the smallest prime factor p of n
+   into p transforms of n/p and combined with the roots of unity of the ntop-point table */
+static void fft_mixed(const cplx *x, int n, int s, cplx *y, const cplx *w, int ntop) {
+  if (n == 1) {
+    y[0] = x[0];
+    return;
+  }
+  int p = 2;
+  while (n % p) p++;
+  const int m = n / p;
+  for (int r = 0; r < p; r++) fft_mixed(x + (size_t)r * s, m, s * p, y + (size_t)r * m, w, ntop);
+  cplx *t = malloc(sizeof(cplx) * n);
+  for (int k = 0; k < m; k++)
+    for (int u = 0; u < p; u++) {
+      cplx acc = 0;
+      for (int r = 0; r < p; r++) acc += y[r * m + k] * root_of(w, ntop, ((long)r * (k + u * m) % n) * (ntop / n));
+      t[k + u * m] = acc;
+    }
+  memcpy(y, t, sizeof(cplx) * n);
+  free(t);
+}
+
 static void fft1d(cplx *a, int n, const cplx *w) {
+  if (n & (n - 1)) {
+    cplx *x = malloc(sizeof(cplx) * n);
+    memcpy(x, a, sizeof(cplx) * n);
+    fft_mixed(x, n, 1, a, w, n);
+    free(x);
+    return;
+  }
   /* iterative radix-2, decimation in time; w[m] = exp(sign*2*pi*i*m/n) */
   for (int i = 1, j = 0; i < n; i++) {
     int bit = n >> 1;
