@@ -533,8 +533,12 @@ __device__ __forceinline__ void gstage(const cx_t<T>* src, cx_t<T>* dst, const c
   using C2 = cx_t<T>;
   constexpr int CP = C + 1;
   const int S = L / R, tm = L / (P * R);
+  // k = i mod P without an integer division: floor((i + 1/2) / P) in FP32 is exact for
+  // i, P < 2^11 (the quotient stays 1/(2P) away from every integer)
+  const float rp = 1.0f / (float)P;
   for (int q = threadIdx.x; q < C * S; q += nt) {
-    const int c = q % C, i = q / C, k = i % P;
+    const int c = q % C, i = q / C;
+    const int k = i - P * (int)(((float)i + 0.5f) * rp);
     C2 v[R];
 #pragma unroll
     for (int t = 0; t < R; ++t) v[t] = src[(i + t * S) * CP + c];

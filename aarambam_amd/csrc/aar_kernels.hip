@@ -375,7 +375,7 @@ template <typename T> static int gen_cols(int L, int N) {
 template <typename T> static size_t gen_lds(int L, int N, int C) {
   return (2 * (size_t)L * (C + 1) + N) * sizeof(cx_t<T>);
 }
-constexpr int kGenThreads = 256;
+constexpr int kGenThreads = 512;
 
 // x or y pass: tile t of C adjacent kz columns (t < nmain) or C adjacent rows of the Nyquist
 // plane, field blockIdx.y; twg: the N-point table (tw_offset(N))
@@ -392,14 +392,14 @@ __global__ void __launch_bounds__(kGenThreads)
   for (int m = threadIdx.x; m < N; m += kGenThreads) tw[m] = twg[m];
   C2* f = base + (long long)blockIdx.y * g.fs;
   const int t = blockIdx.x;
+  const bool main_tile = t < nmain;
+  const int per = H / C;
+  const int q = main_tile ? t / per : 0, kz0 = main_tile ? (t % per) * C : 0;  // q: fixed y (x) or x (y)
+  const int rc0 = main_tile ? 0 : (t - nmain) * C;  // first fixed y (x pass) or x (y pass) of a Nyquist tile
   // offset of element pos of column c of this tile
   auto off = [&](int pos, int c) -> long long {
-    if (t < nmain) {
-      const int per = H / C, q = t / per, kz = (t % per) * C + c;  // q: fixed y (x pass) or x (y pass)
-      return (AXIS == 0 ? row_of(N, pos, q) : row_of(N, q, pos)) * H + kz;
-    }
-    const int rc = (t - nmain) * C + c;  // fixed y (x pass) or x (y pass) of a Nyquist row
-    return g.nq + (AXIS == 0 ? row_of(N, pos, rc) : row_of(N, rc, pos));
+    if (main_tile) return (AXIS == 0 ? row_of(N, pos, q) : row_of(N, q, pos)) * H + kz0 + c;
+    return g.nq + (AXIS == 0 ? row_of(N, pos, rc0 + c) : row_of(N, rc0 + c, pos));
   };
   for (int q = threadIdx.x; q < C * N; q += kGenThreads) {
     const int c = q % C, pos = q / C;
@@ -457,8 +457,8 @@ __global__ void __launch_bounds__(kGenThreads)
   C2* rows = f + r0 * H;
   if constexpr (DIR > 0) {
     // Y[k] = (X_k + X*_{H-k}) + i (X_k - X*_{H-k}) e^{+2 pi i k/N}; X_H from the Nyquist plane
-    for (int q = threadIdx.x; q < C * H; q += kGenThreads) {
-      const int c = q / H, k = q % H;
+    for (int c = 0; c < C; ++c)
+    for (int k = threadIdx.x; k < H; k += kGenThreads) {
       const C2* row = rows + (long long)c * H;
       C2 y;
       if (k == 0) {
@@ -476,20 +476,16 @@ __global__ void __launch_bounds__(kGenThreads)
     }
     __syncthreads();
     const C2* r = gfft<T, +1, C>(a, b, tw, 2, pl, kGenThreads);
-    for (int q = threadIdx.x; q < C * H; q += kGenThreads) {
-      const int c = q / H, m = q % H;
-      rows[(long long)c * H + m] = r[m * CP + c];
-    }
+    for (int c = 0; c < C; ++c)
+      for (int m = threadIdx.x; m < H; m += kGenThreads) rows[(long long)c * H + m] = r[m * CP + c];
   } else {
-    for (int q = threadIdx.x; q < C * H; q += kGenThreads) {
-      const int c = q / H, m = q % H;
-      a[m * CP + c] = rows[(long long)c * H + m];
-    }
+    for (int c = 0; c < C; ++c)
+      for (int m = threadIdx.x; m < H; m += kGenThreads) a[m * CP + c] = rows[(long long)c * H + m];
     __syncthreads();
     const C2* r = gfft<T, -1, C>(a, b, tw, 2, pl, kGenThreads);
     // X_k = E_k + e^{-2 pi i k/N} O_k from Z_k and Z_{H-k}; X_0 and the Nyquist entry from Z_0
-    for (int q = threadIdx.x; q < C * H; q += kGenThreads) {
-      const int c = q / H, k = q % H;
+    for (int c = 0; c < C; ++c)
+    for (int k = threadIdx.x; k < H; k += kGenThreads) {
       C2* out = rows + (long long)c * H;
       if (k == 0) {
         const C2 z = r[c];

@@ -51,6 +51,8 @@ MIX = [  # (N, template, seed, fNL, precision, schedule)
     (32, "equil", 15, -80.0, aar.AAR_FP32, aar.AAR_SCHED_BATCHED),
     (64, "scalari", 16, 120.0, aar.AAR_FP32, aar.AAR_SCHED_BATCHED),
     (128, "local", 17, -20.0, aar.AAR_FP32, aar.AAR_SCHED_BATCHED),
+    (96, "equil", 18, 64.0, aar.AAR_FP32, aar.AAR_SCHED_BATCHED),  # mixed-radix passes
+    (48, "scalari", 19, -33.0, aar.AAR_FP64, aar.AAR_SCHED_PLAIN),
 ]
 
 
