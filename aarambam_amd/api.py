@@ -62,7 +62,7 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_read_params_res", "aar_l
            "aar_job_ng_statistics", "aar_mode_bispectra_covariance", "aar_mode_QS",
            "aar_mode_basis_expansion", "aar_job_reseed", "aar_ctx_cached_plans",
            "aar_job_run_noalloc", "aar_autotune_plans", "aar_xy_radix_plan", "aar_set_xy_radix_plan",
-           "aar_sym_eig_extremes"]
+           "aar_sym_eig_extremes", "aar_ctx_set_jit", "aar_job_jit"]
 KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
 
 
@@ -136,6 +136,8 @@ def lib():
             "aar_bench_pass": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp]),
             "aar_job_set_profiling": (C.c_int, [vp, C.c_int]),
             "aar_job_set_graph": (C.c_int, [vp, C.c_int]),
+            "aar_ctx_set_jit": (C.c_int, [vp, C.c_int]),
+            "aar_job_jit": (C.c_int, [vp]),
             "aar_job_kernel_stats": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), dp, dp]),
             "aar_server_create": (C.c_int, [vp, C.c_longlong, C.c_int, C.POINTER(vp)]),
             "aar_server_submit": (C.c_int, [vp, PP, TP, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_longlong)]),
@@ -250,6 +252,11 @@ class Context:
 
     def sync(self):
         _check(lib().aar_ctx_sync(self.h), "aar_ctx_sync")
+
+    def set_jit(self, on=True):
+        """Batched jobs prepared from now on compile their fused z pass for their own product
+        schedule (on) or run the library's kernel (off); results are identical."""
+        _check(lib().aar_ctx_set_jit(self.h, int(on)), "aar_ctx_set_jit")
 
     def close(self):
         if self.h:
@@ -443,6 +450,11 @@ class Job:
 
     def set_graph(self, on=True):
         _check(lib().aar_job_set_graph(self.h, int(on)), "aar_job_set_graph")
+
+    def jit(self):
+        """1: the NG sum's fused z pass runs a kernel compiled for this job's schedule; 0: the
+        library's; otherwise the status of a failed compile."""
+        return lib().aar_job_jit(self.h)
 
     def set_profiling(self, on=True):
         _check(lib().aar_job_set_profiling(self.h, int(on)), "aar_job_set_profiling")

@@ -17,7 +17,9 @@ EXE_RES = os.path.join(HERE, "bin", "2LPTResBasis")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["aar_kernels.hip", "aar_stats.hip", "aar_modal.hip", "aar_job.cpp", "aar_host.cpp", "aar_serve.cpp",
-           "aar_modal_host.cpp"]
+           "aar_modal_host.cpp", "aar_jit.cpp"]
+# headers the run-time compiled fused z kernels include (aar_jit.cpp), embedded as string literals
+JIT_HEADERS = ["aar_device.h", "aar_fft.h", "aar_zng.h"]
 # -ffp-contract=off keeps the reference's operation order (no fused multiply-add
 # contraction); packed FP32 instructions are disabled by target feature.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -33,22 +35,42 @@ def _run(cmd):
     return err
 
 
+def write_jit_headers():
+    """_build/aar_jit_headers.inc: the headers of the run-time compiled kernels as raw string
+    literals (kJitHeaderNames, kJitHeaderText, kJitHeaderCount)."""
+    names, texts = [], []
+    for h in JIT_HEADERS:
+        text = open(os.path.join(CSRC, h)).read()
+        if ')AARJIT"' in text:
+            raise RuntimeError(f"{h} holds the literal delimiter")
+        names.append(f'"{h}"')
+        texts.append('R"AARJIT(' + text + ')AARJIT"')
+    out = (f"static const int kJitHeaderCount = {len(JIT_HEADERS)};\n"
+           f"static const char* kJitHeaderNames[] = {{{', '.join(names)}}};\n"
+           f"static const char* kJitHeaderText[] = {{\n" + ",\n".join(texts) + "};\n")
+    path = os.path.join(BUILD, "aar_jit_headers.inc")
+    with open(path + ".tmp", "w") as f:
+        f.write(out)
+    os.replace(path + ".tmp", path)
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(os.path.dirname(EXE), exist_ok=True)
+    write_jit_headers()
 
     def compile_one(src):
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
-            cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+            cmd = [HIPCC] + FLAGS + ["-I" + BUILD, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
         _run(cmd)
         return obj
 
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs + ["-lhiprtc"])
     os.replace(tmp, LIB)
     main_src = os.path.join(CSRC, "aar_main.cpp")
     if os.path.exists(main_src):

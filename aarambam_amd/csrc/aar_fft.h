@@ -12,19 +12,30 @@
 // and stores to HBM are coalesced across the C adjacent columns and the
 // intermediate stages stay in LDS.
 #pragma once
-#include <type_traits>
+#include "aar_device.h"
 
-#include "aar_common.h"
-
-// Loaders may take (pos, col) or (pos, col, e), e = the thread's element slot.
-template <class F> __device__ __forceinline__ auto call_load(F& f, int pos, int col, int e) {
-  if constexpr (std::is_invocable_v<F&, int, int, int>) return f(pos, col, e);
-  else return f(pos, col);
+// Loaders may take (pos, col) or (pos, col, e), e = the thread's element slot; storers
+// (pos, col, v) or (pos, col, v, e), e = the thread's output slot.  The form with the slot is
+// chosen when it exists (overload on int, the exact match, over long).
+template <class F> __device__ __forceinline__ auto call_load_(F& f, int pos, int col, int e, int) -> decltype(f(pos, col, e)) {
+  return f(pos, col, e);
 }
-// Storers may take (pos, col, v) or (pos, col, v, e), e = the thread's output slot.
+template <class F> __device__ __forceinline__ auto call_load_(F& f, int pos, int col, int, long) -> decltype(f(pos, col)) {
+  return f(pos, col);
+}
+template <class F> __device__ __forceinline__ auto call_load(F& f, int pos, int col, int e) {
+  return call_load_(f, pos, col, e, 0);
+}
+template <class F, class V>
+__device__ __forceinline__ auto call_store_(F& f, int pos, int col, V v, int e, int) -> decltype(f(pos, col, v, e), void()) {
+  f(pos, col, v, e);
+}
+template <class F, class V>
+__device__ __forceinline__ auto call_store_(F& f, int pos, int col, V v, int, long) -> decltype(f(pos, col, v), void()) {
+  f(pos, col, v);
+}
 template <class F, class V> __device__ __forceinline__ void call_store(F& f, int pos, int col, V v, int e) {
-  if constexpr (std::is_invocable_v<F&, int, int, V, int>) f(pos, col, v, e);
-  else f(pos, col, v);
+  call_store_(f, pos, col, v, e, 0);
 }
 
 __host__ __device__ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
@@ -467,7 +478,8 @@ inline GPlan make_gplan(int L) {
   while (n % 8 == 0 && p.ns < 12) p.r[p.ns++] = 8, n /= 8;
   if (n % 4 == 0) p.r[p.ns++] = 4, n /= 4;
   if (n % 2 == 0) p.r[p.ns++] = 2, n /= 2;
-  for (int q : {3, 5, 7})
+  const int odd[3] = {3, 5, 7};
+  for (int q : odd)
     while (n % q == 0 && p.ns < 12) p.r[p.ns++] = q, n /= q;
   if (n != 1) p.ns = 0;  // not 7-smooth: no plan
   return p;

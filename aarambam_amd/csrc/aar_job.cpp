@@ -21,6 +21,7 @@
 #include "../../include/aarambam.h"
 #include "aar_common.h"
 #include "aar_internal.h"
+#include "aar_jit.h"
 #include "aar_kernels.h"
 
 int aar_host_transfer_tables(const aar_params* p, const aar_tables* t, std::vector<double>& logk,
@@ -49,6 +50,8 @@ struct Plan {
   std::vector<int> modes_h, kmodes_h, groups_h;
   std::vector<NgTerm> terms_h;
   std::vector<PairWork> pairs;
+  std::vector<int> fgroups_h;  // the factored product schedule (host copy, for aar_jit.cpp)
+  std::vector<NgRec4> frecs_h;
   std::vector<char> image;  // host image of the block
   size_t bytes = 0;
   void* block = nullptr;
@@ -70,6 +73,9 @@ struct Plan {
 struct aar_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // batched jobs run the NG sum's fused z pass compiled for their own product schedule
+  // (aar_jit.cpp) unless turned off (aar_ctx_set_jit)
+  bool jit = true;
   float2* tw32 = nullptr;
   double2* tw64 = nullptr;
   // cosmology scalars by the parameters they depend on: the Romberg sigma8
@@ -154,6 +160,10 @@ struct aar_job {
   bool stage_used[2] = {false, false};
   int stage_next = 0, stage_pending = -1;
   bool fused_lpt = false;
+  // the NG sum's fused z pass compiled for this job's schedule (jit_status: the status of a
+  // failed compile, when the library's k_z_ng runs instead)
+  JitZng jit;
+  int jit_status = 0;
   // slot map: phi_G, Phi, the work region; ZA and 2LPT displacement fields
   int s_phig = 0, s_phi = 1, s_reg = 2, s_disp = 8, s_disp2 = 2;
   void* slot(int s) const { return (char*)pool + (size_t)s * g.fs * cxsize; }
@@ -356,8 +366,10 @@ template <typename T> struct Exec {
           // x, y inverse passes of the M mode fields, then the fused z pass
           // (C2R, products, R2C), then y, x forward passes of the K products
           if ((rc = xy(reg, j->M, +1))) return rc;
-          if ((rc = timed(j, st, AAR_K_PRODUCTS, (j->M + j->K) * S,
-                          [&] { return P::z_ng(reg, j->M, j->K, j->fgroups, j->frecs, g, tw(j), st); })))
+          if ((rc = timed(j, st, AAR_K_PRODUCTS, (j->M + j->K) * S, [&] {
+                 return j->jit.fn ? jit_zng_launch(j->jit, reg, j->M, j->K, g, tw(j), st)
+                                  : P::z_ng(reg, j->M, j->K, j->fgroups, j->frecs, g, tw(j), st);
+               })))
             return rc;
           if ((rc = xy(reg, j->K, -1))) return rc;
         } else {
@@ -832,6 +844,8 @@ static int plan_layout(const aar_params* p, const aar_tables* t, int schedule, P
   pl.M = (int)s.modes.size();
   pl.K = (int)s.kmodes.size();
   pl.pairs = s.pairs;
+  pl.fgroups_h = s.fz.groups;
+  pl.frecs_h = s.fz.recs;
   if (pl.M > 32) return AAR_ERR_UNSUPPORTED;
   std::vector<int> modes_dev = s.modes, groups = s.groups;
   std::vector<NgTerm> terms = s.terms;
@@ -1146,6 +1160,22 @@ int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, i
   aar_seedtable(N, p->Seed, seeds.data());
   if ((rc = job_upload(j, &j->seeds, seeds))) return fail(rc);
   j->fused_lpt = schedule == AAR_SCHED_BATCHED && N >= 64 && is_pow2(N);
+  // the fused z pass of the NG sum (Exec::run's condition), compiled for this schedule; not in
+  // the allocation-free path (loading a module allocates)
+  if (ctx->jit && !plan_in_ws && schedule == AAR_SCHED_BATCHED && j->M >= 1 && j->M <= 16 && N >= 64 &&
+      is_pow2(N)) {
+    const Plan& pl = *j->plan;
+    const std::string src = jit_zng_source(precision == AAR_FP64, N / 2, j->M, j->K, pl.fgroups_h.data(),
+                                           pl.frecs_h.data());
+    jit_zng_shape(precision == AAR_FP64, N / 2, j->M, &j->jit.rows, &j->jit.threads);
+    std::string log;
+    j->jit_status = jit_zng_kernel(ctx->device, src, &j->jit, &log);
+    if (j->jit_status) {  // the library's k_z_ng runs instead
+      j->jit.fn = nullptr;
+      fprintf(stderr, "aarambam: fused z kernel for this schedule not compiled (status %d): %.400s\n",
+              j->jit_status, log.c_str());
+    }
+  }
   j->s_disp = j->s_reg + 6;
   j->s_disp2 = j->s_reg + (j->fused_lpt ? 1 : 0);
   j->npart = (long long)p->GlassTileFac * p->GlassTileFac * p->GlassTileFac * t->nglass;
@@ -1327,6 +1357,17 @@ extern "C" int aar_job_set_profiling(aar_job* j, int on) {
     j->k_bytes[k] = 0;
   }
   return AAR_OK;
+}
+
+extern "C" int aar_ctx_set_jit(aar_ctx* ctx, int on) {
+  if (!ctx) return AAR_ERR_BAD_ARG;
+  ctx->jit = on != 0;
+  return AAR_OK;
+}
+
+extern "C" int aar_job_jit(const aar_job* j) {
+  if (!j) return AAR_ERR_BAD_ARG;
+  return j->jit.fn ? 1 : j->jit_status;
 }
 
 extern "C" int aar_job_kernel_stats(aar_job* j, int kind, int* launches, double* seconds, double* bytes) {

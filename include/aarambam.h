@@ -236,6 +236,16 @@ int aar_job_set_profiling(aar_job *job, int on);
 int aar_job_set_graph(aar_job *job, int on);
 int aar_job_kernel_stats(aar_job *job, int kind, int *launches, double *seconds, double *bytes);
 int aar_job_destroy(aar_job *job);
+/* The batched schedule's fused z pass of the NG sum (z C2R of the mode fields, products, z R2C)
+ * is compiled at aar_job_prepare for the job's own product schedule (hiprtc; cached per device
+ * and schedule, so one compile serves every job of a table), with the same operations in the
+ * same order as the library's kernel: results are bit-identical either way.  No counterpart in
+ * the reference (FFTW's planner is the nearest).  aar_ctx_set_jit(ctx, 0) keeps later jobs of
+ * the context on the library's kernel.  aar_job_jit: 1 if the job runs a compiled kernel, 0 if
+ * it runs the library's (pairwise schedule, other mesh sizes, the allocation-free path, or off),
+ * else the status of a compile that failed (the library's kernel then runs). */
+int aar_ctx_set_jit(aar_ctx *ctx, int on);
+int aar_job_jit(const aar_job *job);
 
 /* ---- template check of a finished job (no counterpart in the reference) ----
  * Binned statistics of the potentials on the device.  Bins are |k| in
