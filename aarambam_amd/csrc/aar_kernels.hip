@@ -1334,19 +1334,18 @@ __device__ __forceinline__ void cic_particle(int n, int t, const cx_t<T>* __rest
   reinterpret_cast<float3*>(vel)[p] = make_float3(V[0], V[1], V[2]);
 }
 
-// Block b of the flattened grid covers 256 consecutive glass particles of one tile: with the
-// lattice glass, four z-rows at one x, so blocks b and b + 1 share a y-row of corners and b and
-// b + 64 / 256 an x-plane.  The grid is remapped so that each XCD takes a contiguous range of
-// blocks (xcd_tile): neighbouring blocks then read their shared rows through one L2.
+// Block (b, t) covers 256 consecutive glass particles b of tile t: with the lattice glass, four
+// z-rows at one x, so blocks b and b + 1 share a y-row of corners and b and b + 16 an x-plane.
+// Blocks run in their natural order, consecutive blocks on different XCDs: at 512^3 the gather
+// reads 8.6 spectra of HBM and takes 2.01 ms, where giving each XCD a contiguous range of blocks
+// read 12.2 and took 2.17 ms (1024^3: 15.9 against 16.1 ms).
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_cic(const cx_t<T>* __restrict__ disp, const cx_t<T>* __restrict__ disp2, Geom g,
           const float* __restrict__ glass, int nglass, CicConst cc, float* __restrict__ pos, float* __restrict__ vel) {
-  const int bpt = gridDim.x;  // blocks per glass tile
-  const int lb = xcd_tile(blockIdx.y * bpt + blockIdx.x, bpt * gridDim.y);
-  const int n = (lb % bpt) * blockDim.x + threadIdx.x;  // glass particle
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;  // glass particle
   if (n >= nglass) return;
-  cic_particle<T>(n, lb / bpt, disp, disp2, g, glass, nglass, cc, pos, vel);
+  cic_particle<T>(n, blockIdx.y, disp, disp2, g, glass, nglass, cc, pos, vel);
 }
 
 template <typename T>
