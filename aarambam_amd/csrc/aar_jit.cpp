@@ -100,8 +100,9 @@ std::string jit_zng_source(int fp64, int H, int M, int K, const int* groups, con
   }
   s += "      default: break;\n    }\n  }\n};\n";
   s += "extern \"C\" __global__ void __launch_bounds__((ZNGCfg<T, kH, kM, 0>::NT))\n"
+       "__attribute__((amdgpu_waves_per_eu(ZNGCfg<T, kH, kM, 0>::WPE)))\n"
        "    aar_zng_jit(C2* base, int M, int K, Geom g, const C2* __restrict__ twg) {\n"
-       "  z_ng_body<T, kH, kM, 0>(base, M, K, g, twg, Products{});\n}\n";
+       "  z_ng_run<T, kH, kM, 0>(base, M, K, g, twg, Products{});\n}\n";
   return s;
 }
 

@@ -548,9 +548,10 @@ template <typename T> struct RecordProducts {
 
 template <typename T, int H, int MMAX, int FIX = 0>
 __global__ void __launch_bounds__((ZNGCfg<T, H, MMAX, FIX>::NT))
+__attribute__((amdgpu_waves_per_eu(ZNGCfg<T, H, MMAX, FIX>::WPE)))
     k_z_ng(cx_t<T>* base, int M, int K, const int* __restrict__ gs, const NgRec4* __restrict__ recs, Geom g,
            const cx_t<T>* __restrict__ twg) {
-  z_ng_body<T, H, MMAX, FIX>(base, M, K, g, twg, RecordProducts<T>{gs, recs});
+  z_ng_run<T, H, MMAX, FIX>(base, M, K, g, twg, RecordProducts<T>{gs, recs});
 }
 
 template <typename T, int H, int MMAX>

@@ -42,6 +42,13 @@ __device__ __forceinline__ void r2c_post(const cx_t<T>* tile, const cx_t<T>* tw,
   }
 }
 
+// The paired form applies to FP32 at H = 128 and 256 (z_ng_body_paired, below).
+template <typename T, int H, int FIX> struct ZPair {
+  static constexpr bool value = FIX == 0 && sizeof(T) == 4 && (H == 128 || H == 256);
+  static constexpr int N = 2 * H, EF = 8, C = 4, NT = C * N / EF;
+  static constexpr bool WL = wave_local(N, EF) && NT % 64 == 0;
+};
+
 // For C grid rows at a time: half-complex -> real along z for each of the M
 // mode fields (kept in registers), the products h_K = sum c f_i f_j of the
 // regrouped NG sum, and real -> half-complex along z of each h_K, written
@@ -59,6 +66,9 @@ template <typename T, int H, int MMAX, int FIX = 0> struct ZNGCfg {
   static constexpr int NT = C * H / E;
   // every row's exchanges inside one wavefront: no workgroup barriers
   static constexpr bool WL = wave_local(H, E) && NT % 64 == 0;
+  // the paired form holds two fields' raw inputs and their prefetch: its allocation is held to
+  // 256 registers (two waves per SIMD, one spilled value) where it would take 262 (one wave)
+  static constexpr int WPE = ZPair<T, H, FIX>::value ? 2 : 1;
 };
 
 // FIX = 0: the NG sum, M mode fields and K products, those of output kk formed by
@@ -211,3 +221,142 @@ __device__ __forceinline__ void z_ng_body(cx_t<T>* base, int M, int K, const Geo
   }
 }
 
+
+// Paired form of the NG sum's fused z pass (FP32, H = 128 or 256, the 256^3 and 512^3 meshes).
+// Two mode fields a, b go through one N-point complex inverse transform of Y = A + i B, with A
+// and B the Hermitian half spectra (Y_p = A_p + i B_p for p <= H, conj(A_{N-p}) + i conj(B_{N-p})
+// above; the imaginary parts of X_0 and X_H are ignored as FFTW's half-complex convention
+// does): the real and imaginary parts of the result are a and b.  Two products h1, h2 go
+// through one forward transform of h1 + i h2, split as H1_k = (Z_k + Z*_{N-k}) / 2 and
+// H2_k = (Z_k - Z*_{N-k}) / (2 i).  Per field this drops the half-length transform's pre- and
+// post-twiddles and one LDS exchange; the launch shape is the unpaired one's (C = 4 rows,
+// C N / 8 threads: eight elements per thread, one wavefront per 512-point row).
+
+template <typename T, int H, int MMAX, class Prod>
+__device__ __forceinline__ void z_ng_body_paired(cx_t<T>* base, int M, int K, const Geom& g,
+                                                 const cx_t<T>* __restrict__ twg, const Prod& prod) {
+  using C2 = cx_t<T>;
+  using Cfg = ZPair<T, H, 0>;
+  constexpr int N = Cfg::N, EF = Cfg::EF, C = Cfg::C, NT = Cfg::NT;
+  constexpr bool WL = Cfg::WL;
+  constexpr int E = EF / 2;  // the products see the eight real samples as four pairs
+  static_assert(NT == ZNGCfg<T, H, MMAX, 0>::NT && C == ZNGCfg<T, H, MMAX, 0>::C, "launch shape");
+  using TI = TileIdx<T, N, C, true, EF>;
+  __shared__ C2 tile[TI::SIZE];
+  __shared__ C2 tw[N];
+  load_twiddles(tw, twg + N, N, NT);
+  __syncthreads();
+  const long long r0 = (long long)blockIdx.x * C;
+  // real samples of the mode fields at the thread's EF last-stage positions: sample e of slot m
+  // in fx<e/2>[m] (e even) or fy<e/2>[m] (e odd), indexed by the uniform mode number
+  T fx0[MMAX], fx1[MMAX], fx2[MMAX], fx3[MMAX], fy0[MMAX], fy1[MMAX], fy2[MMAX], fy3[MMAX];
+  auto put = [&](int m, int e, T v) {
+    if (e == 0) fx0[m] = v;
+    if (e == 1) fy0[m] = v;
+    if (e == 2) fx1[m] = v;
+    if (e == 3) fy1[m] = v;
+    if (e == 4) fx2[m] = v;
+    if (e == 5) fy2[m] = v;
+    if (e == 6) fx3[m] = v;
+    if (e == 7) fy3[m] = v;
+  };
+  auto get = [&](int m, C2* v) {
+    v[0] = cmake<C2>(fx0[m], fy0[m]);
+    v[1] = cmake<C2>(fx1[m], fy1[m]);
+    v[2] = cmake<C2>(fx2[m], fy2[m]);
+    v[3] = cmake<C2>(fx3[m], fy3[m]);
+  };
+  // X_p of field m at the thread's first-stage positions p (X_{N-p} above H, X_H from the
+  // Nyquist plane), prefetched one pair ahead
+  C2 ra[EF], rb[EF], na[EF], nb[EF];
+  auto fetch = [&](int m, C2* out) {
+    const C2* fm = base + (long long)m * g.fs;
+#pragma unroll
+    for (int e = 0; e < EF; ++e) {
+      int pos, col;
+      first_slot<T, N, C, NT, true>(e, pos, col);
+      const C2* row = fm + (r0 + col) * H;
+      out[e] = pos < H ? row[pos] : (pos == H ? fm[g.nq + r0 + col] : row[N - pos]);
+    }
+  };
+  auto zero = [&](C2* out) {
+#pragma unroll
+    for (int e = 0; e < EF; ++e) out[e] = cmake<C2>(0, 0);
+  };
+  // fields m and m + 1 (m + 1 == M: a zero field) -> slots m, m + 1; prefetch the pair at next
+  auto c2r_pair = [&](int m, int next) {
+    __builtin_amdgcn_sched_barrier(0);
+    if (next >= 0) {
+      fetch(next, na);
+      if (next + 1 < M) fetch(next + 1, nb);
+      else zero(nb);
+    }
+    auto load = [&](int p, int, int e) {
+      const C2 A = ra[e], B = rb[e];
+      if (p == 0 || p == H) return cmake<C2>(A.x, B.x);
+      if (p < H) return cmake<C2>(A.x - B.y, A.y + B.x);
+      return cmake<C2>(A.x + B.y, B.x - A.y);
+    };
+    auto store = [&](int, int, C2 v, int e) {
+      put(m, e, v.x);
+      if (m + 1 < MMAX) put(m + 1, e, v.y);
+    };
+    fft_tile<T, N, C, NT, +1, 1, false, true, WL>(tile, tw, load, store);
+#pragma unroll
+    for (int e = 0; e < EF; ++e) {
+      ra[e] = na[e];
+      rb[e] = nb[e];
+    }
+  };
+  // products kk and kk + 1 (kk + 1 == K: none) -> slots kk, kk + 1
+  auto r2c_pair = [&](int kk, const C2* h1, const C2* h2) {
+    tile_sync<WL>();  // the tile is free
+#pragma unroll
+    for (int e = 0; e < EF; ++e) {
+      int pos, col;
+      last_slot<T, N, C, NT, true>(e, pos, col);
+      const T a = (e & 1) ? h1[e / 2].y : h1[e / 2].x, b = (e & 1) ? h2[e / 2].y : h2[e / 2].x;
+      tile[TI::at(pos, col)] = cmake<C2>(a, b);
+    }
+    auto load = [&](int pos, int c) { return tile[TI::at(pos, c)]; };
+    auto store = [&](int pos, int c, C2 v) { tile[TI::at(pos, c)] = v; };
+    fft_tile<T, N, C, NT, -1, 1, true, true, WL>(tile, tw, load, store);
+    tile_sync<WL>();
+    // bins k in [0, H] of the thread's rows (with wavefront-local rows, of its wavefront's rows)
+    constexpr int GT = WL ? 64 : NT, GC = WL ? C * 64 / NT : C;
+    const int gt = WL ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+    const int gc0 = WL ? (int)(threadIdx.x >> 6) * GC : 0;
+    C2* f1 = base + (long long)kk * g.fs;
+    C2* f2 = base + (long long)(kk + 1) * g.fs;
+    const bool two = kk + 1 < K;
+    for (int q = gt; q < GC * (H + 1); q += GT) {
+      const int c = gc0 + q / (H + 1), k = q % (H + 1);
+      const C2 Z = tile[TI::at(k, c)], W = tile[TI::at(k == 0 ? 0 : N - k, c)];
+      const C2 s = cmake<C2>((Z.x + W.x) * (T)0.5, (Z.y - W.y) * (T)0.5);  // (Z + W*) / 2
+      const C2 d = cmake<C2>((Z.y + W.y) * (T)0.5, (W.x - Z.x) * (T)0.5);  // (Z - W*) / (2 i)
+      const long long o = k < H ? (r0 + c) * H + k : g.nq + r0 + c;
+      f1[o] = s;
+      if (two) f2[o] = d;
+    }
+  };
+  fetch(0, ra);
+  if (1 < M) fetch(1, rb);
+  else zero(rb);
+  for (int m = 0; m < M; m += 2) c2r_pair(m, m + 2 < M ? m + 2 : -1);
+  for (int kk = 0; kk < K; kk += 2) {
+    C2 h1[E], h2[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) h1[e] = h2[e] = cmake<C2>(0, 0);
+    prod.template operator()<E>(kk, get, h1);
+    if (kk + 1 < K) prod.template operator()<E>(kk + 1, get, h2);
+    r2c_pair(kk, h1, h2);
+  }
+}
+
+// The fused z pass of a product sum: the paired form where it applies, else z_ng_body.
+template <typename T, int H, int MMAX, int FIX, class Prod>
+__device__ __forceinline__ void z_ng_run(cx_t<T>* base, int M, int K, const Geom& g, const cx_t<T>* __restrict__ twg,
+                                         const Prod& prod) {
+  if constexpr (ZPair<T, H, FIX>::value) z_ng_body_paired<T, H, MMAX>(base, M, K, g, twg, prod);
+  else z_ng_body<T, H, MMAX, FIX>(base, M, K, g, twg, prod);
+}

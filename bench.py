@@ -403,6 +403,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=64)
+    ap.add_argument("--serve-max-active", type=int, default=4, help="jobs the server keeps in flight at most")
     ap.add_argument("--no-decomp", action="store_true")
     ap.add_argument("--no-1024", action="store_true", help="skip the 1024^3 local+equilateral leg")
     ap.add_argument("--xy-group", type=int, default=-1,
@@ -502,7 +503,7 @@ def main():
     if not args.no_1024:
         out["north_star_1024"] = north_star_1024(aar, ctx)
     if not args.no_serve:
-        out["serving"] = serving(aar, ctx, args.serve_jobs)
+        out["serving"] = serving(aar, ctx, args.serve_jobs, max_active=args.serve_max_active)
     if not args.no_decomp:
         out["decomposition"] = decomposition(aar, ctx)
     ctx.close()
