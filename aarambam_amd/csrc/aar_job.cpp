@@ -1694,7 +1694,8 @@ extern "C" int aar_gaussian_field(aar_ctx* ctx, const aar_params* p, const aar_t
 }
 
 // Microbenchmark of one building block on nf fields of an N^3 grid:
-// kind 0 x-pass, 1 y-pass, 2 z c2r, 3 z r2c, 4 plain copy kernel over nf spectra.
+// kind 0 x-pass, 1 y-pass, 2 z c2r, 3 z r2c, 4 plain copy kernel over nf spectra, 5 / 6 the x / y
+// pass's tile access without the transform (k_xy_pattern).
 // Returns the mean seconds per launch over `iters` launches.
 template <typename T> static int bench_pass_t(aar_ctx* ctx, int N, int kind, int nf, int iters, double* sec) {
   using C2 = cx_t<T>;
@@ -1713,6 +1714,8 @@ template <typename T> static int bench_pass_t(aar_ctx* ctx, int N, int kind, int
       case 1: return Pipeline<T>::xy_pass(d, nf, g, tw, 1, +1, ctx->stream);
       case 2: return Pipeline<T>::z_pass(d, nf, g, tw, +1, ctx->stream);
       case 3: return Pipeline<T>::z_pass(d, nf, g, tw, -1, ctx->stream);
+      case 5: return aar_xy_pattern<T>(d, nf, g, 0, ctx->stream);
+      case 6: return aar_xy_pattern<T>(d, nf, g, 1, ctx->stream);
       default:
         return aar_copy16(d, d2, (long long)nf * g.fs * (long long)sizeof(C2), ctx->stream);
     }
@@ -1736,7 +1739,7 @@ template <typename T> static int bench_pass_t(aar_ctx* ctx, int N, int kind, int
 }
 
 extern "C" int aar_bench_pass(aar_ctx* ctx, int n, int precision, int kind, int nf, int iters, double* sec) {
-  if (!ctx || !sec || !pow2(n) || nf < 1 || iters < 1 || kind < 0 || kind > 4) return AAR_ERR_BAD_ARG;
+  if (!ctx || !sec || !pow2(n) || nf < 1 || iters < 1 || kind < 0 || kind > 6) return AAR_ERR_BAD_ARG;
   AAR_HIP(hipSetDevice(ctx->device));
   return precision == AAR_FP32 ? bench_pass_t<float>(ctx, n, kind, nf, iters, sec)
                                : bench_pass_t<double>(ctx, n, kind, nf, iters, sec);

@@ -121,6 +121,8 @@ template <typename T> struct Pipeline {
 // zero n doubles (graph-capturable; no memset nodes)
 int aar_zero_fill(double* p, long long n, hipStream_t st);
 int aar_copy16(const void* src, void* dst, long long bytes, hipStream_t st);
+// read and write back nf fields with the x (axis 0) or y (axis 1) pass's tile addressing, no transform
+template <typename T> int aar_xy_pattern(cx_t<T>* f, int nf, const Geom& g, int axis, hipStream_t st);
 
 // ---- modal decomposition (aar_modal.hip): QQ = (w Q) Q^T and QS = S (w Q)^T over tetrapyd points
 int modal_cov_launch(const double* w, const int* i1, const int* i2, const int* i3, int npts, const int* p1,

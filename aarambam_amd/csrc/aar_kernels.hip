@@ -1385,6 +1385,42 @@ int aar_copy16(const void* src, void* dst, long long bytes, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
 }
 
+// The x / y passes' memory access without the transform: each workgroup reads one 128-byte
+// column tile (16 adjacent kz columns of FP32 values, or 8 of FP64, at every position along the
+// axis, the addresses of tile shape 0) into registers and writes it back in place.  Its rate
+// bounds what a pass with that access pattern can reach (the reference for the x / y passes'
+// roofline at 1024^3, where the tile no longer stays in the last-level cache).
+template <typename T, int AXIS>
+__global__ void __launch_bounds__(1024) k_xy_pattern(cx_t<T>* __restrict__ f, int N, int H) {
+  constexpr int C = sizeof(T) == 4 ? 16 : 8, E = 16;
+  const int t = blockIdx.x, per = H / C, a = t / per, c0 = (t % per) * C;
+  const int col = threadIdx.x % C, p0 = threadIdx.x / C, rows = N / E;
+  cx_t<T>* fb = f + (long long)blockIdx.y * ((long long)N * N * (H + 1));
+  cx_t<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int pos = p0 + e * rows;
+    v[e] = fb[(AXIS == 0 ? row_of(N, pos, a) : row_of(N, a, pos)) * H + c0 + col];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int pos = p0 + e * rows;
+    fb[(AXIS == 0 ? row_of(N, pos, a) : row_of(N, a, pos)) * H + c0 + col] = v[e];
+  }
+}
+
+template <typename T> int aar_xy_pattern(cx_t<T>* f, int nf, const Geom& g, int axis, hipStream_t st) {
+  constexpr int C = sizeof(T) == 4 ? 16 : 8;
+  if (!g.pow2 || g.N < 2 * C || g.N / 16 * C > 1024) return AAR_ERR_UNSUPPORTED;
+  const dim3 grid(g.N * (g.H / C), nf), block(C * g.N / 16);
+  if (axis == 0) hipLaunchKernelGGL((k_xy_pattern<T, 0>), grid, block, 0, st, f, g.N, g.H);
+  else hipLaunchKernelGGL((k_xy_pattern<T, 1>), grid, block, 0, st, f, g.N, g.H);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+template int aar_xy_pattern<float>(float2*, int, const Geom&, int, hipStream_t);
+template int aar_xy_pattern<double>(double2*, int, const Geom&, int, hipStream_t);
+
 __global__ void k_zero(double* p, long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;

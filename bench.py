@@ -193,6 +193,7 @@ def north_star_1024(aar, ctx, steps=3, fp64_steps=2):
                          "xy_tile_variant": [aar.lib().aar_xy_tile_variant(N, aar.AAR_FP32, a) for a in (0, 1)],
                          "xy_pair_group_cols": aar.lib().aar_xy_group_cols(N, aar.AAR_FP32)},
             "job_roofline": {"bytes_per_job": B, "frac": round(B / (ms32 * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)},
+            "xy_access_pattern_gbs": xy_pattern_gbs(ctx, N, nf=4),
             "dominant_kernel": {"kernel": dom, "achieved_gbs": round(achieved, 1),
                                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                                 "avg_launch_us": round(d["seconds"] / d["launches"] * 1e6, 2)},
@@ -345,6 +346,13 @@ def copy_peak(ctx, N=512, nf=8, iters=10):
     sec = ctx.bench_pass(N, 4, nf, iters)
     S = N * N * (N // 2 + 1) * 8
     return 2 * nf * S / sec / 1e9
+
+
+def xy_pattern_gbs(ctx, N, nf=8, iters=5):
+    """Rate of the x and y passes' tile access with no transform (aar_bench_pass kinds 5, 6):
+    reads and writes of the kz < N/2 block of nf FP32 fields."""
+    nbytes = 2 * nf * N * N * (N // 2) * 8
+    return {ax: round(nbytes / ctx.bench_pass(N, k, nf, iters) / 1e9, 1) for ax, k in (("x", 5), ("y", 6))}
 
 
 def profiled_traffic(kernel, N, template):
@@ -500,6 +508,8 @@ def main():
                                       "plans": "0: radices spread evenly (512 = 8 8 8, 1024 = 16 8 8), "
                                                "1: 16 first (16 16 2, 16 16 4), 2: 16 last (2 16 16, 4 16 16)"}}
     out["copy_peak_gbs"] = round(copy_peak(ctx, 512), 1)
+    if pow2:
+        out["xy_access_pattern_gbs"] = xy_pattern_gbs(ctx, N)
     if not args.no_1024:
         out["north_star_1024"] = north_star_1024(aar, ctx)
     if not args.no_serve:

@@ -387,7 +387,9 @@ int aar_fft3d(aar_ctx *ctx, int n, int precision, int dir, const double *in, dou
 int aar_gaussian_field(aar_ctx *ctx, const aar_params *p, const aar_tables *t, int precision,
                        double *out);
 /* Mean seconds per launch of one FFT pass (kind 0 x, 1 y, 2 z-C2R, 3 z-R2C)
- * or of a plain streaming copy kernel (kind 4, reads and writes nf spectra) over nf fields of an N^3 grid. */
+ * or of a plain streaming copy kernel (kind 4, reads and writes nf spectra) over nf fields of an N^3 grid;
+ * kinds 5 / 6 read and write back the kz < N/2 block of the nf fields with the x / y pass's tile
+ * addressing and no transform (the access pattern's own rate, a bound for the passes). */
 int aar_bench_pass(aar_ctx *ctx, int n, int precision, int kind, int nf, int iters, double *sec);
 /* Device ranlxd draws for tests: `n` uniforms of stream `seed`. */
 int aar_ranlxd_device(aar_ctx *ctx, uint32_t seed, int n, double *out);
