@@ -62,8 +62,10 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_read_params_res", "aar_l
            "aar_job_ng_statistics", "aar_mode_bispectra_covariance", "aar_mode_QS",
            "aar_mode_basis_expansion", "aar_job_reseed", "aar_ctx_cached_plans",
            "aar_job_run_noalloc", "aar_autotune_plans", "aar_xy_radix_plan", "aar_set_xy_radix_plan",
-           "aar_sym_eig_extremes", "aar_ctx_set_jit", "aar_job_jit"]
-KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate"]
+           "aar_sym_eig_extremes", "aar_ctx_set_jit", "aar_job_jit", "aar_autotune_report",
+           "aar_xy_split_group_cols", "aar_set_xy_split_group_cols"]
+KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate",
+         "xy_split"]
 
 
 class AarError(RuntimeError):
@@ -157,6 +159,9 @@ def lib():
             "aar_set_xy_tile_variant": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
             "aar_xy_group_cols": (C.c_int, [C.c_int, C.c_int]),
             "aar_set_xy_group_cols": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+            "aar_autotune_report": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp]),
+            "aar_xy_split_group_cols": (C.c_int, [C.c_int, C.c_int]),
+            "aar_set_xy_split_group_cols": (C.c_int, [C.c_int, C.c_int, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -301,6 +306,15 @@ class Context:
         pl = C.c_int(0)
         _check(lib().aar_autotune_plans(self.h, n, precision, nf, _dptr(sec), C.byref(pl)), "aar_autotune_plans")
         return sec, pl.value
+
+    def autotune_report(self, n, precision=AAR_FP32, nf=8):
+        """One autotuner search with every timing: (tile shapes [2][4], radix plans [3], column
+        groups of the multi-output pairs [5]: whole passes, all columns, halves, quarters,
+        eighths), seconds, -1 where not timed."""
+        ts, ps, ss = np.full(8, -1.0), np.full(3, -1.0), np.full(5, -1.0)
+        _check(lib().aar_autotune_report(self.h, n, precision, nf, _dptr(ts), _dptr(ps), _dptr(ss)),
+               "aar_autotune_report")
+        return ts.reshape(2, 4), ps, ss
 
     def cached_plans(self):
         """(count, device bytes) of the job tables the context keeps resident for later jobs."""

@@ -56,7 +56,7 @@ struct Plan {
   size_t bytes = 0;
   void* block = nullptr;
   bool owned = false;  // hipMalloc'd by the cache (freed with the plan)
-  size_t o_kv = 0, o_sinc = 0, o_x0 = 0, o_outc = 0, o_modes = 0, o_kmodes = 0, o_groups = 0, o_terms = 0,
+  size_t o_kv = 0, o_sinc = 0, o_icic = 0, o_x0 = 0, o_outc = 0, o_modes = 0, o_kmodes = 0, o_groups = 0, o_terms = 0,
          o_weights = 0, o_fgroups = 0, o_frecs = 0, o_logk = 0, o_logT = 0,
          o_glass = 0;
   int nglass = 0;
@@ -100,7 +100,7 @@ struct aar_job {
   void* pool = nullptr;
   int nslots = 0;
   uint32_t* seeds = nullptr;
-  double *kv = nullptr, *sinc = nullptr, *x0 = nullptr, *logk = nullptr, *logT = nullptr;
+  double *kv = nullptr, *sinc = nullptr, *icic = nullptr, *x0 = nullptr, *logk = nullptr, *logT = nullptr;
   unsigned char* outc = nullptr;
   int *modes = nullptr, *kmodes = nullptr, *groups = nullptr;
   NgTerm* terms = nullptr;
@@ -335,6 +335,7 @@ template <typename T> struct Exec {
     const double phase_shift = p.PhaseFlip == 1 ? AAR_PI : 0.0;
     // x / y pass pairs, in the tuned column groups; one timed unit of `dispatches` launches
     const int xyg = aar_xy_group(sizeof(T) == 8, g.N);
+    const int xsg = aar_xy_split_group(sizeof(T) == 8, g.N);
     auto xy = [&](C2* f, int nf, int dir) {
       return timed(j, st, AAR_K_XY, 4 * nf * S, [&] { return P::xy_pair(f, nf, g, tw(j), dir, st, xyg); },
                    aar_xy_pair_dispatches(nf, g.H, xyg));
@@ -380,27 +381,38 @@ template <typename T> struct Exec {
           if ((rc = r2c(reg, j->K))) return rc;
         }
       }
-      // combine + the nine 2LPT spectra in one elementwise pass
-      if ((rc = timed(j, st, AAR_K_COMBINE, (j->K + 11) * S, [&] {
-             return P::ng_lpt(F(j, j->s_phig), reg, j->K, j->kmodes, F(j, j->s_phi), reg, p.Fnl, g, j->ks, j->tf,
-                              c.Dplus, c.Beta, st);
+      // combine + the nine 2LPT spectra in one elementwise pass (with the fused z passes: only
+      // the five x-pass inputs of the nine, the ky and kz factors applied by the x pass)
+      const bool split = j->fused_lpt;
+      if ((rc = timed(j, st, AAR_K_COMBINE, (j->K + (split ? 7 : 11)) * S, [&] {
+             return P::ng_lpt(F(j, j->s_phig), reg, j->K, j->kmodes, F(j, j->s_phi), split ? reg + 9 * g.fs : reg,
+                              p.Fnl, g, j->ks, j->tf, c.Dplus, c.Beta, st, split);
            })))
         return rc;
       C2* disp = F(j, j->s_disp);
       C2* disp2 = F(j, j->s_disp2);
-      if (j->fused_lpt) {
-        // x, y inverse passes of all nine spectra; z C2R of the three ZA fields;
-        // the six gradient fields go through the fused z pass, whose one
-        // product is the 2LPT source (never stored in real space), then y, x
-        // forward passes of the source
-        if ((rc = xy(reg, 9, +1))) return rc;
+      // x then y inverse passes of fields made by XSplit xs: nin x-pass inputs -> nout spectra
+      auto xys = [&](C2* f, const XSplit& xs) {
+        return timed(j, st, AAR_K_XY_SPLIT, (3.0 * xs.nin + xs.nout) * S,
+                     [&] { return P::xy_split(f, xs, g, tw(j), st, xsg); }, aar_xy_pair_dispatches(xs.nin, g.H, xsg));
+      };
+      if (split) {
+        // x (five inputs, nine outputs), y inverse passes of the nine spectra; z C2R of the
+        // three ZA fields; the six gradient fields go through the fused z pass, whose one
+        // product is the 2LPT source (never stored in real space), then y, x forward passes
+        // of the source; its two x-pass inputs of the three 2LPT displacements
+        if ((rc = xys(reg, lpt_xsplit(j->ks)))) return rc;
         if ((rc = timed(j, st, AAR_K_Z, 3 * (S + R), [&] { return P::z_pass(disp, 3, g, tw(j), +1, st); })))
           return rc;
         if ((rc = timed(j, st, AAR_K_PRODUCTS, 7 * S,
                         [&] { return P::z_source(reg, g, tw(j), st); })))
           return rc;
         if ((rc = xy(reg, 1, -1))) return rc;
-        if ((rc = timed(j, st, AAR_K_LPT, 4 * S, [&] { return P::disp2_spec(reg, disp2, g, j->ks, st); })))
+        if ((rc = timed(j, st, AAR_K_LPT, 3 * S,
+                        [&] { return P::disp2_spec(reg, disp2 + 3 * g.fs, g, j->ks, st, true); })))
+          return rc;
+        if ((rc = xys(disp2, disp2_xsplit(j->ks)))) return rc;
+        if ((rc = timed(j, st, AAR_K_Z, 3 * (S + R), [&] { return P::z_pass(disp2, 3, g, tw(j), +1, st); })))
           return rc;
       } else {
         if ((rc = c2r(reg, 9))) return rc;
@@ -408,8 +420,8 @@ template <typename T> struct Exec {
         if ((rc = r2c(reg + 3 * g.fs, 1))) return rc;
         if ((rc = timed(j, st, AAR_K_LPT, 4 * S, [&] { return P::disp2_spec(reg + 3 * g.fs, disp2, g, j->ks, st); })))
           return rc;
+        if ((rc = c2r(disp2, 3))) return rc;
       }
-      if ((rc = c2r(disp2, 3))) return rc;
       return timed(j, st, AAR_K_CIC, 6 * R + 24.0 * j->npart, [&] {
         return P::cic(disp, disp2, g, j->glass, j->nglass, j->glass_box, j->p.GlassTileFac, p.Box,
                       c.vel_prefac, c.vel_prefac2, j->pos, j->vel, j->npart, st);
@@ -639,13 +651,16 @@ static std::mutex g_tune_mu;
 static bool g_tuned[2][11] = {};
 
 template <typename T>
-static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, double* sec, int* chosen,
-                      double* plan_sec = nullptr) {
+static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, int nslots, double* sec, int* chosen,
+                      double* plan_sec = nullptr, double* split_sec = nullptr) {
   using C2 = cx_t<T>;
   const Geom g = make_geom(N);
   C2* d = (C2*)scratch;
-  if (!d && device_alloc((void**)&d, (size_t)nf * g.fs * sizeof(C2))) return AAR_ERR_ALLOC;
-  int rc = aar_zero_fill((double*)d, (long long)nf * g.fs * (long long)(sizeof(C2) / 8), ctx->stream);
+  // fields of d: nf for the pass timings, at least fourteen for the multi-output pairs (scratch,
+  // the job's own pool, holds nslots)
+  const int nall = scratch ? nslots : std::max(nf, 14);
+  if (!d && device_alloc((void**)&d, (size_t)nall * g.fs * sizeof(C2))) return AAR_ERR_ALLOC;
+  int rc = aar_zero_fill((double*)d, (long long)nall * g.fs * (long long)(sizeof(C2) / 8), ctx->stream);
   const C2* tw = sizeof(T) == 4 ? (const C2*)ctx->tw32 : (const C2*)ctx->tw64;
   hipEvent_t a = nullptr, b = nullptr;
   hipEventCreate(&a);
@@ -682,21 +697,29 @@ static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, double* sec, i
   const int nplan = (N == 512 || N == 1024) ? kXYPlanCount : 1;
   double best = 1e30;
   int bg = 0, bp = 0;
+  // seconds per call of fn(q) (q = 0, 1, ...): one call untimed, then the least mean of three
+  // runs of `iters` calls (the choices below are a few percent apart)
+  auto time_calls = [&](auto&& fn, double& t) {
+    t = 1e30;
+    int r = fn(0);
+    for (int rep = 0; rep < 3 && !r; ++rep) {
+      hipEventRecord(a, ctx->stream);
+      for (int q = 0; q < iters && !r; ++q) r = fn(q);
+      hipEventRecord(b, ctx->stream);
+      if (hipEventSynchronize(b) != hipSuccess) r = AAR_ERR_HIP;
+      float ms = 0;
+      if (!r && hipEventElapsedTime(&ms, a, b) == hipSuccess) t = std::min(t, ms * 1e-3 / iters);
+    }
+    return r;
+  };
   for (int pl = 0; pl < nplan && !rc; ++pl) {
     aar_set_xy_plan(sizeof(T) == 8, N, pl);
     double bestp = 1e30;
     for (int gc : {0, g.H, g.H / 2, g.H / 4}) {
       if (rc || (gc > 0 && gc < 16)) break;
-      int r = Pipeline<T>::xy_pair(d, nf, g, tw, +1, ctx->stream, gc);
-      double t = 1e30;
-      if (!r) {
-        hipEventRecord(a, ctx->stream);
-        for (int q = 0; q < iters && !r; ++q) r = Pipeline<T>::xy_pair(d, nf, g, tw, q % 2 ? -1 : +1, ctx->stream, gc);
-        hipEventRecord(b, ctx->stream);
-        if (hipEventSynchronize(b) != hipSuccess) r = AAR_ERR_HIP;
-        float ms = 0;
-        if (!r && hipEventElapsedTime(&ms, a, b) == hipSuccess) t = ms * 1e-3 / iters;
-      }
+      double t;
+      const int r = time_calls(
+          [&](int q) { return Pipeline<T>::xy_pair(d, nf, g, tw, q % 2 ? -1 : +1, ctx->stream, gc); }, t);
       if (r) rc = r;
       bestp = std::min(bestp, t);
       if (t < best) {
@@ -709,6 +732,31 @@ static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, double* sec, i
   }
   aar_set_xy_plan(sizeof(T) == 8, N, rc ? 0 : bp);
   if (!rc) aar_set_xy_group(sizeof(T) == 8, N, bg);
+  // the column groups of the multi-output pairs of the 2LPT spectra (five inputs, nine outputs:
+  // a group's outputs are larger than its input, so the best group can differ), on fourteen
+  // fields with unit factor tables
+  double* ones = nullptr;
+  if (!rc && nall >= 14 && !device_alloc((void**)&ones, sizeof(double) * N)) {
+    const std::vector<double> one(N, 1.0);
+    XSplit xs = lpt_xsplit(KSpace{});
+    xs.kv = xs.icic = ones;
+    double bests = 1e30;
+    int bs = bg;
+    if (hipMemcpy(ones, one.data(), sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess) rc = AAR_ERR_HIP;
+    for (int gc : {0, g.H, g.H / 2, g.H / 4, g.H / 8}) {
+      if (rc || (gc > 0 && gc < 16)) break;
+      double t;
+      const int r = time_calls([&](int) { return Pipeline<T>::xy_split(d, xs, g, tw, ctx->stream, gc); }, t);
+      if (r) rc = r;
+      if (split_sec) split_sec[gc ? 1 + __builtin_ctz(g.H / gc) : 0] = t < 1e30 ? t : -1.0;
+      if (t < bests) {
+        bests = t;
+        bs = gc;
+      }
+    }
+    if (!rc) aar_set_xy_split_group(sizeof(T) == 8, N, bs);
+    hipFree(ones);
+  }
   hipEventDestroy(a);
   hipEventDestroy(b);
   if (!scratch) hipFree(d);
@@ -734,8 +782,8 @@ extern "C" int aar_set_xy_group_cols(int n, int precision, int cols) {
 extern "C" int aar_autotune(aar_ctx* ctx, int n, int precision, int nf, double* seconds, int* variant) {
   if (!ctx || !pow2(n) || nf < 1 || (precision != AAR_FP32 && precision != AAR_FP64)) return AAR_ERR_BAD_ARG;
   AAR_HIP(hipSetDevice(ctx->device));
-  return precision == AAR_FP32 ? autotune_t<float>(ctx, n, nf, nullptr, seconds, variant)
-                               : autotune_t<double>(ctx, n, nf, nullptr, seconds, variant);
+  return precision == AAR_FP32 ? autotune_t<float>(ctx, n, nf, nullptr, 0, seconds, variant)
+                               : autotune_t<double>(ctx, n, nf, nullptr, 0, seconds, variant);
 }
 
 extern "C" int aar_autotune_plans(aar_ctx* ctx, int n, int precision, int nf, double* plan_seconds, int* plan) {
@@ -743,10 +791,35 @@ extern "C" int aar_autotune_plans(aar_ctx* ctx, int n, int precision, int nf, do
   AAR_HIP(hipSetDevice(ctx->device));
   if (plan_seconds)
     for (int q = 0; q < kXYPlanCount; ++q) plan_seconds[q] = -1.0;
-  const int rc = precision == AAR_FP32 ? autotune_t<float>(ctx, n, nf, nullptr, nullptr, nullptr, plan_seconds)
-                                       : autotune_t<double>(ctx, n, nf, nullptr, nullptr, nullptr, plan_seconds);
+  const int rc = precision == AAR_FP32 ? autotune_t<float>(ctx, n, nf, nullptr, 0, nullptr, nullptr, plan_seconds)
+                                       : autotune_t<double>(ctx, n, nf, nullptr, 0, nullptr, nullptr, plan_seconds);
   if (plan) *plan = aar_xy_plan(precision == AAR_FP64, n);
   return rc;
+}
+
+extern "C" int aar_autotune_report(aar_ctx* ctx, int n, int precision, int nf, double* tile_seconds,
+                                   double* plan_seconds, double* split_seconds) {
+  if (!ctx || !pow2(n) || nf < 1 || (precision != AAR_FP32 && precision != AAR_FP64)) return AAR_ERR_BAD_ARG;
+  AAR_HIP(hipSetDevice(ctx->device));
+  if (plan_seconds)
+    for (int q = 0; q < kXYPlanCount; ++q) plan_seconds[q] = -1.0;
+  if (split_seconds)
+    for (int q = 0; q < 5; ++q) split_seconds[q] = -1.0;
+  return precision == AAR_FP32
+             ? autotune_t<float>(ctx, n, nf, nullptr, 0, tile_seconds, nullptr, plan_seconds, split_seconds)
+             : autotune_t<double>(ctx, n, nf, nullptr, 0, tile_seconds, nullptr, plan_seconds, split_seconds);
+}
+
+extern "C" int aar_xy_split_group_cols(int n, int precision) {
+  if (!pow2(n) || (precision != AAR_FP32 && precision != AAR_FP64)) return AAR_ERR_BAD_ARG;
+  return aar_xy_split_group(precision == AAR_FP64, n);
+}
+
+extern "C" int aar_set_xy_split_group_cols(int n, int precision, int cols) {
+  if (!pow2(n) || (precision != AAR_FP32 && precision != AAR_FP64)) return AAR_ERR_BAD_ARG;
+  if (cols < -1 || cols > n / 2 || (cols > 0 && cols % 16)) return AAR_ERR_BAD_ARG;
+  aar_set_xy_split_group(precision == AAR_FP64, n, cols);
+  return AAR_OK;
 }
 
 extern "C" int aar_xy_radix_plan(int n, int precision) {
@@ -827,9 +900,13 @@ static int validate(const aar_params* p, const aar_tables* t) {
   return AAR_OK;
 }
 
-static int slots_needed(int M, int K, int sched) {
+// Field slots of a job: phi_G, Phi and a region of the mode fields, the products and the 2LPT
+// spectra; with the fused z passes (batched, N >= 64, powers of two) the nine 2LPT spectra and
+// the five x-pass inputs they are made from (lpt_xsplit).
+static int slots_needed(int M, int K, int sched, int N) {
   if (sched == AAR_SCHED_PLAIN) return 2 + 9;
-  return 2 + std::max(std::max(M, K), 9);
+  const bool fused = N >= 64 && is_pow2(N);
+  return 2 + std::max(std::max(M, K), fused ? 14 : 9);
 }
 
 // The host half of a plan: schedule, k-space and transfer tables, glass, and the image of the
@@ -857,7 +934,7 @@ static int plan_layout(const aar_params* p, const aar_tables* t, int schedule, P
     groups = {0, 1};
   }
   // k-space tables (main.c:412-452, 1323-1343)
-  std::vector<double> kv(N), sinc(N), x0(Nm);
+  std::vector<double> kv(N), sinc(N), icic(N), x0(Nm);
   std::vector<unsigned char> outc(N);
   for (int i = 0; i < N; ++i) {
     kv[i] = (i < N / 2) ? i * 2 * AAR_PI / p->Box : -(N - i) * 2 * AAR_PI / p->Box;
@@ -868,6 +945,7 @@ static int plan_layout(const aar_params* p, const aar_tables* t, int schedule, P
       f = sin(f) / f;
     }
     sinc[i] = f;
+    icic[i] = (1 / f) * (1 / f);
   }
   for (int q = 0; q < Nm; ++q) x0[q] = t->x0[2 * q];
   std::vector<double> logk, logT;
@@ -884,6 +962,7 @@ static int plan_layout(const aar_params* p, const aar_tables* t, int schedule, P
   const Part parts[] = {
       {kv.data(), bytes_of(kv), &pl.o_kv},
       {sinc.data(), bytes_of(sinc), &pl.o_sinc},
+      {icic.data(), bytes_of(icic), &pl.o_icic},
       {x0.data(), bytes_of(x0), &pl.o_x0},
       {outc.data(), bytes_of(outc), &pl.o_outc},
       {modes_dev.data(), bytes_of(modes_dev), &pl.o_modes},
@@ -921,7 +1000,7 @@ static size_t job_own_bytes(const aar_params* p, const aar_tables* t, int precis
   const size_t npart = (size_t)p->GlassTileFac * p->GlassTileFac * p->GlassTileFac * t->nglass;
   size_t b = 0;
   auto add = [&](size_t n) { b = ((b + 255) & ~(size_t)255) + (n ? n : 1); };
-  add(slots_needed(M, K, schedule) * (size_t)g.fs * cx);
+  add(slots_needed(M, K, schedule, N) * (size_t)g.fs * cx);
   if (schedule == AAR_SCHED_PLAIN) add((size_t)g.fs * 16);
   add(4 * N * N);
   if (particles) add(12 * npart), add(12 * npart);
@@ -960,6 +1039,7 @@ static void plan_views(const aar_params* p, Plan& pl, aar_job* j) {
   char* b = (char*)pl.block;
   j->kv = (double*)(b + pl.o_kv);
   j->sinc = (double*)(b + pl.o_sinc);
+  j->icic = (double*)(b + pl.o_icic);
   j->x0 = (double*)(b + pl.o_x0);
   j->outc = (unsigned char*)(b + pl.o_outc);
   j->modes = (int*)(b + pl.o_modes);
@@ -986,6 +1066,7 @@ static void plan_views(const aar_params* p, Plan& pl, aar_job* j) {
   ks.kv = j->kv;
   ks.outc = j->outc;
   ks.sinc = j->sinc;
+  ks.icic = j->icic;
   ks.sphere = p->SphereMode;
   ks.box = p->Box;
   ks.nsh = (double)(p->Nsample / 2);
@@ -1141,7 +1222,7 @@ int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, i
   }
   plan_views(p, *j->plan, j);
   // the job's own buffers
-  j->nslots = slots_needed(j->M, j->K, schedule);
+  j->nslots = slots_needed(j->M, j->K, schedule, N);
   if ((rc = job_alloc(j, &j->pool, (size_t)j->nslots * j->g.fs * j->cxsize))) return fail(rc);
   if (schedule == AAR_SCHED_PLAIN && (rc = job_alloc(j, (void**)&j->acc, (size_t)j->g.fs * 16)))
     return fail(rc);
@@ -1152,8 +1233,8 @@ int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, i
       tuned = g_tuned[precision == AAR_FP64][ilog2i(N)];
     }
     const int nf = std::min(j->nslots, 8);
-    if (!tuned && (rc = precision == AAR_FP32 ? autotune_t<float>(ctx, N, nf, j->pool, nullptr, nullptr)
-                                              : autotune_t<double>(ctx, N, nf, j->pool, nullptr, nullptr)))
+    if (!tuned && (rc = precision == AAR_FP32 ? autotune_t<float>(ctx, N, nf, j->pool, j->nslots, nullptr, nullptr)
+                                              : autotune_t<double>(ctx, N, nf, j->pool, j->nslots, nullptr, nullptr)))
       return fail(rc);
   }
   std::vector<uint32_t> seeds((size_t)N * N);

@@ -8,6 +8,7 @@ struct KSpace {
   const double* kv;           // [N] kvec component of grid index i
   const unsigned char* outc;  // [N] 1 when |kv|*Box/(2 pi) > Nsample/2 (cube cut)
   const double* sinc;         // [N] CIC window sin(kv L/2N)/(kv L/2N), 1 at kv = 0
+  const double* icic;         // [N] 1 / sinc^2, the CIC deconvolution of one axis
   int sphere;                 // SphereMode
   double box;                 // Box
   double nsh;                 // (double)(Nsample/2)
@@ -62,19 +63,51 @@ struct XYRange {
   bool nyq = true;
 };
 
+// A multi-output y pass (inverse): the spectra that go into real space are products of one
+// non-separable spectrum and per-axis factors, so fields that differ only in their factors of
+// ky and kz share one x transform (and one field in HBM until the y pass).  The nin inputs sit
+// in slots in0 .. in0 + nin - 1 after the nout outputs (slots 0 .. nout - 1); the x pass
+// transforms them in place; the y pass transforms input src[o] once per output o, loaded times
+// the factor code[o] of its (y, kz) grid indices:
+//   bits 0-1 power of ky, bits 2-3 power of kz, bit 4 the CIC deconvolution of y and z
+//   1 / (sinc_y^2 sinc_z^2), bit 5 times i, bit 6 negated.
+struct XSplit {
+  int nin = 0, nout = 0, in0 = 0;
+  signed char src[9] = {};
+  unsigned char code[9] = {};
+  const double* kv = nullptr;    // [N] kvec component of a grid index (KSpace::kv)
+  const double* icic = nullptr;  // [N] 1 / sinc^2 (KSpace::icic)
+};
+// The y pass of n outputs of a split in one launch: pair i writes slot dst[i] from slot src[i]
+// times the factor code[i]; the pairs of one tile are adjacent workgroups.
+struct XOut {
+  int n = 0;
+  signed char src[9] = {}, dst[9] = {};
+  unsigned char code[9] = {};
+  const double* kv = nullptr;
+  const double* icic = nullptr;
+};
+enum : unsigned char { kXsKy = 1, kXsKy2 = 2, kXsKz = 4, kXsKz2 = 8, kXsCic = 16, kXsI = 32, kXsNeg = 64 };
+
 template <typename T> struct Pipeline {
   // FFT passes on nf consecutive fields starting at f (stride g.fs)
   static int c2r(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st);
   static int r2c(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st);
   // variant < 0: the tile shape from the variant table (aar_set_xy_variant)
+  // xs: the y pass (axis 1, dir > 0) of the xs->n outputs of a split (nf = 1)
   static int xy_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int axis, int dir,
-                     hipStream_t st, int variant = -1, XYRange r = XYRange());
+                     hipStream_t st, int variant = -1, XYRange r = XYRange(), const XOut* xs = nullptr);
   // x then y pass (dir > 0) or y then x (dir < 0).  group_cols > 0: field by field, in groups of
   // group_cols kz columns (the last group may be narrower), so the first pass's output of a group
   // can still be in the last-level cache when the second pass reads it; 0: each pass over all nf
   // fields at once; < 0: the tuned choice for this precision and N (aar_set_xy_group).
   static int xy_pair(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int dir, hipStream_t st,
                      int group_cols);
+  // inverse x pass of xs's inputs (in place), then the multi-output y pass into the outputs
+  // (XSplit, slots from f).  group_cols as in xy_pair (per input: its x pass, then the y pass of
+  // its outputs, group by group).  Powers of two only.
+  static int xy_split(cx_t<T>* f, const XSplit& xs, const Geom& g, const cx_t<T>* tw, hipStream_t st,
+                      int group_cols);
   static int z_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int dir, hipStream_t st);
 
   // fused z pass of the NG sum: M mode fields (x, y already inverse transformed)
@@ -106,17 +139,26 @@ template <typename T> struct Pipeline {
   static int lpt_spec(const cx_t<T>* phi, cx_t<T>* out9, const Geom& g, const KSpace& ks,
                       const TransferDev& tf, double dplus, double beta, hipStream_t st);
   static int source(cx_t<T>* dg6, const Geom& g, hipStream_t st);  // writes into dg6[3]
-  // batched path: combine and the nine 2LPT spectra in one pass (out9 may alias h)
+  // batched path: combine and the nine 2LPT spectra in one pass (out9 may alias h); split: only
+  // the five x-pass inputs of lpt_xsplit() (slots 0-4 of out9), the rest of each spectrum's
+  // factors applied by the multi-output y pass
   static int ng_lpt(const cx_t<T>* phig, const cx_t<T>* h, int K, const int* kmodes, cx_t<T>* phi,
                     cx_t<T>* out9, double fnl, const Geom& g, const KSpace& ks, const TransferDev& tf,
-                    double dplus, double beta, hipStream_t st);
+                    double dplus, double beta, hipStream_t st, bool split = false);
+  // split: the two x-pass inputs of disp2_xsplit() (slots 0, 1 of out3)
   static int disp2_spec(const cx_t<T>* src, cx_t<T>* out3, const Geom& g, const KSpace& ks,
-                        hipStream_t st);
+                        hipStream_t st, bool split = false);
   // disp3 / disp2_3: three consecutive real fields each (ZA, 2LPT)
   static int cic(const cx_t<T>* disp3, const cx_t<T>* disp2_3, const Geom& g, const float* glass,
                  int nglass, double gbox, int tile, double box, double vp1, double vp2, float* pos,
                  float* vel, long long npart, hipStream_t st);
 };
+
+// The splits of the nine 2LPT spectra (outputs: the gradients 00, 01, 02, 11, 12, 22 and the
+// CIC-deconvolved ZA displacements x, y, z; inputs, ng_lpt's split outputs, in slots 9-13) and
+// of the three 2LPT displacements (inputs, disp2_spec's split outputs, in slots 3-4).
+XSplit lpt_xsplit(const KSpace& ks);
+XSplit disp2_xsplit(const KSpace& ks);
 
 // zero n doubles (graph-capturable; no memset nodes)
 int aar_zero_fill(double* p, long long n, hipStream_t st);
@@ -164,7 +206,10 @@ constexpr int kXYPlanCount = 3;
 int aar_xy_plan(int fp64, int n);
 void aar_set_xy_plan(int fp64, int n, int plan);
 void aar_set_xy_group(int fp64, int n, int cols);
-// kernel launches of one xy_pair call
+// column groups of the multi-output pass pairs (xy_split): the tuned choice, else the pairs'
+int aar_xy_split_group(int fp64, int n);
+void aar_set_xy_split_group(int fp64, int n, int cols);  // -1: follow the pairs' choice
+// kernel launches of one xy_pair call (of one xy_split call: nf = its inputs)
 int aar_xy_pair_dispatches(int nf, int H, int group_cols);
 // n uniforms of one ranlxd1 stream on the device (test hook)
 int aar_ranlxd_draws_dev(uint32_t seed, int n, double* out, hipStream_t st);

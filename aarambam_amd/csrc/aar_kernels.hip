@@ -112,9 +112,27 @@ void aar_set_xy_group(int fp64, int n, int cols) {
   while ((1 << l) < n) ++l;
   if (l < 11 && cols >= 0) g_xy_group[fp64 ? 1 : 0][l].store(cols, std::memory_order_relaxed);
 }
+// Column groups of the multi-output x / y pass pairs (xy_split), as g_xy_group; -1: the pair's
+static std::atomic<int> g_xy_split_group[2][11] = {{-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1},
+                                                   {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}};
+int aar_xy_split_group(int fp64, int n) {
+  if (!is_pow2(n)) return 0;
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  if (l >= 11) return 0;
+  const int v = g_xy_split_group[fp64 ? 1 : 0][l].load(std::memory_order_relaxed);
+  return v >= 0 ? v : aar_xy_group(fp64, n);
+}
+void aar_set_xy_split_group(int fp64, int n, int cols) {
+  if (!is_pow2(n)) return;
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  if (l < 11 && cols >= -1) g_xy_split_group[fp64 ? 1 : 0][l].store(cols, std::memory_order_relaxed);
+}
 int aar_xy_pair_dispatches(int nf, int H, int group_cols) {
   return group_cols > 0 ? 2 * nf * ((H + group_cols - 1) / group_cols) : 2;
 }
+
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; give
 // each XCD a contiguous range of tiles so neighbouring tiles share its L2.
@@ -134,18 +152,59 @@ template <typename T, int H> struct ZCfg {
 
 // ------------------------------------------------------------------ x / y passes
 // One tile = C adjacent columns of one field.  Tiles [0, nmain) cover the
-// kz < N/2 block, the rest the Nyquist plane.
-template <typename T, int L, int AXIS, int DIR, int V, int PL>
+// kz < N/2 block, the rest the Nyquist plane.  MO: the y pass of the xo.n outputs
+// of a multi-output split (XSplit), workgroup w (in the XCD-aware order) doing
+// pair w % xo.n of tiles w / xo.n, read from its input's slot times its factor
+// of (y, kz): the pairs of one tile run side by side on one XCD, so an input
+// read by several outputs comes from its L2 after the first.
+template <typename T, int L, int AXIS, int DIR, int V, int PL, bool MO = false>
 __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_waves_per_eu(XYCfg<T, L, V>::WPE)))
-    k_xy_pass(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg, int nmain, int gper, int k0) {
+    k_xy_pass(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg, int nmain, int gper, int k0,
+              XOut xo) {
   using C2 = cx_t<T>;
   constexpr int C = XYCfg<T, L, V>::C, NT = XYCfg<T, L, V>::NT;
+  static_assert(!MO || (AXIS == 1 && DIR > 0), "multi-output: inverse y passes");
   __shared__ T tile_r[L * (C + 1)];  // real-valued exchange tile (one component at a time)
   __shared__ C2 tw[L];
+  __shared__ T fyt[MO ? L : 1];  // MO: the output's factor of ky, per y
   load_twiddles(tw, twg + L, L, NT);
-  C2* f = base + (long long)blockIdx.y * g.fs;
+  // MO: this block's output, its input and its code, picked with constant indices (the index is
+  // uniform; a runtime index into the argument struct would go through scratch)
+  int src = blockIdx.y, dst = blockIdx.y;
+  unsigned code = 0;
+  const int w = xcd_tile(blockIdx.x, gridDim.x);  // this workgroup's (tile block, pair)
+  const int wt = MO ? w / xo.n : w;
+  if constexpr (MO) {
+    const int q = w % xo.n;
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+      if (i == q) {
+        src = xo.src[i];
+        dst = xo.dst[i];
+        code = xo.code[i];
+      }
+    for (int y = threadIdx.x; y < L; y += NT) {
+      double fy = 1;
+      if (code & kXsKy) fy *= xo.kv[y];
+      if (code & kXsKy2) fy *= xo.kv[y] * xo.kv[y];
+      if (code & kXsCic) fy *= xo.icic[y];
+      fyt[y] = (T)fy;
+    }
+  }
+  const C2* fi = base + (long long)src * g.fs;
+  C2* fo = base + (long long)dst * g.fs;
   constexpr int TPW = XYCfg<T, L, V>::TPW;
   const int N = g.N, H = g.H;
+  // MO: the factor of kz (with the sign) of this thread's column of tile t (y pass: the column's
+  // kz, or kz = N/2 on the Nyquist plane)
+  auto factor_z = [&](int t) -> T {
+    const int zi = t < nmain ? k0 + (t % gper) * C + (int)(threadIdx.x % C) : H;
+    double fz = (code & kXsNeg) ? -1.0 : 1.0;
+    if (code & kXsKz) fz *= xo.kv[zi];
+    if (code & kXsKz2) fz *= xo.kv[zi] * xo.kv[zi];
+    if (code & kXsCic) fz *= xo.icic[zi];
+    return (T)fz;
+  };
   // The offset of element (pos, col) of tile t is (pos / B) s1 + (pos % B) s2 + ct, B = kXB,
   // with tile-uniform strides s1, s2 and a column term ct that the thread computes once per
   // tile (each thread of the split-exchange tile keeps one column, col = threadIdx.x % C).
@@ -175,35 +234,52 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
   auto off = [](const Addr& A, int pos) -> unsigned {
     return (unsigned)(pos >> kXBLog) * A.s1 + (unsigned)(pos & (kXB - 1)) * A.s2 + A.ct;
   };
+  // the tile's input element at pos (MO: times fz fy(pos), and i where the code says so)
+  auto load = [&](const Addr& A, T fz, int pos) -> C2 {
+    const C2 v = fi[off(A, pos)];
+    if constexpr (MO) {
+      const T s = fz * fyt[pos];
+      return (code & kXsI) ? cmake<C2>(-s * v.y, s * v.x) : cmake<C2>(s * v.x, s * v.y);
+    } else {
+      return v;
+    }
+  };
   if constexpr (TPW == 1) {
-    const Addr A = tile_addr(xcd_tile(blockIdx.x, gridDim.x));
-    __syncthreads();  // twiddles
+    const int t = wt;
+    const Addr A = tile_addr(t);
+    const T fz = MO ? factor_z(t) : (T)1;
+    __syncthreads();  // twiddles (and the factor table)
     fft_tile_split<T, L, C, NT, DIR, PL>(
-        tile_r, tw, [&](int pos, int) { return f[off(A, pos)]; }, [&](int pos, int, C2 v) { f[off(A, pos)] = v; });
+        tile_r, tw, [&](int pos, int) { return load(A, fz, pos); }, [&](int pos, int, C2 v) { fo[off(A, pos)] = v; });
   } else {
     // tiles t0 .. t0 + TPW - 1 (the last workgroup may hold fewer); the condition of every
     // branch and loop below is uniform over the workgroup, so every barrier is reached
     constexpr int E = C * L / NT;
     const int ntiles = nmain + N / C;
-    const int t0 = xcd_tile(blockIdx.x, gridDim.x) * TPW;
+    const int t0 = wt * TPW;
     C2 v[E], nv[E];
     Addr A = tile_addr(t0);
-    split_load<T, L, C, NT, PL>(nv, [&](int pos, int) { return f[off(A, pos)]; });
+    if constexpr (MO) __syncthreads();  // the factor table
+    T fz = MO ? factor_z(t0) : (T)1;
+    split_load<T, L, C, NT, PL>(nv, [&](int pos, int) { return load(A, fz, pos); });
     for (int i = 0; i < TPW && t0 + i < ntiles; ++i) {
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = nv[e];
       const Addr Ac = A;
       if (i + 1 < TPW && t0 + i + 1 < ntiles) {
         A = tile_addr(t0 + i + 1);
-        split_load<T, L, C, NT, PL>(nv, [&](int pos, int) { return f[off(A, pos)]; });
+        if constexpr (MO) fz = factor_z(t0 + i + 1);
+        split_load<T, L, C, NT, PL>(nv, [&](int pos, int) { return load(A, fz, pos); });
       }
-      split_transform<T, L, C, NT, DIR, PL>(tile_r, tw, v, [&](int pos, int, C2 x) { f[off(Ac, pos)] = x; });
+      split_transform<T, L, C, NT, DIR, PL>(tile_r, tw, v, [&](int pos, int, C2 x) { fo[off(Ac, pos)] = x; });
     }
   }
 }
 
+// xo: the y pass of the outputs of a multi-output split (inverse y passes only, nf = 1)
 template <typename T, int L, int AXIS, int DIR, int V, int PL>
-static int launch_xy_v(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, const XYRange& r) {
+static int launch_xy_v(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, const XYRange& r,
+                       const XOut* xo) {
   using Cfg = XYCfg<T, L, V>;
   if constexpr (!Cfg::VALID) {
     return AAR_ERR_UNSUPPORTED;
@@ -215,8 +291,22 @@ static int launch_xy_v(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hip
     const int nmain = g.N * gper;
     const int nnyq = r.nyq ? g.N / C : 0;
     if (nmain + nnyq == 0) return AAR_OK;
-    dim3 grid((nmain + nnyq + Cfg::TPW - 1) / Cfg::TPW, nf);
-    hipLaunchKernelGGL((k_xy_pass<T, L, AXIS, DIR, V, PL>), grid, dim3(NT), 0, st, f, g, tw, nmain, gper, r.k0);
+    const int nwg = (nmain + nnyq + Cfg::TPW - 1) / Cfg::TPW;
+    if (xo) {
+      // the factor table takes L more values of LDS
+      constexpr bool MO_FITS =
+          (long)L * (C + 1) * sizeof(T) + (long)L * sizeof(cx_t<T>) + (long)L * sizeof(T) <= 160 * 1024;
+      if (nf != 1 || xo->n < 1 || xo->n > 9) return AAR_ERR_UNSUPPORTED;
+      if constexpr (AXIS == 1 && DIR > 0 && MO_FITS) {
+        hipLaunchKernelGGL((k_xy_pass<T, L, AXIS, DIR, V, PL, true>), dim3(nwg * xo->n, 1), dim3(NT), 0, st, f, g,
+                           tw, nmain, gper, r.k0, *xo);
+      } else {
+        return AAR_ERR_UNSUPPORTED;
+      }
+    } else {
+      hipLaunchKernelGGL((k_xy_pass<T, L, AXIS, DIR, V, PL>), dim3(nwg, nf), dim3(NT), 0, st, f, g, tw, nmain, gper,
+                         r.k0, XOut());
+    }
     return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
   }
 }
@@ -225,28 +315,29 @@ static int launch_xy_v(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hip
 // than 0 for L = 512 and 1024 (three stages of radix 16, 8, 4 or 2 in different orders).
 template <typename T, int L, int AXIS, int DIR, int PL>
 static int launch_xy_p(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, int v,
-                       const XYRange& r) {
+                       const XYRange& r, const XOut* xs) {
   if constexpr (L >= 256) {
     switch (v) {
-      case 1: return launch_xy_v<T, L, AXIS, DIR, 1, PL>(f, nf, g, tw, st, r);
-      case 2: return launch_xy_v<T, L, AXIS, DIR, 2, PL>(f, nf, g, tw, st, r);
-      case 3: return launch_xy_v<T, L, AXIS, DIR, 3, PL>(f, nf, g, tw, st, r);
+      case 1: return launch_xy_v<T, L, AXIS, DIR, 1, PL>(f, nf, g, tw, st, r, xs);
+      case 2: return launch_xy_v<T, L, AXIS, DIR, 2, PL>(f, nf, g, tw, st, r, xs);
+      case 3: return launch_xy_v<T, L, AXIS, DIR, 3, PL>(f, nf, g, tw, st, r, xs);
       default: break;
     }
   }
-  return launch_xy_v<T, L, AXIS, DIR, 0, PL>(f, nf, g, tw, st, r);
+  return launch_xy_v<T, L, AXIS, DIR, 0, PL>(f, nf, g, tw, st, r, xs);
 }
 
 template <typename T, int L, int AXIS, int DIR>
-static int launch_xy(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, int v, const XYRange& r) {
+static int launch_xy(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, int v, const XYRange& r,
+                     const XOut* xs) {
   if constexpr (L >= 512) {
     switch (aar_xy_plan(sizeof(T) == 8, L)) {
-      case 1: return launch_xy_p<T, L, AXIS, DIR, 1>(f, nf, g, tw, st, v, r);
-      case 2: return launch_xy_p<T, L, AXIS, DIR, 2>(f, nf, g, tw, st, v, r);
+      case 1: return launch_xy_p<T, L, AXIS, DIR, 1>(f, nf, g, tw, st, v, r, xs);
+      case 2: return launch_xy_p<T, L, AXIS, DIR, 2>(f, nf, g, tw, st, v, r, xs);
       default: break;
     }
   }
-  return launch_xy_p<T, L, AXIS, DIR, 0>(f, nf, g, tw, st, v, r);
+  return launch_xy_p<T, L, AXIS, DIR, 0>(f, nf, g, tw, st, v, r, xs);
 }
 
 // ------------------------------------------------------------------ z passes
@@ -610,23 +701,24 @@ int Pipeline<T>::z_ng(cx_t<T>* f, int M, int K, const int* gs, const NgRec4* rec
 
 template <typename T>
 int Pipeline<T>::xy_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int axis, int dir,
-                         hipStream_t st, int variant, XYRange r) {
+                         hipStream_t st, int variant, XYRange r, const XOut* xs) {
   if (!g.pow2) {  // whole passes only: column groups belong to the power-of-two tiles
-    if (r.k0 != 0 || (r.cols != 0 && r.cols != g.H) || !r.nyq) return AAR_ERR_UNSUPPORTED;
+    if (r.k0 != 0 || (r.cols != 0 && r.cols != g.H) || !r.nyq || xs) return AAR_ERR_UNSUPPORTED;
     if (axis == 0) return dir > 0 ? launch_xy_gen<T, 0, 1>(f, nf, g, tw, st) : launch_xy_gen<T, 0, -1>(f, nf, g, tw, st);
     return dir > 0 ? launch_xy_gen<T, 1, 1>(f, nf, g, tw, st) : launch_xy_gen<T, 1, -1>(f, nf, g, tw, st);
   }
   const int v = variant >= 0 ? variant : aar_xy_variant(sizeof(T) == 8, g.N, axis);
   if (variant < 0 && v != 0) {  // a pinned shape that is not built for this size runs as shape 0
-    const int rc = xy_pass(f, nf, g, tw, axis, dir, st, v, r);
-    return rc == AAR_ERR_UNSUPPORTED ? xy_pass(f, nf, g, tw, axis, dir, st, 0, r) : rc;
+    const int rc = xy_pass(f, nf, g, tw, axis, dir, st, v, r, xs);
+    return rc == AAR_ERR_UNSUPPORTED ? xy_pass(f, nf, g, tw, axis, dir, st, 0, r, xs) : rc;
   }
 #define XY_CASE(LL)                                                                                 \
   case LL:                                                                                          \
     if (axis == 0)                                                                                  \
-      return dir > 0 ? launch_xy<T, LL, 0, 1>(f, nf, g, tw, st, v, r)                               \
-                     : launch_xy<T, LL, 0, -1>(f, nf, g, tw, st, v, r);                             \
-    return dir > 0 ? launch_xy<T, LL, 1, 1>(f, nf, g, tw, st, v, r) : launch_xy<T, LL, 1, -1>(f, nf, g, tw, st, v, r);
+      return dir > 0 ? launch_xy<T, LL, 0, 1>(f, nf, g, tw, st, v, r, xs)                       \
+                     : launch_xy<T, LL, 0, -1>(f, nf, g, tw, st, v, r, xs);                     \
+    return dir > 0 ? launch_xy<T, LL, 1, 1>(f, nf, g, tw, st, v, r, xs)                         \
+                   : launch_xy<T, LL, 1, -1>(f, nf, g, tw, st, v, r, xs);
   switch (g.N) {
     XY_CASE(16)
     XY_CASE(32)
@@ -659,6 +751,82 @@ int Pipeline<T>::xy_pair(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, i
         return rc;
     }
   return AAR_OK;
+}
+
+// The y pass of the outputs of inputs [q_lo, q_hi) of a split, as one launch list.
+static XOut split_outputs(const XSplit& xs, int q_lo, int q_hi) {
+  XOut o;
+  o.kv = xs.kv;
+  o.icic = xs.icic;
+  for (int d = 0; d < xs.nout; ++d)
+    if (xs.src[d] >= q_lo && xs.src[d] < q_hi) {
+      o.src[o.n] = (signed char)(xs.in0 + xs.src[d]);
+      o.dst[o.n] = (signed char)d;
+      o.code[o.n] = xs.code[d];
+      ++o.n;
+    }
+  return o;
+}
+
+template <typename T>
+int Pipeline<T>::xy_split(cx_t<T>* f, const XSplit& xs, const Geom& g, const cx_t<T>* tw, hipStream_t st,
+                          int group_cols) {
+  // inputs after the outputs, each output reading one of them
+  if (!g.pow2 || xs.nin < 1 || xs.nout < xs.nin || xs.nout > 9 || xs.in0 < xs.nout) return AAR_ERR_UNSUPPORTED;
+  for (int d = 0; d < xs.nout; ++d)
+    if (xs.src[d] < 0 || xs.src[d] >= xs.nin) return AAR_ERR_UNSUPPORTED;
+  int rc;
+  cx_t<T>* in = f + (long long)xs.in0 * g.fs;
+  if (group_cols < 0) group_cols = aar_xy_split_group(sizeof(T) == 8, g.N);
+  if (group_cols <= 0) {
+    const XOut o = split_outputs(xs, 0, xs.nin);
+    if ((rc = xy_pass(in, xs.nin, g, tw, 0, +1, st))) return rc;
+    return xy_pass(f, 1, g, tw, 1, +1, st, -1, XYRange(), &o);
+  }
+  for (int q = 0; q < xs.nin; ++q) {
+    const XOut o = split_outputs(xs, q, q + 1);
+    for (int k0 = 0; k0 < g.H; k0 += group_cols) {
+      const XYRange r{k0, k0 + group_cols < g.H ? group_cols : g.H - k0, k0 == 0};
+      if ((rc = xy_pass(in + (long long)q * g.fs, 1, g, tw, 0, +1, st, -1, r)) ||
+          (o.n && (rc = xy_pass(f, 1, g, tw, 1, +1, st, -1, r, &o))))
+        return rc;
+    }
+  }
+  return AAR_OK;
+}
+
+XSplit lpt_xsplit(const KSpace& ks) {
+  // inputs (ng_lpt, split) in slots 9-13: kx^2 B, kx B, B, i kx sx B, sx B with B = T(k)/(D+ Beta)
+  // Phi and sx = 1/sinc_x^2; outputs -kx kx B, -kx ky B, -kx kz B, -ky ky B, -ky kz B, -kz kz B
+  // and i k_a B sx sy sz (main.c:1207-1211, 1238-1277, 1311-1369)
+  XSplit xs;
+  xs.nin = 5;
+  xs.nout = 9;
+  xs.in0 = 9;
+  const signed char src[9] = {0, 1, 1, 2, 2, 2, 3, 4, 4};
+  const unsigned char code[9] = {kXsNeg,         kXsKy | kXsNeg,         kXsKz | kXsNeg,
+                                 kXsKy2 | kXsNeg, kXsKy | kXsKz | kXsNeg, kXsKz2 | kXsNeg,
+                                 kXsCic,         kXsKy | kXsCic | kXsI,  kXsKz | kXsCic | kXsI};
+  for (int o = 0; o < 9; ++o) xs.src[o] = src[o], xs.code[o] = code[o];
+  xs.kv = ks.kv;
+  xs.icic = ks.icic;
+  return xs;
+}
+
+XSplit disp2_xsplit(const KSpace& ks) {
+  // inputs (disp2_spec, split) in slots 3-4: -i kx sx S/k^2, sx S/k^2 (S the 2LPT source);
+  // outputs -i k_a S/k^2 sx sy sz (main.c:1353-1366)
+  XSplit xs;
+  xs.nin = 2;
+  xs.nout = 3;
+  xs.in0 = 3;
+  xs.src[0] = 0, xs.src[1] = 1, xs.src[2] = 1;
+  xs.code[0] = kXsCic;
+  xs.code[1] = kXsKy | kXsCic | kXsI | kXsNeg;
+  xs.code[2] = kXsKz | kXsCic | kXsI | kXsNeg;
+  xs.kv = ks.kv;
+  xs.icic = ks.icic;
+  return xs;
 }
 
 template <typename T>
@@ -1115,7 +1283,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
     k_ng_lpt(const cx_t<T>* __restrict__ phig, const cx_t<T>* h, int K, const int* __restrict__ kmodes,
              cx_t<T>* __restrict__ phi, cx_t<T>* out, double fnl, double n3, Geom g, KSpace ks,
-             TransferDev tf, double dplus, double beta) {
+             TransferDev tf, double dplus, double beta, int split) {
   using C2 = cx_t<T>;
   using RT = T;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1146,6 +1314,17 @@ __global__ void __launch_bounds__(256)
   const C2 p = cmake<C2>((T)re, (T)im);
   phi[e] = p;
   const double twb = (double)transfer_dev<T>(tf, (T)kmag1) / dplus / beta;
+  if (split) {
+    // the x-pass inputs of lpt_xsplit: B = twb Phi times the factors of kx (all reads of h at e
+    // are done; the outputs may overwrite them)
+    const double bx = twb * (double)p.x, by = twb * (double)p.y, kx = kv[0], sx = ks.icic[x];
+    out[e] = cmake<C2>((T)(kx * kx * bx), (T)(kx * kx * by));
+    out[g.fs + e] = cmake<C2>((T)(kx * bx), (T)(kx * by));
+    out[2LL * g.fs + e] = cmake<C2>((T)bx, (T)by);
+    out[3LL * g.fs + e] = cmake<C2>((T)(-kx * sx * by), (T)(kx * sx * bx));
+    out[4LL * g.fs + e] = cmake<C2>((T)(sx * bx), (T)(sx * by));
+    return;
+  }
   double cr[3], ci[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -1171,10 +1350,10 @@ __global__ void __launch_bounds__(256)
 template <typename T>
 int Pipeline<T>::ng_lpt(const cx_t<T>* phig, const cx_t<T>* h, int K, const int* kmodes, cx_t<T>* phi,
                         cx_t<T>* out9, double fnl, const Geom& g, const KSpace& ks, const TransferDev& tf,
-                        double dplus, double beta, hipStream_t st) {
+                        double dplus, double beta, hipStream_t st, bool split) {
   const double n3 = (double)((unsigned)g.N * (unsigned)g.N * (unsigned)g.N);
   hipLaunchKernelGGL((k_ng_lpt<T>), dim3((unsigned)((g.fs + 255) / 256)), dim3(256), 0, st, phig, h, K, kmodes,
-                     phi, out9, fnl, n3, g, ks, tf, dplus, beta);
+                     phi, out9, fnl, n3, g, ks, tf, dplus, beta, split ? 1 : 0);
   return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
 }
 
@@ -1200,7 +1379,7 @@ template <typename T> int Pipeline<T>::source(cx_t<T>* dg6, const Geom& g, hipSt
 // psi2_a = src k_a / (i k^2), CIC-deconvolved (main.c:1311-1369)
 template <typename T>
 __global__ void __launch_bounds__(256)
-    k_disp2_spec(const cx_t<T>* __restrict__ src, cx_t<T>* __restrict__ out, Geom g, KSpace ks) {
+    k_disp2_spec(const cx_t<T>* __restrict__ src, cx_t<T>* __restrict__ out, Geom g, KSpace ks, int split) {
   using C2 = cx_t<T>;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= g.fs) return;
@@ -1209,6 +1388,13 @@ __global__ void __launch_bounds__(256)
     using RT = T;
     const RT kv[3] = {(RT)ks.kv[x], (RT)ks.kv[y], (RT)ks.kv[z]};
     const RT kmag2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+    if (split) {  // the x-pass inputs of disp2_xsplit: -i kx sx S/k^2 and sx S/k^2
+      const RT f = kmag2 > (RT)0 ? (RT)ks.icic[x] / kmag2 : (RT)0;
+      const C2 s = src[e];
+      out[e] = cmake<C2>(s.y * kv[0] * f, -s.x * kv[0] * f);
+      out[g.fs + e] = cmake<C2>(s.x * f, s.y * f);
+      return;
+    }
     const RT ff = (RT)1 / ((RT)ks.sinc[x] * (RT)ks.sinc[y] * (RT)ks.sinc[z]);
     const RT f = kmag2 > (RT)0 ? ff * ff / kmag2 : (RT)0;  // CIC deconvolution / k^2
     const C2 s = src[e];
@@ -1219,9 +1405,9 @@ __global__ void __launch_bounds__(256)
 
 template <typename T>
 int Pipeline<T>::disp2_spec(const cx_t<T>* src, cx_t<T>* out3, const Geom& g, const KSpace& ks,
-                            hipStream_t st) {
+                            hipStream_t st, bool split) {
   hipLaunchKernelGGL((k_disp2_spec<T>), dim3((unsigned)((g.fs + 255) / 256)), dim3(256), 0, st, src,
-                     out3, g, ks);
+                     out3, g, ks, split ? 1 : 0);
   return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
 }
 

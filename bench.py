@@ -162,6 +162,11 @@ def rel_errors(a, b):
     return {"max": float(f"{max(err.values()):.3g}"), "fields": {n: float(f"{v:.3g}") for n, v in err.items()}}
 
 
+def kind_gbs(stats):
+    """Algorithmic GB/s of every kernel kind: its summed bytes over its summed HIP-event time."""
+    return {k: round(v["bytes"] / v["seconds"] / 1e9, 1) for k, v in stats.items() if v["seconds"] > 0}
+
+
 def roofline_of(stats):
     dom = max(stats, key=lambda k: stats[k]["seconds"])
     d = stats[dom]
@@ -191,12 +196,14 @@ def north_star_1024(aar, ctx, steps=3, fp64_steps=2):
             "fp32_over_fp64": round(ms64 / ms32, 3),
             "autotune": {"radix_plan": plan, "us_per_xy_pair_8_fields": [round(v * 1e6, 1) for v in psec],
                          "xy_tile_variant": [aar.lib().aar_xy_tile_variant(N, aar.AAR_FP32, a) for a in (0, 1)],
-                         "xy_pair_group_cols": aar.lib().aar_xy_group_cols(N, aar.AAR_FP32)},
+                         "xy_pair_group_cols": aar.lib().aar_xy_group_cols(N, aar.AAR_FP32),
+                         "xy_split_group_cols": aar.lib().aar_xy_split_group_cols(N, aar.AAR_FP32)},
             "job_roofline": {"bytes_per_job": B, "frac": round(B / (ms32 * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)},
             "xy_access_pattern_gbs": xy_pattern_gbs(ctx, N, nf=4),
             "dominant_kernel": {"kernel": dom, "achieved_gbs": round(achieved, 1),
                                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                                 "avg_launch_us": round(d["seconds"] / d["launches"] * 1e6, 2)},
+            "kernel_gbs": kind_gbs(r32["stats"]),
             "kernel_time_share": {k: round(v["seconds"] / sum(x["seconds"] for x in r32["stats"].values()), 4)
                                   for k, v in r32["stats"].items()},
             "fp32_rel_l2_vs_fp64": dict(sub_box=256, **rel_errors(r32["box"], r64["box"]))}
@@ -419,6 +426,9 @@ def main():
                          "counter runs serialize every launch, which skews the autotuner, so they pin it")
     ap.add_argument("--xy-plan", type=int, default=-1,
                     help="pin the x/y radix plan (default: the autotuner's choice); counter runs pin it too")
+    ap.add_argument("--xy-split-group", type=int, default=-2,
+                    help="pin the kz column groups of the multi-output x/y pass pairs (-1: the pairs' choice; "
+                         "default: the autotuner's choice)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: measures one GPU only; --gpus above 1 is not supported (one IC job runs on one "
@@ -432,15 +442,19 @@ def main():
     # x / y tile shapes and x / y pair column groups for this mesh, picked by the autotuner
     # before the timed region
     pow2 = N & (N - 1) == 0  # other mesh sizes run the mixed-radix passes, which have no tuned shapes
-    tsec, tvar, psec, pchosen = ([[0] * 4] * 2, [0, 0], [0] * 3, 0)
+    tsec, tvar, psec, pchosen, ssec = ([[0] * 4] * 2, [0, 0], [0] * 3, 0, [0] * 5)
     if pow2:
-        tsec, tvar = ctx.autotune(N, aar.AAR_FP32, nf=8)
-        psec, pchosen = ctx.autotune_plans(N, aar.AAR_FP32, nf=8)  # the same search, reporting the plans
+        tsec, psec, ssec = ctx.autotune_report(N, aar.AAR_FP32, nf=8)
+        tvar = [aar.lib().aar_xy_tile_variant(N, aar.AAR_FP32, a) for a in (0, 1)]
+        pchosen = aar.lib().aar_xy_radix_plan(N, aar.AAR_FP32)
     if args.xy_group >= 0 and aar.lib().aar_set_xy_group_cols(N, aar.AAR_FP32, args.xy_group) != 0:
         sys.exit(f"--xy-group {args.xy_group}: not a valid column group for N = {N}")
     if args.xy_plan >= 0 and aar.lib().aar_set_xy_radix_plan(N, aar.AAR_FP32, args.xy_plan) != 0:
         sys.exit(f"--xy-plan {args.xy_plan}: not a valid radix plan for N = {N}")
+    if args.xy_split_group >= -1 and aar.lib().aar_set_xy_split_group_cols(N, aar.AAR_FP32, args.xy_split_group) != 0:
+        sys.exit(f"--xy-split-group {args.xy_split_group}: not a valid column group for N = {N}")
     xy_group = aar.lib().aar_xy_group_cols(N, aar.AAR_FP32)
+    xy_split_group = aar.lib().aar_xy_split_group_cols(N, aar.AAR_FP32)
     # timed region: a stream of jobs with fresh seeds, eager launches each bracketed by HIP events
     eb = 0 if args.no_fp64 else min(128, N)
     r32 = time_stream(aar, ctx, cfg, tabs, aar.AAR_FP32, aar.AAR_SCHED_BATCHED, args.steps, args.warmup, True,
@@ -476,6 +490,7 @@ def main():
                      "algorithmic_bytes_per_launch": round(d["bytes"] / d["launches"]),
                      "avg_launch_us": round(d["seconds"] / d["launches"] * 1e6, 2)},
         "kernel_time_share": {k: round(v["seconds"] / ktime, 4) for k, v in stats.items()},
+        "kernel_gbs": kind_gbs(stats),
         "job_roofline": {"bytes_per_job": survey_bytes(N, M, K, 4),
                          "frac": round(survey_bytes(N, M, K, 4) / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)},
         "device_pipeline": {"ms_per_job": round(r32["device"] * 1e3, 3),
@@ -503,6 +518,10 @@ def main():
                                                   for i, ax in enumerate(("x", "y"))},
                        "variants": "0: 128-B tile columns, 1: 64-B, 2: 256-B, 3: 128-B, four tiles per workgroup with the next tile's loads in flight (same radix plan)",
                        "xy_pair_group_cols": xy_group,
+                       "xy_split_group_cols": {"chosen": xy_split_group,
+                                               "us_per_call_9_outputs": [round(v * 1e6, 1) if v > 0 else None
+                                                                         for v in ssec],
+                                               "groups": "whole passes, all columns, halves, quarters, eighths"},
                        "radix_plan": {"chosen": pchosen,
                                       "us_per_xy_pair_8_fields": [round(v * 1e6, 1) if v > 0 else None for v in psec],
                                       "plans": "0: radices spread evenly (512 = 8 8 8, 1024 = 16 8 8), "

@@ -226,9 +226,11 @@ double aar_job_last_seconds(aar_job *job);
 /* Per-kernel-kind timing: when on, every launch of the next runs is bracketed
  * by HIP events on the job's stream; stats give launches, summed seconds and
  * summed algorithmic bytes (reads + writes of the data each launch must move). */
+/* AAR_K_XY_SPLIT: the batched schedule's multi-output x / y pass pairs of the 2LPT spectra
+ * (aar_xy_split_group_cols), whose algorithmic bytes count each input read once. */
 enum {
   AAR_K_GAUSS = 0, AAR_K_XY = 1, AAR_K_Z = 2, AAR_K_MODES = 3, AAR_K_PRODUCTS = 4,
-  AAR_K_COMBINE = 5, AAR_K_LPT = 6, AAR_K_CIC = 7, AAR_K_ACCUM = 8, AAR_NKINDS = 9
+  AAR_K_COMBINE = 5, AAR_K_LPT = 6, AAR_K_CIC = 7, AAR_K_ACCUM = 8, AAR_K_XY_SPLIT = 9, AAR_NKINDS = 10
 };
 int aar_job_set_profiling(aar_job *job, int on);
 /* Graph mode: the first run captures every launch of the job into one
@@ -362,6 +364,13 @@ int aar_autotune(aar_ctx *ctx, int n, int precision, int nf, double *seconds, in
  * the tile shapes, plans change the rounding of the transforms (by a few units in the last place).
  * Counterpart of FFTW's planner choosing among algorithms (rfftw3d_mpi_create_plan, main.c:1566). */
 int aar_autotune_plans(aar_ctx *ctx, int n, int precision, int nf, double *plan_seconds, int *plan);
+/* aar_autotune reporting every timing of its one search (each optional): tile_seconds [2][4] as
+ * aar_autotune's, plan_seconds [3] as aar_autotune_plans', split_seconds [5] mean seconds per
+ * multi-output pair call over whole passes, all columns, halves, quarters, eighths (-1: not
+ * timed).  The choices kept: aar_xy_tile_variant, aar_xy_radix_plan, aar_xy_group_cols,
+ * aar_xy_split_group_cols. */
+int aar_autotune_report(aar_ctx *ctx, int n, int precision, int nf, double *tile_seconds,
+                        double *plan_seconds, double *split_seconds);
 /* radix plan in use for (n, precision), or a negative status; the setter pins one (0 only for
  * meshes other than 512 and 1024) */
 int aar_xy_radix_plan(int n, int precision);
@@ -377,6 +386,16 @@ int aar_set_xy_tile_variant(int n, int precision, int axis, int variant);
  * in use for (n, precision) or a negative status. */
 int aar_xy_group_cols(int n, int precision);
 int aar_set_xy_group_cols(int n, int precision, int cols);
+/* Column groups of the batched schedule's multi-output x / y pass pairs: the nine 2LPT spectra
+ * (main.c:1158-1277, 1311-1369) are one spectrum times factors of kx, ky and kz, so they leave
+ * k-space as five x transforms, each read by the y pass once per spectrum it feeds, times that
+ * spectrum's ky and kz factors (the three 2LPT displacements as two x transforms).  aar_autotune
+ * (and a job's first prepare) also times these pairs over whole passes, all columns, halves,
+ * quarters and eighths on nine fields and keeps the fastest.  cols = -1 in the setter follows
+ * the pairs' choice (aar_xy_group_cols).  Results are bit-identical for every choice.
+ * Counterpart of FFTW's planning (main.c:1556-1599). */
+int aar_xy_split_group_cols(int n, int precision);
+int aar_set_xy_split_group_cols(int n, int precision, int cols);
 
 /* ---- building blocks (tests, tools) ---- */
 /* 3-D transforms with FFTW 2 rfftwnd conventions on a host array in the
