@@ -308,13 +308,13 @@ class Context:
         return sec, pl.value
 
     def autotune_report(self, n, precision=AAR_FP32, nf=8):
-        """One autotuner search with every timing: (tile shapes [2][4], radix plans [3], column
-        groups of the multi-output pairs [5]: whole passes, all columns, halves, quarters,
-        eighths), seconds, -1 where not timed."""
-        ts, ps, ss = np.full(8, -1.0), np.full(3, -1.0), np.full(5, -1.0)
+        """One autotuner search with every timing (seconds, -1 where not timed): tile shapes [2][4],
+        x + y pass pairs per radix plan and column group [3][4] (whole passes, all columns, halves,
+        quarters), multi-output pairs per column group [5] (the same and eighths)."""
+        ts, ps, ss = np.full(8, -1.0), np.full(12, -1.0), np.full(5, -1.0)
         _check(lib().aar_autotune_report(self.h, n, precision, nf, _dptr(ts), _dptr(ps), _dptr(ss)),
                "aar_autotune_report")
-        return ts.reshape(2, 4), ps, ss
+        return ts.reshape(2, 4), ps.reshape(3, 4), ss
 
     def cached_plans(self):
         """(count, device bytes) of the job tables the context keeps resident for later jobs."""

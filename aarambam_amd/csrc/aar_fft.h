@@ -402,22 +402,6 @@ __device__ __forceinline__ void split_slot(int e, int& pos, int& col) {
   pos = threadIdx.x / C + (e / R) * ROWS + (e % R) * STR;
 }
 
-// Output slots of the last stage of fft_tile_split: the (pos, col) that slot e
-// of this thread is stored to.
-template <typename T, int L, int C, int NT, int PL = 0>
-__device__ __forceinline__ void split_last_slot(int e, int& pos, int& col) {
-  constexpr int E = C * L / NT;
-  constexpr int RMAX = E < 16 ? E : 16;
-  constexpr int NS = plan_nstages(L, RMAX);
-  constexpr int R = plan_radix(L, RMAX, NS - 1, PL);
-  constexpr int P = plan_prefix(L, RMAX, NS - 1, PL);
-  constexpr int ROWS = NT / C;
-  col = threadIdx.x % C;
-  const int i = threadIdx.x / C + (e / R) * ROWS;
-  const int k = i % P;
-  pos = (i - k) * R + k + (e % R) * P;
-}
-
 // Input slots of the first stage of fft_tile: the (pos, col) that slot e of
 // this thread is loaded from.
 template <typename T, int L, int C, int NT, bool RM>

@@ -652,7 +652,7 @@ static bool g_tuned[2][11] = {};
 
 template <typename T>
 static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, int nslots, double* sec, int* chosen,
-                      double* plan_sec = nullptr, double* split_sec = nullptr) {
+                      double* plan_sec = nullptr, double* split_sec = nullptr, double* pair_sec = nullptr) {
   using C2 = cx_t<T>;
   const Geom g = make_geom(N);
   C2* d = (C2*)scratch;
@@ -660,7 +660,7 @@ static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, int nslots, do
   // the job's own pool, holds nslots)
   const int nall = scratch ? nslots : std::max(nf, 14);
   if (!d && device_alloc((void**)&d, (size_t)nall * g.fs * sizeof(C2))) return AAR_ERR_ALLOC;
-  int rc = aar_zero_fill((double*)d, (long long)nall * g.fs * (long long)(sizeof(C2) / 8), ctx->stream);
+  int rc = aar_noise_fill<T>((T*)d, (long long)nall * g.fs * 2, ctx->stream);
   const C2* tw = sizeof(T) == 4 ? (const C2*)ctx->tw32 : (const C2*)ctx->tw64;
   hipEvent_t a = nullptr, b = nullptr;
   hipEventCreate(&a);
@@ -715,12 +715,15 @@ static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, int nslots, do
   for (int pl = 0; pl < nplan && !rc; ++pl) {
     aar_set_xy_plan(sizeof(T) == 8, N, pl);
     double bestp = 1e30;
+    int gi = 0;
     for (int gc : {0, g.H, g.H / 2, g.H / 4}) {
       if (rc || (gc > 0 && gc < 16)) break;
       double t;
       const int r = time_calls(
           [&](int q) { return Pipeline<T>::xy_pair(d, nf, g, tw, q % 2 ? -1 : +1, ctx->stream, gc); }, t);
       if (r) rc = r;
+      if (pair_sec) pair_sec[pl * 4 + gi] = t < 1e30 ? t : -1.0;
+      ++gi;
       bestp = std::min(bestp, t);
       if (t < best) {
         best = t;
@@ -760,6 +763,8 @@ static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, int nslots, do
   hipEventDestroy(a);
   hipEventDestroy(b);
   if (!scratch) hipFree(d);
+  else if (!rc)  // a job's pool: leave it as a fresh allocation is left to the job (zeroed)
+    rc = aar_zero_fill((double*)d, (long long)nall * g.fs * (long long)(sizeof(C2) / 8), ctx->stream);
   if (!rc) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[sizeof(T) == 8][ilog2i(N)] = true;
@@ -798,16 +803,16 @@ extern "C" int aar_autotune_plans(aar_ctx* ctx, int n, int precision, int nf, do
 }
 
 extern "C" int aar_autotune_report(aar_ctx* ctx, int n, int precision, int nf, double* tile_seconds,
-                                   double* plan_seconds, double* split_seconds) {
+                                   double* pair_seconds, double* split_seconds) {
   if (!ctx || !pow2(n) || nf < 1 || (precision != AAR_FP32 && precision != AAR_FP64)) return AAR_ERR_BAD_ARG;
   AAR_HIP(hipSetDevice(ctx->device));
-  if (plan_seconds)
-    for (int q = 0; q < kXYPlanCount; ++q) plan_seconds[q] = -1.0;
+  if (pair_seconds)
+    for (int q = 0; q < 4 * kXYPlanCount; ++q) pair_seconds[q] = -1.0;
   if (split_seconds)
     for (int q = 0; q < 5; ++q) split_seconds[q] = -1.0;
   return precision == AAR_FP32
-             ? autotune_t<float>(ctx, n, nf, nullptr, 0, tile_seconds, nullptr, plan_seconds, split_seconds)
-             : autotune_t<double>(ctx, n, nf, nullptr, 0, tile_seconds, nullptr, plan_seconds, split_seconds);
+             ? autotune_t<float>(ctx, n, nf, nullptr, 0, tile_seconds, nullptr, nullptr, split_seconds, pair_seconds)
+             : autotune_t<double>(ctx, n, nf, nullptr, 0, tile_seconds, nullptr, nullptr, split_seconds, pair_seconds);
 }
 
 extern "C" int aar_xy_split_group_cols(int n, int precision) {

@@ -162,6 +162,8 @@ XSplit disp2_xsplit(const KSpace& ks);
 
 // zero n doubles (graph-capturable; no memset nodes)
 int aar_zero_fill(double* p, long long n, hipStream_t st);
+// n values of a fixed pattern in [-1/2, 1/2) (autotuner data)
+template <typename T> int aar_noise_fill(T* p, long long n, hipStream_t st);
 int aar_copy16(const void* src, void* dst, long long bytes, hipStream_t st);
 // read and write back nf fields with the x (axis 0) or y (axis 1) pass's tile addressing, no transform
 template <typename T> int aar_xy_pattern(cx_t<T>* f, int nf, const Geom& g, int axis, hipStream_t st);

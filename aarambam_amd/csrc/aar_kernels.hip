@@ -257,19 +257,29 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
     constexpr int E = C * L / NT;
     const int ntiles = nmain + N / C;
     const int t0 = wt * TPW;
+    // MO: the next tile's loads stay raw while in flight; its factors are applied when it is
+    // taken up (factors of the slots in flight would hold E more registers)
     C2 v[E], nv[E];
     Addr A = tile_addr(t0);
     if constexpr (MO) __syncthreads();  // the factor table
-    T fz = MO ? factor_z(t0) : (T)1;
-    split_load<T, L, C, NT, PL>(nv, [&](int pos, int) { return load(A, fz, pos); });
+    split_load<T, L, C, NT, PL>(nv, [&](int pos, int) { return fi[off(A, pos)]; });
     for (int i = 0; i < TPW && t0 + i < ntiles; ++i) {
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = nv[e];
       const Addr Ac = A;
       if (i + 1 < TPW && t0 + i + 1 < ntiles) {
         A = tile_addr(t0 + i + 1);
-        if constexpr (MO) fz = factor_z(t0 + i + 1);
-        split_load<T, L, C, NT, PL>(nv, [&](int pos, int) { return load(A, fz, pos); });
+        split_load<T, L, C, NT, PL>(nv, [&](int pos, int) { return fi[off(A, pos)]; });
+      }
+      if constexpr (MO) {
+        const T fz = factor_z(t0 + i);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          int pos, col;
+          split_slot<T, L, C, NT, PL>(e, pos, col);
+          const T sc = fz * fyt[pos];
+          v[e] = (code & kXsI) ? cmake<C2>(-sc * v[e].y, sc * v[e].x) : cmake<C2>(sc * v[e].x, sc * v[e].y);
+        }
       }
       split_transform<T, L, C, NT, DIR, PL>(tile_r, tw, v, [&](int pos, int, C2 x) { fo[off(Ac, pos)] = x; });
     }
@@ -1616,6 +1626,22 @@ int aar_zero_fill(double* p, long long n, hipStream_t st) {
   hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
   return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
 }
+
+// n values of a fixed hash pattern in [-1/2, 1/2): timing data for the autotuner (FFT passes
+// over non-zero data, as in a job)
+template <typename T> __global__ void k_noise(T* p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned h = (unsigned)(i * 2654435761ull) ^ (unsigned)(i >> 29);
+  p[i] = (T)((h >> 8) & 0xffffu) * (T)(1.0 / 65536) - (T)0.5;
+}
+
+template <typename T> int aar_noise_fill(T* p, long long n, hipStream_t st) {
+  hipLaunchKernelGGL((k_noise<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
+}
+template int aar_noise_fill<float>(float*, long long, hipStream_t);
+template int aar_noise_fill<double>(double*, long long, hipStream_t);
 
 __global__ void k_ranlxd_draws(uint32_t seed, int n, double* out) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;

@@ -442,9 +442,10 @@ def main():
     # x / y tile shapes and x / y pair column groups for this mesh, picked by the autotuner
     # before the timed region
     pow2 = N & (N - 1) == 0  # other mesh sizes run the mixed-radix passes, which have no tuned shapes
-    tsec, tvar, psec, pchosen, ssec = ([[0] * 4] * 2, [0, 0], [0] * 3, 0, [0] * 5)
+    tsec, tvar, psec, pchosen, ssec, gsec = ([[0] * 4] * 2, [0, 0], [0] * 3, 0, [0] * 5, [[0] * 4] * 3)
     if pow2:
-        tsec, psec, ssec = ctx.autotune_report(N, aar.AAR_FP32, nf=8)
+        tsec, gsec, ssec = ctx.autotune_report(N, aar.AAR_FP32, nf=8)
+        psec = [min([v for v in row if v > 0], default=-1.0) for row in gsec]
         tvar = [aar.lib().aar_xy_tile_variant(N, aar.AAR_FP32, a) for a in (0, 1)]
         pchosen = aar.lib().aar_xy_radix_plan(N, aar.AAR_FP32)
     if args.xy_group >= 0 and aar.lib().aar_set_xy_group_cols(N, aar.AAR_FP32, args.xy_group) != 0:
@@ -518,6 +519,9 @@ def main():
                                                   for i, ax in enumerate(("x", "y"))},
                        "variants": "0: 128-B tile columns, 1: 64-B, 2: 256-B, 3: 128-B, four tiles per workgroup with the next tile's loads in flight (same radix plan)",
                        "xy_pair_group_cols": xy_group,
+                       "us_per_xy_pair_8_fields_by_group": {
+                           "groups": "whole passes, all columns, halves, quarters",
+                           "per_plan": [[round(v * 1e6, 1) if v > 0 else None for v in row] for row in gsec]},
                        "xy_split_group_cols": {"chosen": xy_split_group,
                                                "us_per_call_9_outputs": [round(v * 1e6, 1) if v > 0 else None
                                                                          for v in ssec],

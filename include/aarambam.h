@@ -364,13 +364,14 @@ int aar_autotune(aar_ctx *ctx, int n, int precision, int nf, double *seconds, in
  * the tile shapes, plans change the rounding of the transforms (by a few units in the last place).
  * Counterpart of FFTW's planner choosing among algorithms (rfftw3d_mpi_create_plan, main.c:1566). */
 int aar_autotune_plans(aar_ctx *ctx, int n, int precision, int nf, double *plan_seconds, int *plan);
-/* aar_autotune reporting every timing of its one search (each optional): tile_seconds [2][4] as
- * aar_autotune's, plan_seconds [3] as aar_autotune_plans', split_seconds [5] mean seconds per
- * multi-output pair call over whole passes, all columns, halves, quarters, eighths (-1: not
- * timed).  The choices kept: aar_xy_tile_variant, aar_xy_radix_plan, aar_xy_group_cols,
- * aar_xy_split_group_cols. */
+/* aar_autotune reporting every timing of its one search (each optional, -1 where not timed):
+ * tile_seconds [2][4] as aar_autotune's; pair_seconds [3][4] mean seconds of one x + y pass over
+ * nf fields per radix plan and column group (whole passes, all columns, halves, quarters);
+ * split_seconds [5] mean seconds per multi-output pair call over whole passes, all columns,
+ * halves, quarters, eighths.  The choices kept: aar_xy_tile_variant, aar_xy_radix_plan,
+ * aar_xy_group_cols, aar_xy_split_group_cols. */
 int aar_autotune_report(aar_ctx *ctx, int n, int precision, int nf, double *tile_seconds,
-                        double *plan_seconds, double *split_seconds);
+                        double *pair_seconds, double *split_seconds);
 /* radix plan in use for (n, precision), or a negative status; the setter pins one (0 only for
  * meshes other than 512 and 1024) */
 int aar_xy_radix_plan(int n, int precision);

@@ -92,5 +92,5 @@ def test_split_autotune_reports_every_choice(ctx):
     the multi-output pairs and keeps one of them."""
     ts, ps, ss = ctx.autotune_report(256, aar.AAR_FP32, nf=2)
     assert np.all(ss > 0), ss
-    assert np.all(ts[:, 0] > 0) and ps[0] > 0
+    assert np.all(ts[:, 0] > 0) and np.all(ps[0] > 0)
     assert aar.lib().aar_xy_split_group_cols(256, aar.AAR_FP32) in (0, 128, 64, 32, 16)
