@@ -1792,7 +1792,8 @@ template <typename T> static int bench_pass_t(aar_ctx* ctx, int N, int kind, int
     hipFree(d);
     return AAR_ERR_ALLOC;
   }
-  AAR_HIP(hipMemsetAsync(d, 0, (size_t)nf * g.fs * sizeof(C2), ctx->stream));
+  // non-zero data: HBM moves zero-filled buffers measurably faster than job data
+  int rc = aar_noise_fill<T>((T*)d, (long long)nf * g.fs * 2, ctx->stream);
   const C2* tw = sizeof(T) == 4 ? (const C2*)ctx->tw32 : (const C2*)ctx->tw64;
   auto launch = [&]() -> int {
     switch (kind) {
@@ -1806,7 +1807,7 @@ template <typename T> static int bench_pass_t(aar_ctx* ctx, int N, int kind, int
         return aar_copy16(d, d2, (long long)nf * g.fs * (long long)sizeof(C2), ctx->stream);
     }
   };
-  int rc = launch();
+  if (!rc) rc = launch();
   hipEvent_t a, b;
   hipEventCreate(&a);
   hipEventCreate(&b);
