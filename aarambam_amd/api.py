@@ -63,9 +63,9 @@ EXPORTS = ["aar_status_string", "aar_read_params", "aar_read_params_res", "aar_l
            "aar_mode_basis_expansion", "aar_job_reseed", "aar_ctx_cached_plans",
            "aar_job_run_noalloc", "aar_autotune_plans", "aar_xy_radix_plan", "aar_set_xy_radix_plan",
            "aar_sym_eig_extremes", "aar_ctx_set_jit", "aar_job_jit", "aar_autotune_report",
-           "aar_xy_split_group_cols", "aar_set_xy_split_group_cols"]
+           "aar_xy_split_group_cols", "aar_set_xy_split_group_cols", "aar_ctx_set_modes_x"]
 KINDS = ["gauss", "xy_pass", "z_pass", "modes_spec", "products", "combine", "lpt_spec", "cic", "accumulate",
-         "xy_split"]
+         "xy_split", "modes_xy"]
 
 
 class AarError(RuntimeError):
@@ -139,6 +139,7 @@ def lib():
             "aar_job_set_profiling": (C.c_int, [vp, C.c_int]),
             "aar_job_set_graph": (C.c_int, [vp, C.c_int]),
             "aar_ctx_set_jit": (C.c_int, [vp, C.c_int]),
+            "aar_ctx_set_modes_x": (C.c_int, [vp, C.c_int]),
             "aar_job_jit": (C.c_int, [vp]),
             "aar_job_kernel_stats": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), dp, dp]),
             "aar_server_create": (C.c_int, [vp, C.c_longlong, C.c_int, C.POINTER(vp)]),
@@ -262,6 +263,12 @@ class Context:
         """Batched jobs prepared from now on compile their fused z pass for their own product
         schedule (on) or run the library's kernel (off); results are identical."""
         _check(lib().aar_ctx_set_jit(self.h, int(on)), "aar_ctx_set_jit")
+
+    def set_modes_x(self, mode=1):
+        """Batched jobs run from now on filter their mode fields in the loads of their x pass when
+        the filters are short (1, the default), always (2) or never (0, the mode fields written
+        in k-space first); results are identical."""
+        _check(lib().aar_ctx_set_modes_x(self.h, int(mode)), "aar_ctx_set_modes_x")
 
     def close(self):
         if self.h:

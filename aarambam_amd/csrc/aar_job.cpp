@@ -76,6 +76,9 @@ struct aar_ctx {
   // batched jobs run the NG sum's fused z pass compiled for their own product schedule
   // (aar_jit.cpp) unless turned off (aar_ctx_set_jit)
   bool jit = true;
+  // batched jobs filter the mode fields in the x pass's loads (Pipeline::modes_xy): 0 never,
+  // 1 when the filters' Legendre recurrences are short (aar_ctx_set_modes_x), 2 always
+  int modes_x = 1;
   float2* tw32 = nullptr;
   double2* tw64 = nullptr;
   // cosmology scalars by the parameters they depend on: the Romberg sigma8
@@ -160,6 +163,10 @@ struct aar_job {
   bool stage_used[2] = {false, false};
   int stage_next = 0, stage_pending = -1;
   bool fused_lpt = false;
+  // the mode fields' filters need few Legendre steps per value (modes_xy pays off, at 1024^3 L+E;
+  // at 512^3 with 16 modes, 102 steps per value over the fields, the separate filter pass is
+  // faster since it advances one recurrence through all fields)
+  bool short_filters = false;
   // the NG sum's fused z pass compiled for this job's schedule (jit_status: the status of a
   // failed compile, when the library's k_z_ng runs instead)
   JitZng jit;
@@ -360,13 +367,22 @@ template <typename T> struct Exec {
     const double n3 = (double)((unsigned)g.N * (unsigned)g.N * (unsigned)g.N);
     if (j->sched == AAR_SCHED_BATCHED) {
       if (j->M > 0) {
-        if ((rc = timed(j, st, AAR_K_MODES, (1 + j->M) * S,
-                        [&] { return P::modes_spec(F(j, j->s_phig), reg, j->M, j->modes, g, j->ks, st); })))
+        const bool fused = j->M <= 16 && g.N >= 64 && g.pow2;
+        const bool modes_x = fused && (j->ctx->modes_x == 2 || (j->ctx->modes_x == 1 && j->short_filters));
+        if (modes_x) {
+          // the mode fields filtered from phi_G in the x pass's loads, then their y passes
+          if ((rc = timed(j, st, AAR_K_MODES_XY, (1 + 3.0 * j->M) * S,
+                          [&] { return P::modes_xy(F(j, j->s_phig), reg, j->M, j->modes, g, j->ks, tw(j), st, xyg); },
+                          aar_xy_pair_dispatches(1, g.H, xyg))))
+            return rc;
+        } else if ((rc = timed(j, st, AAR_K_MODES, (1 + j->M) * S,
+                               [&] { return P::modes_spec(F(j, j->s_phig), reg, j->M, j->modes, g, j->ks, st); }))) {
           return rc;
-        if (j->M <= 16 && g.N >= 64 && g.pow2) {
+        }
+        if (fused) {
           // x, y inverse passes of the M mode fields, then the fused z pass
           // (C2R, products, R2C), then y, x forward passes of the K products
-          if ((rc = xy(reg, j->M, +1))) return rc;
+          if (!modes_x && (rc = xy(reg, j->M, +1))) return rc;
           if ((rc = timed(j, st, AAR_K_PRODUCTS, (j->M + j->K) * S, [&] {
                  return j->jit.fn ? jit_zng_launch(j->jit, reg, j->M, j->K, g, tw(j), st)
                                   : P::z_ng(reg, j->M, j->K, j->fgroups, j->frecs, g, tw(j), st);
@@ -1246,6 +1262,11 @@ int aar_job_prepare_ws(aar_ctx* ctx, const aar_params* p, const aar_tables* t, i
   aar_seedtable(N, p->Seed, seeds.data());
   if ((rc = job_upload(j, &j->seeds, seeds))) return fail(rc);
   j->fused_lpt = schedule == AAR_SCHED_BATCHED && N >= 64 && is_pow2(N);
+  {
+    int steps = 0;  // recurrence steps of P_{n-1} per value over the mode fields (n >= 4)
+    for (int n : j->plan->modes_h) steps += n >= 4 ? n - 2 : 0;
+    j->short_filters = steps <= 2 * (int)j->plan->modes_h.size();
+  }
   // the fused z pass of the NG sum (Exec::run's condition), compiled for this schedule; not in
   // the allocation-free path (loading a module allocates)
   if (ctx->jit && !plan_in_ws && schedule == AAR_SCHED_BATCHED && j->M >= 1 && j->M <= 16 && N >= 64 &&
@@ -1448,6 +1469,12 @@ extern "C" int aar_job_set_profiling(aar_job* j, int on) {
 extern "C" int aar_ctx_set_jit(aar_ctx* ctx, int on) {
   if (!ctx) return AAR_ERR_BAD_ARG;
   ctx->jit = on != 0;
+  return AAR_OK;
+}
+
+extern "C" int aar_ctx_set_modes_x(aar_ctx* ctx, int mode) {
+  if (!ctx || mode < 0 || mode > 2) return AAR_ERR_BAD_ARG;
+  ctx->modes_x = mode;
   return AAR_OK;
 }
 

@@ -87,6 +87,14 @@ struct XOut {
   const double* kv = nullptr;
   const double* icic = nullptr;
 };
+// The x pass of the mode fields made from phi_G (modes_x): output m (slot m) is the inverse x
+// transform of q_{modes[m]}(k) phi_G(k) (k_modes_spec's filter, cut and origin zero).
+struct XModes {
+  int M = 0;
+  const void* phig = nullptr;
+  const int* modes = nullptr;  // device, M mode numbers
+  KSpace ks{};
+};
 enum : unsigned char { kXsKy = 1, kXsKy2 = 2, kXsKz = 4, kXsKz2 = 8, kXsCic = 16, kXsI = 32, kXsNeg = 64 };
 
 template <typename T> struct Pipeline {
@@ -94,9 +102,11 @@ template <typename T> struct Pipeline {
   static int c2r(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st);
   static int r2c(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st);
   // variant < 0: the tile shape from the variant table (aar_set_xy_variant)
-  // xs: the y pass (axis 1, dir > 0) of the xs->n outputs of a split (nf = 1)
+  // xs: the y pass (axis 1, dir > 0) of the xs->n outputs of a split (nf = 1); xm: the x pass
+  // (axis 0, dir > 0) of the xm->M mode fields of phi_G into f (nf = 1)
   static int xy_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int axis, int dir,
-                     hipStream_t st, int variant = -1, XYRange r = XYRange(), const XOut* xs = nullptr);
+                     hipStream_t st, int variant = -1, XYRange r = XYRange(), const XOut* xs = nullptr,
+                     const XModes* xm = nullptr);
   // x then y pass (dir > 0) or y then x (dir < 0).  group_cols > 0: field by field, in groups of
   // group_cols kz columns (the last group may be narrower), so the first pass's output of a group
   // can still be in the last-level cache when the second pass reads it; 0: each pass over all nf
@@ -108,6 +118,11 @@ template <typename T> struct Pipeline {
   // its outputs, group by group).  Powers of two only.
   static int xy_split(cx_t<T>* f, const XSplit& xs, const Geom& g, const cx_t<T>* tw, hipStream_t st,
                       int group_cols);
+  // the M mode fields of phi_G into real space along x and y: the x pass filters each tile of
+  // phi_G by q_n on its loads (the mode fields are never stored in k-space), then the y passes;
+  // group_cols as in xy_pair.  Powers of two only.
+  static int modes_xy(const cx_t<T>* phig, cx_t<T>* out, int M, const int* modes, const Geom& g,
+                      const KSpace& ks, const cx_t<T>* tw, hipStream_t st, int group_cols);
   static int z_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int dir, hipStream_t st);
 
   // fused z pass of the NG sum: M mode fields (x, y already inverse transformed)

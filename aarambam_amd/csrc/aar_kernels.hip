@@ -157,24 +157,31 @@ template <typename T, int H> struct ZCfg {
 // pair w % xo.n of tiles w / xo.n, read from its input's slot times its factor
 // of (y, kz): the pairs of one tile run side by side on one XCD, so an input
 // read by several outputs comes from its L2 after the first.
-template <typename T, int L, int AXIS, int DIR, int V, int PL, bool MO = false>
+template <typename T, int L, int AXIS, int DIR, int V, int PL, int MO = 0>
 __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_waves_per_eu(XYCfg<T, L, V>::WPE)))
     k_xy_pass(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg, int nmain, int gper, int k0,
-              XOut xo) {
+              XOut xo, XModes xm) {
   using C2 = cx_t<T>;
   constexpr int C = XYCfg<T, L, V>::C, NT = XYCfg<T, L, V>::NT;
-  static_assert(!MO || (AXIS == 1 && DIR > 0), "multi-output: inverse y passes");
+  static_assert(MO != 1 || (AXIS == 1 && DIR > 0), "multi-output: inverse y passes");
+  static_assert(MO != 2 || (AXIS == 0 && DIR > 0), "mode filters: inverse x passes");
   __shared__ T tile_r[L * (C + 1)];  // real-valued exchange tile (one component at a time)
   __shared__ C2 tw[L];
-  __shared__ T fyt[MO ? L : 1];  // MO: the output's factor of ky, per y
+  __shared__ T fyt[MO == 1 ? L : 1];  // MO 1: the output's factor of ky, per y
   load_twiddles(tw, twg + L, L, NT);
-  // MO: this block's output, its input and its code, picked with constant indices (the index is
-  // uniform; a runtime index into the argument struct would go through scratch)
+  // MO 1: this block's output, its input and its code, picked with constant indices (the index
+  // is uniform; a runtime index into the argument struct would go through scratch)
   int src = blockIdx.y, dst = blockIdx.y;
   unsigned code = 0;
-  const int w = xcd_tile(blockIdx.x, gridDim.x);  // this workgroup's (tile block, pair)
-  const int wt = MO ? w / xo.n : w;
-  if constexpr (MO) {
+  const int w = xcd_tile(blockIdx.x, gridDim.x);  // this workgroup's (tile block, output)
+  const int nouts = MO == 1 ? xo.n : (MO == 2 ? xm.M : 1);
+  const int wt = MO ? w / nouts : w;
+  int nmode = 0;  // MO 2: the mode number of this block's output
+  if constexpr (MO == 2) {
+    dst = w % nouts;
+    nmode = xm.modes[dst];
+  }
+  if constexpr (MO == 1) {
     const int q = w % xo.n;
 #pragma unroll
     for (int i = 0; i < 9; ++i)
@@ -191,7 +198,7 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
       fyt[y] = (T)fy;
     }
   }
-  const C2* fi = base + (long long)src * g.fs;
+  const C2* fi = MO == 2 ? (const C2*)xm.phig : base + (long long)src * g.fs;
   C2* fo = base + (long long)dst * g.fs;
   constexpr int TPW = XYCfg<T, L, V>::TPW;
   const int N = g.N, H = g.H;
@@ -234,10 +241,60 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
   auto off = [](const Addr& A, int pos) -> unsigned {
     return (unsigned)(pos >> kXBLog) * A.s1 + (unsigned)(pos & (kXB - 1)) * A.s2 + A.ct;
   };
-  // the tile's input element at pos (MO: times fz fy(pos), and i where the code says so)
+  // MO 2: the (y, kz) grid indices of this thread's column of tile t and their parts of |k|^2 and
+  // of the cut (x pass: the tile's y, the column's kz, or kz = N/2 on the Nyquist plane)
+  struct ColK {
+    int yz;          // y + kz
+    double kyy, kzz;  // kv[y]^2, kv[kz]^2
+    int oyz;         // cube cut of y or kz
+  };
+  auto col_k = [&](int t) {
+    ColK c;
+    const int col = threadIdx.x % C;
+    int yi, zi;
+    if (t < nmain) {
+      yi = t / gper;
+      zi = k0 + (t % gper) * C + col;
+    } else {
+      yi = (t - nmain) * C + col;
+      zi = H;
+    }
+    c.yz = yi + zi;
+    const double b = xm.ks.kv[yi], cz = xm.ks.kv[zi];
+    c.kyy = b * b;
+    c.kzz = cz * cz;
+    c.oyz = xm.ks.outc[yi] | xm.ks.outc[zi];
+    return c;
+  };
+  // MO 2: phi_G at (x = pos, y, kz) times q_n (k_modes_spec's arithmetic: the same operations in
+  // the same order, so the mode fields are those of k_modes_spec followed by an x pass)
+  auto mode_filter = [&](const ColK& c, int pos, C2 v) -> C2 {
+    using RT = T;
+    const KSpace& ks = xm.ks;
+    const double a = ks.kv[pos];
+    const double kmag2 = a * a + c.kyy + c.kzz;
+    const double kmag1 = (sizeof(T) == 8 || ks.sphere == 1) ? sqrt(kmag2) : 0.0;
+    const bool cut = ks.sphere == 1 ? kmag1 * ks.box / (2 * AAR_PI) > ks.nsh : (ks.outc[pos] | c.oyz) != 0;
+    if (pos + c.yz == 0 || cut) return cmake<C2>(0, 0);
+    const RT k2 = (RT)kmag2;
+    const RT kss = kpow_s<RT>(kmag2, kmag1, ks.s);
+    RT q;
+    if (nmode <= 3) {
+      q = 1;
+      for (int u = 0; u < nmode; ++u) q *= kss;
+    } else {
+      const RT xb = -1 + 2 * (kss - (RT)ks.kmin_s) / (RT)(ks.kmax_s - ks.kmin_s);
+      const RT k3s = kss * kss * kss;  // k^{4-ns}
+      LegRecT<RT> leg;
+      leg.init(xb);
+      q = (leg.at(nmode - 1) - (RT)ks.x0[nmode - 1]) * (k3s / k2);
+    }
+    return cmake<C2>(q * v.x, q * v.y);
+  };
+  // the tile's input element at pos (MO 1: times fz fy(pos), and i where the code says so)
   auto load = [&](const Addr& A, T fz, int pos) -> C2 {
     const C2 v = fi[off(A, pos)];
-    if constexpr (MO) {
+    if constexpr (MO == 1) {
       const T s = fz * fyt[pos];
       return (code & kXsI) ? cmake<C2>(-s * v.y, s * v.x) : cmake<C2>(s * v.x, s * v.y);
     } else {
@@ -247,10 +304,17 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
   if constexpr (TPW == 1) {
     const int t = wt;
     const Addr A = tile_addr(t);
-    const T fz = MO ? factor_z(t) : (T)1;
+    const T fz = MO == 1 ? factor_z(t) : (T)1;
     __syncthreads();  // twiddles (and the factor table)
-    fft_tile_split<T, L, C, NT, DIR, PL>(
-        tile_r, tw, [&](int pos, int) { return load(A, fz, pos); }, [&](int pos, int, C2 v) { fo[off(A, pos)] = v; });
+    if constexpr (MO == 2) {
+      const ColK ck = col_k(t);
+      fft_tile_split<T, L, C, NT, DIR, PL>(
+          tile_r, tw, [&](int pos, int) { return mode_filter(ck, pos, fi[off(A, pos)]); },
+          [&](int pos, int, C2 v) { fo[off(A, pos)] = v; });
+    } else {
+      fft_tile_split<T, L, C, NT, DIR, PL>(
+          tile_r, tw, [&](int pos, int) { return load(A, fz, pos); }, [&](int pos, int, C2 v) { fo[off(A, pos)] = v; });
+    }
   } else {
     // tiles t0 .. t0 + TPW - 1 (the last workgroup may hold fewer); the condition of every
     // branch and loop below is uniform over the workgroup, so every barrier is reached
@@ -261,7 +325,7 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
     // taken up (factors of the slots in flight would hold E more registers)
     C2 v[E], nv[E];
     Addr A = tile_addr(t0);
-    if constexpr (MO) __syncthreads();  // the factor table
+    if constexpr (MO == 1) __syncthreads();  // the factor table
     split_load<T, L, C, NT, PL>(nv, [&](int pos, int) { return fi[off(A, pos)]; });
     for (int i = 0; i < TPW && t0 + i < ntiles; ++i) {
 #pragma unroll
@@ -271,7 +335,16 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
         A = tile_addr(t0 + i + 1);
         split_load<T, L, C, NT, PL>(nv, [&](int pos, int) { return fi[off(A, pos)]; });
       }
-      if constexpr (MO) {
+      if constexpr (MO == 2) {
+        const ColK ck = col_k(t0 + i);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          int pos, col;
+          split_slot<T, L, C, NT, PL>(e, pos, col);
+          v[e] = mode_filter(ck, pos, v[e]);
+        }
+      }
+      if constexpr (MO == 1) {
         const T fz = factor_z(t0 + i);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -289,7 +362,7 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
 // xo: the y pass of the outputs of a multi-output split (inverse y passes only, nf = 1)
 template <typename T, int L, int AXIS, int DIR, int V, int PL>
 static int launch_xy_v(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, const XYRange& r,
-                       const XOut* xo) {
+                       const XOut* xo, const XModes* xm) {
   using Cfg = XYCfg<T, L, V>;
   if constexpr (!Cfg::VALID) {
     return AAR_ERR_UNSUPPORTED;
@@ -308,14 +381,22 @@ static int launch_xy_v(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hip
           (long)L * (C + 1) * sizeof(T) + (long)L * sizeof(cx_t<T>) + (long)L * sizeof(T) <= 160 * 1024;
       if (nf != 1 || xo->n < 1 || xo->n > 9) return AAR_ERR_UNSUPPORTED;
       if constexpr (AXIS == 1 && DIR > 0 && MO_FITS) {
-        hipLaunchKernelGGL((k_xy_pass<T, L, AXIS, DIR, V, PL, true>), dim3(nwg * xo->n, 1), dim3(NT), 0, st, f, g,
-                           tw, nmain, gper, r.k0, *xo);
+        hipLaunchKernelGGL((k_xy_pass<T, L, AXIS, DIR, V, PL, 1>), dim3(nwg * xo->n, 1), dim3(NT), 0, st, f, g,
+                           tw, nmain, gper, r.k0, *xo, XModes());
+      } else {
+        return AAR_ERR_UNSUPPORTED;
+      }
+    } else if (xm) {
+      if (nf != 1 || xm->M < 1 || (long long)nwg * xm->M > 0x7fffffffLL) return AAR_ERR_UNSUPPORTED;
+      if constexpr (AXIS == 0 && DIR > 0) {
+        hipLaunchKernelGGL((k_xy_pass<T, L, AXIS, DIR, V, PL, 2>), dim3(nwg * xm->M, 1), dim3(NT), 0, st, f, g,
+                           tw, nmain, gper, r.k0, XOut(), *xm);
       } else {
         return AAR_ERR_UNSUPPORTED;
       }
     } else {
       hipLaunchKernelGGL((k_xy_pass<T, L, AXIS, DIR, V, PL>), dim3(nwg, nf), dim3(NT), 0, st, f, g, tw, nmain, gper,
-                         r.k0, XOut());
+                         r.k0, XOut(), XModes());
     }
     return hipGetLastError() == hipSuccess ? AAR_OK : AAR_ERR_HIP;
   }
@@ -325,29 +406,29 @@ static int launch_xy_v(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hip
 // than 0 for L = 512 and 1024 (three stages of radix 16, 8, 4 or 2 in different orders).
 template <typename T, int L, int AXIS, int DIR, int PL>
 static int launch_xy_p(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, int v,
-                       const XYRange& r, const XOut* xs) {
+                       const XYRange& r, const XOut* xs, const XModes* xm) {
   if constexpr (L >= 256) {
     switch (v) {
-      case 1: return launch_xy_v<T, L, AXIS, DIR, 1, PL>(f, nf, g, tw, st, r, xs);
-      case 2: return launch_xy_v<T, L, AXIS, DIR, 2, PL>(f, nf, g, tw, st, r, xs);
-      case 3: return launch_xy_v<T, L, AXIS, DIR, 3, PL>(f, nf, g, tw, st, r, xs);
+      case 1: return launch_xy_v<T, L, AXIS, DIR, 1, PL>(f, nf, g, tw, st, r, xs, xm);
+      case 2: return launch_xy_v<T, L, AXIS, DIR, 2, PL>(f, nf, g, tw, st, r, xs, xm);
+      case 3: return launch_xy_v<T, L, AXIS, DIR, 3, PL>(f, nf, g, tw, st, r, xs, xm);
       default: break;
     }
   }
-  return launch_xy_v<T, L, AXIS, DIR, 0, PL>(f, nf, g, tw, st, r, xs);
+  return launch_xy_v<T, L, AXIS, DIR, 0, PL>(f, nf, g, tw, st, r, xs, xm);
 }
 
 template <typename T, int L, int AXIS, int DIR>
 static int launch_xy(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, hipStream_t st, int v, const XYRange& r,
-                     const XOut* xs) {
+                     const XOut* xs, const XModes* xm) {
   if constexpr (L >= 512) {
     switch (aar_xy_plan(sizeof(T) == 8, L)) {
-      case 1: return launch_xy_p<T, L, AXIS, DIR, 1>(f, nf, g, tw, st, v, r, xs);
-      case 2: return launch_xy_p<T, L, AXIS, DIR, 2>(f, nf, g, tw, st, v, r, xs);
+      case 1: return launch_xy_p<T, L, AXIS, DIR, 1>(f, nf, g, tw, st, v, r, xs, xm);
+      case 2: return launch_xy_p<T, L, AXIS, DIR, 2>(f, nf, g, tw, st, v, r, xs, xm);
       default: break;
     }
   }
-  return launch_xy_p<T, L, AXIS, DIR, 0>(f, nf, g, tw, st, v, r, xs);
+  return launch_xy_p<T, L, AXIS, DIR, 0>(f, nf, g, tw, st, v, r, xs, xm);
 }
 
 // ------------------------------------------------------------------ z passes
@@ -711,24 +792,24 @@ int Pipeline<T>::z_ng(cx_t<T>* f, int M, int K, const int* gs, const NgRec4* rec
 
 template <typename T>
 int Pipeline<T>::xy_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, int axis, int dir,
-                         hipStream_t st, int variant, XYRange r, const XOut* xs) {
+                         hipStream_t st, int variant, XYRange r, const XOut* xs, const XModes* xm) {
   if (!g.pow2) {  // whole passes only: column groups belong to the power-of-two tiles
-    if (r.k0 != 0 || (r.cols != 0 && r.cols != g.H) || !r.nyq || xs) return AAR_ERR_UNSUPPORTED;
+    if (r.k0 != 0 || (r.cols != 0 && r.cols != g.H) || !r.nyq || xs || xm) return AAR_ERR_UNSUPPORTED;
     if (axis == 0) return dir > 0 ? launch_xy_gen<T, 0, 1>(f, nf, g, tw, st) : launch_xy_gen<T, 0, -1>(f, nf, g, tw, st);
     return dir > 0 ? launch_xy_gen<T, 1, 1>(f, nf, g, tw, st) : launch_xy_gen<T, 1, -1>(f, nf, g, tw, st);
   }
   const int v = variant >= 0 ? variant : aar_xy_variant(sizeof(T) == 8, g.N, axis);
   if (variant < 0 && v != 0) {  // a pinned shape that is not built for this size runs as shape 0
-    const int rc = xy_pass(f, nf, g, tw, axis, dir, st, v, r, xs);
-    return rc == AAR_ERR_UNSUPPORTED ? xy_pass(f, nf, g, tw, axis, dir, st, 0, r, xs) : rc;
+    const int rc = xy_pass(f, nf, g, tw, axis, dir, st, v, r, xs, xm);
+    return rc == AAR_ERR_UNSUPPORTED ? xy_pass(f, nf, g, tw, axis, dir, st, 0, r, xs, xm) : rc;
   }
 #define XY_CASE(LL)                                                                                 \
   case LL:                                                                                          \
     if (axis == 0)                                                                                  \
-      return dir > 0 ? launch_xy<T, LL, 0, 1>(f, nf, g, tw, st, v, r, xs)                       \
-                     : launch_xy<T, LL, 0, -1>(f, nf, g, tw, st, v, r, xs);                     \
-    return dir > 0 ? launch_xy<T, LL, 1, 1>(f, nf, g, tw, st, v, r, xs)                         \
-                   : launch_xy<T, LL, 1, -1>(f, nf, g, tw, st, v, r, xs);
+      return dir > 0 ? launch_xy<T, LL, 0, 1>(f, nf, g, tw, st, v, r, xs, xm)                       \
+                     : launch_xy<T, LL, 0, -1>(f, nf, g, tw, st, v, r, xs, xm);                     \
+    return dir > 0 ? launch_xy<T, LL, 1, 1>(f, nf, g, tw, st, v, r, xs, xm)                         \
+                   : launch_xy<T, LL, 1, -1>(f, nf, g, tw, st, v, r, xs, xm);
   switch (g.N) {
     XY_CASE(16)
     XY_CASE(32)
@@ -801,6 +882,31 @@ int Pipeline<T>::xy_split(cx_t<T>* f, const XSplit& xs, const Geom& g, const cx_
           (o.n && (rc = xy_pass(f, 1, g, tw, 1, +1, st, -1, r, &o))))
         return rc;
     }
+  }
+  return AAR_OK;
+}
+
+template <typename T>
+int Pipeline<T>::modes_xy(const cx_t<T>* phig, cx_t<T>* out, int M, const int* modes, const Geom& g,
+                          const KSpace& ks, const cx_t<T>* tw, hipStream_t st, int group_cols) {
+  if (!g.pow2 || M < 1) return AAR_ERR_UNSUPPORTED;
+  XModes xm;
+  xm.M = M;
+  xm.phig = phig;
+  xm.modes = modes;
+  xm.ks = ks;
+  int rc;
+  if (group_cols < 0) group_cols = aar_xy_group(sizeof(T) == 8, g.N);
+  // tile shape 0: the filter's per-element recurrence in the loads of the pipelined shape 3
+  // would hold all of a tile's in flight at once
+  if (group_cols <= 0) {
+    if ((rc = xy_pass(out, 1, g, tw, 0, +1, st, 0, XYRange(), nullptr, &xm))) return rc;
+    return xy_pass(out, M, g, tw, 1, +1, st);
+  }
+  for (int k0 = 0; k0 < g.H; k0 += group_cols) {
+    const XYRange r{k0, k0 + group_cols < g.H ? group_cols : g.H - k0, k0 == 0};
+    if ((rc = xy_pass(out, 1, g, tw, 0, +1, st, 0, r, nullptr, &xm)) || (rc = xy_pass(out, M, g, tw, 1, +1, st, -1, r)))
+      return rc;
   }
   return AAR_OK;
 }

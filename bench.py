@@ -426,6 +426,8 @@ def main():
                          "counter runs serialize every launch, which skews the autotuner, so they pin it")
     ap.add_argument("--xy-plan", type=int, default=-1,
                     help="pin the x/y radix plan (default: the autotuner's choice); counter runs pin it too")
+    ap.add_argument("--no-modes-x", action="store_true",
+                    help="write the mode fields in k-space before their x pass (default: filter them in its loads)")
     ap.add_argument("--xy-split-group", type=int, default=-2,
                     help="pin the kz column groups of the multi-output x/y pass pairs (-1: the pairs' choice; "
                          "default: the autotuner's choice)")
@@ -437,6 +439,8 @@ def main():
     import aarambam_amd as aar
     N = args.nmesh
     ctx = aar.Context(0)
+    if args.no_modes_x:
+        ctx.set_modes_x(0)
     cfg, alpha, x0, glass, gbox = workload(N, args.box, args.template, ctx=ctx)
     tabs = aar.tables(alpha, x0, glass, gbox)
     # x / y tile shapes and x / y pair column groups for this mesh, picked by the autotuner

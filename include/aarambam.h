@@ -227,10 +227,13 @@ double aar_job_last_seconds(aar_job *job);
  * by HIP events on the job's stream; stats give launches, summed seconds and
  * summed algorithmic bytes (reads + writes of the data each launch must move). */
 /* AAR_K_XY_SPLIT: the batched schedule's multi-output x / y pass pairs of the 2LPT spectra
- * (aar_xy_split_group_cols), whose algorithmic bytes count each input read once. */
+ * (aar_xy_split_group_cols), whose algorithmic bytes count each input read once; AAR_K_MODES_XY:
+ * the mode fields filtered in their x pass's loads (aar_ctx_set_modes_x) and their y passes,
+ * phi_G read once. */
 enum {
   AAR_K_GAUSS = 0, AAR_K_XY = 1, AAR_K_Z = 2, AAR_K_MODES = 3, AAR_K_PRODUCTS = 4,
-  AAR_K_COMBINE = 5, AAR_K_LPT = 6, AAR_K_CIC = 7, AAR_K_ACCUM = 8, AAR_K_XY_SPLIT = 9, AAR_NKINDS = 10
+  AAR_K_COMBINE = 5, AAR_K_LPT = 6, AAR_K_CIC = 7, AAR_K_ACCUM = 8, AAR_K_XY_SPLIT = 9, AAR_K_MODES_XY = 10,
+  AAR_NKINDS = 11
 };
 int aar_job_set_profiling(aar_job *job, int on);
 /* Graph mode: the first run captures every launch of the job into one
@@ -248,6 +251,14 @@ int aar_job_destroy(aar_job *job);
  * else the status of a compile that failed (the library's kernel then runs). */
 int aar_ctx_set_jit(aar_ctx *ctx, int on);
 int aar_job_jit(const aar_job *job);
+/* Batched jobs at powers of two N >= 64 can filter their mode fields (q_n(k) phi_G(k),
+ * main.c:571-639) in the loads of the x pass that takes them towards real space, so the mode
+ * fields are never stored in k-space; each value's filter is then evaluated on its own, where
+ * the separate filter pass advances one Legendre recurrence through all the fields.  mode 1 (the
+ * default): when the recurrences are short (at most two steps per field on average, as for the
+ * local and equilateral tables); 2: always; 0: never.  The same operations in the same order
+ * either way: results are bit-identical.  No counterpart in the reference. */
+int aar_ctx_set_modes_x(aar_ctx *ctx, int mode);
 
 /* ---- template check of a finished job (no counterpart in the reference) ----
  * Binned statistics of the potentials on the device.  Bins are |k| in

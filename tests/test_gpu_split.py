@@ -94,3 +94,24 @@ def test_split_autotune_reports_every_choice(ctx):
     assert np.all(ss > 0), ss
     assert np.all(ts[:, 0] > 0) and np.all(ps[0] > 0)
     assert aar.lib().aar_xy_split_group_cols(256, aar.AAR_FP32) in (0, 128, 64, 32, 16)
+
+
+@pytest.mark.parametrize("prec,N", [(aar.AAR_FP64, 64), (aar.AAR_FP32, 128), (aar.AAR_FP32, 256)])
+def test_mode_fields_filtered_in_the_x_pass_match_the_k_space_filter(prec, N):
+    """The mode fields filtered in the loads of their x pass and written in k-space
+    first by k_modes_spec: bitwise equal jobs (the same operations in the same order), with the
+    shipped 15-mode table and the reference example's cube cut (Nsample = N/2)."""
+    cfg = cases.base_config(Nmesh=N, Nsample=N // 2, Nmax=N, N_modes=15, Seed=5)
+    alpha, x0, glass, gbox = cases.job_inputs(cfg, cases.alpha_rows_file())
+    tabs = aar.tables(alpha, x0, glass, gbox)
+    res = []
+    for mode in (2, 0):  # always, never
+        c = aar.Context(0)
+        c.set_modes_x(mode)
+        res.append(run_job(c, cfg, tabs, prec))
+        c.close()
+    for n in res[0]:
+        np.testing.assert_array_equal(res[0][n], res[1][n], err_msg=n)
+    if prec == aar.AAR_FP64 and N == 64:
+        ref = orc.run(cfg, alpha, x0, glass, gbox)
+        assert rel_l2(res[0]["phing_real"], ref["phing_real"]) < 1e-10
