@@ -426,8 +426,9 @@ def main():
                          "counter runs serialize every launch, which skews the autotuner, so they pin it")
     ap.add_argument("--xy-plan", type=int, default=-1,
                     help="pin the x/y radix plan (default: the autotuner's choice); counter runs pin it too")
-    ap.add_argument("--no-modes-x", action="store_true",
-                    help="write the mode fields in k-space before their x pass (default: filter them in its loads)")
+    ap.add_argument("--modes-x", type=int, default=1, choices=(0, 1, 2),
+                    help="filter the mode fields in the loads of their x pass: 1 when the filters are short "
+                         "(default), 2 always, 0 never (written in k-space first)")
     ap.add_argument("--xy-split-group", type=int, default=-2,
                     help="pin the kz column groups of the multi-output x/y pass pairs (-1: the pairs' choice; "
                          "default: the autotuner's choice)")
@@ -439,8 +440,7 @@ def main():
     import aarambam_amd as aar
     N = args.nmesh
     ctx = aar.Context(0)
-    if args.no_modes_x:
-        ctx.set_modes_x(0)
+    ctx.set_modes_x(args.modes_x)
     cfg, alpha, x0, glass, gbox = workload(N, args.box, args.template, ctx=ctx)
     tabs = aar.tables(alpha, x0, glass, gbox)
     # x / y tile shapes and x / y pair column groups for this mesh, picked by the autotuner
