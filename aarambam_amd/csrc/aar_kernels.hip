@@ -1072,7 +1072,7 @@ __global__ void __launch_bounds__(kGaussRows)
   ranlxd_seed(rs, seeds[pid]);
   bool redo = false;
   for (int kb = 0; kb < H; kb += 6) {
-    ranlxd_update(rs);
+    ranlxd_update_dev(rs);
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
       const int k = kb + q;
@@ -1093,7 +1093,7 @@ __global__ void __launch_bounds__(kGaussRows)
     int pos = 12;
     auto next = [&]() {
       if (pos == 12) {
-        ranlxd_update(rr);
+        ranlxd_update_dev(rr);
         pos = 0;
       }
       uint64_t v = rr.s[0];
@@ -1759,7 +1759,7 @@ __global__ void k_ranlxd_draws(uint32_t seed, int n, double* out) {
   Ranlxd r;
   ranlxd_seed(r, seed);
   for (int b = 0; b < n; b += 12) {
-    ranlxd_update(r);
+    ranlxd_update_dev(r);
 #pragma unroll
     for (int q = 0; q < 12; ++q)
       if (b + q < n) out[b + q] = (double)r.s[q] * kTwoM48;

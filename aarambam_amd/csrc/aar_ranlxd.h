@@ -72,6 +72,43 @@ AAR_HD void ranlxd_update(Ranlxd& r) {
   r.carry = c;
 }
 
+#if defined(__HIPCC__)
+// The same update on the device, the 48-bit words held as 32-bit halves: a step is two
+// subtractions with borrow (the borrow of the high half is the next step's c, since the words
+// are below 2^48) and a mask, where the 64-bit form takes five instructions.
+__device__ __forceinline__ void ranlxd_update_dev(Ranlxd& r) {
+  uint32_t lo[12], hi[12];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) {
+    lo[q] = (uint32_t)r.s[q];
+    hi[q] = (uint32_t)(r.s[q] >> 32);
+  }
+  uint32_t c = (uint32_t)r.carry;
+  auto step = [&](int q) {
+    const int l = (q + 7) % 12;
+    uint32_t b0, b1;
+    const uint32_t yl = __builtin_subc(lo[l], lo[q], c, &b0);
+    const uint32_t yh = __builtin_subc(hi[l], hi[q], b0, &b1);
+    c = b1;
+    lo[q] = yl;
+    hi[q] = yh & 0xffffu;
+  };
+#pragma unroll 1
+  for (int b = 0; b < 16; ++b) {
+#pragma unroll
+    for (int q = 0; q < 12; ++q) step(q);
+  }
+#pragma unroll
+  for (int q = 0; q < 10; ++q) step(q);
+#pragma unroll
+  for (int q = 0; q < 12; ++q) {
+    const int o = (q + 10) % 12;
+    r.s[q] = ((uint64_t)hi[o] << 32) | lo[o];
+  }
+  r.carry = c;
+}
+#endif
+
 // Sequential generator with the GSL calling semantics (host seed table and
 // the rare redraw path of the Gaussian kernel).
 struct RanlxdStream {
