@@ -152,11 +152,14 @@ template <typename T, int H> struct ZCfg {
 
 // ------------------------------------------------------------------ x / y passes
 // One tile = C adjacent columns of one field.  Tiles [0, nmain) cover the
-// kz < N/2 block, the rest the Nyquist plane.  MO: the y pass of the xo.n outputs
-// of a multi-output split (XSplit), workgroup w (in the XCD-aware order) doing
-// pair w % xo.n of tiles w / xo.n, read from its input's slot times its factor
-// of (y, kz): the pairs of one tile run side by side on one XCD, so an input
-// read by several outputs comes from its L2 after the first.
+// kz < N/2 block, the rest the Nyquist plane.
+// MO = 1: the y pass of the xo.n outputs of a multi-output split (XSplit),
+// workgroup w (in the XCD-aware order) doing pair w % xo.n of tiles w / xo.n,
+// read from its input's slot times its factor of (y, kz): the pairs of one tile
+// run side by side on one XCD, so an input read by several outputs comes from
+// its L2 after the first.
+// MO = 2: the x pass of the xm.M mode fields of phi_G (XModes), workgroup w doing
+// mode w % M of tiles w / M, phi_G read times the mode's filter q_n(|k|).
 template <typename T, int L, int AXIS, int DIR, int V, int PL, int MO = 0>
 __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_waves_per_eu(XYCfg<T, L, V>::WPE)))
     k_xy_pass(cx_t<T>* __restrict__ base, Geom g, const cx_t<T>* __restrict__ twg, int nmain, int gper, int k0,
@@ -202,8 +205,8 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
   C2* fo = base + (long long)dst * g.fs;
   constexpr int TPW = XYCfg<T, L, V>::TPW;
   const int N = g.N, H = g.H;
-  // MO: the factor of kz (with the sign) of this thread's column of tile t (y pass: the column's
-  // kz, or kz = N/2 on the Nyquist plane)
+  // MO 1: the factor of kz (with the sign) of this thread's column of tile t (y pass: the
+  // column's kz, or kz = N/2 on the Nyquist plane)
   auto factor_z = [&](int t) -> T {
     const int zi = t < nmain ? k0 + (t % gper) * C + (int)(threadIdx.x % C) : H;
     double fz = (code & kXsNeg) ? -1.0 : 1.0;
@@ -321,8 +324,8 @@ __global__ void __launch_bounds__((XYCfg<T, L, V>::NT)) __attribute__((amdgpu_wa
     constexpr int E = C * L / NT;
     const int ntiles = nmain + N / C;
     const int t0 = wt * TPW;
-    // MO: the next tile's loads stay raw while in flight; its factors are applied when it is
-    // taken up (factors of the slots in flight would hold E more registers)
+    // MO: the next tile's loads stay raw while in flight; its factors (or filters) are applied
+    // when it is taken up (factors of the slots in flight would hold E more registers)
     C2 v[E], nv[E];
     Addr A = tile_addr(t0);
     if constexpr (MO == 1) __syncthreads();  // the factor table
