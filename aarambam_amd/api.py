@@ -317,8 +317,9 @@ class Context:
     def autotune_report(self, n, precision=AAR_FP32, nf=8):
         """One autotuner search with every timing (seconds, -1 where not timed): tile shapes [2][4],
         x + y pass pairs per radix plan and column group [3][4] (whole passes, all columns, halves,
-        quarters), multi-output pairs per column group [5] (the same and eighths)."""
-        ts, ps, ss = np.full(8, -1.0), np.full(12, -1.0), np.full(5, -1.0)
+        quarters), multi-output pairs [9]: per column group (the same and eighths), then per tile
+        shape of their y pass at the group kept."""
+        ts, ps, ss = np.full(8, -1.0), np.full(12, -1.0), np.full(9, -1.0)
         _check(lib().aar_autotune_report(self.h, n, precision, nf, _dptr(ts), _dptr(ps), _dptr(ss)),
                "aar_autotune_report")
         return ts.reshape(2, 4), ps.reshape(3, 4), ss

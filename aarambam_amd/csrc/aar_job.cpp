@@ -762,6 +762,7 @@ static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, int nslots, do
     double bests = 1e30;
     int bs = bg;
     if (hipMemcpy(ones, one.data(), sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess) rc = AAR_ERR_HIP;
+    aar_set_xy_variant(sizeof(T) == 8, N, 2, -1);  // the groups timed with the y pass's tile shape
     for (int gc : {0, g.H, g.H / 2, g.H / 4, g.H / 8}) {
       if (rc || (gc > 0 && gc < 16)) break;
       double t;
@@ -774,6 +775,22 @@ static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, int nslots, do
       }
     }
     if (!rc) aar_set_xy_split_group(sizeof(T) == 8, N, bs);
+    // then the tile shape of the pairs' y pass (its loads apply the output factors, one output
+    // per workgroup, so the shape that suits the plain y pass need not suit it), at that group
+    double bestv = 1e30;
+    int bv = -1;
+    for (int v = 0; v < kXYVariantCount && !rc; ++v) {
+      aar_set_xy_variant(sizeof(T) == 8, N, 2, v);
+      double t = 1e30;
+      const int r = time_calls([&](int) { return Pipeline<T>::xy_split(d, xs, g, tw, ctx->stream, bs); }, t);
+      if (r) rc = r;
+      if (split_sec) split_sec[5 + v] = t < 1e30 ? t : -1.0;
+      if (t < bestv) {
+        bestv = t;
+        bv = v;
+      }
+    }
+    aar_set_xy_variant(sizeof(T) == 8, N, 2, rc ? -1 : bv);
     hipFree(ones);
   }
   hipEventDestroy(a);
@@ -825,7 +842,7 @@ extern "C" int aar_autotune_report(aar_ctx* ctx, int n, int precision, int nf, d
   if (pair_seconds)
     for (int q = 0; q < 4 * kXYPlanCount; ++q) pair_seconds[q] = -1.0;
   if (split_seconds)
-    for (int q = 0; q < 5; ++q) split_seconds[q] = -1.0;
+    for (int q = 0; q < 9; ++q) split_seconds[q] = -1.0;
   return precision == AAR_FP32
              ? autotune_t<float>(ctx, n, nf, nullptr, 0, tile_seconds, nullptr, nullptr, split_seconds, pair_seconds)
              : autotune_t<double>(ctx, n, nf, nullptr, 0, tile_seconds, nullptr, nullptr, split_seconds, pair_seconds);
@@ -856,7 +873,8 @@ extern "C" int aar_set_xy_radix_plan(int n, int precision, int plan) {
 }
 
 extern "C" int aar_set_xy_tile_variant(int n, int precision, int axis, int variant) {
-  if (!pow2(n) || axis < 0 || axis > 1 || variant < 0 || variant >= kXYVariantCount) return AAR_ERR_BAD_ARG;
+  if (!pow2(n) || axis < 0 || axis > 2 || variant < (axis == 2 ? -1 : 0) || variant >= kXYVariantCount)
+    return AAR_ERR_BAD_ARG;
   if (precision != AAR_FP32 && precision != AAR_FP64) return AAR_ERR_BAD_ARG;
   aar_set_xy_variant(precision == AAR_FP64, n, axis, variant);
   std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -865,7 +883,7 @@ extern "C" int aar_set_xy_tile_variant(int n, int precision, int axis, int varia
 }
 
 extern "C" int aar_xy_tile_variant(int n, int precision, int axis) {
-  if (!pow2(n) || axis < 0 || axis > 1) return -1;
+  if (!pow2(n) || axis < 0 || axis > 2) return -1;
   return aar_xy_variant(precision == AAR_FP64, n, axis);
 }
 

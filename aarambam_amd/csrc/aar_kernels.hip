@@ -69,18 +69,28 @@ template <typename T, int L, int V = 0> struct XYCfg {
 // Variant table: [precision 0 FP32 / 1 FP64][log2 N][axis]
 // Tuning tables, read by every launch and written by the autotuner or a pin from any thread.
 // Only powers of two have entries; other mesh sizes run the mixed-radix passes (0 everywhere).
-static std::atomic<int> g_xy_variant[2][11][2] = {};
+// Axis 2 is the y pass of the multi-output pairs (xy_split), whose loads also apply the output
+// factors; -1 there follows axis 1.
+static std::atomic<int> g_xy_variant[2][11][3] = {{{0, 0, -1}, {0, 0, -1}, {0, 0, -1}, {0, 0, -1}, {0, 0, -1},
+                                                   {0, 0, -1}, {0, 0, -1}, {0, 0, -1}, {0, 0, -1}, {0, 0, -1},
+                                                   {0, 0, -1}},
+                                                  {{0, 0, -1}, {0, 0, -1}, {0, 0, -1}, {0, 0, -1}, {0, 0, -1},
+                                                   {0, 0, -1}, {0, 0, -1}, {0, 0, -1}, {0, 0, -1}, {0, 0, -1},
+                                                   {0, 0, -1}}};
 int aar_xy_variant(int fp64, int n, int axis) {
   if (!is_pow2(n)) return 0;
   int l = 0;
   while ((1 << l) < n) ++l;
-  return (l < 11 && axis >= 0 && axis < 2) ? g_xy_variant[fp64 ? 1 : 0][l][axis].load(std::memory_order_relaxed) : 0;
+  if (l >= 11 || axis < 0 || axis > 2) return 0;
+  const int v = g_xy_variant[fp64 ? 1 : 0][l][axis].load(std::memory_order_relaxed);
+  return v >= 0 ? v : g_xy_variant[fp64 ? 1 : 0][l][1].load(std::memory_order_relaxed);
 }
 void aar_set_xy_variant(int fp64, int n, int axis, int v) {
   if (!is_pow2(n)) return;
   int l = 0;
   while ((1 << l) < n) ++l;
-  if (l < 11 && axis >= 0 && axis < 2 && v >= 0 && v < kXYVariantCount) g_xy_variant[fp64 ? 1 : 0][l][axis].store(v, std::memory_order_relaxed);
+  if (l < 11 && axis >= 0 && axis < 3 && v >= (axis == 2 ? -1 : 0) && v < kXYVariantCount)
+    g_xy_variant[fp64 ? 1 : 0][l][axis].store(v, std::memory_order_relaxed);
 }
 
 // Radix plans of the x / y passes: [precision][log2 N] (aar_fft.h plan_radix)
@@ -801,7 +811,7 @@ int Pipeline<T>::xy_pass(cx_t<T>* f, int nf, const Geom& g, const cx_t<T>* tw, i
     if (axis == 0) return dir > 0 ? launch_xy_gen<T, 0, 1>(f, nf, g, tw, st) : launch_xy_gen<T, 0, -1>(f, nf, g, tw, st);
     return dir > 0 ? launch_xy_gen<T, 1, 1>(f, nf, g, tw, st) : launch_xy_gen<T, 1, -1>(f, nf, g, tw, st);
   }
-  const int v = variant >= 0 ? variant : aar_xy_variant(sizeof(T) == 8, g.N, axis);
+  const int v = variant >= 0 ? variant : aar_xy_variant(sizeof(T) == 8, g.N, xs ? 2 : axis);
   if (variant < 0 && v != 0) {  // a pinned shape that is not built for this size runs as shape 0
     const int rc = xy_pass(f, nf, g, tw, axis, dir, st, v, r, xs, xm);
     return rc == AAR_ERR_UNSUPPORTED ? xy_pass(f, nf, g, tw, axis, dir, st, 0, r, xs, xm) : rc;

@@ -195,7 +195,7 @@ def north_star_1024(aar, ctx, steps=3, fp64_steps=2):
                            "steps": fp64_steps},
             "fp32_over_fp64": round(ms64 / ms32, 3),
             "autotune": {"radix_plan": plan, "us_per_xy_pair_8_fields": [round(v * 1e6, 1) for v in psec],
-                         "xy_tile_variant": [aar.lib().aar_xy_tile_variant(N, aar.AAR_FP32, a) for a in (0, 1)],
+                         "xy_tile_variant": [aar.lib().aar_xy_tile_variant(N, aar.AAR_FP32, a) for a in (0, 1, 2)],
                          "xy_pair_group_cols": aar.lib().aar_xy_group_cols(N, aar.AAR_FP32),
                          "xy_split_group_cols": aar.lib().aar_xy_split_group_cols(N, aar.AAR_FP32)},
             "job_roofline": {"bytes_per_job": B, "frac": round(B / (ms32 * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)},
@@ -446,11 +446,11 @@ def main():
     # x / y tile shapes and x / y pair column groups for this mesh, picked by the autotuner
     # before the timed region
     pow2 = N & (N - 1) == 0  # other mesh sizes run the mixed-radix passes, which have no tuned shapes
-    tsec, tvar, psec, pchosen, ssec, gsec = ([[0] * 4] * 2, [0, 0], [0] * 3, 0, [0] * 5, [[0] * 4] * 3)
+    tsec, tvar, psec, pchosen, ssec, gsec = ([[0] * 4] * 2, [0, 0, 0], [0] * 3, 0, [0] * 9, [[0] * 4] * 3)
     if pow2:
         tsec, gsec, ssec = ctx.autotune_report(N, aar.AAR_FP32, nf=8)
         psec = [min([v for v in row if v > 0], default=-1.0) for row in gsec]
-        tvar = [aar.lib().aar_xy_tile_variant(N, aar.AAR_FP32, a) for a in (0, 1)]
+        tvar = [aar.lib().aar_xy_tile_variant(N, aar.AAR_FP32, a) for a in (0, 1, 2)]
         pchosen = aar.lib().aar_xy_radix_plan(N, aar.AAR_FP32)
     if args.xy_group >= 0 and aar.lib().aar_set_xy_group_cols(N, aar.AAR_FP32, args.xy_group) != 0:
         sys.exit(f"--xy-group {args.xy_group}: not a valid column group for N = {N}")
@@ -518,7 +518,7 @@ def main():
         # accuracy of the timed FP32 path: relative L2 error against the FP64 pairwise path, same seed,
         # on a box^3 corner of every displacement field and of the non-Gaussian potential
         out["fp32_rel_l2_vs_fp64"] = dict(sub_box=eb, seed=ERR_SEED, **rel_errors(r32["box"], r64["box"]))
-    out["autotune"] = None if not pow2 else {"xy_tile_variant": {"x": tvar[0], "y": tvar[1]},
+    out["autotune"] = None if not pow2 else {"xy_tile_variant": {"x": tvar[0], "y": tvar[1], "y_split": tvar[2]},
                        "us_per_launch_8_fields": {ax: [round(v * 1e6, 1) if v > 0 else None for v in tsec[i]]
                                                   for i, ax in enumerate(("x", "y"))},
                        "variants": "0: 128-B tile columns, 1: 64-B, 2: 256-B, 3: 128-B, four tiles per workgroup with the next tile's loads in flight (same radix plan)",
@@ -528,8 +528,11 @@ def main():
                            "per_plan": [[round(v * 1e6, 1) if v > 0 else None for v in row] for row in gsec]},
                        "xy_split_group_cols": {"chosen": xy_split_group,
                                                "us_per_call_9_outputs": [round(v * 1e6, 1) if v > 0 else None
-                                                                         for v in ssec],
+                                                                         for v in ssec[:5]],
                                                "groups": "whole passes, all columns, halves, quarters, eighths"},
+                       "xy_split_y_tile_variant": {"chosen": tvar[2],
+                                                   "us_per_call_9_outputs": [round(v * 1e6, 1) if v > 0 else None
+                                                                             for v in ssec[5:]]},
                        "radix_plan": {"chosen": pchosen,
                                       "us_per_xy_pair_8_fields": [round(v * 1e6, 1) if v > 0 else None for v in psec],
                                       "plans": "0: radices spread evenly (512 = 8 8 8, 1024 = 16 8 8), "

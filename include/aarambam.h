@@ -378,18 +378,21 @@ int aar_autotune_plans(aar_ctx *ctx, int n, int precision, int nf, double *plan_
 /* aar_autotune reporting every timing of its one search (each optional, -1 where not timed):
  * tile_seconds [2][4] as aar_autotune's; pair_seconds [3][4] mean seconds of one x + y pass over
  * nf fields per radix plan and column group (whole passes, all columns, halves, quarters);
- * split_seconds [5] mean seconds per multi-output pair call over whole passes, all columns,
- * halves, quarters, eighths.  The choices kept: aar_xy_tile_variant, aar_xy_radix_plan,
- * aar_xy_group_cols, aar_xy_split_group_cols. */
+ * split_seconds [9] mean seconds per multi-output pair call over whole passes, all columns,
+ * halves, quarters, eighths, then at the group kept per tile shape 0-3 of the pairs' y pass.  The
+ * choices kept: aar_xy_tile_variant (axes 0, 1, 2), aar_xy_radix_plan, aar_xy_group_cols,
+ * aar_xy_split_group_cols. */
 int aar_autotune_report(aar_ctx *ctx, int n, int precision, int nf, double *tile_seconds,
                         double *pair_seconds, double *split_seconds);
 /* radix plan in use for (n, precision), or a negative status; the setter pins one (0 only for
  * meshes other than 512 and 1024) */
 int aar_xy_radix_plan(int n, int precision);
 int aar_set_xy_radix_plan(int n, int precision, int plan);
-/* tile shape in use for (n, precision, axis 0 = x, 1 = y) */
+/* tile shape in use for (n, precision, axis 0 = x, 1 = y, 2 = the y pass of the multi-output
+ * pairs, whose loads also apply the output factors) */
 int aar_xy_tile_variant(int n, int precision, int axis);
-/* pin a tile shape (0, 1, 2, 3); a shape not built for n falls back to 0 at launch */
+/* pin a tile shape (0, 1, 2, 3; for axis 2 also -1: follow axis 1); a shape not built for n falls
+ * back to 0 at launch.  Every shape gives bit-identical results. */
 int aar_set_xy_tile_variant(int n, int precision, int axis, int variant);
 /* Column groups of the x / y pass pairs, also picked by aar_autotune: 0 runs each pass over all
  * fields at once; cols > 0 (a multiple of 16, at most n / 2) runs them field by field over kz

@@ -79,6 +79,41 @@ def test_split_column_groups_give_identical_results(ctx, prec, N, groups):
         assert rel_l2(res[0]["vel"], ref["vel"]) < tol
 
 
+@pytest.mark.parametrize("prec,N", [(aar.AAR_FP64, 64), (aar.AAR_FP32, 512)])
+def test_split_y_tile_shapes_give_identical_results(ctx, prec, N):
+    """The multi-output pairs' y pass has a tile shape of its own (axis 2 of the variant table):
+    every shape, and -1 (follow the y pass), gives bitwise equal jobs.  At 512^3 FP32 every shape
+    is built; the real fields are compared on an all-x, all-y slab of four z planes, which every
+    (x, kz) column of the y pass reaches through the z transform."""
+    lib = aar.lib()
+    saved = lib.aar_xy_tile_variant(N, prec, 2)
+    cfg = cases.base_config(Nmesh=N, Nsample=N, Nmax=N, N_modes=12, Seed=91)
+    alpha, x0, glass, gbox = cases.job_inputs(cfg, cases.le_rows())
+    tabs = aar.tables(alpha, x0, glass, gbox)
+    res = []
+    try:
+        for v in (-1, 0, 1, 2, 3):
+            assert lib.aar_set_xy_tile_variant(N, prec, 2, v) == 0
+            assert lib.aar_xy_tile_variant(N, prec, 2) == (lib.aar_xy_tile_variant(N, prec, 1) if v < 0 else v)
+            job = aar.Job(ctx, aar.params_from_dict(cfg), tabs, prec, aar.AAR_SCHED_BATCHED)
+            job.run()
+            job.sync()
+            if N <= 64:
+                res.append({n: job.field(n) for n in FIELDS})
+            else:
+                res.append({n: job.field_box(n, (0, 0, 5), (N, N, 4)) for n in FIELDS[:6]})
+            job.close()
+    finally:
+        lib.aar_set_xy_tile_variant(N, prec, 2, saved)
+    assert lib.aar_set_xy_tile_variant(N, prec, 2, 4) == AAR_ERR_BAD_ARG
+    assert lib.aar_set_xy_tile_variant(N, prec, 2, -2) == AAR_ERR_BAD_ARG
+    assert lib.aar_set_xy_tile_variant(N, prec, 1, -1) == AAR_ERR_BAD_ARG
+    assert np.abs(res[0]["disp2_x"]).max() > 0
+    for r in res[1:]:
+        for n in res[0]:
+            np.testing.assert_array_equal(r[n], res[0][n], err_msg=n)
+
+
 def test_split_group_setter_rejects_bad_widths():
     lib = aar.lib()
     assert lib.aar_set_xy_split_group_cols(512, aar.AAR_FP32, 8) == AAR_ERR_BAD_ARG
@@ -89,9 +124,10 @@ def test_split_group_setter_rejects_bad_widths():
 
 def test_split_autotune_reports_every_choice(ctx):
     """One tuner search at 256^3 times whole passes, all columns, halves, quarters and eighths of
-    the multi-output pairs and keeps one of them."""
+    the multi-output pairs, then the tile shapes of their y pass, and keeps one of each."""
     ts, ps, ss = ctx.autotune_report(256, aar.AAR_FP32, nf=2)
     assert np.all(ss > 0), ss
+    assert aar.lib().aar_xy_tile_variant(256, aar.AAR_FP32, 2) in (0, 1, 2, 3)
     assert np.all(ts[:, 0] > 0) and np.all(ps[0] > 0)
     assert aar.lib().aar_xy_split_group_cols(256, aar.AAR_FP32) in (0, 128, 64, 32, 16)
 
