@@ -682,12 +682,14 @@ static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, int nslots, do
   hipEventCreate(&a);
   hipEventCreate(&b);
   const int iters = 4;
+  bool built_y[kXYVariantCount] = {};  // tile shapes built for the y pass at this size
   for (int axis = 0; axis < 2 && !rc; ++axis) {
     double best = 1e30;
     int bv = 0;
     for (int v = 0; v < kXYVariantCount && !rc; ++v) {
       double t = 1e30;
       int r = Pipeline<T>::xy_pass(d, nf, g, tw, axis, +1, ctx->stream, v);
+      if (axis == 1) built_y[v] = r == AAR_OK;
       if (r == AAR_OK) {
         hipEventRecord(a, ctx->stream);
         for (int q = 0; q < iters && !r; ++q) r = Pipeline<T>::xy_pass(d, nf, g, tw, axis, q % 2 ? -1 : +1, ctx->stream, v);
@@ -780,6 +782,7 @@ static int autotune_t(aar_ctx* ctx, int N, int nf, void* scratch, int nslots, do
     double bestv = 1e30;
     int bv = -1;
     for (int v = 0; v < kXYVariantCount && !rc; ++v) {
+      if (!built_y[v]) continue;  // not built here: it would run as shape 0
       aar_set_xy_variant(sizeof(T) == 8, N, 2, v);
       double t = 1e30;
       const int r = time_calls([&](int) { return Pipeline<T>::xy_split(d, xs, g, tw, ctx->stream, bs); }, t);

@@ -126,8 +126,8 @@ def test_split_autotune_reports_every_choice(ctx):
     """One tuner search at 256^3 times whole passes, all columns, halves, quarters and eighths of
     the multi-output pairs, then the tile shapes of their y pass, and keeps one of each."""
     ts, ps, ss = ctx.autotune_report(256, aar.AAR_FP32, nf=2)
-    assert np.all(ss > 0), ss
-    assert aar.lib().aar_xy_tile_variant(256, aar.AAR_FP32, 2) in (0, 1, 2, 3)
+    assert np.all(ss[:8] > 0) and ss[8] == -1, ss  # y shape 3 is not built at 256^3
+    assert aar.lib().aar_xy_tile_variant(256, aar.AAR_FP32, 2) in (0, 1, 2)
     assert np.all(ts[:, 0] > 0) and np.all(ps[0] > 0)
     assert aar.lib().aar_xy_split_group_cols(256, aar.AAR_FP32) in (0, 128, 64, 32, 16)
 
