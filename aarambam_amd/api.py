@@ -49,7 +49,7 @@ class AarCosmo(C.Structure):
 
 
 EXPORTS = ["aar_status_string", "aar_read_params", "aar_read_params_res", "aar_load_alpha", "aar_load_x0", "aar_load_transfer",
-           "aar_read_glass", "aar_cosmology", "aar_seedtable", "aar_ranlxd_nth", "aar_write_gadget",
+           "aar_read_glass", "aar_cosmology", "aar_seedtable", "aar_seedtable_threads", "aar_ranlxd_nth", "aar_write_gadget",
            "aar_write_gadget_files", "aar_task_layout", "aar_write_gadget_tasks", "aar_write_phi_tasks",
            "aar_write_phi", "aar_write_inputspec", "aar_ctx_create", "aar_ctx_destroy", "aar_ctx_sync", "aar_job_bytes",
            "aar_job_prepare", "aar_job_run", "aar_job_sync", "aar_job_device_output", "aar_job_fetch",
@@ -97,6 +97,7 @@ def lib():
             "aar_read_glass": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), fp, dp, C.POINTER(C.c_uint8)]),
             "aar_cosmology": (C.c_int, [PP, TP, CP]),
             "aar_seedtable": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_uint32)]),
+            "aar_seedtable_threads": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_uint32), C.c_int]),
             "aar_ranlxd_nth": (C.c_ulong, [C.c_ulong, C.c_int, C.c_int]),
             "aar_write_gadget": (C.c_int, [PP, CP, fp, fp, C.c_longlong, C.POINTER(C.c_uint8)]),
             "aar_write_gadget_files": (C.c_int, [PP, CP, fp, fp, C.c_longlong, C.POINTER(C.c_uint8), C.c_int,
@@ -244,9 +245,15 @@ def cosmology(params, tabs):
     return {n: getattr(c, n) for n, _ in AarCosmo._fields_}
 
 
-def seedtable(N, seed):
+def seedtable(N, seed, threads=None):
+    """The reference's N x N per-pencil seed table; threads: host threads that fill it (default:
+    the library's choice), bit-identical for every count."""
     out = np.empty(N * N, np.uint32)
-    _check(lib().aar_seedtable(N, seed, out.ctypes.data_as(C.POINTER(C.c_uint32))), "aar_seedtable")
+    ptr = out.ctypes.data_as(C.POINTER(C.c_uint32))
+    if threads is None:
+        _check(lib().aar_seedtable(N, seed, ptr), "aar_seedtable")
+    else:
+        _check(lib().aar_seedtable_threads(N, seed, ptr, threads), "aar_seedtable_threads")
     return out.reshape(N, N)
 
 

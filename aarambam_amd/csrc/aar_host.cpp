@@ -3,6 +3,7 @@
 // input-spectrum writers.  Semantics follow Aarambam/LPT/Basis for one
 // process holding the whole grid (NTask = NTaskWithN = 1).
 #include <math.h>
+#include <cmath>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -10,6 +11,7 @@
 #include <algorithm>
 #include <string>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "../../include/aarambam.h"
@@ -538,13 +540,191 @@ int aar_host_transfer_tables(const aar_params* p, const aar_tables* t, std::vect
   return AAR_OK;
 }
 
+// ------------------------------------------------------------------ ranlxd jump-ahead
+// The subtract-with-borrow recursion x_n = x_{n-5} - x_{n-12} - c_{n-1} (mod b = 2^48) is a
+// multiplicative congruential generator in another form (Marsaglia and Zaman 1991; Tezuka,
+// L'Ecuyer and Couture 1993).  With m = b^12 - b^5 + 1 and
+//   X_n = sum_{j<12} x_{n-12+j} b^j - sum_{j<5} x_{n-5+j} b^j + c_{n-1},
+// the recursion gives b X_{n+1} = X_n + x_n m exactly, so X_{n+k} = b^-k X_n (mod m), and the
+// outputs follow from the true X: x_n = -X_n mod b (m = 1 mod b).  X_n lies in (-b^5, b^12],
+// so its residue fixes it except within b^5 of a multiple of m, where the jump declines.
+namespace {
+constexpr int kBigLimbs = 20;  // 1280 bits: products of two residues below 2^640
+struct Big {
+  uint64_t w[kBigLimbs] = {};
+};
+int big_cmp(const Big& a, const Big& b) {
+  for (int i = kBigLimbs - 1; i >= 0; --i)
+    if (a.w[i] != b.w[i]) return a.w[i] < b.w[i] ? -1 : 1;
+  return 0;
+}
+void big_add(Big& a, const Big& b) {
+  unsigned __int128 c = 0;
+  for (int i = 0; i < kBigLimbs; ++i) {
+    c += (unsigned __int128)a.w[i] + b.w[i];
+    a.w[i] = (uint64_t)c;
+    c >>= 64;
+  }
+}
+void big_sub(Big& a, const Big& b) {  // a >= b
+  uint64_t br = 0;
+  for (int i = 0; i < kBigLimbs; ++i) {
+    const uint64_t x = a.w[i], y = b.w[i];
+    const uint64_t d = x - y - br;
+    br = (x < y) || (x - y < br);
+    a.w[i] = d;
+  }
+}
+Big big_shl(const Big& a, int bits) {
+  Big r;
+  const int q = bits / 64, s = bits % 64;
+  for (int i = kBigLimbs - 1; i >= q; --i) {
+    uint64_t v = a.w[i - q] << s;
+    if (s && i - q - 1 >= 0) v |= a.w[i - q - 1] >> (64 - s);
+    r.w[i] = v;
+  }
+  return r;
+}
+Big big_shr(const Big& a, int bits) {
+  Big r;
+  const int q = bits / 64, s = bits % 64;
+  for (int i = 0; i + q < kBigLimbs; ++i) {
+    uint64_t v = a.w[i + q] >> s;
+    if (s && i + q + 1 < kBigLimbs) v |= a.w[i + q + 1] << (64 - s);
+    r.w[i] = v;
+  }
+  return r;
+}
+Big big_low(const Big& a, int bits) {  // a mod 2^bits
+  Big r = a;
+  const int q = bits / 64, s = bits % 64;
+  for (int i = q; i < kBigLimbs; ++i) r.w[i] = (i == q && s) ? (r.w[i] & ((1ull << s) - 1)) : 0;
+  return r;
+}
+Big big_mul(const Big& a, const Big& b) {  // low 1280 bits of a b
+  Big r;
+  for (int i = 0; i < kBigLimbs; ++i) {
+    if (!a.w[i]) continue;
+    unsigned __int128 c = 0;
+    for (int j = 0; i + j < kBigLimbs; ++j) {
+      c += (unsigned __int128)a.w[i] * b.w[j] + r.w[i + j];
+      r.w[i + j] = (uint64_t)c;
+      c >>= 64;
+    }
+  }
+  return r;
+}
+Big big_digits(const uint64_t* d, int n) {  // sum_j d[j] 2^(48 j), d[j] < 2^48
+  Big r;
+  for (int j = n - 1; j >= 0; --j) {
+    r = big_shl(r, 48);
+    r.w[0] |= d[j];
+  }
+  return r;
+}
+const Big& swb_m() {  // 2^576 - 2^240 + 1
+  static const Big m = [] {
+    Big one, t;
+    one.w[0] = 1;
+    t = big_shl(one, 576);
+    big_sub(t, big_shl(one, 240));
+    big_add(t, one);
+    return t;
+  }();
+  return m;
+}
+void big_reduce(Big& p) {  // p mod m, folding 2^576 = 2^240 - 1 (mod m)
+  for (Big h = big_shr(p, 576); big_cmp(h, Big()) != 0; h = big_shr(p, 576)) {
+    Big t = big_low(p, 576);
+    big_add(t, big_shl(h, 240));
+    big_sub(t, h);
+    p = t;
+  }
+  while (big_cmp(p, swb_m()) >= 0) big_sub(p, swb_m());
+}
+Big big_mulmod(const Big& a, const Big& b) {
+  Big p = big_mul(a, b);
+  big_reduce(p);
+  return p;
+}
+// b^-k mod m, b = 2^48: b^-1 = m - (m - 1) / b
+Big swb_inv_pow(uint64_t k) {
+  Big mm1 = swb_m(), one;
+  one.w[0] = 1;
+  big_sub(mm1, one);
+  Big base = swb_m();
+  big_sub(base, big_shr(mm1, 48));
+  Big r = one;
+  while (k) {
+    if (k & 1) r = big_mulmod(r, base);
+    base = big_mulmod(base, base);
+    k >>= 1;
+  }
+  return r;
+}
+}  // namespace
+
+// The generator `steps` (>= 12) recursion steps further on, registers oldest first as after
+// ranlxd_update; false (r unchanged) where the jump declines.
+bool aar_ranlxd_jump(Ranlxd& r, uint64_t steps) {
+  if (steps < 12) return false;
+  const Big& m = swb_m();
+  // residue of X_n = A - B + c
+  Big x = big_digits(r.s, 12);
+  Big c;
+  c.w[0] = r.carry;
+  big_add(x, c);
+  big_reduce(x);
+  const Big bb = big_digits(r.s + 7, 5);
+  if (big_cmp(x, bb) < 0) big_add(x, m);
+  big_sub(x, bb);
+  // residue of X_{n+steps-12}, then the true value
+  x = big_mulmod(x, swb_inv_pow(steps - 12));
+  Big lo, hi = m;
+  lo.w[0] = 1;
+  lo = big_shl(lo, 240);
+  big_sub(hi, lo);
+  if (big_cmp(x, lo) < 0 || big_cmp(x, hi) > 0) return false;
+  // twelve outputs x_n = -X_n mod b, X_{n+1} = (X_n + x_n m) / b
+  uint64_t s[12];
+  for (int j = 0; j < 12; ++j) {
+    const uint64_t xn = (0 - x.w[0]) & kMask48;
+    s[j] = xn;
+    Big t = m;
+    Big k;
+    k.w[0] = xn;
+    t = big_mul(t, k);
+    big_add(x, t);
+    x = big_shr(x, 48);
+  }
+  // the borrow: c = X - A + B, 0 or 1
+  big_add(x, big_digits(s + 7, 5));
+  const Big a = big_digits(s, 12);
+  if (big_cmp(x, a) < 0) return false;
+  big_sub(x, a);
+  Big one;
+  one.w[0] = 1;
+  if (big_cmp(x, one) > 0) return false;
+  for (int j = 0; j < 12; ++j) r.s[j] = s[j];
+  r.carry = x.w[0];
+  return true;
+}
+
 // ------------------------------------------------------------------ seed table
-extern "C" int aar_seedtable(int N, int seed, uint32_t* st) {
-  if (N < 2 || !st) return AAR_ERR_BAD_ARG;
+// Draws of shells [i0, i1) of the fill order of main.c:321-349; shell i takes 8 i + 4 draws, so
+// shell i starts at draw 4 i^2.
+static void seed_shells(int N, uint32_t seed, int i0, int i1, uint32_t* st) {
   RanlxdStream g;
-  g.seed((uint32_t)seed);
+  g.seed(seed);
+  const uint64_t d0 = 4ull * i0 * i0;
+  if (d0) {  // the state after d0 / 12 + 1 updates, at output d0 % 12
+    const uint64_t u = d0 / 12 + 1;
+    if (!aar_ranlxd_jump(g.r, 202 * u))
+      for (uint64_t q = 0; q < u; ++q) ranlxd_update(g.r);
+    g.pos = (int)(d0 % 12);
+  }
   auto draw = [&]() { return (uint32_t)(0x7fffffff * g.uniform()); };
-  for (int i = 0; i < N / 2; i++) {  // main.c:321-349
+  for (int i = i0; i < i1; i++) {
     for (int j = 0; j < i; j++) st[i * N + j] = draw();
     for (int j = 0; j < i + 1; j++) st[j * N + i] = draw();
     for (int j = 0; j < i; j++) st[(N - 1 - i) * N + j] = draw();
@@ -554,7 +734,32 @@ extern "C" int aar_seedtable(int N, int seed, uint32_t* st) {
     for (int j = 0; j < i; j++) st[(N - 1 - i) * N + (N - 1 - j)] = draw();
     for (int j = 0; j < i + 1; j++) st[(N - 1 - j) * N + (N - 1 - i)] = draw();
   }
+}
+
+// One serial stream of N^2 draws, filled by up to eight threads that each jump the generator to
+// the first draw of their shells (equal numbers of draws: shell boundaries at h sqrt(t / T)).
+extern "C" int aar_seedtable_threads(int N, int seed, uint32_t* st, int threads) {
+  if (N < 2 || !st || threads < 1) return AAR_ERR_BAD_ARG;
+  const int h = N / 2;
+  const int T = std::min(threads, std::max(1, h / 16));
+  if (T == 1) {
+    seed_shells(N, (uint32_t)seed, 0, h, st);
+    return AAR_OK;
+  }
+  std::vector<std::thread> pool;
+  int i0 = 0;
+  for (int t = 1; t <= T; ++t) {
+    const int i1 = t == T ? h : std::min(h, (int)std::lround(h * std::sqrt((double)t / T)));
+    if (i1 > i0) pool.emplace_back(seed_shells, N, (uint32_t)seed, i0, i1, st);
+    i0 = std::max(i0, i1);
+  }
+  for (auto& th : pool) th.join();
   return AAR_OK;
+}
+
+extern "C" int aar_seedtable(int N, int seed, uint32_t* st) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  return aar_seedtable_threads(N, seed, st, N >= 512 ? (int)std::min(8u, std::max(1u, hw)) : 1);
 }
 
 extern "C" unsigned long aar_ranlxd_nth(unsigned long seed, int luxury, int n) {

@@ -143,6 +143,12 @@ int aar_load_transfer(const char *path, int *nrows, double *k, double *t);
 int aar_read_glass(const char *path, int *nglass, float *pos, double *box, uint8_t *header_out);
 int aar_cosmology(const aar_params *p, const aar_tables *t, aar_cosmo *out);
 int aar_seedtable(int nmesh, int seed, uint32_t *out /* nmesh^2 */);
+/* The same table filled by up to `threads` host threads (aar_seedtable uses up to eight from
+ * nmesh 512 on): each jumps the one ranlxd stream to the first draw of its share (the
+ * subtract-with-borrow recursion as a congruential generator modulo 2^576 - 2^240 + 1), so the
+ * table is bit-identical for every thread count.  No counterpart in the reference, whose
+ * main.c:317-349 draws the table serially. */
+int aar_seedtable_threads(int nmesh, int seed, uint32_t *out, int threads);
 /* n-th output of gsl_rng_get for ranlxd (luxury 202 or 397), GSL rng_test semantics */
 unsigned long aar_ranlxd_nth(unsigned long seed, int luxury, int n);
 int aar_write_gadget(const aar_params *p, const aar_cosmo *c, const float *pos, const float *vel,

@@ -38,6 +38,24 @@ def test_seedtable_matches_oracle(N, seed):
     np.testing.assert_array_equal(api.seedtable(N, seed), orc.seedtable(N, seed))
 
 
+@pytest.mark.parametrize("N,seed", [(96, 5), (512, 2 ** 31 - 1), (1024, 0), (1024, 77)])
+def test_threaded_seedtable_matches_the_serial_stream(N, seed):
+    """Threads that jump the ranlxd stream ahead to their first draw (shell i starts at draw 4 i^2:
+    the jumps land on every register position the fill order reaches, after up to 87,000
+    updates) give the oracle's serial table, whatever the thread count."""
+    ref = orc.seedtable(N, seed)
+    np.testing.assert_array_equal(api.seedtable(N, seed), ref)
+    for t in (1, 2, 3, 8, 16):
+        np.testing.assert_array_equal(api.seedtable(N, seed, threads=t), ref, err_msg=f"threads={t}")
+
+
+def test_seedtable_threads_rejects_bad_arguments():
+    out = np.empty(16, np.uint32)
+    ptr = out.ctypes.data_as(C.POINTER(C.c_uint32))
+    assert aar.lib().aar_seedtable_threads(4, 1, ptr, 0) == 100001
+    assert aar.lib().aar_seedtable_threads(1, 1, ptr, 2) == 100001
+
+
 @pytest.mark.parametrize("which", [1, 2])
 def test_cosmology_bitwise_equal_to_oracle(which):
     tk, tt = cases.load_transfer()
